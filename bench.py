@@ -1,0 +1,200 @@
+"""Benchmark: global Pearson repetition map, 1024x1024 image, 11x11 blocks.
+
+One step = the whole global-repetition pipeline on an image already in HBM
+(local statistics, noise variance, relevance mask, compaction, block gather,
+split-BF16 MFMA contraction with the fused epilogue, FP64 row reduction,
+finalize and normalisation): sred_global_map_2d_dev.
+
+Unit: one ordered (reference block, test block) pair over the interior,
+N^2 with N = (1024-10)^2 = 1,028,196; value = N^2 / t / 1e9 (Gblock-pairs/s).
+
+The JSON line also carries: the plain-FP32 form timed the same way and the
+speed-up; max/mean abs error of both normalised maps against the FP64
+oracle on sampled pixels (plus the map extremes) and between the two forms
+over the whole map; the roofline of the dominant kernel (the contraction,
+timed with HIP events on its own stream); and the oracle's float32
+restatement of the reference timed on host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W = H = 1024
+B = 11
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--oracle-rows", type=int, default=96)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def host_cores():
+    n = os.environ.get("OMP_NUM_THREADS")
+    if n and n.isdigit():
+        return int(n)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the contraction from the committed PMC summary."""
+    p = os.path.join(ROOT, "profiles", "pmc_global_pearson_bf16x3_1024_b11.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (see DESIGN.md for sharding)"}))
+        sys.exit(2)
+    import torch
+
+    import oracle as O
+    import sred_amd
+    from sred_amd import _lib as L
+    from sred_amd import synthetic
+    from sred_amd.commands import global_map_device
+
+    img = O.normalize_input(synthetic.lattice_and_blobs(W, seed=2))
+    N = (W - 2 * (B // 2)) * (H - 2 * (B // 2))
+    Kb = sred_amd.block_size_2d(B, B)
+    pairs = float(N) * N
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)  # default priority
+    d_img = torch.from_numpy(img).to(dev)
+    d_out = {p: torch.zeros_like(d_img) for p in ("bf16x3", "fp32")}
+    ctx = sred_amd.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+
+    def step(prec):
+        global_map_device(ctx, L.PEARSON, L.BF16X3 if prec == "bf16x3" else L.FP32, d_img.data_ptr(), W, H, B, B, 0.0,
+                          True, d_out[prec].data_ptr())
+
+    def timed(prec, steps, warmup):
+        for _ in range(warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        kern = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(prec)
+            kern.append(ctx.last_contraction()[0])
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, float(np.mean(kern)), ctx.last_contraction()[2]
+
+    with torch.cuda.stream(stream):
+        t_mfma, k_mfma, pairs_eval = timed("bf16x3", args.steps, args.warmup)
+        print(f"# bf16x3: {t_mfma * 1e3:.1f} ms/step, contraction {k_mfma:.1f} ms", file=sys.stderr, flush=True)
+        t_f32, k_f32, _ = timed("fp32", args.fp32_steps, 1)
+        print(f"# fp32:   {t_f32 * 1e3:.1f} ms/step, contraction {k_f32:.1f} ms", file=sys.stderr, flush=True)
+    map_mfma = d_out["bf16x3"].cpu().numpy()
+    map_f32 = d_out["fp32"].cpu().numpy()
+    ctx.close()
+
+    # ---- error vs the FP64 oracle on sampled pixels plus the map extremes ----
+    r = B // 2
+    rng = np.random.default_rng(123)
+    inner = np.zeros((H, W), bool)
+    inner[r:H - r, r:W - r] = True
+    flat_idx = np.flatnonzero(inner)
+    order = np.argsort(map_f32.ravel()[flat_idx])
+    extremes = np.concatenate([flat_idx[order[:4]], flat_idx[order[-4:]]])
+    sample = np.unique(np.concatenate([rng.choice(flat_idx, args.oracle_rows, replace=False), extremes]))
+    m32, s32 = O.local_stats_f32(img, B, B)
+    nv = O.noise_var(img)
+    t0 = time.perf_counter()
+    S64, W64 = O.global_accum_f64(O.PEARSON, img, s32, B, B, nv, 0.0, rows=sample)
+    t_or = time.perf_counter() - t0
+    npx = float(np.count_nonzero((s32 * s32 > 0) & inner))
+    rep64 = S64.ravel()[sample] / (W64.ravel()[sample] * npx + 1e-7)
+    ex = np.isin(sample, extremes)
+    lo, hi = rep64[ex].min(), rep64[ex].max()
+    norm64 = (rep64 - lo) / (hi - lo + 1e-7)
+    err_mfma = np.abs(map_mfma.ravel()[sample] - norm64)
+    err_f32 = np.abs(map_f32.ravel()[sample] - norm64)
+    diff_forms = np.abs(map_mfma - map_f32)
+    print(f"# oracle64 on {sample.size} rows: {t_or:.1f} s", file=sys.stderr, flush=True)
+
+    # ---- CPU baseline: the oracle's float32 restatement on host cores ----
+    cpu = None
+    if not args.no_cpu:
+        cores = host_cores()
+        rows_done, t_cpu, batch = 0, 0.0, 16
+        ref_rows = flat_idx  # row-major from the first interior pixel
+        while t_cpu < args.cpu_seconds and rows_done < ref_rows.size:
+            rows = ref_rows[rows_done:rows_done + batch]
+            t0 = time.perf_counter()
+            O.global_accum_f32(O.PEARSON, img, m32, s32, B, B, nv, 0.0, rows=rows, nthreads=cores)
+            t_cpu += time.perf_counter() - t0
+            rows_done += rows.size
+            batch = min(batch * 2, 4096)
+        cpu = {"value": rows_done * float(N) / t_cpu / 1e9, "unit": "Gblock-pairs/s", "cores": cores, "kind": "port",
+               "sample": f"oracle float32 restatement of kernelGetGlobalPearson2D.cl: first {rows_done} interior "
+                         f"reference pixels (row-major) x all {N} test pixels, {t_cpu:.1f} s on {cores} threads"}
+        print(f"# cpu: {cpu['value']:.4f} Gpairs/s", file=sys.stderr, flush=True)
+
+    algo_flops = pairs * 2 * Kb
+    achieved = algo_flops / (k_mfma * 1e-3) / 1e12
+    n_pad = int(np.sqrt(pairs_eval)) if pairs_eval else N
+    kpad = (Kb + 31) // 32 * 32
+    result = {
+        "metric": "Gblock-pairs/s, global Pearson map 1024² b11: MFMA vs FP32 at err≤1e-4",
+        "value": pairs / t_mfma / 1e9,
+        "unit": "Gblock-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mfma * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3",
+        "data": "synthetic: ring lattice (pitch 24, rotated) + 900 Gaussian blobs, noise 0.05, seed 2",
+        "config": {"workload": "global repetition, Pearson, 1024x1024, 11x11 blocks (K=81), relevance 0, normalised",
+                   "image": [W, H], "block": [B, B], "K": Kb, "pairs": pairs, "precision": "bf16x3 (split BF16 on MFMA, FP32 accumulate, FP64 row sums)"},
+        "fp32": {"value": pairs / t_f32 / 1e9, "ms_per_step": t_f32 * 1e3, "contraction_ms": k_f32},
+        "speedup_vs_fp32": t_f32 / t_mfma,
+        "contraction_ms": k_mfma,
+        "error_vs_fp64": {"samples": int(sample.size), "bf16x3_max": float(err_mfma.max()),
+                          "bf16x3_mean": float(err_mfma.mean()), "fp32_max": float(err_f32.max()),
+                          "fp32_mean": float(err_f32.mean()), "bf16x3_vs_fp32_full_map_max": float(diff_forms.max())},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic_from_profile(),
+                     "issued_tflops": float(n_pad) ** 2 * 2 * kpad * 3 / (k_mfma * 1e-3) / 1e12,
+                     "kernel": "k_contract_mfma<Pearson, KS=3>"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

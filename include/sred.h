@@ -1,0 +1,146 @@
+/*
+ * sred.h -- C ABI of libsred_hip.so, the MI355X (gfx950) implementation of
+ * SReD's 2D block-correlation path.
+ *
+ * The library replaces the device side of SReD's CLUtils (JOCL/OpenCL) for
+ * 2D global and block repetition.  Every entry point is extern "C", takes
+ * plain pointers and sizes, returns an int status (SRED_OK == 0, < 0 on
+ * error) and never throws; sred_last_error() gives the message of the last
+ * failure on the calling thread.
+ *
+ * Buffers: entry points without a _dev suffix take HOST pointers (what a
+ * JNI binding of CLUtils passes: Java float[] arrays) and are synchronous.
+ * _dev entry points take DEVICE pointers and enqueue on the context's stream
+ * (sred_ctx_set_stream); they return when the work is enqueued, except where
+ * stated.  All images are row-major float32, index y*width+x.
+ *
+ * Citations name the reference (SReD) file:line each entry point replaces.
+ */
+#ifndef SRED_H
+#define SRED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define SRED_OK 0
+#define SRED_ERR_ARG (-1)            /* invalid argument (even block size, NULL, size out of range) */
+#define SRED_ERR_HIP (-2)            /* HIP runtime failure */
+#define SRED_ERR_NONBINARY_MASK (-3) /* Utils.java:1002-1004 IllegalArgumentException */
+#define SRED_ERR_UNKNOWN_METRIC (-4) /* CLUtils.java:2193-2195 returns null */
+#define SRED_ERR_NONFINITE (-5)      /* Utils.java:861-884 NaN/Inf in a noise tile */
+#define SRED_ERR_NO_DEVICE (-6)      /* no HIP device visible */
+
+/* ---- metrics (GlobalRepetition2D_.java:20-25, BlockRepetition2D_.java:18-23) ---- */
+#define SRED_PEARSON 0
+#define SRED_SSIM 1
+#define SRED_NRMSE 2
+#define SRED_ABSDIFF 3
+
+/* ---- contraction precision of the global maps ---- */
+#define SRED_FP32 0   /* plain FP32 vector-ALU contraction */
+#define SRED_BF16X3 1 /* split-BF16 (hi*hi + hi*lo + lo*hi) on MFMA, FP32 accumulate */
+
+/* Largest supported block side (pixels); K <= 3200. */
+#define SRED_MAX_BLOCK_SIDE 63
+
+typedef struct sred_ctx sred_ctx;
+
+/* ---- library / context ---- */
+int sred_version(void);
+const char *sred_last_error(void);
+/* Replaces CLUtils.getOpenCLResources (CLUtils.java:169-238): one context per
+ * device, reusable (unlike OpenCLResources, which each global call releases,
+ * CLUtils.java:1773).  Not thread-safe: one context per caller thread. */
+int sred_ctx_create(int device, sred_ctx **out);
+int sred_ctx_destroy(sred_ctx *ctx);
+/* Stream for _dev entry points (a hipStream_t; NULL = the null stream). */
+int sred_ctx_set_stream(sred_ctx *ctx, void *stream);
+int sred_ctx_synchronize(sred_ctx *ctx);
+/* Column-chunk override for the global contraction (0 = automatic). */
+int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols);
+
+/* Metric from the reference's dialog string; block != 0 uses the block
+ * command's list ("NRMSE (inverted)", "Abs. Diff. of StdDevs.").
+ * Mirrors the string dispatch of CLUtils.java:2174-2196 and :995-1010. */
+int sred_metric_from_name(const char *name, int block);
+
+/* ---- host-side mirrors of Utils (plain C++ on the host, no GPU) ---- */
+/* Elliptical block size K: GlobalRepetition2D_.java:102-111. */
+int sred_block_size_2d(int bw, int bh);
+/* Min-max input normalisation: Utils.getInputImage2D, Utils.java:708-722. */
+int sred_normalize_input_2d(float *img, int64_t n);
+/* Utils.getReferenceBlock2D (Utils.java:557-639): px receives K pixels,
+ * returns K (>0) or SRED_ERR_ARG for even sizes. */
+int sred_reference_block_2d(const float *block, int bw, int bh, float *px, float *mean, float *std_);
+/* Utils.getMeanNoiseVar2D (Utils.java:790-837). */
+int sred_noise_var_2d(const float *img, int w, int h, float *noise_var);
+/* Utils.getRelevanceMask (Utils.java:944-968) + nPixels
+ * (GlobalRepetition2D_.java:133-140). */
+int sred_relevance_mask_2d(const float *stds, int w, int h, int brw, int brh, float noise_var, float c,
+                           float *mask, float *threshold, float *n_pixels);
+/* Utils.applyMask2D (Utils.java:915-925), in place. */
+int sred_apply_mask_2d(float *map, int w, int h, const float *mask);
+/* Utils.normalizeImage2D (Utils.java:985-1031), in place; mask may be NULL. */
+int sred_normalize_map_2d(float *map, int w, int h, int brw, int brh, const float *mask);
+
+/* ---- device path, host buffers (the CLUtils methods a JNI binding replaces) ---- */
+/* CLUtils.getLocalStatistics2D (CLUtils.java:376-467) with
+ * kernelGetLocalStatistics2D.cl: float32 block mean and unbiased std. */
+int sred_local_stats_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float *means,
+                        float *stds);
+/* Raw accumulators of kernelGetGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}2D.cl
+ * over all 64x64 launch tiles of CLUtils.java:1727-1740: S (similarity sum)
+ * and Wsum (weight sum) per pixel.  filter_param and threshold are the values
+ * the reference renders into the FILTER_PARAM/THRESHOLD macros
+ * (noise variance, noise variance * relevance constant). */
+int sred_global_accum_2d(sred_ctx *ctx, int metric, int precision, const float *img, const float *means,
+                         const float *stds, int w, int h, int bw, int bh, float filter_param, float threshold,
+                         float *S, float *Wsum);
+/* CLUtils.calculateGlobalRepetitionMap2D (CLUtils.java:2174-2196) ->
+ * getGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}2D: accumulators plus the host
+ * finalize (division by Wsum*nPixels+eps, inversion for NRMSE/AbsDiff, mask
+ * if relevance_constant > 0, masked normalisation).  The inputs are those of
+ * the Java call: the image, its local statistics, the relevance mask with its
+ * constant, threshold and noise variance, and nPixels. */
+int sred_global_repetition_2d(sred_ctx *ctx, int metric, int precision, const float *img, const float *means,
+                              const float *stds, int w, int h, int bw, int bh, const float *mask,
+                              float relevance_constant, float threshold, float noise_var, float n_pixels,
+                              int normalize, float *out);
+/* CLUtils.calculateBlockRepetitionMap2D (CLUtils.java:995-1010) ->
+ * getBlock{Pearson,Ssim,Nrmse,AbsDiffStds}2D (CLUtils.java:486-981): local
+ * statistics, template comparison, relevance mask and normalisation.
+ * tmpl holds the K template pixels (ReferenceBlock2D.getPixels()). */
+int sred_block_repetition_2d(sred_ctx *ctx, int metric, const float *img, int w, int h, const float *tmpl,
+                             int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
+                             int normalize, float *out);
+
+/* ---- device path, device buffers ---- */
+/* Whole global-repetition pipeline of GlobalRepetition2D_.run
+ * (GlobalRepetition2D_.java:124-145, input already normalised, VST off):
+ * statistics, noise variance, relevance mask, nPixels, accumulators and
+ * finalize.  d_img and d_out are device pointers (w*h floats).  Optional
+ * device outputs (NULL to skip): d_means, d_stds, d_mask, d_S, d_Wsum.
+ * Synchronises once on the stream (to read the 4-byte noise variance). */
+int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int bw,
+                           int bh, float relevance_constant, int normalize, float *d_out, float *d_means,
+                           float *d_stds, float *d_mask, float *d_S, float *d_Wsum);
+/* Whole block-repetition pipeline (BlockRepetition2D_.java:141-145 ->
+ * CLUtils.java:995-1010) on device buffers; tmpl is a HOST array of K floats.
+ * Synchronises once when relevance_constant != 0 (noise variance). */
+int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl,
+                          int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
+                          int normalize, float *d_out);
+
+/* ---- introspection for measurement ---- */
+/* Average duration (ms, HIP events) and launch count of the dominant
+ * contraction kernel in the last global call on this context. */
+int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

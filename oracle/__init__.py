@@ -1,0 +1,258 @@
+"""CPU oracle for SReD's 2D block-correlation path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` import it; the product library
+(``sred_amd``) never does.
+
+It wraps ``liboracle.so`` (built from ``sred_oracle.c`` by ``oracle/Makefile``)
+which restates the reference in two forms:
+
+* ``refsim``   float32-faithful restatement of the OpenCL kernels and Java host
+  steps (the reference's own map);
+* ``oracle64`` FP64 magnitudes with compensated sums, discrete decisions taken
+  from the float32 statistics as the reference takes them.
+
+Every C function cites the reference file:line it follows.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+PEARSON, SSIM, NRMSE, ABSDIFF = 0, 1, 2, 3
+_F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liboracle.so with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_float, c_i64, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_int64, ctypes.c_double
+        vp = ctypes.c_void_p
+        L.or_footprint.argtypes = [c_int, c_int, vp, vp]
+        L.or_java_float_literal.argtypes = [c_float]
+        L.or_java_float_literal.restype = c_double
+        L.or_normalize_input.argtypes = [_F, c_i64]
+        L.or_reference_block.argtypes = [_F, c_int, c_int, _F, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_local_stats_f32.argtypes = [_F, c_int, c_int, c_int, c_int, _F, _F]
+        L.or_local_stats_f64.argtypes = [_F, c_int, c_int, c_int, c_int, _D, _D]
+        L.or_mean_var_block.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int,
+                                        ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_noise_var.argtypes = [_F, c_int, c_int]
+        L.or_noise_var.restype = c_float
+        L.or_relevance_mask.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, c_float, _F,
+                                        ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_apply_mask.argtypes = [_F, c_i64, _F]
+        L.or_normalize_map.argtypes = [_F, c_int, c_int, c_int, c_int, vp]
+        L.or_global_accum_f32.argtypes = [c_int, _F, _F, _F, c_int, c_int, c_int, c_int, c_float, c_float,
+                                          _F, _F, vp, c_i64, c_int]
+        L.or_global_accum_f64.argtypes = [c_int, _F, _F, c_int, c_int, c_int, c_int, c_float, c_float,
+                                          _D, _D, vp, c_i64, c_int]
+        L.or_global_finalize.argtypes = [c_int, _F, _F, _F, c_float, c_float, c_int, c_int, c_int, c_int, c_int, _F]
+        L.or_global_finalize_f64.argtypes = [c_int, _D, _D, _F, c_double, c_float, c_int, c_int, c_int, c_int,
+                                             c_int, _D]
+        L.or_global_map.argtypes = [c_int, c_int, _F, c_int, c_int, c_int, c_int, c_float, c_int, vp, vp, c_int]
+        L.or_block_map.argtypes = [c_int, _F, c_int, c_int, _F, c_int, c_int, c_float, c_float, c_float, c_int, _F]
+        L.or_block_raw.argtypes = [c_int, _F, _F, _F, c_int, c_int, _F, c_int, c_int, c_float, c_float, _F]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def footprint(bw, bh):
+    """(K, dx, dy) of the elliptical block, reference row-major order."""
+    k = lib().or_footprint(bw, bh, None, None)
+    dx = (ctypes.c_int * k)()
+    dy = (ctypes.c_int * k)()
+    lib().or_footprint(bw, bh, dx, dy)
+    return k, np.array(dx[:], dtype=np.int32), np.array(dy[:], dtype=np.int32)
+
+
+def java_float_literal(f):
+    return lib().or_java_float_literal(float(np.float32(f)))
+
+
+def normalize_input(img):
+    out = _f32(img).copy()
+    lib().or_normalize_input(out.reshape(-1), out.size)
+    return out
+
+
+def reference_block(block):
+    block = _f32(block)
+    bh, bw = block.shape
+    px = np.zeros(bw * bh, np.float32)
+    m, s = ctypes.c_float(), ctypes.c_float()
+    k = lib().or_reference_block(block.reshape(-1), bw, bh, px, ctypes.byref(m), ctypes.byref(s))
+    if k < 0:
+        return None
+    return px[:k].copy(), m.value, s.value
+
+
+def local_stats_f32(img, bw, bh):
+    img = _f32(img)
+    h, w = img.shape
+    m = np.zeros_like(img)
+    s = np.zeros_like(img)
+    lib().or_local_stats_f32(img.reshape(-1), w, h, bw, bh, m.reshape(-1), s.reshape(-1))
+    return m, s
+
+
+def local_stats_f64(img, bw, bh):
+    img = _f32(img)
+    h, w = img.shape
+    m = np.zeros(img.shape, np.float64)
+    v = np.zeros(img.shape, np.float64)
+    lib().or_local_stats_f64(img.reshape(-1), w, h, bw, bh, m.reshape(-1), v.reshape(-1))
+    return m, v
+
+
+def mean_var_block(img, x0, y0, x1, y1):
+    img = _f32(img)
+    h, w = img.shape
+    m, v = ctypes.c_float(), ctypes.c_float()
+    rc = lib().or_mean_var_block(img.reshape(-1), w, h, x0, y0, x1, y1, ctypes.byref(m), ctypes.byref(v))
+    if rc:
+        raise ValueError("invalid block")
+    return m.value, v.value
+
+
+def noise_var(img):
+    img = _f32(img)
+    h, w = img.shape
+    return lib().or_noise_var(img.reshape(-1), w, h)
+
+
+def relevance_mask(stds, brw, brh, noise_variance, c):
+    stds = _f32(stds)
+    h, w = stds.shape
+    mask = np.zeros_like(stds)
+    thr, npx = ctypes.c_float(), ctypes.c_float()
+    lib().or_relevance_mask(stds.reshape(-1), w, h, brw, brh, noise_variance, c, mask.reshape(-1),
+                            ctypes.byref(thr), ctypes.byref(npx))
+    return mask, thr.value, npx.value
+
+
+def apply_mask(img, mask):
+    out = _f32(img).copy()
+    lib().or_apply_mask(out.reshape(-1), out.size, _f32(mask).reshape(-1))
+    return out
+
+
+def normalize_map(img, brw, brh, mask=None):
+    out = _f32(img).copy()
+    h, w = out.shape
+    mk = None if mask is None else _f32(mask)
+    rc = lib().or_normalize_map(out.reshape(-1), w, h, brw, brh,
+                                None if mk is None else mk.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise ValueError("Relevance mask must be binary (0.0f or 1.0f)")
+    return out
+
+
+def _rows(rows):
+    if rows is None:
+        return None, 0, None
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    return r.ctypes.data_as(ctypes.c_void_p), r.size, r
+
+
+def global_accum_f32(metric, img, means, stds, bw, bh, filter_param, threshold, rows=None, nthreads=0):
+    img = _f32(img)
+    h, w = img.shape
+    S = np.zeros_like(img)
+    W = np.zeros_like(img)
+    rp, nr, _keep = _rows(rows)
+    lib().or_global_accum_f32(metric, img.reshape(-1), _f32(means).reshape(-1), _f32(stds).reshape(-1), w, h,
+                              bw, bh, filter_param, threshold, S.reshape(-1), W.reshape(-1), rp, nr, nthreads)
+    return S, W
+
+
+def global_accum_f64(metric, img, stds32, bw, bh, filter_param, threshold, rows=None, nthreads=0):
+    img = _f32(img)
+    h, w = img.shape
+    S = np.zeros(img.shape, np.float64)
+    W = np.zeros(img.shape, np.float64)
+    rp, nr, _keep = _rows(rows)
+    lib().or_global_accum_f64(metric, img.reshape(-1), _f32(stds32).reshape(-1), w, h, bw, bh, filter_param,
+                              threshold, S.reshape(-1), W.reshape(-1), rp, nr, nthreads)
+    return S, W
+
+
+def global_finalize(metric, S, W, mask, n_pixels, c, normalize, bw, bh):
+    S = _f32(S)
+    h, w = S.shape
+    out = np.zeros_like(S)
+    rc = lib().or_global_finalize(metric, S.reshape(-1), _f32(W).reshape(-1), _f32(mask).reshape(-1), n_pixels,
+                                  c, int(normalize), w, h, bw, bh, out.reshape(-1))
+    if rc:
+        raise ValueError("finalize failed")
+    return out
+
+
+def global_finalize_f64(metric, S, W, mask, n_pixels, c, normalize, bw, bh):
+    S = np.ascontiguousarray(S, np.float64)
+    h, w = S.shape
+    out = np.zeros_like(S)
+    rc = lib().or_global_finalize_f64(metric, S.reshape(-1), np.ascontiguousarray(W, np.float64).reshape(-1),
+                                      _f32(mask).reshape(-1), float(n_pixels), c, int(normalize), w, h, bw, bh,
+                                      out.reshape(-1))
+    if rc:
+        raise ValueError("finalize failed")
+    return out
+
+
+def global_map(metric, img, bw, bh, c=0.0, normalize=True, fp64=False, nthreads=0):
+    """Full global-repetition map (refsim float32, or oracle64 with fp64=True)."""
+    img = _f32(img)
+    h, w = img.shape
+    if fp64:
+        out = np.zeros(img.shape, np.float64)
+        rc = lib().or_global_map(metric, 1, img.reshape(-1), w, h, bw, bh, c, int(normalize), None,
+                                 out.ctypes.data_as(ctypes.c_void_p), nthreads)
+    else:
+        out = np.zeros(img.shape, np.float32)
+        rc = lib().or_global_map(metric, 0, img.reshape(-1), w, h, bw, bh, c, int(normalize),
+                                 out.ctypes.data_as(ctypes.c_void_p), None, nthreads)
+    if rc:
+        raise ValueError("global map failed")
+    return out
+
+
+def block_raw(metric, img, means, stds, tmpl, bw, bh, tmpl_mean, tmpl_std):
+    img = _f32(img)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    rc = lib().or_block_raw(metric, img.reshape(-1), _f32(means).reshape(-1), _f32(stds).reshape(-1), w, h,
+                            _f32(tmpl), bw, bh, tmpl_mean, tmpl_std, out.reshape(-1))
+    if rc:
+        raise ValueError("block raw failed")
+    return out
+
+
+def block_map(metric, img, tmpl, bw, bh, tmpl_mean, tmpl_std, c=0.0, normalize=True):
+    img = _f32(img)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    rc = lib().or_block_map(metric, img.reshape(-1), w, h, _f32(tmpl), bw, bh, tmpl_mean, tmpl_std, c,
+                            int(normalize), out.reshape(-1))
+    if rc:
+        raise ValueError("block map failed")
+    return out

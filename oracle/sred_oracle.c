@@ -1,0 +1,647 @@
+/*
+ * sred_oracle.c -- CPU restatement of SReD's 2D block-correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  See sred_oracle.h for the two
+ * forms (refsim float32-faithful, oracle64 FP64) and their rules.  Built
+ * with -ffp-contract=off so every float expression rounds exactly as written
+ * (Java never contracts; the reference's OpenCL compiler may, which is the
+ * one unpinned freedom of the kernels' float order).
+ *
+ * Paths in citations are relative to the SReD repository root.
+ */
+#include "sred_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EPS_F 1e-7f   /* Utils.EPSILON, Utils.java:20 */
+
+/* ---- footprint: GlobalRepetition2D_.java:102-111, CLUtils.java:391-400 ---- */
+int or_footprint(int bw, int bh, int *dx, int *dy)
+{
+    int brw = bw / 2, brh = bh / 2, k = 0;
+    for (int y = 0; y < bh; y++) {
+        for (int x = 0; x < bw; x++) {
+            float fx = (float)(x - brw), fy = (float)(y - brh);
+            /* float division by (float)(r*r): with r=0 this is x/0 -> inf/nan,
+             * exactly as the reference (only odd sizes >= 3 are meaningful) */
+            if (((fx * fx) / (float)(brw * brw)) + ((fy * fy) / (float)(brh * brh)) <= 1.0f) {
+                if (dx) { dx[k] = x - brw; dy[k] = y - brh; }
+                k++;
+            }
+        }
+    }
+    return k;
+}
+
+/* ---- Java ""+float -> OpenCL double literal (CLUtils.java:1705-1712) ---- */
+double or_java_float_literal(float f)
+{
+    char buf[64];
+    if (f == 0.0f) return 0.0;
+    for (int p = 1; p <= 9; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, (double)f);
+        if (strtof(buf, NULL) == f) return strtod(buf, NULL);
+    }
+    return (double)f;
+}
+
+/* ---- Utils.java:708-722 ---- */
+void or_normalize_input(float *img, int64_t n)
+{
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        mn = fminf(mn, img[i]);
+        mx = fmaxf(mx, img[i]);
+    }
+    for (int64_t i = 0; i < n; i++) img[i] = (img[i] - mn) / (mx - mn + EPS_F);
+}
+
+/* ---- Utils.java:557-639 ---- */
+int or_reference_block(const float *blk, int bw, int bh, float *px, float *mean, float *std_)
+{
+    if (bw % 2 == 0 || bh % 2 == 0) return -1;               /* Utils.java:574-577 */
+    int brw = bw / 2, brh = bh / 2, k = 0;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) {
+            float fx = (float)(i - brw), fy = (float)(j - brh);
+            if (((fx * fx) / (float)(brw * brw)) + ((fy * fy) / (float)(brh * brh)) <= 1.0f) px[k++] = blk[j * bw + i];
+        }
+    float bmin = FLT_MAX, bmax = -FLT_MAX;
+    for (int i = 0; i < k; i++) { bmin = fminf(bmin, px[i]); bmax = fmaxf(bmax, px[i]); }
+    float m = 0.0f;
+    for (int i = 0; i < k; i++) {
+        px[i] = (px[i] - bmin) / (bmax - bmin + EPS_F);
+        m += px[i];
+    }
+    m /= (float)k;
+    for (int i = 0; i < k; i++) px[i] = px[i] - m;
+    float s = 0.0f;
+    for (int i = 0; i < k; i++) s += (px[i] - m) * (px[i] - m);  /* Utils.java:632-634: mean subtracted twice */
+    s = (float)sqrt((double)(s / (float)(k - 1)));
+    *mean = m;
+    *std_ = s;
+    return k;
+}
+
+/* ---- kernelGetLocalStatistics2D.cl:9-55 ---- */
+void or_local_stats_f32(const float *img, int w, int h, int bw, int bh, float *means, float *stds)
+{
+    int brw = bw / 2, brh = bh / 2;
+    int K = or_footprint(bw, bh, NULL, NULL);
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < h; gy++) {
+        float blk[1024];
+        for (int gx = 0; gx < w; gx++) {
+            int64_t o = (int64_t)gy * w + gx;
+            if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { means[o] = 0.0f; stds[o] = 0.0f; continue; }
+            for (int i = 0; i < K; i++) blk[i] = img[(int64_t)(gy + dy[i]) * w + gx + dx[i]];
+            float m = 0.0f;
+            for (int i = 0; i < K; i++) m += blk[i];
+            m /= (float)K;
+            float v = 0.0f;
+            for (int i = 0; i < K; i++) v += (blk[i] - m) * (blk[i] - m);
+            v /= (float)(K - 1);
+            means[o] = m;
+            stds[o] = sqrtf(v);
+        }
+    }
+    free(dx); free(dy);
+}
+
+void or_local_stats_f64(const float *img, int w, int h, int bw, int bh, double *means, double *vars)
+{
+    int brw = bw / 2, brh = bh / 2;
+    int K = or_footprint(bw, bh, NULL, NULL);
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < h; gy++)
+        for (int gx = 0; gx < w; gx++) {
+            int64_t o = (int64_t)gy * w + gx;
+            if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { means[o] = 0.0; vars[o] = 0.0; continue; }
+            double m = 0.0;
+            for (int i = 0; i < K; i++) m += img[(int64_t)(gy + dy[i]) * w + gx + dx[i]];
+            m /= K;
+            double v = 0.0;
+            for (int i = 0; i < K; i++) {
+                double d = img[(int64_t)(gy + dy[i]) * w + gx + dx[i]] - m;
+                v += d * d;
+            }
+            means[o] = m;
+            vars[o] = v / (K - 1);
+        }
+    free(dx); free(dy);
+}
+
+/* ---- Utils.java:854-900 ---- */
+int or_mean_var_block(const float *img, int w, int h, int x0, int y0, int x1, int y1, float *mean, float *var)
+{
+    int bw = x1 - x0, bh = y1 - y0, n = bw * bh;
+    if (bw <= 0 || bh <= 0 || n <= 0) return -1;
+    if (x0 < 0 || y0 < 0 || x1 > w || y1 > h) return -1;
+    float m = 0.0f, sq = 0.0f;
+    for (int y = y0; y < y1; y++)
+        for (int x = x0; x < x1; x++) {
+            float v = img[(int64_t)y * w + x];
+            if (isnan(v) || isinf(v)) return -1;
+            m += v;
+            sq += v * v;
+        }
+    m /= (float)n;
+    float va = sq / (float)n - m * m;
+    if (isnan(va)) return -1;
+    *mean = m;
+    *var = va;
+    return 0;
+}
+
+static int cmp_float(const void *a, const void *b)
+{
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* ---- Utils.java:790-837 ---- */
+float or_noise_var(const float *img, int w, int h)
+{
+    int bw, bh;
+    if (w * h <= 352 * 288) { bw = 8; bh = 8; } else { bw = 16; bh = 16; }
+    int nbx = w / bw, nby = h / bh, nb = nbx * nby;
+    float *lv = calloc(nb > 0 ? nb : 1, sizeof(float));
+    int idx = 0;
+    for (int y = 0; y < nby; y++)
+        for (int x = 0; x < nbx; x++) {
+            float m, v;
+            or_mean_var_block(img, w, h, x * bw, y * bh, (x + 1) * bw, (y + 1) * bh, &m, &v);
+            lv[idx++] = v;
+        }
+    qsort(lv, nb, sizeof(float), cmp_float);
+    int nvars = (int)(0.03f * (float)nb + 1.0f);
+    float nv = 0.0f;
+    for (int i = 0; i < nvars && i < nb; i++) nv += lv[i];
+    nv = fabsf(nv / (float)nvars);
+    nv = (1.0f + 0.001f * (nv - 40.0f)) * nv;
+    free(lv);
+    return nv;
+}
+
+/* ---- Utils.java:944-968 and GlobalRepetition2D_.java:133-140 ---- */
+void or_relevance_mask(const float *stds, int w, int h, int brw, int brh, float noise_var, float c,
+                       float *mask, float *threshold, float *n_pixels)
+{
+    float thr = noise_var * c;
+    memset(mask, 0, sizeof(float) * (size_t)w * h);
+    float np = 0.0f;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            float s = stds[(int64_t)y * w + x];
+            mask[(int64_t)y * w + x] = (s * s <= thr) ? 0.0f : 1.0f;
+            if (mask[(int64_t)y * w + x] > 0.0f) np += 1.0f;
+        }
+    *threshold = thr;
+    *n_pixels = np;
+}
+
+void or_apply_mask(float *map, int64_t n, const float *mask)
+{
+    for (int64_t i = 0; i < n; i++) map[i] *= mask[i];
+}
+
+/* ---- Utils.java:985-1031 ---- */
+int or_normalize_map(float *map, int w, int h, int brw, int brh, const float *mask)
+{
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask) {
+                if (mask[i] != 0.0f && mask[i] != 1.0f) return -1;
+                if (!(mask[i] > 0.0f)) continue;
+            }
+            mn = fminf(map[i], mn);
+            mx = fmaxf(map[i], mx);
+        }
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask && !(mask[i] > 0.0f)) continue;
+            map[i] = (map[i] - mn) / (mx - mn + EPS_F);
+        }
+    return 0;
+}
+
+int or_normalize_map_f64(double *map, int w, int h, int brw, int brh, const float *mask)
+{
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask) {
+                if (mask[i] != 0.0f && mask[i] != 1.0f) return -1;
+                if (!(mask[i] > 0.0f)) continue;
+            }
+            mn = fmin(map[i], mn);
+            mx = fmax(map[i], mx);
+        }
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask && !(mask[i] > 0.0f)) continue;
+            map[i] = (map[i] - mn) / (mx - mn + 1e-7);
+        }
+    return 0;
+}
+
+/* ---- global accumulators, float32 refsim ----
+ * kernelGetGlobalPearson2D.cl:67-166, kernelGetGlobalSsim2D.cl:11-118,
+ * kernelGetGlobalNrmse2D.cl:11-125, kernelGetGlobalAbsDiffStds2D.cl:10-53.
+ * The kernels re-evaluate the ellipse predicate per pair; its outcome is
+ * data-independent, so the offsets are evaluated once (identical membership
+ * and order).  Accumulation over test pixels is sequential in row-major
+ * order, as in each work-item of the reference. */
+void or_global_accum_f32(int metric, const float *img, const float *means, const float *stds, int w, int h,
+                         int bw, int bh, float filter_param, float threshold, float *S, float *Wsum,
+                         const int32_t *rows, int64_t nrows, int nthreads)
+{
+    const int brw = bw / 2, brh = bh / 2;
+    const int K = or_footprint(bw, bh, NULL, NULL);
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+    const double fp_d = or_java_float_literal(filter_param);
+    const double thr_d = or_java_float_literal(threshold);
+    const double eps_d = or_java_float_literal(EPS_F);
+    int64_t *off = malloc(sizeof(int64_t) * K);
+    for (int i = 0; i < K; i++) off[i] = (int64_t)dy[i] * w + dx[i];
+    int64_t total = rows ? nrows : (int64_t)w * h;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (int64_t r = 0; r < total; r++) {
+        int64_t o = rows ? rows[r] : r;
+        int gx = (int)(o % w), gy = (int)(o / w);
+        float acc = 0.0f, wacc = 0.0f;
+        float rb[1024], tb[1024];
+        if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
+            /* border: map written 0 (NRMSE leaves it unwritten), weights untouched */
+            S[o] = 0.0f; Wsum[o] = 0.0f;
+            continue;
+        }
+        const float rs = stds[o];
+        const float rv = rs * rs;
+        if ((double)rv < thr_d) { S[o] = 0.0f; Wsum[o] = 0.0f; continue; }
+        const float rm = (metric == OR_ABSDIFF) ? 0.0f : means[o];
+        if (metric != OR_ABSDIFF) {
+            for (int i = 0; i < K; i++) rb[i] = img[o + off[i]];
+            for (int i = 0; i < K; i++) rb[i] = rb[i] - rm;
+        }
+        for (int y = brh; y < h - brh; y++) {
+            for (int x = brw; x < w - brw; x++) {
+                int64_t t = (int64_t)y * w + x;
+                const float ts = stds[t];
+                const float tv = ts * ts;
+                if ((double)tv < thr_d) continue;
+                if (metric == OR_ABSDIFF) {
+                    /* kernelGetGlobalAbsDiffStds2D.cl:46-50: exp evaluated in double */
+                    float d = rs - ts;
+                    float wt = (float)exp(-1.0 * ((double)(d * d) / (fp_d + eps_d)));
+                    wacc += wt;
+                    acc += fabsf(rs - ts) * wt;
+                    continue;
+                }
+                const float tm = means[t];
+                for (int i = 0; i < K; i++) tb[i] = img[t + off[i]];
+                for (int i = 0; i < K; i++) tb[i] = tb[i] - tm;
+                float d = rs - ts;
+                float q = (float)((double)(d * d) / (fp_d + eps_d));
+                float wt = (float)expf((float)(-1.0f) * q);
+                wacc += wt;
+                if (metric == OR_NRMSE) {
+                    float mse = 0.0f;
+                    for (int i = 0; i < K; i++) mse += (rb[i] - tb[i]) * (rb[i] - tb[i]);
+                    mse /= (float)K;
+                    acc += sqrtf(mse);
+                    continue;
+                }
+                float cov = 0.0f;
+                for (int i = 0; i < K; i++) cov += rb[i] * tb[i];
+                cov /= (float)(K - 1);
+                if (metric == OR_PEARSON) {
+                    if (rs == 0.0f && ts == 0.0f) acc += 1.0f * wt;
+                    else if (rs == 0.0f || ts == 0.0f) acc += 0.0f;
+                    else {
+                        float r = (float)((double)cov / ((double)(rs * ts) + eps_d));
+                        acc += fmaxf(0.0f, r) * wt;
+                    }
+                } else { /* SSIM, kernelGetGlobalSsim2D.cl:100-115 */
+                    const float c1 = 0.0001f, c2 = 0.0009f;
+                    if (rs == 0.0f && ts == 0.0f) acc += 1.0f * wt;
+                    else if (rs == 0.0f || ts == 0.0f) acc += 0.0f;
+                    else {
+                        float num = (2.0f * rm * tm + c1) * (2.0f * cov + c2);
+                        float den = ((rm * rm) * (tm * tm) + c1) * ((rs * rs) + (ts * ts) + c2);
+                        float ssim = num / den * wt;
+                        acc += fmaxf(0.0f, ssim);
+                    }
+                }
+            }
+        }
+        S[o] = acc;
+        Wsum[o] = wacc;
+    }
+    free(dx); free(dy); free(off);
+}
+
+/* ---- global accumulators, oracle64 ---- */
+typedef struct { double s, c; } ksum;
+static inline void kadd(ksum *k, double v)
+{   /* Neumaier compensated summation */
+    double t = k->s + v;
+    if (fabs(k->s) >= fabs(v)) k->c += (k->s - t) + v; else k->c += (v - t) + k->s;
+    k->s = t;
+}
+
+void or_global_accum_f64(int metric, const float *img, const float *stds32, int w, int h, int bw, int bh,
+                         float filter_param, float threshold, double *S, double *Wsum,
+                         const int32_t *rows, int64_t nrows, int nthreads)
+{
+    const int brw = bw / 2, brh = bh / 2;
+    const int K = or_footprint(bw, bh, NULL, NULL);
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+    const double fp_d = or_java_float_literal(filter_param);
+    const double thr_d = or_java_float_literal(threshold);
+    const double eps = 1e-7;
+    int64_t n = (int64_t)w * h;
+    double *m64 = malloc(sizeof(double) * n), *v64 = malloc(sizeof(double) * n);
+    or_local_stats_f64(img, w, h, bw, bh, m64, v64);
+    /* interior valid columns, compacted, with centred FP64 block vectors */
+    int64_t nval = 0;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            float s = stds32[(int64_t)y * w + x];
+            if (!((double)(s * s) < thr_d)) nval++;
+        }
+    int64_t *vidx = malloc(sizeof(int64_t) * (nval ? nval : 1));
+    double *vec = malloc(sizeof(double) * (nval ? nval : 1) * K);
+    nval = 0;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t t = (int64_t)y * w + x;
+            float s = stds32[t];
+            if ((double)(s * s) < thr_d) continue;
+            for (int i = 0; i < K; i++) vec[nval * K + i] = (double)img[t + (int64_t)dy[i] * w + dx[i]] - m64[t];
+            vidx[nval++] = t;
+        }
+    int64_t total = rows ? nrows : n;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (int64_t r = 0; r < total; r++) {
+        int64_t o = rows ? rows[r] : r;
+        int gx = (int)(o % w), gy = (int)(o / w);
+        if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { S[o] = 0.0; Wsum[o] = 0.0; continue; }
+        float rs32 = stds32[o];
+        if ((double)(rs32 * rs32) < thr_d) { S[o] = 0.0; Wsum[o] = 0.0; continue; }
+        double rs = sqrt(v64[o]), rm = m64[o];
+        double rb[1024];
+        for (int i = 0; i < K; i++) rb[i] = (double)img[o + (int64_t)dy[i] * w + dx[i]] - rm;
+        ksum acc = {0, 0}, wacc = {0, 0};
+        for (int64_t j = 0; j < nval; j++) {
+            int64_t t = vidx[j];
+            double ts = sqrt(v64[t]), tm = m64[t];
+            double d = rs - ts;
+            double wt = exp(-(d * d) / (fp_d + eps));
+            kadd(&wacc, wt);
+            if (metric == OR_ABSDIFF) { kadd(&acc, fabs(d) * wt); continue; }
+            const double *tb = vec + j * K;
+            if (metric == OR_NRMSE) {
+                double mse = 0.0;
+                for (int i = 0; i < K; i++) { double e = rb[i] - tb[i]; mse += e * e; }
+                kadd(&acc, sqrt(mse / K));
+                continue;
+            }
+            double cov = 0.0;
+            for (int i = 0; i < K; i++) cov += rb[i] * tb[i];
+            cov /= (K - 1);
+            int rz = (rs32 == 0.0f), tz = (stds32[t] == 0.0f);
+            if (rz && tz) { kadd(&acc, wt); continue; }
+            if (rz || tz) continue;
+            if (metric == OR_PEARSON) {
+                double rr = cov / (rs * ts + eps);
+                kadd(&acc, (rr > 0.0 ? rr : 0.0) * wt);
+            } else {
+                const double c1 = 0.0001f, c2 = 0.0009f;  /* float constants, as in the kernel */
+                double ss = (2.0 * rm * tm + c1) * (2.0 * cov + c2) / ((rm * rm) * (tm * tm) + c1) /
+                            (rs * rs + ts * ts + c2) * wt;
+                kadd(&acc, ss > 0.0 ? ss : 0.0);
+            }
+        }
+        S[o] = acc.s + acc.c;
+        Wsum[o] = wacc.s + wacc.c;
+    }
+    free(dx); free(dy); free(m64); free(v64); free(vidx); free(vec);
+}
+
+/* ---- host finalize: CLUtils.java:1747-1770 (Pearson), 1865-1887 (SSIM),
+ *      1983-2020 (NRMSE), 2112-2149 (AbsDiffStds) ---- */
+int or_global_finalize(int metric, const float *S, const float *Wsum, const float *mask, float n_pixels,
+                       float c, int normalize, int w, int h, int bw, int bh, float *out)
+{
+    int brw = bw / 2, brh = bh / 2;
+    memset(out, 0, sizeof(float) * (size_t)w * h);
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            out[i] = S[i] / (Wsum[i] * n_pixels + EPS_F);
+        }
+    if (metric == OR_NRMSE || metric == OR_ABSDIFF) {
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) {
+                int64_t i = (int64_t)y * w + x;
+                if (mask[i] > 0.0f) out[i] = (out[i] == 0.0f) ? 1.0f : 1.0f / out[i];
+            }
+    }
+    if (c > 0.0f) or_apply_mask(out, (int64_t)w * h, mask);
+    if (normalize) return or_normalize_map(out, w, h, brw, brh, mask);
+    return 0;
+}
+
+int or_global_finalize_f64(int metric, const double *S, const double *Wsum, const float *mask, double n_pixels,
+                           float c, int normalize, int w, int h, int bw, int bh, double *out)
+{
+    int brw = bw / 2, brh = bh / 2;
+    memset(out, 0, sizeof(double) * (size_t)w * h);
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            out[i] = S[i] / (Wsum[i] * n_pixels + 1e-7);
+        }
+    if (metric == OR_NRMSE || metric == OR_ABSDIFF) {
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) {
+                int64_t i = (int64_t)y * w + x;
+                if (mask[i] > 0.0f) out[i] = (out[i] == 0.0) ? 1.0 : 1.0 / out[i];
+            }
+    }
+    if (c > 0.0f)
+        for (int64_t i = 0; i < (int64_t)w * h; i++) out[i] *= mask[i];
+    if (normalize) return or_normalize_map_f64(out, w, h, brw, brh, mask);
+    return 0;
+}
+
+/* ---- GlobalRepetition2D_.java:124-145 (VST off, input already normalised) ---- */
+int or_global_map(int metric, int fp64, const float *img, int w, int h, int bw, int bh, float c, int normalize,
+                  float *out32, double *out64, int nthreads)
+{
+    int64_t n = (int64_t)w * h;
+    float *means = malloc(sizeof(float) * n), *stds = malloc(sizeof(float) * n), *mask = malloc(sizeof(float) * n);
+    or_local_stats_f32(img, w, h, bw, bh, means, stds);
+    float nv = or_noise_var(img, w, h), thr, np;
+    or_relevance_mask(stds, w, h, bw / 2, bh / 2, nv, c, mask, &thr, &np);
+    int rc;
+    if (!fp64) {
+        float *S = malloc(sizeof(float) * n), *W = malloc(sizeof(float) * n);
+        or_global_accum_f32(metric, img, means, stds, w, h, bw, bh, nv, thr, S, W, NULL, 0, nthreads);
+        rc = or_global_finalize(metric, S, W, mask, np, c, normalize, w, h, bw, bh, out32);
+        free(S); free(W);
+    } else {
+        double *S = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n);
+        or_global_accum_f64(metric, img, stds, w, h, bw, bh, nv, thr, S, W, NULL, 0, nthreads);
+        rc = or_global_finalize_f64(metric, S, W, mask, (double)np, c, normalize, w, h, bw, bh, out64);
+        free(S); free(W);
+    }
+    free(means); free(stds); free(mask);
+    return rc;
+}
+
+/* ---- block repetition kernels: kernelGetBlockPearson2D.cl:12-90,
+ *      kernelGetBlockSsim2D.cl:103-172, kernelGetBlockNrmse2D.cl:12-75,
+ *      kernelGetBlockAbsDiffStds2D.cl:84-106 ---- */
+int or_block_raw(int metric, const float *img, const float *means, const float *stds, int w, int h,
+                 const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std, float *out)
+{
+    const int brw = bw / 2, brh = bh / 2;
+    const int K = or_footprint(bw, bh, NULL, NULL);
+    if (K > 1024) return -1;
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+    const double eps_d = or_java_float_literal(EPS_F);
+    const double std_d = or_java_float_literal(tmpl_std);
+    const double mean_d = or_java_float_literal(tmpl_mean);
+#pragma omp parallel for schedule(static)
+    for (int gy = 0; gy < h; gy++) {
+        float cb[1024];
+        for (int gx = 0; gx < w; gx++) {
+            int64_t o = (int64_t)gy * w + gx;
+            if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { out[o] = 0.0f; continue; }
+            if (metric == OR_ABSDIFF) { out[o] = fabsf(tmpl_std - stds[o]); continue; }
+            const float cm = means[o];
+            for (int i = 0; i < K; i++) cb[i] = img[(int64_t)(gy + dy[i]) * w + gx + dx[i]];
+            for (int i = 0; i < K; i++) cb[i] = cb[i] - cm;
+            if (metric == OR_NRMSE) {
+                float mse = 0.0f;
+                for (int i = 0; i < K; i++) mse += (tmpl[i] - cb[i]) * (tmpl[i] - cb[i]);
+                mse /= (float)K;
+                out[o] = sqrtf(mse);
+                continue;
+            }
+            float cov = 0.0f;
+            for (int i = 0; i < K; i++) cov += tmpl[i] * cb[i];
+            cov /= (float)(K - 1);
+            const float cs = stds[o];
+            if (std_d == 0.0 && cs == 0.0f) { out[o] = 1.0f; continue; }
+            if (std_d == 0.0 || cs == 0.0f) { out[o] = 0.0f; continue; }
+            if (metric == OR_PEARSON) {
+                float r = (float)((double)cov / ((std_d * (double)cs) + eps_d));
+                out[o] = fmaxf(0.0f, r);
+            } else {
+                const float c1 = 0.0001f, c2 = 0.0009f;
+                /* ref_mean/ref_std are double literals: the expression is double */
+                double num = (2.0 * mean_d * (double)cm + (double)c1) * (double)(2.0f * cov + c2);
+                double den = ((mean_d * mean_d) + (double)(cm * cm) + (double)c1) *
+                             ((std_d * std_d) + (double)(cs * cs) + (double)c2);
+                float ss = (float)(num / den);
+                out[o] = fmaxf(0.0f, ss);
+            }
+        }
+    }
+    free(dx); free(dy);
+    return 0;
+}
+
+/* CLUtils.java:486-981 host sequences around the block kernels */
+int or_block_map(int metric, const float *img, int w, int h, const float *tmpl, int bw, int bh, float tmpl_mean,
+                 float tmpl_std, float c, int normalize, float *out)
+{
+    const int brw = bw / 2, brh = bh / 2;
+    int64_t n = (int64_t)w * h;
+    float *means = malloc(sizeof(float) * n), *stds = malloc(sizeof(float) * n);
+    or_local_stats_f32(img, w, h, bw, bh, means, stds);
+    int rc = or_block_raw(metric, img, means, stds, w, h, tmpl, bw, bh, tmpl_mean, tmpl_std, out);
+    float *mask = NULL;
+    float thr, np;
+    if (rc) goto done;
+    if (metric == OR_PEARSON || metric == OR_SSIM) {            /* CLUtils.java:565-577 */
+        if (c > 0.0f) {
+            mask = malloc(sizeof(float) * n);
+            or_relevance_mask(stds, w, h, brw, brh, or_noise_var(img, w, h), c, mask, &thr, &np);
+            or_apply_mask(out, n, mask);
+        }
+        if (normalize) rc = or_normalize_map(out, w, h, brw, brh, mask);
+    } else if (metric == OR_NRMSE) {                            /* CLUtils.java:788-841 */
+        /* readback copies only the interior; border stays 0 (CLUtils.java:789-795) */
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++)
+                if (x < brw || x >= w - brw || y < brh || y >= h - brh) out[(int64_t)y * w + x] = 0.0f;
+        mask = malloc(sizeof(float) * n);
+        if (c == 0.0f) {
+            memset(mask, 0, sizeof(float) * n);
+            for (int y = brh; y < h - brh; y++)
+                for (int x = brw; x < w - brw; x++) mask[(int64_t)y * w + x] = 1.0f;
+        } else {
+            or_relevance_mask(stds, w, h, brw, brh, or_noise_var(img, w, h), c, mask, &thr, &np);
+        }
+        or_apply_mask(out, n, mask);
+        rc = or_normalize_map(out, w, h, brw, brh, mask);
+        for (int y = brh; y < h - brh && !rc; y++)
+            for (int x = brw; x < w - brw; x++) {
+                int64_t i = (int64_t)y * w + x;
+                if (mask[i] > 0.0f) out[i] = (out[i] == 0.0f) ? 1.0f : 1.0f / out[i];
+            }
+        if (normalize && !rc) rc = or_normalize_map(out, w, h, brw, brh, mask);
+    } else {                                                    /* CLUtils.java:938-975 */
+        float mn = FLT_MAX, mx = FLT_TRUE_MIN;                 /* Float.MIN_VALUE, CLUtils.java:940 */
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) {
+                mn = fminf(mn, out[(int64_t)y * w + x]);
+                mx = fmaxf(mx, out[(int64_t)y * w + x]);
+            }
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) {
+                int64_t i = (int64_t)y * w + x;
+                out[i] = (out[i] - mn) / (mx - mn + EPS_F);
+            }
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) out[(int64_t)y * w + x] = 1.0f - out[(int64_t)y * w + x];
+        if (c > 0.0f) {
+            mask = malloc(sizeof(float) * n);
+            or_relevance_mask(stds, w, h, brw, brh, or_noise_var(img, w, h), c, mask, &thr, &np);
+            or_apply_mask(out, n, mask);
+        }
+        if (normalize) rc = or_normalize_map(out, w, h, brw, brh, mask);
+    }
+done:
+    free(means); free(stds); free(mask);
+    return rc;
+}

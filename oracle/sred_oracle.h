@@ -1,0 +1,116 @@
+/*
+ * sred_oracle.h -- CPU restatement of SReD's 2D block-correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests,
+ * the smoke check and the CPU-baseline leg of the benchmark load it to
+ * compare against the HIP library (sred_amd/libsred_hip.so).  No product code
+ * path links, loads or calls it.
+ *
+ * Two forms are restated from the reference's text (Java host +
+ * OpenCL C kernels):
+ *
+ *   refsim  -- float32-faithful: the kernels' loop orders, sequential float
+ *              accumulation, the double promotions caused by the kernels'
+ *              unsuffixed macro literals, and the Java float host steps.
+ *              This is "the reference's own map".
+ *   oracle64 -- the same mathematics in FP64 with compensated sums; discrete
+ *              decisions (relevance mask, threshold skip, sigma==0 special
+ *              cases) are taken from the float32 statistics exactly as the
+ *              reference takes them, magnitudes are FP64.
+ *
+ * Every function cites the reference file:line it follows (paths relative to
+ * the SReD repository root).  Pinning of each function is described in the
+ * test files under tests/ (known-answer vectors from the reference's own
+ * JUnit tests, closed forms, invariants, brute force on tiny inputs).
+ */
+#ifndef SRED_ORACLE_H
+#define SRED_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_PEARSON = 0, OR_SSIM = 1, OR_NRMSE = 2, OR_ABSDIFF = 3 };
+
+/* Elliptical footprint, GlobalRepetition2D_.java:102-111 and the kernel
+ * predicate kernelGetGlobalPearson2D.cl:44-50.  Writes the row-major offsets
+ * (dx, dy) of the K member pixels if dx/dy are non-NULL; returns K. */
+int or_footprint(int bw, int bh, int *dx, int *dy);
+
+/* The double an OpenCL compiler reads from Java's ""+float rendering of f
+ * (CLUtils.java:1705-1712): the shortest decimal that round-trips to f,
+ * parsed as double. */
+double or_java_float_literal(float f);
+
+/* Utils.getInputImage2D min-max normalisation, Utils.java:708-722. */
+void or_normalize_input(float *img, int64_t n);
+
+/* Utils.getReferenceBlock2D, Utils.java:557-639 (including the double
+ * mean subtraction in the std, Utils.java:626-636).  px receives the K
+ * mean-subtracted template pixels.  Returns K, or -1 for even sizes. */
+int or_reference_block(const float *blk, int bw, int bh, float *px, float *mean, float *std_);
+
+/* kernelGetLocalStatistics2D.cl:9-55 in float32 (sequential sums, border 0). */
+void or_local_stats_f32(const float *img, int w, int h, int bw, int bh, float *means, float *stds);
+/* The same statistics in FP64 (mean and unbiased variance). */
+void or_local_stats_f64(const float *img, int w, int h, int bw, int bh, double *means, double *vars);
+
+/* Utils.getMeanAndVarBlock2D, Utils.java:854-900 (one-pass float).  Returns
+ * 0, or -1 for invalid coordinates / non-finite input (the reference throws). */
+int or_mean_var_block(const float *img, int w, int h, int x0, int y0, int x1, int y1, float *mean, float *var);
+/* Utils.getMeanNoiseVar2D, Utils.java:790-837. */
+float or_noise_var(const float *img, int w, int h);
+/* Utils.getRelevanceMask, Utils.java:944-968, plus the nPixels count of
+ * GlobalRepetition2D_.java:133-140.  mask is W*H {0,1}. */
+void or_relevance_mask(const float *stds, int w, int h, int brw, int brh, float noise_var, float c,
+                       float *mask, float *threshold, float *n_pixels);
+/* Utils.applyMask2D, Utils.java:915-925. */
+void or_apply_mask(float *map, int64_t n, const float *mask);
+/* Utils.normalizeImage2D, Utils.java:985-1031, in place (mask may be NULL).
+ * Returns 0, or -1 for a non-binary mask (the reference throws). */
+int or_normalize_map(float *map, int w, int h, int brw, int brh, const float *mask);
+int or_normalize_map_f64(double *map, int w, int h, int brw, int brh, const float *mask);
+
+/* Global accumulators S (similarity sum) and Wsum (weight sum), float32,
+ * kernelGetGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}2D.cl.  filter_param and
+ * threshold are the float values the host renders into the kernel macros.
+ * rows: optional list of reference pixel indices (y*w+x) to compute; NULL
+ * computes every pixel.  S/Wsum are W*H (only listed pixels are written).
+ * nthreads <= 0 uses all OpenMP threads. */
+void or_global_accum_f32(int metric, const float *img, const float *means, const float *stds, int w, int h,
+                         int bw, int bh, float filter_param, float threshold, float *S, float *Wsum,
+                         const int32_t *rows, int64_t nrows, int nthreads);
+/* FP64 accumulators; stats recomputed in FP64; validity and sigma==0
+ * decisions from the float32 stds (stds32). */
+void or_global_accum_f64(int metric, const float *img, const float *stds32, int w, int h, int bw, int bh,
+                         float filter_param, float threshold, double *S, double *Wsum,
+                         const int32_t *rows, int64_t nrows, int nthreads);
+
+/* Host finalize of CLUtils.getGlobal*2D (CLUtils.java:1747-1770, 1983-2020,
+ * 2112-2149): rep = S/(Wsum*nPixels+eps) on the interior, inversion for
+ * NRMSE/AbsDiff where mask>0, mask if c>0, masked normalisation. */
+int or_global_finalize(int metric, const float *S, const float *Wsum, const float *mask, float n_pixels,
+                       float c, int normalize, int w, int h, int bw, int bh, float *out);
+int or_global_finalize_f64(int metric, const double *S, const double *Wsum, const float *mask, double n_pixels,
+                           float c, int normalize, int w, int h, int bw, int bh, double *out);
+
+/* Full global-repetition pipeline from a normalised input image:
+ * GlobalRepetition2D_.java:124-145 with VST off.  fp64 selects oracle64. */
+int or_global_map(int metric, int fp64, const float *img, int w, int h, int bw, int bh, float c, int normalize,
+                  float *out32, double *out64, int nthreads);
+
+/* Block repetition, CLUtils.getBlock{Pearson,Ssim,Nrmse,AbsDiffStds}2D
+ * (CLUtils.java:486-981) with their kernels.  tmpl is the K template pixels
+ * as handed to the kernel (ReferenceBlock2D.getPixels()). */
+int or_block_map(int metric, const float *img, int w, int h, const float *tmpl, int bw, int bh, float tmpl_mean,
+                 float tmpl_std, float c, int normalize, float *out);
+/* Raw per-window kernel output only (before any host post-processing). */
+int or_block_raw(int metric, const float *img, const float *means, const float *stds, int w, int h,
+                 const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,99 @@
+"""Mirror of SReD's ``CLUtils`` 2D entry points over the HIP library.
+
+The Java CLUtils builds OpenCL programs and launches kernels through JOCL;
+these functions keep its method names (snake_case), argument meaning, return
+values and error behaviour, and call libsred_hip.so (include/sred.h) instead.
+Unknown metric strings return None, as CLUtils.java:2193-2195 does.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .utils import InputImage2D, ReferenceBlock2D, RelevanceMask
+
+GLOBAL_METRICS = ["Pearson's R", "SSIM", "NRMSE", "Abs. diff. of StdDevs"]          # GlobalRepetition2D_.java:20-25
+BLOCK_METRICS = ["Pearson's R", "SSIM", "NRMSE (inverted)", "Abs. Diff. of StdDevs."]  # BlockRepetition2D_.java:18-23
+PRECISIONS = {"fp32": L.FP32, "bf16x3": L.BF16X3}
+
+
+@dataclass
+class LocalStatistics:
+    """CLUtils.CLLocalStatistics (CLUtils.java:84-154), host arrays."""
+
+    local_means: np.ndarray
+    local_stds: np.ndarray
+
+
+def get_opencl_resources(device=0):
+    """CLUtils.getOpenCLResources counterpart: a reusable HIP device context."""
+    return L.Context(device)
+
+
+def get_local_statistics_2d(resources, input_image, block_width, block_height):
+    """CLUtils.getLocalStatistics2D (CLUtils.java:376-467)."""
+    img = L.f32(input_image.image_array).reshape(-1)
+    means = np.zeros(input_image.size, np.float32)
+    stds = np.zeros(input_image.size, np.float32)
+    L.check(L.load().sred_local_stats_2d(resources.handle, L.fptr(img), input_image.width, input_image.height,
+                                         block_width, block_height, L.fptr(means), L.fptr(stds)))
+    return LocalStatistics(means, stds)
+
+
+def _metric_id(metric, block):
+    rc = L.load().sred_metric_from_name(metric.encode(), 1 if block else 0)
+    return None if rc == L.SRED_ERR_UNKNOWN_METRIC else L.check(rc)
+
+
+def get_global_accumulators_2d(resources, metric_id, input_image, local_statistics, block_width, block_height,
+                               filter_param, threshold, precision="bf16x3"):
+    """Raw outputs of kernelGetGlobal*2D (similarity sum S, weight sum Wsum)."""
+    img = L.f32(input_image.image_array).reshape(-1)
+    S = np.zeros(input_image.size, np.float32)
+    W = np.zeros(input_image.size, np.float32)
+    L.check(L.load().sred_global_accum_2d(
+        resources.handle, metric_id, PRECISIONS[precision], L.fptr(img), L.fptr(L.f32(local_statistics.local_means)),
+        L.fptr(L.f32(local_statistics.local_stds)), input_image.width, input_image.height, block_width, block_height,
+        filter_param, threshold, L.fptr(S), L.fptr(W)))
+    return S, W
+
+
+def calculate_global_repetition_map_2d(metric, input_image, block_width, block_height, block_size, local_statistics,
+                                       relevance_mask, n_pixels, normalize_output, resources, precision="bf16x3"):
+    """CLUtils.calculateGlobalRepetitionMap2D (CLUtils.java:2174-2196).
+
+    precision selects the contraction: "bf16x3" (split-BF16 MFMA) or "fp32".
+    Returns a float32 array of width*height, or None for an unknown metric.
+    """
+    mid = _metric_id(metric, block=False)
+    if mid is None:
+        return None
+    if block_size != L.load().sred_block_size_2d(block_width, block_height):
+        raise ValueError("block_size does not match the elliptical block of the given width/height")
+    out = np.zeros(input_image.size, np.float32)
+    img = L.f32(input_image.image_array).reshape(-1)
+    L.check(L.load().sred_global_repetition_2d(
+        resources.handle, mid, PRECISIONS[precision], L.fptr(img), L.fptr(L.f32(local_statistics.local_means)),
+        L.fptr(L.f32(local_statistics.local_stds)), input_image.width, input_image.height, block_width, block_height,
+        L.fptr(L.f32(relevance_mask.relevance_mask)), relevance_mask.relevance_constant,
+        relevance_mask.relevance_threshold, relevance_mask.noise_mean_variance, n_pixels, int(bool(normalize_output)),
+        L.fptr(out)))
+    return out
+
+
+def calculate_block_repetition_map_2d(metric, input_image, reference_block, relevance_constant, normalize_output,
+                                      resources):
+    """CLUtils.calculateBlockRepetitionMap2D (CLUtils.java:995-1010).
+
+    Returns a float32 array of width*height, or None for an unknown metric.
+    """
+    mid = _metric_id(metric, block=True)
+    if mid is None:
+        return None
+    out = np.zeros(input_image.size, np.float32)
+    img = L.f32(input_image.image_array).reshape(-1)
+    L.check(L.load().sred_block_repetition_2d(
+        resources.handle, mid, L.fptr(img), input_image.width, input_image.height,
+        L.fptr(L.f32(reference_block.pixels)), reference_block.width, reference_block.height, reference_block.mean,
+        reference_block.std, relevance_constant, int(bool(normalize_output)), L.fptr(out)))
+    return out

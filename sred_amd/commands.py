@@ -1,0 +1,70 @@
+"""The two 2D plugin commands on the HIP path.
+
+``global_repetition_2d`` follows GlobalRepetition2D_.run
+(GlobalRepetition2D_.java:35-157) and ``block_repetition_2d`` follows
+BlockRepetition2D_.run (BlockRepetition2D_.java:32-157, single image) with
+the dialog parameters as keyword arguments and the same defaults (block 5x5,
+relevance constant 0, normalise on).  The ImageJ dialog, window lookup,
+display and variance stabilisation stay outside this library.
+"""
+import numpy as np
+
+from . import clutils, utils
+from . import _lib as L
+
+
+def global_repetition_2d(image, block_width=5, block_height=5, relevance_constant=0.0, metric="Pearson's R",
+                         normalize_output=True, precision="bf16x3", resources=None):
+    """Repetition map of every block against every other block (float32, HxW)."""
+    if block_width % 2 == 0 or block_height % 2 == 0:  # GlobalRepetition2D_.java:86-89
+        raise ValueError("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.")
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        brw, brh = block_width // 2, block_height // 2
+        block_size = utils.block_size_2d(block_width, block_height)
+        inp = utils.get_input_image_2d(image, False, True)
+        stats = clutils.get_local_statistics_2d(res, inp, block_width, block_height)
+        mask = utils.get_relevance_mask(inp.image_array, inp.width, inp.height, brw, brh, stats.local_stds,
+                                        relevance_constant)
+        n_pixels = utils.count_relevant_pixels(mask.relevance_mask, inp.width, inp.height, brw, brh)
+        out = clutils.calculate_global_repetition_map_2d(metric, inp, block_width, block_height, block_size, stats,
+                                                         mask, n_pixels, normalize_output, res, precision)
+        return None if out is None else out.reshape(inp.height, inp.width)
+    finally:
+        if own:
+            res.close()
+
+
+def block_repetition_2d(block, image, relevance_constant=0.0, metric="Pearson's R", normalize_output=True,
+                        resources=None):
+    """Repetition map of one reference block against every window (float32, HxW)."""
+    ref = utils.get_reference_block_2d(block)
+    if ref is None:
+        raise ValueError("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.")
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        inp = utils.get_input_image_2d(image, False, True)
+        out = clutils.calculate_block_repetition_map_2d(metric, inp, ref, relevance_constant, normalize_output, res)
+        return None if out is None else out.reshape(inp.height, inp.width)
+    finally:
+        if own:
+            res.close()
+
+
+def global_map_device(ctx, metric_id, precision_id, d_img, width, height, block_width, block_height,
+                      relevance_constant, normalize_output, d_out, d_means=0, d_stds=0, d_mask=0, d_S=0, d_W=0):
+    """sred_global_map_2d_dev on device pointers (ints, e.g. torch data_ptr())."""
+    L.check(L.load().sred_global_map_2d_dev(ctx.handle, metric_id, precision_id, d_img, width, height, block_width,
+                                            block_height, relevance_constant, int(bool(normalize_output)), d_out,
+                                            d_means or None, d_stds or None, d_mask or None, d_S or None, d_W or None))
+
+
+def block_map_device(ctx, metric_id, d_img, width, height, reference_block, relevance_constant, normalize_output,
+                     d_out):
+    """sred_block_map_2d_dev on device pointers; the template stays a host array."""
+    L.check(L.load().sred_block_map_2d_dev(ctx.handle, metric_id, d_img, width, height,
+                                           L.fptr(L.f32(reference_block.pixels)), reference_block.width,
+                                           reference_block.height, reference_block.mean, reference_block.std,
+                                           relevance_constant, int(bool(normalize_output)), d_out))

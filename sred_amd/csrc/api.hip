@@ -1,0 +1,639 @@
+// api.hip -- the extern "C" boundary of libsred_hip.so (declared in
+// include/sred.h) and the host-side helpers it needs.
+//
+// The host-pointer entry points reproduce the CLUtils methods a JNI binding
+// replaces; the _dev entry points run the same pipelines on device buffers.
+// Host-side mirrors of Utils (noise variance, relevance mask, normalisation,
+// reference block) are plain C++ with Java float semantics, compiled without
+// FP contraction (the file is built with -ffp-contract=off for the host).
+#include <float.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sred_internal.h"
+
+#define SRED_VERSION 100
+
+namespace sred {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int DevBuf::ensure(size_t n)
+{
+    if (n <= bytes && ptr) return SRED_OK;
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) {
+            set_error(std::string("hipFree: ") + hipGetErrorString(e));
+            return SRED_ERR_HIP;
+        }
+    }
+    size_t want = std::max<size_t>(n, 256);
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        set_error(std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+        return SRED_ERR_HIP;
+    }
+    bytes = want;
+    return SRED_OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// GlobalRepetition2D_.java:102-111 / CLUtils.java:391-400 / kernel predicate.
+int footprint(int bw, int bh, std::vector<int> *dx, std::vector<int> *dy)
+{
+#pragma clang fp contract(off)
+    const int brw = bw / 2, brh = bh / 2;
+    int k = 0;
+    if (dx) { dx->clear(); dy->clear(); }
+    for (int y = 0; y < bh; y++)
+        for (int x = 0; x < bw; x++) {
+            float fx = (float)(x - brw), fy = (float)(y - brh);
+            if (((fx * fx) / (float)(brw * brw)) + ((fy * fy) / (float)(brh * brh)) <= 1.0f) {
+                if (dx) { dx->push_back(x - brw); dy->push_back(y - brh); }
+                k++;
+            }
+        }
+    return k;
+}
+
+double java_float_literal(float f)
+{
+    if (f == 0.0f) return 0.0;
+    char buf[64];
+    for (int p = 1; p <= 9; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, (double)f);
+        if (strtof(buf, nullptr) == f) return strtod(buf, nullptr);
+    }
+    return (double)f;
+}
+
+int make_geometry(int w, int h, int bw, int bh, Geometry *g)
+{
+    if (w <= 0 || h <= 0) {
+        set_error("image dimensions must be positive");
+        return SRED_ERR_ARG;
+    }
+    if (bw < 3 || bh < 3 || bw % 2 == 0 || bh % 2 == 0) {
+        set_error("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.");
+        return SRED_ERR_ARG;
+    }
+    if (bw > SRED_MAX_BLOCK_SIDE || bh > SRED_MAX_BLOCK_SIDE) {
+        set_error("block side larger than SRED_MAX_BLOCK_SIDE");
+        return SRED_ERR_ARG;
+    }
+    if ((int64_t)w * h >= (int64_t)1 << 31) {
+        set_error("image larger than 2^31 pixels");
+        return SRED_ERR_ARG;
+    }
+    g->w = w; g->h = h; g->bw = bw; g->bh = bh; g->brw = bw / 2; g->brh = bh / 2;
+    g->K = footprint(bw, bh, nullptr, nullptr);
+    if (g->K < 2 || g->K > kMaxK) {
+        set_error("unsupported block footprint");
+        return SRED_ERR_ARG;
+    }
+    return SRED_OK;
+}
+
+int upload_offsets(sred_ctx *ctx, const Geometry &g, int pitch, DevBuf *buf, int *key)
+{
+    if (key[0] == g.bw && key[1] == g.bh && key[2] == pitch && buf->ptr) return SRED_OK;
+    std::vector<int> dx, dy;
+    footprint(g.bw, g.bh, &dx, &dy);
+    std::vector<int> off(g.K);
+    for (int k = 0; k < g.K; k++) off[k] = dy[k] * pitch + dx[k];
+    SRED_TRY(buf->ensure(sizeof(int) * g.K));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    SRED_HIP_CHECK(hipMemcpy(buf->ptr, off.data(), sizeof(int) * g.K, hipMemcpyHostToDevice));
+    key[0] = g.bw; key[1] = g.bh; key[2] = pitch;
+    return SRED_OK;
+}
+
+static int check_ctx(sred_ctx *ctx)
+{
+    if (!ctx) {
+        set_error("null context");
+        return SRED_ERR_ARG;
+    }
+    SRED_HIP_CHECK(hipSetDevice(ctx->device));
+    return SRED_OK;
+}
+
+static int upload(sred_ctx *ctx, DevBuf &b, const float *h, int64_t n)
+{
+    SRED_TRY(b.ensure(sizeof(float) * n));
+    SRED_HIP_CHECK(hipMemcpyAsync(b.ptr, h, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    return SRED_OK;
+}
+
+static int download(sred_ctx *ctx, float *h, const DevBuf &b, int64_t n)
+{
+    SRED_HIP_CHECK(hipMemcpyAsync(h, b.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SRED_OK;
+}
+
+// Block-repetition host sequence of CLUtils.getBlock*2D on device maps:
+// Pearson/SSIM CLUtils.java:565-577, NRMSE :788-841, AbsDiff :938-975.
+static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *tmpl,
+                          float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out)
+{
+    const int64_t n = g.n();
+    const bool need_stats = (c != 0.0f);
+    if (need_stats) {
+        SRED_TRY(ctx->means.ensure(sizeof(float) * n));
+        SRED_TRY(ctx->stds.ensure(sizeof(float) * n));
+    }
+    SRED_TRY(block_raw_dev(ctx, g, metric, d_img, need_stats ? ctx->means.as<float>() : nullptr,
+                           need_stats ? ctx->stds.as<float>() : nullptr, tmpl, tmpl_mean, tmpl_std, d_out));
+    float *d_mask = nullptr;
+    if (need_stats) {
+        float nv, np;
+        SRED_TRY(noise_var_dev(ctx, d_img, g.w, g.h, &nv));
+        SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
+        d_mask = ctx->mask.as<float>();
+        SRED_TRY(relevance_mask_dev(ctx, g, ctx->stds.as<float>(), nv * c, d_mask, &np));
+    }
+    if (metric == SRED_PEARSON || metric == SRED_SSIM) {
+        if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
+        if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, c > 0.0f ? d_mask : nullptr, 0));
+    } else if (metric == SRED_NRMSE) {
+        if (c == 0.0f) {
+            SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
+            d_mask = ctx->mask.as<float>();
+            SRED_TRY(interior_ones_mask_dev(ctx, g, d_mask));
+        }
+        SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
+        SRED_TRY(normalize_map_dev(ctx, g, d_out, d_mask, 0));
+        SRED_TRY(invert_masked_dev(ctx, g, d_out, d_mask));
+        if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, d_mask, 0));
+    } else {
+        SRED_TRY(normalize_map_dev(ctx, g, d_out, nullptr, 1));  // Float.MIN_VALUE max initialiser
+        SRED_TRY(one_minus_dev(ctx, g, d_out));
+        if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
+        if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, c > 0.0f ? d_mask : nullptr, 0));
+    }
+    return SRED_OK;
+}
+
+}  // namespace sred
+
+using namespace sred;
+
+extern "C" {
+
+int sred_version(void) { return SRED_VERSION; }
+
+const char *sred_last_error(void) { return g_last_error.c_str(); }
+
+int sred_ctx_create(int device, sred_ctx **out)
+{
+    if (!out) {
+        set_error("null output pointer");
+        return SRED_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return SRED_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        set_error("device index out of range");
+        return SRED_ERR_ARG;
+    }
+    SRED_HIP_CHECK(hipSetDevice(device));
+    sred_ctx *c = new sred_ctx();
+    c->device = device;
+    *out = c;
+    return SRED_OK;
+}
+
+int sred_ctx_destroy(sred_ctx *ctx)
+{
+    if (!ctx) return SRED_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->img, &ctx->means, &ctx->stds, &ctx->mask, &ctx->S, &ctx->W, &ctx->out,
+                      &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->u32, &ctx->uhi,
+                      &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl};
+    for (DevBuf *b : bufs) b->release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+    return SRED_OK;
+}
+
+int sred_ctx_set_stream(sred_ctx *ctx, void *stream)
+{
+    SRED_TRY(check_ctx(ctx));
+    ctx->stream = static_cast<hipStream_t>(stream);
+    return SRED_OK;
+}
+
+int sred_ctx_synchronize(sred_ctx *ctx)
+{
+    SRED_TRY(check_ctx(ctx));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return SRED_OK;
+}
+
+int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols)
+{
+    if (!ctx || cols < 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    ctx->chunk_cols = cols;
+    return SRED_OK;
+}
+
+int sred_metric_from_name(const char *name, int block)
+{
+    if (!name) {
+        set_error("null metric name");
+        return SRED_ERR_ARG;
+    }
+    static const char *global_names[4] = {"Pearson's R", "SSIM", "NRMSE", "Abs. diff. of StdDevs"};
+    static const char *block_names[4] = {"Pearson's R", "SSIM", "NRMSE (inverted)", "Abs. Diff. of StdDevs."};
+    const char **names = block ? block_names : global_names;
+    for (int i = 0; i < 4; i++)
+        if (strcmp(name, names[i]) == 0) return i;
+    set_error(std::string("unknown metric: ") + name);
+    return SRED_ERR_UNKNOWN_METRIC;
+}
+
+int sred_block_size_2d(int bw, int bh)
+{
+    if (bw < 3 || bh < 3 || bw > SRED_MAX_BLOCK_SIDE || bh > SRED_MAX_BLOCK_SIDE) {
+        set_error("block side out of range");
+        return SRED_ERR_ARG;
+    }
+    return footprint(bw, bh, nullptr, nullptr);
+}
+
+int sred_normalize_input_2d(float *img, int64_t n)
+{
+#pragma clang fp contract(off)
+    if (!img || n <= 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        mn = std::min(mn, img[i]);
+        mx = std::max(mx, img[i]);
+    }
+    for (int64_t i = 0; i < n; i++) img[i] = (img[i] - mn) / (mx - mn + kEps);
+    return SRED_OK;
+}
+
+int sred_reference_block_2d(const float *block, int bw, int bh, float *px, float *mean, float *std_)
+{
+#pragma clang fp contract(off)
+    if (!block || !px || !mean || !std_) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    if (bw % 2 == 0 || bh % 2 == 0 || bw < 3 || bh < 3 || bw > SRED_MAX_BLOCK_SIDE || bh > SRED_MAX_BLOCK_SIDE) {
+        set_error("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.");
+        return SRED_ERR_ARG;
+    }
+    std::vector<int> dx, dy;
+    int K = footprint(bw, bh, &dx, &dy);
+    const int brw = bw / 2, brh = bh / 2;
+    for (int k = 0; k < K; k++) px[k] = block[(dy[k] + brh) * bw + dx[k] + brw];
+    float bmin = FLT_MAX, bmax = -FLT_MAX;
+    for (int k = 0; k < K; k++) { bmin = std::min(bmin, px[k]); bmax = std::max(bmax, px[k]); }
+    float m = 0.0f;
+    for (int k = 0; k < K; k++) {
+        px[k] = (px[k] - bmin) / (bmax - bmin + kEps);
+        m += px[k];
+    }
+    m /= (float)K;
+    for (int k = 0; k < K; k++) px[k] = px[k] - m;
+    float s = 0.0f;
+    for (int k = 0; k < K; k++) s += (px[k] - m) * (px[k] - m);  // Utils.java:632-634 subtracts the mean again
+    s = (float)std::sqrt((double)(s / (float)(K - 1)));
+    *mean = m;
+    *std_ = s;
+    return K;
+}
+
+int sred_noise_var_2d(const float *img, int w, int h, float *noise_var)
+{
+#pragma clang fp contract(off)
+    if (!img || !noise_var || w <= 0 || h <= 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    int tbw = 8, tbh = 8;
+    if ((int64_t)w * h > 352 * 288) { tbw = 16; tbh = 16; }
+    int nbx = w / tbw, nby = h / tbh, nb = nbx * nby;
+    if (nb <= 0) {
+        set_error("image smaller than one noise tile (the reference indexes an empty variance array)");
+        return SRED_ERR_ARG;
+    }
+    std::vector<float> lv(nb);
+    for (int by = 0; by < nby; by++)
+        for (int bx = 0; bx < nbx; bx++) {
+            float m = 0.0f, sq = 0.0f;
+            for (int y = by * tbh; y < (by + 1) * tbh; y++)
+                for (int x = bx * tbw; x < (bx + 1) * tbw; x++) {
+                    float v = img[(int64_t)y * w + x];
+                    if (!std::isfinite(v)) {
+                        set_error("Input contains invalid (NaN or Infinite) values.");
+                        return SRED_ERR_NONFINITE;
+                    }
+                    m += v;
+                    sq += v * v;
+                }
+            m /= (float)(tbw * tbh);
+            lv[by * nbx + bx] = sq / (float)(tbw * tbh) - m * m;
+        }
+    std::sort(lv.begin(), lv.end());
+    int nvars = (int)(0.03f * (float)nb + 1.0f);
+    float nv = 0.0f;
+    for (int i = 0; i < nvars && i < nb; i++) nv += lv[i];
+    nv = std::fabs(nv / (float)nvars);
+    nv = (1.0f + 0.001f * (nv - 40.0f)) * nv;
+    *noise_var = nv;
+    return SRED_OK;
+}
+
+int sred_relevance_mask_2d(const float *stds, int w, int h, int brw, int brh, float noise_var, float c,
+                           float *mask, float *threshold, float *n_pixels)
+{
+#pragma clang fp contract(off)
+    if (!stds || !mask || w <= 0 || h <= 0 || brw < 0 || brh < 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    float thr = noise_var * c;
+    float np = 0.0f;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int64_t i = (int64_t)y * w + x;
+            float m = 0.0f;
+            if (x >= brw && x < w - brw && y >= brh && y < h - brh) m = (stds[i] * stds[i] <= thr) ? 0.0f : 1.0f;
+            mask[i] = m;
+            if (m > 0.0f) np += 1.0f;
+        }
+    if (threshold) *threshold = thr;
+    if (n_pixels) *n_pixels = np;
+    return SRED_OK;
+}
+
+int sred_apply_mask_2d(float *map, int w, int h, const float *mask)
+{
+    if (!map || !mask || w <= 0 || h <= 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    for (int64_t i = 0; i < (int64_t)w * h; i++) map[i] *= mask[i];
+    return SRED_OK;
+}
+
+int sred_normalize_map_2d(float *map, int w, int h, int brw, int brh, const float *mask)
+{
+#pragma clang fp contract(off)
+    if (!map || w <= 0 || h <= 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask) {
+                if (mask[i] != 0.0f && mask[i] != 1.0f) {
+                    set_error("Relevance mask must be binary (0.0f or 1.0f)");
+                    return SRED_ERR_NONBINARY_MASK;
+                }
+                if (!(mask[i] > 0.0f)) continue;
+            }
+            mn = std::min(map[i], mn);
+            mx = std::max(map[i], mx);
+        }
+    for (int y = brh; y < h - brh; y++)
+        for (int x = brw; x < w - brw; x++) {
+            int64_t i = (int64_t)y * w + x;
+            if (mask && !(mask[i] > 0.0f)) continue;
+            map[i] = (map[i] - mn) / (mx - mn + kEps);
+        }
+    return SRED_OK;
+}
+
+int sred_local_stats_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float *means, float *stds)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !means || !stds) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(ctx->means.ensure(sizeof(float) * g.n()));
+    SRED_TRY(ctx->stds.ensure(sizeof(float) * g.n()));
+    SRED_TRY(launch_local_stats(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>()));
+    SRED_HIP_CHECK(hipMemcpyAsync(means, ctx->means.ptr, sizeof(float) * g.n(), hipMemcpyDeviceToHost, ctx->stream));
+    SRED_TRY(download(ctx, stds, ctx->stds, g.n()));
+    return SRED_OK;
+}
+
+int sred_global_accum_2d(sred_ctx *ctx, int metric, int precision, const float *img, const float *means,
+                         const float *stds, int w, int h, int bw, int bh, float filter_param, float threshold,
+                         float *S, float *Wsum)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !means || !stds || !S || !Wsum) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (precision != SRED_FP32 && precision != SRED_BF16X3) {
+        set_error("unknown precision");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(upload(ctx, ctx->means, means, g.n()));
+    SRED_TRY(upload(ctx, ctx->stds, stds, g.n()));
+    SRED_TRY(ctx->S.ensure(sizeof(float) * g.n()));
+    SRED_TRY(ctx->W.ensure(sizeof(float) * g.n()));
+    GlobalParams p;
+    p.metric = metric;
+    p.precision = precision;
+    p.filter_param = filter_param;
+    p.threshold = threshold;
+    SRED_TRY(global_accum_dev(ctx, g, p, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>(),
+                              ctx->S.as<float>(), ctx->W.as<float>()));
+    SRED_HIP_CHECK(hipMemcpyAsync(S, ctx->S.ptr, sizeof(float) * g.n(), hipMemcpyDeviceToHost, ctx->stream));
+    SRED_TRY(download(ctx, Wsum, ctx->W, g.n()));
+    return SRED_OK;
+}
+
+int sred_global_repetition_2d(sred_ctx *ctx, int metric, int precision, const float *img, const float *means,
+                              const float *stds, int w, int h, int bw, int bh, const float *mask,
+                              float relevance_constant, float threshold, float noise_var, float n_pixels,
+                              int normalize, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !means || !stds || !mask || !out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric < 0 || metric > 3) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    if (precision != SRED_FP32 && precision != SRED_BF16X3) {
+        set_error("unknown precision");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(upload(ctx, ctx->means, means, g.n()));
+    SRED_TRY(upload(ctx, ctx->stds, stds, g.n()));
+    SRED_TRY(upload(ctx, ctx->mask, mask, g.n()));
+    if (normalize) {
+        int ok;
+        SRED_TRY(mask_is_binary_dev(ctx, g, ctx->mask.as<float>(), &ok));
+        if (!ok) {
+            set_error("Relevance mask must be binary (0.0f or 1.0f)");
+            return SRED_ERR_NONBINARY_MASK;
+        }
+    }
+    SRED_TRY(ctx->S.ensure(sizeof(float) * g.n()));
+    SRED_TRY(ctx->W.ensure(sizeof(float) * g.n()));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * g.n()));
+    GlobalParams p;
+    p.metric = metric;
+    p.precision = precision;
+    p.filter_param = noise_var;
+    p.threshold = threshold;
+    SRED_TRY(global_accum_dev(ctx, g, p, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>(),
+                              ctx->S.as<float>(), ctx->W.as<float>()));
+    SRED_TRY(global_finalize_dev(ctx, g, metric, ctx->S.as<float>(), ctx->W.as<float>(), ctx->mask.as<float>(),
+                                 n_pixels, relevance_constant, normalize, ctx->out.as<float>()));
+    SRED_TRY(download(ctx, out, ctx->out, g.n()));
+    return SRED_OK;
+}
+
+int sred_block_repetition_2d(sred_ctx *ctx, int metric, const float *img, int w, int h, const float *tmpl, int bw,
+                             int bh, float tmpl_mean, float tmpl_std, float relevance_constant, int normalize,
+                             float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !tmpl || !out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric < 0 || metric > 3) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * g.n()));
+    SRED_TRY(block_pipeline(ctx, g, metric, ctx->img.as<float>(), tmpl, tmpl_mean, tmpl_std, relevance_constant,
+                            normalize, ctx->out.as<float>()));
+    SRED_TRY(download(ctx, out, ctx->out, g.n()));
+    return SRED_OK;
+}
+
+int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int bw,
+                           int bh, float relevance_constant, int normalize, float *d_out, float *d_means,
+                           float *d_stds, float *d_mask, float *d_S, float *d_Wsum)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!d_img || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric < 0 || metric > 3) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    if (precision != SRED_FP32 && precision != SRED_BF16X3) {
+        set_error("unknown precision");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = g.n();
+    if (!d_means) { SRED_TRY(ctx->means.ensure(sizeof(float) * n)); d_means = ctx->means.as<float>(); }
+    if (!d_stds) { SRED_TRY(ctx->stds.ensure(sizeof(float) * n)); d_stds = ctx->stds.as<float>(); }
+    if (!d_mask) { SRED_TRY(ctx->mask.ensure(sizeof(float) * n)); d_mask = ctx->mask.as<float>(); }
+    if (!d_S) { SRED_TRY(ctx->S.ensure(sizeof(float) * n)); d_S = ctx->S.as<float>(); }
+    if (!d_Wsum) { SRED_TRY(ctx->W.ensure(sizeof(float) * n)); d_Wsum = ctx->W.as<float>(); }
+    // GlobalRepetition2D_.java:124-140
+    SRED_TRY(launch_local_stats(ctx, g, d_img, d_means, d_stds));
+    float nv, np;
+    SRED_TRY(noise_var_dev(ctx, d_img, g.w, g.h, &nv));
+    const float thr = nv * relevance_constant;
+    SRED_TRY(relevance_mask_dev(ctx, g, d_stds, thr, d_mask, &np));
+    GlobalParams p;
+    p.metric = metric;
+    p.precision = precision;
+    p.filter_param = nv;
+    p.threshold = thr;
+    SRED_TRY(global_accum_dev(ctx, g, p, d_img, d_means, d_stds, d_S, d_Wsum));
+    SRED_TRY(global_finalize_dev(ctx, g, metric, d_S, d_Wsum, d_mask, np, relevance_constant, normalize, d_out));
+    return SRED_OK;
+}
+
+int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl, int bw,
+                          int bh, float tmpl_mean, float tmpl_std, float relevance_constant, int normalize,
+                          float *d_out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!d_img || !tmpl || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric < 0 || metric > 3) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    return block_pipeline(ctx, g, metric, d_img, tmpl, tmpl_mean, tmpl_std, relevance_constant, normalize, d_out);
+}
+
+int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated)
+{
+    SRED_TRY(check_ctx(ctx));
+    float t = 0.0f;
+    if (ctx->last_launches > 0 && ctx->ev0) {
+        SRED_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        SRED_HIP_CHECK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    }
+    if (ms) *ms = t;
+    if (launches) *launches = ctx->last_launches;
+    if (pairs_evaluated) *pairs_evaluated = ctx->last_pairs;
+    return SRED_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,822 @@
+// global.hip -- global repetition: all-pairs block comparison as a dense
+// contraction with the per-pixel aggregation fused into its epilogue.
+//
+// Reference: kernelGetGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}2D.cl (one
+// work-item per reference pixel looping over every test pixel, gathering
+// both blocks per pair) driven by CLUtils.getGlobal*2D (CLUtils.java:
+// 1680-2156).  Here the work is re-expressed as:
+//
+//   1. compaction  -- interior pixels that pass the relevance threshold
+//                     (kernel line 33/67: var < threshold skips) in row-major
+//                     order, so rows and columns are the same valid set;
+//   2. gather      -- each valid block, mean-subtracted and scaled, becomes
+//                     one row of U (N x K_pad): FP32 (k-major) or split BF16
+//                     hi/lo (row-major), plus a float4 of per-block scalars;
+//   3. contraction -- U * U^T tile by tile (FP32 VALU or bf16x3 MFMA); each
+//                     dot product goes straight into the metric's epilogue
+//                     (exp weight, normalisation, truncation, row sums), so
+//                     the N x N similarity matrix never leaves registers;
+//   4. reduce      -- per-chunk row partials summed in FP64 per pixel;
+//   5. finalize    -- CLUtils' host division, inversion, mask, normalise.
+//
+// Epilogue algebra (exact rewrites of the reference's per-pair formulas):
+//   w_ij = exp(-(s_i-s_j)^2 / (h+eps)) = exp2(-(t_i - t_j)^2), t = s*sqrt(log2 e/(h+eps))
+//   Pearson: u = (x-mu)/(s*sqrt(K-1)), r = u_i.u_j / (1 + eps/(s_i s_j))
+//            (= cov/(s_i s_j + eps)); flat blocks have u = 0, so pairs with one
+//            flat block give 0 as in the reference; a flat row gets the count
+//            of flat columns (both-flat pairs contribute weight 1).
+//   SSIM:    u = (x-mu)/sqrt(K-1) (u_i.u_j = cov); flat blocks carry an
+//            infinite variance so their term is 0; flat rows as Pearson.
+//   NRMSE:   u = (x-mu)/sqrt(K), mse = |u_i|^2 + |u_j|^2 - 2 u_i.u_j.
+//   AbsDiff: |s_i - s_j| w_ij = |t_i - t_j| w_ij / sqrt(log2 e/(h+eps)).
+#include <float.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "sred_internal.h"
+
+namespace sred {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr float kPadT = 1e30f;  // t of padding columns: their weight is exp2(-inf) = 0
+
+struct EpiConst {
+    float eps;  // 1e-7
+    float c1;   // SSIM 1e-4
+    float c2;   // SSIM 9e-4
+};
+
+// ---------------------------------------------------------------------------
+// 1. compaction (deterministic, row-major order preserved)
+// ---------------------------------------------------------------------------
+constexpr int kCompactItems = 4;
+constexpr int kCompactBlock = 256;
+constexpr int kCompactTile = kCompactItems * kCompactBlock;
+
+__device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int64_t p, int64_t n, int w, int h, int brw,
+                                            int brh, double thr_d, int *valid, int *flat)
+{
+#pragma clang fp contract(off)
+    *valid = 0;
+    *flat = 0;
+    if (p >= n) return;
+    int x = (int)(p % w), y = (int)(p / w);
+    if (x < brw || x >= w - brw || y < brh || y >= h - brh) return;
+    float s = stds[p];
+    float v = s * s;
+    if ((double)v < thr_d) return;  // kernelGetGlobalPearson2D.cl:33,67 (double threshold literal)
+    *valid = 1;
+    *flat = (s == 0.0f);
+}
+
+__global__ __launch_bounds__(kCompactBlock) void k_compact_count(const float *__restrict__ stds, int w, int h, int brw,
+                                                                 int brh, double thr_d, int2 *__restrict__ counts)
+{
+    const int64_t n = (int64_t)w * h;
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
+    int cv = 0, cf = 0;
+    for (int i = 0; i < kCompactItems; i++) {
+        int v, f;
+        valid_flags(stds, base + i, n, w, h, brw, brh, thr_d, &v, &f);
+        cv += v;
+        cf += f;
+    }
+    __shared__ int sv[kCompactBlock], sf[kCompactBlock];
+    sv[threadIdx.x] = cv;
+    sf[threadIdx.x] = cf;
+    __syncthreads();
+    for (int s = kCompactBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sv[threadIdx.x] += sv[threadIdx.x + s];
+            sf[threadIdx.x] += sf[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[blockIdx.x] = make_int2(sv[0], sf[0]);
+}
+
+// Exclusive scan of block counts (one workgroup); totals at index nblocks.
+__global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ counts, int nblocks,
+                                                       int2 *__restrict__ offsets)
+{
+    __shared__ int sv[1024], sf[1024];
+    __shared__ int carry_v, carry_f;
+    if (threadIdx.x == 0) { carry_v = 0; carry_f = 0; }
+    __syncthreads();
+    for (int b0 = 0; b0 < nblocks; b0 += 1024) {
+        int i = b0 + threadIdx.x;
+        int2 c = i < nblocks ? counts[i] : make_int2(0, 0);
+        sv[threadIdx.x] = c.x;
+        sf[threadIdx.x] = c.y;
+        __syncthreads();
+        for (int s = 1; s < 1024; s <<= 1) {  // Hillis-Steele inclusive scan
+            int av = threadIdx.x >= s ? sv[threadIdx.x - s] : 0;
+            int af = threadIdx.x >= s ? sf[threadIdx.x - s] : 0;
+            __syncthreads();
+            sv[threadIdx.x] += av;
+            sf[threadIdx.x] += af;
+            __syncthreads();
+        }
+        if (i < nblocks) offsets[i] = make_int2(carry_v + sv[threadIdx.x] - c.x, carry_f + sf[threadIdx.x] - c.y);
+        __syncthreads();
+        if (threadIdx.x == 1023) { carry_v += sv[1023]; carry_f += sf[1023]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offsets[nblocks] = make_int2(carry_v, carry_f);
+}
+
+__global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *__restrict__ stds, int w, int h,
+                                                                   int brw, int brh, double thr_d,
+                                                                   const int2 *__restrict__ offsets,
+                                                                   int *__restrict__ idx)
+{
+    const int64_t n = (int64_t)w * h;
+    const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
+    int flags[kCompactItems], cv = 0;
+    for (int i = 0; i < kCompactItems; i++) {
+        int f;
+        valid_flags(stds, base + i, n, w, h, brw, brh, thr_d, &flags[i], &f);
+        cv += flags[i];
+    }
+    __shared__ int sv[kCompactBlock];
+    sv[threadIdx.x] = cv;
+    __syncthreads();
+    for (int s = 1; s < kCompactBlock; s <<= 1) {
+        int a = threadIdx.x >= s ? sv[threadIdx.x - s] : 0;
+        __syncthreads();
+        sv[threadIdx.x] += a;
+        __syncthreads();
+    }
+    int pos = offsets[blockIdx.x].x + sv[threadIdx.x] - cv;
+    for (int i = 0; i < kCompactItems; i++)
+        if (flags[i]) idx[pos++] = (int)(base + i);
+}
+
+// ---------------------------------------------------------------------------
+// 2. gather
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
+
+// Per-block row scale of u for each metric.
+__device__ __forceinline__ float row_scale(int metric, float sigma, float inv_sqrt_km1, float inv_sqrt_k)
+{
+    if (metric == SRED_PEARSON) return sigma > 0.0f ? inv_sqrt_km1 / sigma : 0.0f;
+    if (metric == SRED_SSIM) return inv_sqrt_km1;
+    return inv_sqrt_k;  // NRMSE
+}
+
+// Split-BF16 rows: U_hi/U_lo[v][K_pad], 8 bf16 (one uint4) per thread.
+__global__ __launch_bounds__(256) void k_gather_bf16(int metric, const float *__restrict__ img,
+                                                     const float *__restrict__ means, const float *__restrict__ stds,
+                                                     const int *__restrict__ idx, int Nv, int Nv_pad, int K, int KS,
+                                                     const int *__restrict__ offs, float inv_sqrt_km1,
+                                                     float inv_sqrt_k, uint4 *__restrict__ Uhi,
+                                                     uint4 *__restrict__ Ulo)
+{
+    const int q4 = KS * 4;  // uint4 per row
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nv_pad * q4) return;
+    int v = (int)(t / q4), q = (int)(t - (int64_t)v * q4);
+    unsigned short hi[8], lo[8];
+    if (v < Nv) {
+        int p = idx[v];
+        float mu = means[p];
+        float sc = row_scale(metric, stds[p], inv_sqrt_km1, inv_sqrt_k);
+        for (int e = 0; e < 8; e++) {
+            int k = q * 8 + e;
+            float u = 0.0f;
+            if (k < K) u = (img[p + offs[k]] - mu) * sc;  // test_block[i] - test_mean (kernel line 84-87)
+            hi[e] = f2bf(u);
+            lo[e] = f2bf(u - bf2f(hi[e]));
+        }
+    } else {
+        for (int e = 0; e < 8; e++) hi[e] = lo[e] = 0;
+    }
+    uint4 H, L;
+    H.x = hi[0] | ((unsigned)hi[1] << 16); H.y = hi[2] | ((unsigned)hi[3] << 16);
+    H.z = hi[4] | ((unsigned)hi[5] << 16); H.w = hi[6] | ((unsigned)hi[7] << 16);
+    L.x = lo[0] | ((unsigned)lo[1] << 16); L.y = lo[2] | ((unsigned)lo[3] << 16);
+    L.z = lo[4] | ((unsigned)lo[5] << 16); L.w = lo[6] | ((unsigned)lo[7] << 16);
+    Uhi[t] = H;
+    Ulo[t] = L;
+}
+
+// FP32 rows, k-major: UT[k][v] for k < K4 (K rounded up to 4).
+__global__ __launch_bounds__(256) void k_gather_f32(int metric, const float *__restrict__ img,
+                                                    const float *__restrict__ means, const float *__restrict__ stds,
+                                                    const int *__restrict__ idx, int Nv, int Nv_pad, int K, int K4,
+                                                    const int *__restrict__ offs, float inv_sqrt_km1,
+                                                    float inv_sqrt_k, float *__restrict__ UT)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Nv_pad * K4) return;
+    int k = (int)(t / Nv_pad), v = (int)(t - (int64_t)k * Nv_pad);
+    float u = 0.0f;
+    if (v < Nv && k < K) {
+        int p = idx[v];
+        u = (img[p + offs[k]] - means[p]) * row_scale(metric, stds[p], inv_sqrt_km1, inv_sqrt_k);
+    }
+    UT[t] = u;
+}
+
+// Per-block scalars (see the layout of the epilogue constants below).
+__global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict__ img, const float *__restrict__ means,
+                                             const float *__restrict__ stds, const int *__restrict__ idx, int Nv,
+                                             int Nv_pad, int K, const int *__restrict__ offs, float tscale,
+                                             float inv_sqrt_km1, float inv_sqrt_k, float4 *__restrict__ aux)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Nv_pad) return;
+    float4 a = make_float4(kPadT, 0.0f, 0.0f, 0.0f);
+    if (v >= Nv) {
+        if (metric == SRED_SSIM) a.z = INFINITY;
+        if (metric == SRED_NRMSE) a.y = -INFINITY;
+        aux[v] = a;
+        return;
+    }
+    int p = idx[v];
+    float s = stds[p], mu = means[p];
+    a.x = s * tscale;
+    if (metric == SRED_PEARSON) {
+        a.y = s > 0.0f ? 1.0f / s : 0.0f;
+    } else if (metric == SRED_SSIM) {
+        a.y = mu;
+        a.z = s == 0.0f ? INFINITY : s * s;
+    } else if (metric == SRED_NRMSE) {
+        float sc = row_scale(metric, s, inv_sqrt_km1, inv_sqrt_k), n = 0.0f;
+        for (int k = 0; k < K; k++) {
+            float u = (img[p + offs[k]] - mu) * sc;
+            n += u * u;
+        }
+        a.y = n;
+    }
+    aux[v] = a;
+}
+
+// ---------------------------------------------------------------------------
+// 3. epilogue
+// ---------------------------------------------------------------------------
+// Row constants R and column constants C from the per-block scalars:
+//   Pearson: R = (t, eps/s),          C = (t, 1/s)
+//   SSIM:    R = (t, 2mu, mu^2, s^2+c2), C = (t, mu, mu^2, s^2)   (inf for flat)
+//   NRMSE:   R = (t, |u|^2),          C = (t, |u|^2)              (-inf for padding)
+//   AbsDiff: R = (t),                 C = (t)
+template <int M> __device__ __forceinline__ float4 row_const(float4 a, const EpiConst &e)
+{
+    if (M == SRED_PEARSON) return make_float4(a.x, e.eps * a.y, 0.0f, 0.0f);
+    if (M == SRED_SSIM) return make_float4(a.x, 2.0f * a.y, a.y * a.y, a.z + e.c2);
+    return a;
+}
+template <int M> __device__ __forceinline__ float4 col_const(float4 a, const EpiConst &)
+{
+    if (M == SRED_SSIM) return make_float4(a.x, a.y, a.y * a.y, a.z);
+    return a;
+}
+
+template <int M>
+__device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, float &W,
+                                         const EpiConst &e)
+{
+    const float d = R.x - C.x;
+    const float w = __builtin_amdgcn_exp2f(-(d * d));
+    if (M == SRED_PEARSON) {
+        float r = dot * __builtin_amdgcn_rcpf(1.0f + R.y * C.y);
+        S = fmaf(fmaxf(r, 0.0f), w, S);
+    } else if (M == SRED_SSIM) {
+        float a = fmaf(R.y, C.y, e.c1);
+        float b = fmaf(2.0f, dot, e.c2);
+        float c = fmaf(R.z, C.z, e.c1);
+        float t = (a * b) * __builtin_amdgcn_rcpf(c * (R.w + C.w)) * w;
+        S += fmaxf(t, 0.0f);
+    } else if (M == SRED_NRMSE) {
+        float mse = fmaf(-2.0f, dot, R.y + C.y);
+        S += __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
+    } else {  // ABSDIFF
+        S = fmaf(fabsf(d), w, S);
+    }
+    W += w;
+}
+
+// ---------------------------------------------------------------------------
+// 3a. split-BF16 MFMA contraction (v_mfma_f32_16x16x32_bf16).
+// Workgroup: 4 waves; each wave owns RG*16 rows whose A fragments (hi and lo)
+// stay in registers for the whole column sweep.  Column tiles of 64 blocks
+// are staged in LDS (hi and lo) and shared by the 4 waves.  MFMA lane layout
+// (16x16 output): lane l holds column l%16 and rows 4*(l/16)+0..3; operand
+// lanes hold row/column l%16 and k-group l/16 (8 consecutive k).  The k
+// assignment is identical for both operands, so it cancels in the dot.
+// Workgroup ids are chunk-major: concurrently resident workgroups sweep the
+// same column chunk (sized to stay in each XCD's L2).
+// ---------------------------------------------------------------------------
+constexpr int kRG = 2;
+constexpr int kMfmaBM = 64 * kRG;
+constexpr int kBN = 64;
+
+template <int M, int KS>
+__global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+                                                       const float4 *__restrict__ aux, int nrowtiles,
+                                                       int cols_per_chunk, int ncols, float2 *__restrict__ partial,
+                                                       int Nv_pad, EpiConst ec)
+{
+    constexpr int ROWU4 = KS * 4;
+    constexpr int PITCH = ROWU4 + 1;  // odd pitch (in 16-byte units): conflict-free fragment reads
+    __shared__ uint4 lds_hi[kBN * PITCH];
+    __shared__ uint4 lds_lo[kBN * PITCH];
+
+    const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int rowbase = rt * kMfmaBM + wave * 16 * kRG;
+
+    bf16x8 ahi[kRG][KS], alo[kRG][KS];
+#pragma unroll
+    for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            size_t o = (size_t)(rowbase + rg * 16 + li) * ROWU4 + ks * 4 + lk;
+            uint4 h = Uhi[o], l = Ulo[o];
+            ahi[rg][ks] = __builtin_bit_cast(bf16x8, h);
+            alo[rg][ks] = __builtin_bit_cast(bf16x8, l);
+        }
+    float4 R[kRG][4];
+#pragma unroll
+    for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) R[rg][r] = row_const<M>(aux[rowbase + rg * 16 + lk * 4 + r], ec);
+
+    double accS[kRG][4], accW[kRG][4];
+#pragma unroll
+    for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0;
+
+    const int c_begin = ch * cols_per_chunk;
+    const int c_end = min(c_begin + cols_per_chunk, ncols);
+    for (int c0 = c_begin; c0 < c_end; c0 += kBN) {
+        __syncthreads();
+        for (int i = tid; i < kBN * ROWU4; i += 256) {
+            int col = i / ROWU4, q = i - col * ROWU4;
+            size_t o = (size_t)(c0 + col) * ROWU4 + q;
+            lds_hi[col * PITCH + q] = Uhi[o];
+            lds_lo[col * PITCH + q] = Ulo[o];
+        }
+        __syncthreads();
+        float pS[kRG][4], pW[kRG][4];
+#pragma unroll
+        for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) pS[rg][r] = pW[rg][r] = 0.0f;
+#pragma unroll
+        for (int cg = 0; cg < 4; cg++) {
+            const int col = cg * 16 + li;
+            const float4 C = col_const<M>(aux[c0 + col], ec);
+            floatx4 acc[kRG];
+#pragma unroll
+            for (int rg = 0; rg < kRG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
+                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
+#pragma unroll
+                for (int rg = 0; rg < kRG; rg++) {
+                    acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, acc[rg], 0, 0, 0);
+                    acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, acc[rg], 0, 0, 0);
+                    acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, acc[rg], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) epilogue<M>(acc[rg][r], R[rg][r], C, pS[rg][r], pW[rg][r], ec);
+        }
+#pragma unroll
+        for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                accS[rg][r] += (double)pS[rg][r];
+                accW[rg][r] += (double)pW[rg][r];
+            }
+    }
+#pragma unroll
+    for (int rg = 0; rg < kRG; rg++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            double s = accS[rg][r], w = accW[rg][r];
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) {
+                s += __shfl_xor(s, m);
+                w += __shfl_xor(w, m);
+            }
+            if (li == 0)
+                partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2((float)s, (float)w);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// 3b. plain FP32 contraction (vector ALU).  64x64 pair tiles, 4x4 pairs per
+// thread; the workgroup's 64 rows stay in LDS (k-major), each column tile is
+// staged k-major, and every k step is two broadcast-free float4 LDS reads
+// and 16 FMAs.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ UT, const float4 *__restrict__ aux,
+                                                      int K4, int Nv_pad, int nrowtiles, int cols_per_chunk,
+                                                      float2 *__restrict__ partial, EpiConst ec)
+{
+    extern __shared__ float sm[];
+    float *As = sm;
+    float *Bs = sm + (size_t)K4 * 64;
+    const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int row0 = rt * 64;
+    for (int i = tid; i < K4 * 16; i += 256) {
+        int k = i >> 4, q = i & 15;
+        *reinterpret_cast<float4 *>(&As[k * 64 + q * 4]) =
+            *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + row0 + q * 4]);
+    }
+    float4 R[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) R[i] = row_const<M>(aux[row0 + ty * 4 + i], ec);
+    double accS[4] = {0, 0, 0, 0}, accW[4] = {0, 0, 0, 0};
+    const int c_begin = ch * cols_per_chunk;
+    const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
+    for (int c0 = c_begin; c0 < c_end; c0 += 64) {
+        __syncthreads();
+        for (int i = tid; i < K4 * 16; i += 256) {
+            int k = i >> 4, q = i & 15;
+            *reinterpret_cast<float4 *>(&Bs[k * 64 + q * 4]) =
+                *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + c0 + q * 4]);
+        }
+        __syncthreads();
+        float d[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) d[i][j] = 0.0f;
+#pragma unroll 4
+        for (int k = 0; k < K4; k++) {
+            float4 a = *reinterpret_cast<const float4 *>(&As[k * 64 + ty * 4]);
+            float4 b = *reinterpret_cast<const float4 *>(&Bs[k * 64 + tx * 4]);
+            float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) d[i][j] = fmaf(av[i], bv[j], d[i][j]);
+        }
+        float4 C[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) C[j] = col_const<M>(aux[c0 + tx * 4 + j], ec);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float s = 0.0f, w = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) epilogue<M>(d[i][j], R[i], C[j], s, w, ec);
+            accS[i] += (double)s;
+            accW[i] += (double)w;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double s = accS[i], w = accW[i];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            s += __shfl_xor(s, m);
+            w += __shfl_xor(w, m);
+        }
+        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = make_float2((float)s, (float)w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3c. AbsDiffStds (kernelGetGlobalAbsDiffStds2D.cl): no block contraction,
+// only the weight and |s_i - s_j| per pair.  64x64 pair tiles, 4x4 per thread.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_contract_absdiff(const float4 *__restrict__ aux, int Nv_pad, int nrowtiles,
+                                                          int cols_per_chunk, float2 *__restrict__ partial,
+                                                          EpiConst ec)
+{
+    const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int row0 = rt * 64;
+    __shared__ float tcol[64];
+    float4 R[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) R[i] = aux[row0 + ty * 4 + i];
+    double accS[4] = {0, 0, 0, 0}, accW[4] = {0, 0, 0, 0};
+    const int c_begin = ch * cols_per_chunk;
+    const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
+    for (int c0 = c_begin; c0 < c_end; c0 += 64) {
+        __syncthreads();
+        if (tid < 64) tcol[tid] = aux[c0 + tid].x;
+        __syncthreads();
+        float4 C[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) C[j] = make_float4(tcol[tx * 4 + j], 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float s = 0.0f, w = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) epilogue<SRED_ABSDIFF>(0.0f, R[i], C[j], s, w, ec);
+            accS[i] += (double)s;
+            accW[i] += (double)w;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        double s = accS[i], w = accW[i];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            s += __shfl_xor(s, m);
+            w += __shfl_xor(w, m);
+        }
+        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = make_float2((float)s, (float)w);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. reduce: FP64 sum of the per-chunk partials, scattered to pixel maps.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce(const float2 *__restrict__ partial, int nchunks, int Nv, int Nv_pad,
+                                                const int *__restrict__ idx, const float *__restrict__ stds,
+                                                int metric, float flat_count, double s_scale,
+                                                float *__restrict__ S_map, float *__restrict__ W_map)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Nv) return;
+    double s = 0.0, w = 0.0;
+    for (int c = 0; c < nchunks; c++) {
+        float2 p = partial[(size_t)c * Nv_pad + v];
+        s += (double)p.x;
+        w += (double)p.y;
+    }
+    int p = idx[v];
+    // both-flat pairs contribute 1*weight with weight exp(-0) = 1
+    // (kernelGetGlobalPearson2D.cl:101-102, kernelGetGlobalSsim2D.cl:108-109)
+    if ((metric == SRED_PEARSON || metric == SRED_SSIM) && stds[p] == 0.0f) s = (double)flat_count;
+    S_map[p] = (float)(s * s_scale);
+    W_map[p] = (float)w;
+}
+
+// ---------------------------------------------------------------------------
+// 5. host finalize of CLUtils.getGlobal*2D (CLUtils.java:1747-1759 and the
+// inversion of :1997-2010 / :2126-2139): rep = S/(Wsum*nPixels + eps) on the
+// interior in Java float order; border pixels stay 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_finalize(const float *__restrict__ S, const float *__restrict__ W,
+                                                  const float *__restrict__ mask, int w, int h, int brw, int brh,
+                                                  float n_pixels, int invert, float *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    int64_t n = (int64_t)w * h;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int x = (int)(i % w), y = (int)(i / w);
+        float r = 0.0f;
+        if (x >= brw && x < w - brw && y >= brh && y < h - brh) {
+            float den = W[i] * n_pixels;
+            den = den + kEps;
+            r = S[i] / den;
+            if (invert && mask[i] > 0.0f) r = (r == 0.0f) ? 1.0f : 1.0f / r;
+        }
+        out[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <int M, int KS>
+static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux, int nrt,
+                           int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+{
+    hipLaunchKernelGGL((k_contract_mfma<M, KS>), dim3(grid), dim3(256), 0, st, hi, lo, aux, nrt, cpc, ncols, part,
+                       Nv_pad, ec);
+}
+
+template <int M>
+static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
+                       int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+{
+    switch (KS) {
+    case 1: launch_mfma_ks<M, 1>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 2: launch_mfma_ks<M, 2>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 3: launch_mfma_ks<M, 3>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 4: launch_mfma_ks<M, 4>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 5: launch_mfma_ks<M, 5>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 6: launch_mfma_ks<M, 6>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 7: launch_mfma_ks<M, 7>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 8: launch_mfma_ks<M, 8>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    default:
+        set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+        return SRED_ERR_ARG;
+    }
+    return SRED_OK;
+}
+
+template <int M>
+static void launch_f32(int grid, size_t lds, hipStream_t st, const float *UT, const float4 *aux, int K4, int Nv_pad,
+                       int nrt, int cpc, float2 *part, EpiConst ec)
+{
+    hipLaunchKernelGGL((k_contract_f32<M>), dim3(grid), dim3(256), lds, st, UT, aux, K4, Nv_pad, nrt, cpc, part, ec);
+}
+
+// Columns per chunk: the chunk's column data (~2 MiB) stays resident in an
+// XCD's L2 while the concurrently running workgroups sweep it; smaller chunks
+// when there are too few row tiles to fill 256 CUs.
+static int choose_chunk_cols(int Nv_pad, int bytes_per_col, int nrowtiles, int override_cols)
+{
+    int cols;
+    if (override_cols > 0) {
+        cols = (override_cols + 63) / 64 * 64;
+    } else {
+        cols = std::max(64, (2 << 20) / std::max(bytes_per_col, 1) / 64 * 64);
+        const int target_wgs = 4096;
+        int want_chunks = (target_wgs + nrowtiles - 1) / nrowtiles;
+        int cols_fill = ((Nv_pad + want_chunks - 1) / want_chunks + 63) / 64 * 64;
+        cols = std::max(64, std::min(cols, cols_fill));
+    }
+    return std::min(cols, Nv_pad);
+}
+
+__global__ void k_zero(unsigned int *p, int64_t nwords)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes)
+{
+    int64_t words = (int64_t)((bytes + 3) / 4);
+    int blocks = (int)std::min<int64_t>((words + 255) / 256, 4096);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, ctx->stream, (unsigned int *)ptr, words);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
+int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
+                     const float *d_means, const float *d_stds, float *d_S, float *d_W)
+{
+    const int metric = p.metric;
+    if (metric < 0 || metric > 3) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    const double thr_d = java_float_literal(p.threshold);
+    const double fp_d = java_float_literal(p.filter_param);
+    const double eps_d = java_float_literal(kEps);
+    const float tscale = (float)std::sqrt(1.4426950408889634 / (fp_d + eps_d));
+    const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(g.K - 1)));
+    const float inv_sqrt_k = (float)(1.0 / std::sqrt((double)g.K));
+    EpiConst ec{kEps, 0.0001f, 0.0009f};
+
+    // 1. compaction
+    const int64_t n = g.n();
+    const int nblk = (int)((n + kCompactTile - 1) / kCompactTile);
+    SRED_TRY(ctx->counts.ensure(sizeof(int2) * (2 * (size_t)nblk + 2)));
+    int2 *d_counts = ctx->counts.as<int2>();
+    int2 *d_offsets = d_counts + nblk;
+    SRED_TRY(ctx->valid_idx.ensure(sizeof(int) * (size_t)n + 256));
+    int *d_idx = ctx->valid_idx.as<int>();
+    hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
+                       g.brh, thr_d, d_counts);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, nblk, d_offsets);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
+                       g.brh, thr_d, d_offsets, d_idx);
+    SRED_HIP_CHECK(hipGetLastError());
+    int2 tot;
+    SRED_HIP_CHECK(hipMemcpyAsync(&tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const int Nv = tot.x;
+    const float flat_count = (float)std::min(tot.y, 16777216);
+
+    SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));
+    SRED_TRY(zero_dev(ctx, d_W, sizeof(float) * n));
+    ctx->last_ms = 0.0f;
+    ctx->last_launches = 0;
+    ctx->last_pairs = (int64_t)Nv * Nv;
+    if (Nv == 0) return SRED_OK;
+    const int Nv_pad = (Nv + 255) / 256 * 256;
+
+    // 2. offsets relative to the image and per-block scalars
+    SRED_TRY(upload_offsets(ctx, g, g.w, &ctx->offs2, ctx->offs2_key));
+    const int *d_offs = ctx->offs2.as<int>();
+    SRED_TRY(ctx->aux.ensure(sizeof(float4) * (size_t)Nv_pad));
+    float4 *d_aux = ctx->aux.as<float4>();
+    hipLaunchKernelGGL(k_aux, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
+                       d_idx, Nv, Nv_pad, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, d_aux);
+
+    const bool mfma = (p.precision == SRED_BF16X3) && metric != SRED_ABSDIFF;
+    const int KS = (g.K + 31) / 32;
+    const int K4 = (g.K + 3) / 4 * 4;
+    int bm, bytes_per_col;
+    if (mfma) { bm = kMfmaBM; bytes_per_col = KS * 32 * 4; }
+    else if (metric == SRED_ABSDIFF) { bm = 64; bytes_per_col = 16; }
+    else { bm = 64; bytes_per_col = K4 * 4; }
+    const int nrowtiles = Nv_pad / bm;
+    const int cpc = choose_chunk_cols(Nv_pad, bytes_per_col, nrowtiles, ctx->chunk_cols);
+    const int nchunks = (Nv_pad + cpc - 1) / cpc;
+    const int64_t grid64 = (int64_t)nrowtiles * nchunks;
+    if (grid64 > 0x7fffffff) {
+        set_error("problem too large for one launch");
+        return SRED_ERR_ARG;
+    }
+    const int grid = (int)grid64;
+    SRED_TRY(ctx->partial.ensure(sizeof(float2) * (size_t)nchunks * Nv_pad));
+    float2 *d_part = ctx->partial.as<float2>();
+
+    if (mfma) {
+        if (KS > 8) {
+            set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+            return SRED_ERR_ARG;
+        }
+        size_t u4 = (size_t)Nv_pad * KS * 4;
+        SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
+        SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
+        hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                           d_means, d_stds, d_idx, Nv, Nv_pad, g.K, KS, d_offs, inv_sqrt_km1, inv_sqrt_k,
+                           ctx->uhi.as<uint4>(), ctx->ulo.as<uint4>());
+    } else if (metric != SRED_ABSDIFF) {
+        size_t nu = (size_t)Nv_pad * K4;
+        SRED_TRY(ctx->u32.ensure(sizeof(float) * nu));
+        hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                           d_means, d_stds, d_idx, Nv, Nv_pad, g.K, K4, d_offs, inv_sqrt_km1, inv_sqrt_k,
+                           ctx->u32.as<float>());
+    }
+    SRED_HIP_CHECK(hipGetLastError());
+
+    // 3. contraction (timed with events on the launch stream)
+    if (!ctx->ev0) {
+        SRED_HIP_CHECK(hipEventCreate(&ctx->ev0));
+        SRED_HIP_CHECK(hipEventCreate(&ctx->ev1));
+    }
+    SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (mfma) {
+        const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
+        int rc = SRED_OK;
+        switch (metric) {
+        case SRED_PEARSON:
+            rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part,
+                                           Nv_pad, ec);
+            break;
+        case SRED_SSIM:
+            rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part, Nv_pad,
+                                        ec);
+            break;
+        default:
+            rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part, Nv_pad,
+                                         ec);
+            break;
+        }
+        SRED_TRY(rc);
+    } else if (metric == SRED_ABSDIFF) {
+        hipLaunchKernelGGL(k_contract_absdiff, dim3(grid), dim3(256), 0, ctx->stream, d_aux, Nv_pad, nrowtiles, cpc,
+                           d_part, ec);
+    } else {
+        size_t lds = sizeof(float) * 2 * (size_t)K4 * 64;
+        if (lds > 160 * 1024) {
+            set_error("block too large for the FP32 contraction tile");
+            return SRED_ERR_ARG;
+        }
+        const float *UT = ctx->u32.as<float>();
+        if (metric == SRED_PEARSON)
+            launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+        else if (metric == SRED_SSIM)
+            launch_f32<SRED_SSIM>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+        else
+            launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+    }
+    SRED_HIP_CHECK(hipGetLastError());
+    SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->last_launches = 1;
+
+    // 4. reduce
+    const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)tscale : 1.0;
+    hipLaunchKernelGGL(k_reduce, dim3((Nv + 255) / 256), dim3(256), 0, ctx->stream, d_part, nchunks, Nv, Nv_pad, d_idx,
+                       d_stds, metric, flat_count, s_scale, d_S, d_W);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
+int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_S, const float *d_W,
+                        const float *d_mask, float n_pixels, float c, int normalize, float *d_out)
+{
+    int blocks = (int)std::min<int64_t>((g.n() + 255) / 256, 8192);
+    int invert = (metric == SRED_NRMSE || metric == SRED_ABSDIFF);
+    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, ctx->stream, d_S, d_W, d_mask, g.w, g.h, g.brw, g.brh,
+                       n_pixels, invert, d_out);
+    SRED_HIP_CHECK(hipGetLastError());
+    if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
+    if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, d_mask, 0));
+    return SRED_OK;
+}
+
+}  // namespace sred

@@ -1,0 +1,120 @@
+// sred_internal.h -- shared declarations of libsred_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sred.h"
+
+namespace sred {
+
+// Per-thread error text behind sred_last_error().
+void set_error(const std::string &msg);
+
+#define SRED_HIP_CHECK(expr)                                                            \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::sred::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+            return SRED_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+#define SRED_TRY(expr)                   \
+    do {                                 \
+        int rc_ = (expr);                \
+        if (rc_ != SRED_OK) return rc_;  \
+    } while (0)
+
+constexpr float kEps = 1e-7f;           // Utils.EPSILON, Utils.java:20
+constexpr int kMaxK = 3200;             // footprint size cap (63x63 ellipse has 3109 px)
+
+// Growable device scratch buffer owned by a context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Geometry of one 2D problem.
+struct Geometry {
+    int w = 0, h = 0, bw = 0, bh = 0, brw = 0, brh = 0, K = 0;
+    int64_t n() const { return (int64_t)w * h; }
+};
+
+int make_geometry(int w, int h, int bw, int bh, Geometry *g);
+// Row-major footprint offsets (dx, dy), reference predicate.
+int footprint(int bw, int bh, std::vector<int> *dx, std::vector<int> *dy);
+// The double an OpenCL compiler reads from Java's ""+f.
+double java_float_literal(float f);
+
+}  // namespace sred
+
+struct sred_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int chunk_cols = 0;
+    // device scratch
+    sred::DevBuf img, means, stds, mask, S, W, out;          // pixel maps (host-pointer paths)
+    sred::DevBuf tile_var, counts, valid_idx, aux, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
+    int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
+    std::vector<float> host_tile_var;
+    // measurement of the last global contraction
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    int last_launches = 0;
+    int64_t last_pairs = 0;
+};
+
+namespace sred {
+
+// Upload the footprint offsets dy*pitch+dx (cached per (bw, bh, pitch)).
+int upload_offsets(sred_ctx *ctx, const Geometry &g, int pitch, DevBuf *buf, int *key);
+// Zero device memory with a kernel (graph-capture safe, unlike memset nodes).
+int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes);
+
+// ---- launchers (stream = ctx->stream) ----
+// Local statistics, reference float order (kernelGetLocalStatistics2D.cl).
+int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds);
+// Java float noise variance (Utils.getMeanNoiseVar2D): tile variances on the
+// GPU, sort and mean of the lowest 3% on the host (synchronises).
+int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var);
+// Relevance mask + Java float nPixels (device); n_pixels returned on host
+// (synchronises).
+int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
+                       float *n_pixels);
+
+struct GlobalParams {
+    int metric = SRED_PEARSON;
+    int precision = SRED_FP32;
+    float filter_param = 0.0f;  // noise variance
+    float threshold = 0.0f;     // noise variance * relevance constant
+};
+// Accumulators S, Wsum (float maps, w*h) of the global kernels.
+int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
+                     const float *d_means, const float *d_stds, float *d_S, float *d_W);
+// Host-finalize of CLUtils.getGlobal*2D on device maps.
+int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_S, const float *d_W,
+                        const float *d_mask, float n_pixels, float c, int normalize, float *d_out);
+// Block-repetition raw kernel output (before host post-processing); window
+// means/stds are written when both output pointers are non-NULL.
+int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
+                  float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out);
+// Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
+// mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
+int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);
+int apply_mask_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask);
+int interior_ones_mask_dev(sred_ctx *ctx, const Geometry &g, float *d_mask);
+// In-place inversion where mask > 0: rep==0 -> 1 else 1/rep (CLUtils.java:1997-2010).
+int invert_masked_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask);
+// 1 - x on the interior (CLUtils.java:962-966).
+int one_minus_dev(sred_ctx *ctx, const Geometry &g, float *d_map);
+// Check that a device mask is binary on the interior.
+int mask_is_binary_dev(sred_ctx *ctx, const Geometry &g, const float *d_mask, int *is_binary);
+
+}  // namespace sred

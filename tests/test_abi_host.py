@@ -1,0 +1,120 @@
+"""CPU-only checks of the C ABI: the library loads and exports every symbol
+include/sred.h declares, and the host-side Utils mirror (plain C++, no GPU
+call) matches the oracle and the reference's known answers."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sred_amd
+from sred_amd import _lib as L
+from tests.helpers import kats, random42_inputs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+K = kats()
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sred_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.load()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names == set(L.EXPORTS)
+
+
+def test_version_and_metric_names():
+    lib = L.load()
+    assert lib.sred_version() >= 100
+    for i, m in enumerate(sred_amd.GLOBAL_METRICS):
+        assert lib.sred_metric_from_name(m.encode(), 0) == i
+    for i, m in enumerate(sred_amd.BLOCK_METRICS):
+        assert lib.sred_metric_from_name(m.encode(), 1) == i
+    assert lib.sred_metric_from_name(b"Hausdorff", 0) == L.SRED_ERR_UNKNOWN_METRIC
+    assert lib.sred_metric_from_name(b"NRMSE", 1) == L.SRED_ERR_UNKNOWN_METRIC  # block list says "NRMSE (inverted)"
+
+
+def test_block_size_table():
+    for b, k in K["block_size_table"]["sizes"].items():
+        assert sred_amd.block_size_2d(int(b), int(b)) == k
+    for bw, bh in [(3, 7), (9, 5), (21, 11), (63, 63)]:
+        assert sred_amd.block_size_2d(bw, bh) == O.footprint(bw, bh)[0]
+
+
+def test_input_normalisation_matches_oracle_bitwise():
+    rng = np.random.default_rng(0)
+    img = (rng.random((37, 29)) * 1000).astype(np.float32)
+    a = sred_amd.get_input_image_2d(img).image_array
+    np.testing.assert_array_equal(a, O.normalize_input(img))
+    k = K["input_image_2d_normalize"]
+    np.testing.assert_allclose(sred_amd.get_input_image_2d(k["image"]).image_array, k["expected"], atol=k["tol"])
+    with pytest.raises(NotImplementedError):
+        sred_amd.get_input_image_2d(img, stabilise_noise_variance=True)
+
+
+def test_reference_block_kat_and_oracle():
+    k = K["reference_block_2d"]
+    rb = sred_amd.get_reference_block_2d(np.array(k["block"], np.float32))
+    assert abs(rb.mean - k["expected_mean"]) <= k["tol"] and abs(rb.std - k["expected_std"]) <= k["tol"]
+    assert sred_amd.get_reference_block_2d(np.array(k["even_block"], np.float32)) is None
+    rng = np.random.default_rng(1)
+    blk = rng.random((15, 11)).astype(np.float32)
+    rb = sred_amd.get_reference_block_2d(blk)
+    px, m, s = O.reference_block(blk)
+    np.testing.assert_array_equal(rb.pixels, px)
+    assert (rb.mean, rb.std) == (m, s)
+
+
+def test_noise_variance_and_mask_random42_kat():
+    k = K["relevance_mask_random42"]
+    img, stds = random42_inputs(k)
+    rm = sred_amd.get_relevance_mask(img, 50, 50, 2, 2, stds, k["relevance_constant"])
+    assert np.float64(np.float32(rm.noise_mean_variance)) == k["expected_noise_var"]
+    assert np.float32(rm.relevance_threshold) == np.float32(k["expected_threshold_f32"])
+    mask, thr, npx = O.relevance_mask(stds, 2, 2, O.noise_var(img), k["relevance_constant"])
+    np.testing.assert_array_equal(rm.relevance_mask.reshape(50, 50), mask)
+
+
+def test_noise_variance_matches_oracle_large_tiles():
+    rng = np.random.default_rng(2)
+    img = rng.random((400, 300)).astype(np.float32)  # > CIF: 16x16 tiles
+    assert sred_amd.get_mean_noise_var_2d(img, 300, 400) == O.noise_var(img)
+    bad = img.copy()
+    bad[5, 5] = np.inf
+    with pytest.raises(sred_amd.SredError):
+        sred_amd.get_mean_noise_var_2d(bad, 300, 400)
+    with pytest.raises(sred_amd.SredError):
+        sred_amd.get_mean_noise_var_2d(np.zeros((5, 5), np.float32), 5, 5)  # no 8x8 tile
+
+
+def test_apply_mask_and_normalise_kats():
+    k = K["apply_mask_2d"]
+    a = np.array(k["image"], np.float32)
+    sred_amd.apply_mask_2d(a, 3, 3, np.array(k["mask"], np.float32))
+    np.testing.assert_array_equal(a, np.array(k["expected"], np.float32))
+    k = K["normalize_image_2d_mask"]
+    out = sred_amd.normalize_image_2d(np.array(k["image"], np.float32), 4, 4, *k["border"], np.array(k["mask"], np.float32))
+    np.testing.assert_allclose(out, np.array(k["expected"]), atol=k["tol"])
+    k = K["normalize_image_2d_nonbinary"]
+    with pytest.raises(ValueError):
+        sred_amd.normalize_image_2d(np.array(k["image"], np.float32), 4, 4, *k["border"], np.array(k["mask"], np.float32))
+
+
+def test_normalise_matches_oracle_bitwise():
+    rng = np.random.default_rng(4)
+    m = rng.random((30, 31)).astype(np.float32)
+    mask = (rng.random((30, 31)) > 0.3).astype(np.float32)
+    for mk in (None, mask):
+        a = sred_amd.normalize_image_2d(m, 31, 30, 3, 2, mk)
+        np.testing.assert_array_equal(a, O.normalize_map(m, 3, 2, mk))
