@@ -43,7 +43,36 @@ def parse():
     ap.add_argument("--oracle-rows", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: bf16x3 steps only, then a 256 MiB device copy (PMC calibration)")
     return ap.parse_args()
+
+
+def profile_run(args):
+    """bf16x3 steps only plus one plain copy of known size, so rocprofv3 PMC
+    counters (FETCH_SIZE, WRITE_SIZE) of the contraction can be scaled by
+    what the same counters read for a copy of 256 MiB."""
+    import torch
+
+    import sred_amd
+    from sred_amd import _lib as L
+    from sred_amd import synthetic
+    from sred_amd.commands import global_map_device
+
+    img = sred_amd.get_input_image_2d(synthetic.lattice_and_blobs(W, seed=2)).image_array
+    d_img = torch.from_numpy(img).cuda()
+    d_out = torch.zeros_like(d_img)
+    with sred_amd.Context(0) as ctx:
+        for i in range(args.warmup + args.steps):
+            global_map_device(ctx, L.PEARSON, L.BF16X3, d_img.data_ptr(), W, H, B, B, 0.0, True, d_out.data_ptr())
+            torch.cuda.synchronize()
+            print(f"# step {i}: contraction {ctx.last_contraction()[0]:.2f} ms", file=sys.stderr, flush=True)
+    n = 64 * 1024 * 1024  # 256 MiB of float32
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile": True, "copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
 
 
 def host_cores():
@@ -73,8 +102,11 @@ def traffic_from_profile():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 (see DESIGN.md for sharding)"}))
+        print(json.dumps({"error": "one GPU only: --gpus must be 1 (row sharding across GPUs is not built)"}))
         sys.exit(2)
+    if args.profile:
+        profile_run(args)
+        return
     import torch
 
     import oracle as O

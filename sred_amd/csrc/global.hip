@@ -42,6 +42,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr float kPadT = 1e30f;  // t of padding columns: their weight is exp2(-inf) = 0
+// Pearson fast path: with every sigma of a tile >= kFastSigma (or 0),
+// delta = eps/(s_i s_j) <= 3.1e-4 and r = dot*(1 - delta) differs from
+// dot/(1 + delta) by delta^2 <= 1e-7 relative; other tiles use the exact form.
+constexpr float kFastSigma = 0.018f;
 
 struct EpiConst {
     float eps;  // 1e-7
@@ -248,6 +252,7 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
     a.x = s * tscale;
     if (metric == SRED_PEARSON) {
         a.y = s > 0.0f ? 1.0f / s : 0.0f;
+        a.z = (s > 0.0f && s < kFastSigma) ? 1.0f : 0.0f;
     } else if (metric == SRED_SSIM) {
         a.y = mu;
         a.z = s == 0.0f ? INFINITY : s * s;
@@ -260,6 +265,19 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
         a.y = n;
     }
     aux[v] = a;
+}
+
+// tile_small[g] = 1 if any block of rows/columns [32g, 32g+32) has 0 < sigma < kFastSigma
+__global__ __launch_bounds__(256) void k_tile_small(const float4 *__restrict__ aux, int ngroups,
+                                                    int *__restrict__ tile_small)
+{
+    int g = (blockIdx.x * blockDim.x + threadIdx.x) / 32;
+    int lane32 = threadIdx.x & 31;
+    if (g >= ngroups) return;
+    int small = aux[g * 32 + lane32].z != 0.0f;
+    unsigned long long b = __ballot(small);
+    unsigned int half = (threadIdx.x & 32) ? (unsigned int)(b >> 32) : (unsigned int)b;
+    if (lane32 == 0) tile_small[g] = half != 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -282,14 +300,15 @@ template <int M> __device__ __forceinline__ float4 col_const(float4 a, const Epi
     return a;
 }
 
-template <int M>
+template <int M, bool EXACT = true>
 __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, float &W,
                                          const EpiConst &e)
 {
     const float d = R.x - C.x;
     const float w = __builtin_amdgcn_exp2f(-(d * d));
     if (M == SRED_PEARSON) {
-        float r = dot * __builtin_amdgcn_rcpf(1.0f + R.y * C.y);
+        const float delta = R.y * C.y;
+        float r = EXACT ? dot * __builtin_amdgcn_rcpf(1.0f + delta) : fmaf(-dot, delta, dot);
         S = fmaf(fmaxf(r, 0.0f), w, S);
     } else if (M == SRED_SSIM) {
         float a = fmaf(R.y, C.y, e.c1);
@@ -317,16 +336,21 @@ __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float
 // Workgroup ids are chunk-major: concurrently resident workgroups sweep the
 // same column chunk (sized to stay in each XCD's L2).
 // ---------------------------------------------------------------------------
-constexpr int kRG = 2;
-constexpr int kMfmaBM = 64 * kRG;
 constexpr int kBN = 64;
+// rows per wave = 16*RG: 4 row groups while the A fragments fit (K_pad <= 128),
+// 2 beyond; every B fragment read from LDS then serves 64 (or 32) rows.
+// (SSIM carries four row constants per row, so it keeps 4 row groups only up to K_pad = 96).
+constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
 
 template <int M, int KS>
-__global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
-                                                       const float4 *__restrict__ aux, int nrowtiles,
+__global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+                                                       const float4 *__restrict__ aux,
+                                                       const int *__restrict__ tile_small, int nrowtiles,
                                                        int cols_per_chunk, int ncols, float2 *__restrict__ partial,
                                                        int Nv_pad, EpiConst ec)
 {
+    constexpr int RG = rg_for(M, KS);
+    constexpr int BM = 64 * RG;
     constexpr int ROWU4 = KS * 4;
     constexpr int PITCH = ROWU4 + 1;  // odd pitch (in 16-byte units): conflict-free fragment reads
     __shared__ uint4 lds_hi[kBN * PITCH];
@@ -335,11 +359,11 @@ __global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__
     const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int rowbase = rt * kMfmaBM + wave * 16 * kRG;
+    const int rowbase = rt * BM + wave * 16 * RG;
 
-    bf16x8 ahi[kRG][KS], alo[kRG][KS];
+    bf16x8 ahi[RG][KS], alo[RG][KS];
 #pragma unroll
-    for (int rg = 0; rg < kRG; rg++)
+    for (int rg = 0; rg < RG; rg++)
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             size_t o = (size_t)(rowbase + rg * 16 + li) * ROWU4 + ks * 4 + lk;
@@ -347,17 +371,22 @@ __global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__
             ahi[rg][ks] = __builtin_bit_cast(bf16x8, h);
             alo[rg][ks] = __builtin_bit_cast(bf16x8, l);
         }
-    float4 R[kRG][4];
+    float4 R[RG][4];
 #pragma unroll
-    for (int rg = 0; rg < kRG; rg++)
+    for (int rg = 0; rg < RG; rg++)
 #pragma unroll
         for (int r = 0; r < 4; r++) R[rg][r] = row_const<M>(aux[rowbase + rg * 16 + lk * 4 + r], ec);
+    // rows of this wave with a small sigma force the exact Pearson epilogue
+    bool row_small = false;
+    if (M == SRED_PEARSON)
+        for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
 
-    double accS[kRG][4], accW[kRG][4];
+    // per-lane FP32 row sums over the chunk (at most cols_per_chunk/16 terms each)
+    float accS[RG][4], accW[RG][4];
 #pragma unroll
-    for (int rg = 0; rg < kRG; rg++)
+    for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0;
+        for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0f;
 
     const int c_begin = ch * cols_per_chunk;
     const int c_end = min(c_begin + cols_per_chunk, ncols);
@@ -370,54 +399,51 @@ __global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__
             lds_lo[col * PITCH + q] = Ulo[o];
         }
         __syncthreads();
-        float pS[kRG][4], pW[kRG][4];
-#pragma unroll
-        for (int rg = 0; rg < kRG; rg++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) pS[rg][r] = pW[rg][r] = 0.0f;
-#pragma unroll
+        const bool exact = row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
+#pragma unroll 1
         for (int cg = 0; cg < 4; cg++) {
             const int col = cg * 16 + li;
             const float4 C = col_const<M>(aux[c0 + col], ec);
-            floatx4 acc[kRG];
+            floatx4 acc[RG];
 #pragma unroll
-            for (int rg = 0; rg < kRG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int rg = 0; rg < RG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
                 bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
                 bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
 #pragma unroll
-                for (int rg = 0; rg < kRG; rg++) {
+                for (int rg = 0; rg < RG; rg++) {
                     acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, acc[rg], 0, 0, 0);
                     acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, acc[rg], 0, 0, 0);
                     acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, acc[rg], 0, 0, 0);
                 }
             }
+            if (exact) {
 #pragma unroll
-            for (int rg = 0; rg < kRG; rg++)
+                for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) epilogue<M>(acc[rg][r], R[rg][r], C, pS[rg][r], pW[rg][r], ec);
-        }
+                    for (int r = 0; r < 4; r++)
+                        epilogue<M, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
+            } else {
 #pragma unroll
-        for (int rg = 0; rg < kRG; rg++)
+                for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                accS[rg][r] += (double)pS[rg][r];
-                accW[rg][r] += (double)pW[rg][r];
+                    for (int r = 0; r < 4; r++)
+                        epilogue<M, false>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
             }
+        }
     }
 #pragma unroll
-    for (int rg = 0; rg < kRG; rg++)
+    for (int rg = 0; rg < RG; rg++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            double s = accS[rg][r], w = accW[rg][r];
+            float s = accS[rg][r], w = accW[rg][r];
 #pragma unroll
             for (int m = 1; m < 16; m <<= 1) {
                 s += __shfl_xor(s, m);
                 w += __shfl_xor(w, m);
             }
-            if (li == 0)
-                partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2((float)s, (float)w);
+            if (li == 0) partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
         }
 }
 
@@ -429,8 +455,9 @@ __global__ __launch_bounds__(256) void k_contract_mfma(const uint4 *__restrict__
 // ---------------------------------------------------------------------------
 template <int M>
 __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ UT, const float4 *__restrict__ aux,
-                                                      int K4, int Nv_pad, int nrowtiles, int cols_per_chunk,
-                                                      float2 *__restrict__ partial, EpiConst ec)
+                                                      const int *__restrict__ tile_small, int K4, int Nv_pad,
+                                                      int nrowtiles, int cols_per_chunk, float2 *__restrict__ partial,
+                                                      EpiConst ec)
 {
     extern __shared__ float sm[];
     float *As = sm;
@@ -446,10 +473,12 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
     float4 R[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) R[i] = row_const<M>(aux[row0 + ty * 4 + i], ec);
+    const bool row_small = (M == SRED_PEARSON) && (tile_small[row0 / 32] | tile_small[row0 / 32 + 1]) != 0;
     double accS[4] = {0, 0, 0, 0}, accW[4] = {0, 0, 0, 0};
     const int c_begin = ch * cols_per_chunk;
     const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
+        const bool exact = row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
         __syncthreads();
         for (int i = tid; i < K4 * 16; i += 256) {
             int k = i >> 4, q = i & 15;
@@ -478,8 +507,13 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             float s = 0.0f, w = 0.0f;
+            if (exact) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) epilogue<M>(d[i][j], R[i], C[j], s, w, ec);
+                for (int j = 0; j < 4; j++) epilogue<M, true>(d[i][j], R[i], C[j], s, w, ec);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) epilogue<M, false>(d[i][j], R[i], C[j], s, w, ec);
+            }
             accS[i] += (double)s;
             accW[i] += (double)w;
         }
@@ -594,26 +628,26 @@ __global__ __launch_bounds__(256) void k_finalize(const float *__restrict__ S, c
 // host side
 // ---------------------------------------------------------------------------
 template <int M, int KS>
-static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux, int nrt,
-                           int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
+                           const int *tsm, int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
 {
-    hipLaunchKernelGGL((k_contract_mfma<M, KS>), dim3(grid), dim3(256), 0, st, hi, lo, aux, nrt, cpc, ncols, part,
-                       Nv_pad, ec);
+    hipLaunchKernelGGL((k_contract_mfma<M, KS>), dim3(grid), dim3(256), 0, st, hi, lo, aux, tsm, nrt, cpc, ncols,
+                       part, Nv_pad, ec);
 }
 
 template <int M>
 static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
-                       int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+                       const int *tsm, int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
 {
     switch (KS) {
-    case 1: launch_mfma_ks<M, 1>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 2: launch_mfma_ks<M, 2>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 3: launch_mfma_ks<M, 3>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 4: launch_mfma_ks<M, 4>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 5: launch_mfma_ks<M, 5>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 6: launch_mfma_ks<M, 6>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 7: launch_mfma_ks<M, 7>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
-    case 8: launch_mfma_ks<M, 8>(grid, st, hi, lo, aux, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 1: launch_mfma_ks<M, 1>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 2: launch_mfma_ks<M, 2>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 3: launch_mfma_ks<M, 3>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 4: launch_mfma_ks<M, 4>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 5: launch_mfma_ks<M, 5>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 6: launch_mfma_ks<M, 6>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 7: launch_mfma_ks<M, 7>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
+    case 8: launch_mfma_ks<M, 8>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
     default:
         set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
         return SRED_ERR_ARG;
@@ -622,26 +656,27 @@ static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const 
 }
 
 template <int M>
-static void launch_f32(int grid, size_t lds, hipStream_t st, const float *UT, const float4 *aux, int K4, int Nv_pad,
-                       int nrt, int cpc, float2 *part, EpiConst ec)
+static void launch_f32(int grid, size_t lds, hipStream_t st, const float *UT, const float4 *aux, const int *tsm,
+                       int K4, int Nv_pad, int nrt, int cpc, float2 *part, EpiConst ec)
 {
-    hipLaunchKernelGGL((k_contract_f32<M>), dim3(grid), dim3(256), lds, st, UT, aux, K4, Nv_pad, nrt, cpc, part, ec);
+    hipLaunchKernelGGL((k_contract_f32<M>), dim3(grid), dim3(256), lds, st, UT, aux, tsm, K4, Nv_pad, nrt, cpc, part,
+                       ec);
 }
 
 // Columns per chunk: the chunk's column data (~2 MiB) stays resident in an
 // XCD's L2 while the concurrently running workgroups sweep it; smaller chunks
 // when there are too few row tiles to fill 256 CUs.
-static int choose_chunk_cols(int Nv_pad, int bytes_per_col, int nrowtiles, int override_cols)
+static int choose_chunk_cols(int Nv_pad, int bytes_per_col, int nrowtiles, int override_cols, int max_cols)
 {
     int cols;
     if (override_cols > 0) {
-        cols = (override_cols + 63) / 64 * 64;
+        cols = std::min((override_cols + 63) / 64 * 64, max_cols);
     } else {
         cols = std::max(64, (2 << 20) / std::max(bytes_per_col, 1) / 64 * 64);
         const int target_wgs = 4096;
         int want_chunks = (target_wgs + nrowtiles - 1) / nrowtiles;
         int cols_fill = ((Nv_pad + want_chunks - 1) / want_chunks + 63) / 64 * 64;
-        cols = std::max(64, std::min(cols, cols_fill));
+        cols = std::max(64, std::min(std::min(cols, cols_fill), max_cols));
     }
     return std::min(cols, Nv_pad);
 }
@@ -713,16 +748,23 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     float4 *d_aux = ctx->aux.as<float4>();
     hipLaunchKernelGGL(k_aux, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
                        d_idx, Nv, Nv_pad, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, d_aux);
+    const int ngroups = Nv_pad / 32;
+    SRED_TRY(ctx->tflags.ensure(sizeof(int) * (size_t)ngroups));
+    const int *d_tsm = ctx->tflags.as<int>();
+    hipLaunchKernelGGL(k_tile_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_aux, ngroups,
+                       ctx->tflags.as<int>());
 
     const bool mfma = (p.precision == SRED_BF16X3) && metric != SRED_ABSDIFF;
     const int KS = (g.K + 31) / 32;
     const int K4 = (g.K + 3) / 4 * 4;
     int bm, bytes_per_col;
-    if (mfma) { bm = kMfmaBM; bytes_per_col = KS * 32 * 4; }
+    if (mfma) { bm = 64 * rg_for(metric, KS); bytes_per_col = KS * 32 * 4; }
     else if (metric == SRED_ABSDIFF) { bm = 64; bytes_per_col = 16; }
     else { bm = 64; bytes_per_col = K4 * 4; }
     const int nrowtiles = Nv_pad / bm;
-    const int cpc = choose_chunk_cols(Nv_pad, bytes_per_col, nrowtiles, ctx->chunk_cols);
+    // the MFMA kernel keeps FP32 row sums over a chunk: cap a chunk at 2048
+    // columns (128 terms per lane) before the FP64 sum over chunks
+    const int cpc = choose_chunk_cols(Nv_pad, bytes_per_col, nrowtiles, ctx->chunk_cols, mfma ? 2048 : 1 << 30);
     const int nchunks = (Nv_pad + cpc - 1) / cpc;
     const int64_t grid64 = (int64_t)nrowtiles * nchunks;
     if (grid64 > 0x7fffffff) {
@@ -764,16 +806,16 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         int rc = SRED_OK;
         switch (metric) {
         case SRED_PEARSON:
-            rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part,
-                                           Nv_pad, ec);
+            rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad,
+                                           d_part, Nv_pad, ec);
             break;
         case SRED_SSIM:
-            rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part, Nv_pad,
-                                        ec);
+            rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad, d_part,
+                                        Nv_pad, ec);
             break;
         default:
-            rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, hi, lo, d_aux, nrowtiles, cpc, Nv_pad, d_part, Nv_pad,
-                                         ec);
+            rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad,
+                                         d_part, Nv_pad, ec);
             break;
         }
         SRED_TRY(rc);
@@ -788,11 +830,11 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         }
         const float *UT = ctx->u32.as<float>();
         if (metric == SRED_PEARSON)
-            launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
         else if (metric == SRED_SSIM)
-            launch_f32<SRED_SSIM>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_SSIM>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
         else
-            launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, UT, d_aux, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
     }
     SRED_HIP_CHECK(hipGetLastError());
     SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));

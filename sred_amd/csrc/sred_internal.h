@@ -61,7 +61,7 @@ struct sred_ctx {
     int chunk_cols = 0;
     // device scratch
     sred::DevBuf img, means, stds, mask, S, W, out;          // pixel maps (host-pointer paths)
-    sred::DevBuf tile_var, counts, valid_idx, aux, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
+    sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
     // measurement of the last global contraction
