@@ -62,6 +62,13 @@ int sred_ctx_set_stream(sred_ctx *ctx, void *stream);
 int sred_ctx_synchronize(sred_ctx *ctx);
 /* Column-chunk override for the global contraction (0 = automatic). */
 int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols);
+/* Tuning options of the global contraction (no counterpart in the
+ * reference): SRED_OPT_CHUNK_COLS (as above) and SRED_OPT_SYMMETRIC (1, the
+ * default: the split-BF16 contraction evaluates each unordered block pair
+ * once and adds it to both rows; 0: every ordered pair). */
+#define SRED_OPT_CHUNK_COLS 1
+#define SRED_OPT_SYMMETRIC 2
+int sred_ctx_set_option(sred_ctx *ctx, int option, int value);
 
 /* Metric from the reference's dialog string; block != 0 uses the block
  * command's list ("NRMSE (inverted)", "Abs. Diff. of StdDevs.").
@@ -136,6 +143,13 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
                           int normalize, float *d_out);
 
 /* ---- introspection for measurement ---- */
+/* Launch plan of the global contraction for n_valid valid blocks of
+ * block_pixels (K) pixels (no counterpart in the reference; host only).
+ * out[8]: padded block count, rows per panel (symmetric) or per workgroup,
+ * columns per chunk, chunks, workgroups, partial-sum slots, block pairs
+ * evaluated, flags (1 = MFMA, 2 = symmetric). */
+int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision, int symmetric, int chunk_cols,
+                        int64_t *out);
 /* Average duration (ms, HIP events) and launch count of the dominant
  * contraction kernel in the last global call on this context. */
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);

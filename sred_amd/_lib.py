@@ -22,15 +22,16 @@ SRED_ERR_NO_DEVICE = -6
 
 PEARSON, SSIM, NRMSE, ABSDIFF = 0, 1, 2, 3
 FP32, BF16X3 = 0, 1
+OPT_CHUNK_COLS, OPT_SYMMETRIC = 1, 2
 
 # every symbol include/sred.h declares
 EXPORTS = [
     "sred_version", "sred_last_error", "sred_ctx_create", "sred_ctx_destroy", "sred_ctx_set_stream",
-    "sred_ctx_synchronize", "sred_ctx_set_chunk_cols", "sred_metric_from_name", "sred_block_size_2d",
+    "sred_ctx_synchronize", "sred_ctx_set_chunk_cols", "sred_ctx_set_option", "sred_metric_from_name", "sred_block_size_2d",
     "sred_normalize_input_2d", "sred_reference_block_2d", "sred_noise_var_2d", "sred_relevance_mask_2d",
     "sred_apply_mask_2d", "sred_normalize_map_2d", "sred_local_stats_2d", "sred_global_accum_2d",
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
-    "sred_ctx_last_contraction",
+    "sred_ctx_last_contraction", "sred_global_plan_2d",
 ]
 
 
@@ -70,6 +71,7 @@ def load():
         "sred_ctx_set_stream": ([_VP, _VP], _I),
         "sred_ctx_synchronize": ([_VP], _I),
         "sred_ctx_set_chunk_cols": ([_VP, _I], _I),
+        "sred_ctx_set_option": ([_VP, _I, _I], _I),
         "sred_metric_from_name": ([ctypes.c_char_p, _I], _I),
         "sred_block_size_2d": ([_I, _I], _I),
         "sred_normalize_input_2d": ([_F, _I64], _I),
@@ -86,6 +88,7 @@ def load():
         "sred_global_map_2d_dev": ([_VP, _I, _I, _VP, _I, _I, _I, _I, _FL, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
         "sred_block_map_2d_dev": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_ctx_last_contraction": ([_VP, _F, ctypes.POINTER(_I), ctypes.POINTER(_I64)], _I),
+        "sred_global_plan_2d": ([_I, _I, _I, _I, _I, _I, ctypes.POINTER(_I64)], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -103,6 +106,14 @@ def check(rc):
             raise NonBinaryMaskError(rc, msg)
         raise SredError(rc, msg)
     return rc
+
+
+def global_plan(n_valid, block_pixels, metric, precision, symmetric=1, chunk_cols=0):
+    """sred_global_plan_2d as a dict (host only, no GPU)."""
+    out = (_I64 * 8)()
+    check(load().sred_global_plan_2d(n_valid, block_pixels, metric, precision, symmetric, chunk_cols, out))
+    keys = ["nv_pad", "rows", "cols_per_chunk", "nchunks", "grid", "partial_elems", "pairs", "flags"]
+    return dict(zip(keys, list(out)))
 
 
 def fptr(a):
@@ -152,6 +163,9 @@ class Context:
 
     def set_chunk_cols(self, cols):
         check(self._lib.sred_ctx_set_chunk_cols(self.handle, int(cols)))
+
+    def set_option(self, option, value):
+        check(self._lib.sred_ctx_set_option(self.handle, int(option), int(value)))
 
     def last_contraction(self):
         """(ms, launches, pairs_evaluated) of the last global contraction."""

@@ -266,6 +266,27 @@ int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols)
     return SRED_OK;
 }
 
+int sred_ctx_set_option(sred_ctx *ctx, int option, int value)
+{
+    if (!ctx) {
+        set_error("null context");
+        return SRED_ERR_ARG;
+    }
+    switch (option) {
+    case SRED_OPT_CHUNK_COLS:
+        if (value < 0) break;
+        ctx->chunk_cols = value;
+        return SRED_OK;
+    case SRED_OPT_SYMMETRIC:
+        ctx->symmetric = value != 0;
+        return SRED_OK;
+    default:
+        break;
+    }
+    set_error("unknown option or invalid value");
+    return SRED_ERR_ARG;
+}
+
 int sred_metric_from_name(const char *name, int block)
 {
     if (!name) {
@@ -620,6 +641,26 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
         return SRED_ERR_UNKNOWN_METRIC;
     }
     return block_pipeline(ctx, g, metric, d_img, tmpl, tmpl_mean, tmpl_std, relevance_constant, normalize, d_out);
+}
+
+int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision, int symmetric, int chunk_cols,
+                        int64_t *out)
+{
+    if (!out) {
+        set_error("null output");
+        return SRED_ERR_ARG;
+    }
+    ContractionPlan pl;
+    SRED_TRY(plan_contraction(n_valid, block_pixels, metric, precision, symmetric, chunk_cols, &pl));
+    out[0] = pl.Nv_pad;
+    out[1] = pl.sym ? pl.P : pl.bm;
+    out[2] = pl.cols_per_chunk;
+    out[3] = pl.nchunks;
+    out[4] = pl.grid;
+    out[5] = pl.partial_elems;
+    out[6] = pl.pairs_evaluated;
+    out[7] = (pl.mfma ? 1 : 0) | (pl.sym ? 2 : 0);
+    return SRED_OK;
 }
 
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated)

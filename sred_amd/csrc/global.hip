@@ -300,29 +300,36 @@ template <int M> __device__ __forceinline__ float4 col_const(float4 a, const Epi
     return a;
 }
 
-template <int M, bool EXACT = true>
+template <int M, bool EXACT = true, bool COLS = false>
 __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, float &W,
-                                         const EpiConst &e)
+                                         const EpiConst &e, float *cS = nullptr, float *cW = nullptr)
 {
     const float d = R.x - C.x;
     const float w = __builtin_amdgcn_exp2f(-(d * d));
     if (M == SRED_PEARSON) {
         const float delta = R.y * C.y;
         float r = EXACT ? dot * __builtin_amdgcn_rcpf(1.0f + delta) : fmaf(-dot, delta, dot);
-        S = fmaf(fmaxf(r, 0.0f), w, S);
+        r = fmaxf(r, 0.0f);
+        S = fmaf(r, w, S);
+        if (COLS) *cS = fmaf(r, w, *cS);
     } else if (M == SRED_SSIM) {
         float a = fmaf(R.y, C.y, e.c1);
         float b = fmaf(2.0f, dot, e.c2);
         float c = fmaf(R.z, C.z, e.c1);
-        float t = (a * b) * __builtin_amdgcn_rcpf(c * (R.w + C.w)) * w;
-        S += fmaxf(t, 0.0f);
+        float t = fmaxf((a * b) * __builtin_amdgcn_rcpf(c * (R.w + C.w)) * w, 0.0f);
+        S += t;
+        if (COLS) *cS += t;
     } else if (M == SRED_NRMSE) {
         float mse = fmaf(-2.0f, dot, R.y + C.y);
-        S += __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
+        float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
+        S += t;
+        if (COLS) *cS += t;
     } else {  // ABSDIFF
         S = fmaf(fabsf(d), w, S);
+        if (COLS) *cS = fmaf(fabsf(d), w, *cS);
     }
     W += w;
+    if (COLS) *cW += w;
 }
 
 // ---------------------------------------------------------------------------
@@ -341,6 +348,9 @@ constexpr int kBN = 64;
 // 2 beyond; every B fragment read from LDS then serves 64 (or 32) rows.
 // (SSIM carries four row constants per row, so it keeps 4 row groups only up to K_pad = 96).
 constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
+// the symmetric kernel also carries column sums: 4 row groups for Pearson and
+// NRMSE up to K_pad = 96, 2 otherwise
+constexpr int rg_sym(int metric, int ks) { return (metric != SRED_SSIM && ks <= 3) ? 4 : 2; }
 
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
@@ -445,6 +455,179 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
             }
             if (li == 0) partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
         }
+}
+
+// ---------------------------------------------------------------------------
+// 3a'. symmetric split-BF16 MFMA contraction.  Every per-pair term is
+// symmetric in (i, j), so only panel pairs (p, c) with c >= p are evaluated.
+// Panels are P rows = P columns of the valid set.  Workgroup (p, c), c > p:
+// the P x P rectangle's row sums go to partial[c][rows of p] and its column
+// sums to partial[p][columns of c]; workgroup (p, p) evaluates its whole
+// diagonal square and writes row sums only.  Every partial slot therefore has
+// exactly one writer (deterministic, no atomics), and the FP64 reduction over
+// all slots gives the full row sums.  Column sums: per lane over its 16 rows,
+// across the 4 k-groups by shuffles, across the 4 waves through an LDS staging
+// buffer folded into an LDS accumulator for the panel's P columns.
+// ---------------------------------------------------------------------------
+template <int M, int KS>
+__global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__restrict__ Uhi,
+                                                              const uint4 *__restrict__ Ulo,
+                                                              const float4 *__restrict__ aux,
+                                                              const int *__restrict__ tile_small, int P,
+                                                              int npanels, float2 *__restrict__ partial, int Nv_pad,
+                                                              EpiConst ec)
+{
+    constexpr int RG = rg_sym(M, KS);
+    constexpr int BM = 64 * RG;
+    constexpr int ROWU4 = KS * 4;
+    constexpr int PITCH = ROWU4 + 1;
+    __shared__ uint4 lds_hi[kBN * PITCH];
+    __shared__ uint4 lds_lo[kBN * PITCH];
+    __shared__ float2 stage[4][kBN];
+    extern __shared__ float2 colacc[];  // P entries
+
+    // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
+    const int q = blockIdx.x;
+    int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+    while ((c + 1) * (c + 2) / 2 <= q) c++;
+    while (c * (c + 1) / 2 > q) c--;
+    const int p = q - c * (c + 1) / 2;
+    if (c >= npanels) return;  // grid larger than the panel-pair count: nothing to do
+    const bool diag = (p == c);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    if (!diag)
+        for (int i = tid; i < P; i += 256) colacc[i] = make_float2(0.0f, 0.0f);
+
+    bool pending = false;  // stage[] holds the column sums of the previous tile
+    int pending_col = 0;
+    const int col0 = c * P;
+    for (int r0 = p * P; r0 < (p + 1) * P; r0 += BM) {
+        const int rowbase = r0 + wave * 16 * RG;
+        bf16x8 ahi[RG][KS], alo[RG][KS];
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                size_t o = (size_t)(rowbase + rg * 16 + li) * ROWU4 + ks * 4 + lk;
+                ahi[rg][ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
+                alo[rg][ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
+            }
+        float4 R[RG][4];
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) R[rg][r] = row_const<M>(aux[rowbase + rg * 16 + lk * 4 + r], ec);
+        bool row_small = false;
+        if (M == SRED_PEARSON)
+            for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
+        float accS[RG][4], accW[RG][4];
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0f;
+
+        for (int c0 = col0; c0 < col0 + P; c0 += kBN) {
+            __syncthreads();
+            if (pending && tid < kBN) {
+                float2 a = stage[0][tid], b = stage[1][tid], d = stage[2][tid], e = stage[3][tid];
+                float2 &t = colacc[pending_col + tid];
+                t.x += (a.x + b.x) + (d.x + e.x);
+                t.y += (a.y + b.y) + (d.y + e.y);
+            }
+            for (int i = tid; i < kBN * ROWU4; i += 256) {
+                int col = i / ROWU4, qq = i - col * ROWU4;
+                size_t o = (size_t)(c0 + col) * ROWU4 + qq;
+                lds_hi[col * PITCH + qq] = Uhi[o];
+                lds_lo[col * PITCH + qq] = Ulo[o];
+            }
+            __syncthreads();
+            const bool exact =
+                row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
+#pragma unroll 1
+            for (int cg = 0; cg < 4; cg++) {
+                const int col = cg * 16 + li;
+                const float4 C = col_const<M>(aux[c0 + col], ec);
+                floatx4 acc[RG];
+#pragma unroll
+                for (int rg = 0; rg < RG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
+                    bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
+#pragma unroll
+                    for (int rg = 0; rg < RG; rg++) {
+                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, acc[rg], 0, 0, 0);
+                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, acc[rg], 0, 0, 0);
+                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, acc[rg], 0, 0, 0);
+                    }
+                }
+                if (diag) {
+                    if (exact) {
+#pragma unroll
+                        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                epilogue<M, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
+                    } else {
+#pragma unroll
+                        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                epilogue<M, false>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
+                    }
+                } else {
+                    float cs = 0.0f, cw = 0.0f;
+                    if (exact) {
+#pragma unroll
+                        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                epilogue<M, true, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
+                                                        &cw);
+                    } else {
+#pragma unroll
+                        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                epilogue<M, false, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
+                                                         &cw);
+                    }
+                    cs += __shfl_xor(cs, 16);
+                    cw += __shfl_xor(cw, 16);
+                    cs += __shfl_xor(cs, 32);
+                    cw += __shfl_xor(cw, 32);
+                    if (lk == 0) stage[wave][col] = make_float2(cs, cw);
+                }
+            }
+            pending = !diag;
+            pending_col = c0 - col0;
+        }
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                float s = accS[rg][r], w = accW[rg][r];
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1) {
+                    s += __shfl_xor(s, m);
+                    w += __shfl_xor(w, m);
+                }
+                if (li == 0) partial[(size_t)c * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
+            }
+    }
+    if (!diag) {
+        __syncthreads();
+        if (tid < kBN) {
+            float2 a = stage[0][tid], b = stage[1][tid], d = stage[2][tid], e = stage[3][tid];
+            float2 &t = colacc[pending_col + tid];
+            t.x += (a.x + b.x) + (d.x + e.x);
+            t.y += (a.y + b.y) + (d.y + e.y);
+        }
+        __syncthreads();
+        for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -635,6 +818,34 @@ static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint
                        part, Nv_pad, ec);
 }
 
+template <int M, int KS>
+static void launch_sym_ks(int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
+                          const int *tsm, int P, float2 *part, int Nv_pad, EpiConst ec)
+{
+    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, hi, lo, aux, tsm, P, Nv_pad / P,
+                       part, Nv_pad, ec);
+}
+
+template <int M>
+static int launch_sym(int KS, int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo,
+                      const float4 *aux, const int *tsm, int P, float2 *part, int Nv_pad, EpiConst ec)
+{
+    switch (KS) {
+    case 1: launch_sym_ks<M, 1>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 2: launch_sym_ks<M, 2>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 3: launch_sym_ks<M, 3>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 4: launch_sym_ks<M, 4>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 5: launch_sym_ks<M, 5>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 6: launch_sym_ks<M, 6>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 7: launch_sym_ks<M, 7>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 8: launch_sym_ks<M, 8>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    default:
+        set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+        return SRED_ERR_ARG;
+    }
+    return SRED_OK;
+}
+
 template <int M>
 static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
                        const int *tsm, int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
@@ -697,6 +908,52 @@ int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes)
     return SRED_OK;
 }
 
+// Launch geometry of the contraction for Nv valid blocks of K pixels.
+int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, ContractionPlan *pl)
+{
+    ContractionPlan q{};
+    if (Nv < 0 || K < 2 || metric < 0 || metric > 3) {
+        set_error("invalid contraction plan arguments");
+        return SRED_ERR_ARG;
+    }
+    q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF;
+    q.KS = (K + 31) / 32;
+    q.K4 = (K + 3) / 4 * 4;
+    if (q.mfma && q.KS > 8) {
+        set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+        return SRED_ERR_ARG;
+    }
+    q.sym = q.mfma && symmetric;
+    // symmetric panels: P rows = P columns, a multiple of 256, up to 2048,
+    // small enough for at least ~2000 panel pairs
+    q.P = q.sym ? std::min(2048, std::max(256, ((Nv + 63) / 64 + 255) / 256 * 256)) : 0;
+    q.Nv_pad = q.sym ? (Nv + q.P - 1) / q.P * q.P : (Nv + 255) / 256 * 256;
+    int bytes_per_col;
+    if (q.sym) { q.bm = 64 * rg_sym(metric, q.KS); bytes_per_col = q.KS * 32 * 4; }
+    else if (q.mfma) { q.bm = 64 * rg_for(metric, q.KS); bytes_per_col = q.KS * 32 * 4; }
+    else if (metric == SRED_ABSDIFF) { q.bm = 64; bytes_per_col = 16; }
+    else { q.bm = 64; bytes_per_col = q.K4 * 4; }
+    q.nrowtiles = q.Nv_pad / q.bm;
+    // the MFMA kernels keep FP32 row sums over a chunk: at most 2048 columns
+    // (128 terms per lane) before the FP64 sum over chunks
+    q.cols_per_chunk = q.sym ? q.P
+                             : choose_chunk_cols(std::max(q.Nv_pad, 64), bytes_per_col, std::max(q.nrowtiles, 1),
+                                                 chunk_cols, q.mfma ? 2048 : 1 << 30);
+    q.nchunks = q.Nv_pad > 0 ? (q.Nv_pad + q.cols_per_chunk - 1) / q.cols_per_chunk : 0;
+    q.grid = q.sym ? (int64_t)q.nchunks * (q.nchunks + 1) / 2 : (int64_t)q.nrowtiles * q.nchunks;
+    q.partial_elems = (int64_t)q.nchunks * q.Nv_pad;
+    q.pairs_evaluated = q.sym ? (int64_t)q.P * q.P * q.grid : (int64_t)q.Nv_pad * q.Nv_pad;
+    const bool ok = q.sym ? (q.Nv_pad % q.P == 0 && q.P % q.bm == 0 && q.nchunks * q.P == q.Nv_pad)
+                          : (q.Nv_pad % q.bm == 0 && q.nrowtiles * q.bm == q.Nv_pad &&
+                             (int64_t)q.nchunks * q.cols_per_chunk >= q.Nv_pad && q.cols_per_chunk % 64 == 0);
+    if (!ok || q.grid > 0x7fffffff) {
+        set_error("contraction geometry out of range");
+        return SRED_ERR_ARG;
+    }
+    *pl = q;
+    return SRED_OK;
+}
+
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W)
 {
@@ -739,7 +996,10 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     ctx->last_launches = 0;
     ctx->last_pairs = (int64_t)Nv * Nv;
     if (Nv == 0) return SRED_OK;
-    const int Nv_pad = (Nv + 255) / 256 * 256;
+    ContractionPlan pl;
+    SRED_TRY(plan_contraction(Nv, g.K, metric, p.precision, ctx->symmetric, ctx->chunk_cols, &pl));
+    const bool mfma = pl.mfma, sym = pl.sym;
+    const int P = pl.P, Nv_pad = pl.Nv_pad;
 
     // 2. offsets relative to the image and per-block scalars
     SRED_TRY(upload_offsets(ctx, g, g.w, &ctx->offs2, ctx->offs2_key));
@@ -754,32 +1014,13 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     hipLaunchKernelGGL(k_tile_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_aux, ngroups,
                        ctx->tflags.as<int>());
 
-    const bool mfma = (p.precision == SRED_BF16X3) && metric != SRED_ABSDIFF;
-    const int KS = (g.K + 31) / 32;
-    const int K4 = (g.K + 3) / 4 * 4;
-    int bm, bytes_per_col;
-    if (mfma) { bm = 64 * rg_for(metric, KS); bytes_per_col = KS * 32 * 4; }
-    else if (metric == SRED_ABSDIFF) { bm = 64; bytes_per_col = 16; }
-    else { bm = 64; bytes_per_col = K4 * 4; }
-    const int nrowtiles = Nv_pad / bm;
-    // the MFMA kernel keeps FP32 row sums over a chunk: cap a chunk at 2048
-    // columns (128 terms per lane) before the FP64 sum over chunks
-    const int cpc = choose_chunk_cols(Nv_pad, bytes_per_col, nrowtiles, ctx->chunk_cols, mfma ? 2048 : 1 << 30);
-    const int nchunks = (Nv_pad + cpc - 1) / cpc;
-    const int64_t grid64 = (int64_t)nrowtiles * nchunks;
-    if (grid64 > 0x7fffffff) {
-        set_error("problem too large for one launch");
-        return SRED_ERR_ARG;
-    }
-    const int grid = (int)grid64;
-    SRED_TRY(ctx->partial.ensure(sizeof(float2) * (size_t)nchunks * Nv_pad));
+    const int KS = pl.KS, K4 = pl.K4;
+    const int nrowtiles = pl.nrowtiles, cpc = pl.cols_per_chunk, nchunks = pl.nchunks;
+    const int grid = (int)pl.grid;
+    SRED_TRY(ctx->partial.ensure(sizeof(float2) * (size_t)pl.partial_elems));
     float2 *d_part = ctx->partial.as<float2>();
 
     if (mfma) {
-        if (KS > 8) {
-            set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
-            return SRED_ERR_ARG;
-        }
         size_t u4 = (size_t)Nv_pad * KS * 4;
         SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
         SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
@@ -801,7 +1042,18 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         SRED_HIP_CHECK(hipEventCreate(&ctx->ev1));
     }
     SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
-    if (mfma) {
+    if (sym) {
+        const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
+        const size_t lds = sizeof(float2) * (size_t)P;
+        int rc;
+        if (metric == SRED_PEARSON)
+            rc = launch_sym<SRED_PEARSON>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
+        else if (metric == SRED_SSIM)
+            rc = launch_sym<SRED_SSIM>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
+        else
+            rc = launch_sym<SRED_NRMSE>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
+        SRED_TRY(rc);
+    } else if (mfma) {
         const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
         int rc = SRED_OK;
         switch (metric) {
@@ -839,6 +1091,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     SRED_HIP_CHECK(hipGetLastError());
     SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->last_launches = 1;
+    ctx->last_pairs = pl.pairs_evaluated;
 
     // 4. reduce
     const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)tscale : 1.0;

@@ -59,6 +59,7 @@ struct sred_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int chunk_cols = 0;
+    int symmetric = 1;  // MFMA contraction over upper-triangle panel pairs
     // device scratch
     sred::DevBuf img, means, stds, mask, S, W, out;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
@@ -88,6 +89,13 @@ int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_
 // (synchronises).
 int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
                        float *n_pixels);
+
+struct ContractionPlan {
+    bool mfma, sym;
+    int KS, K4, P, Nv_pad, bm, nrowtiles, cols_per_chunk, nchunks;
+    int64_t grid, partial_elems, pairs_evaluated;
+};
+int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, ContractionPlan *pl);
 
 struct GlobalParams {
     int metric = SRED_PEARSON;

@@ -118,3 +118,74 @@ def test_normalise_matches_oracle_bitwise():
     for mk in (None, mask):
         a = sred_amd.normalize_image_2d(m, 31, 30, 3, 2, mk)
         np.testing.assert_array_equal(a, O.normalize_map(m, 3, 2, mk))
+
+
+# ---------------------------------------------------------------------------
+# launch geometry of the global contraction (host only)
+# ---------------------------------------------------------------------------
+NV_CASES = [1, 2, 63, 64, 255, 256, 257, 1000, 3876, 8320, 27936, 63504, 254016, 1028196, 4137156]
+K_CASES = [5, 13, 29, 49, 81, 113, 149, 197]
+
+
+@pytest.mark.parametrize("precision", [L.FP32, L.BF16X3])
+@pytest.mark.parametrize("sym", [0, 1])
+def test_contraction_plan_invariants(precision, sym):
+    for nv in NV_CASES:
+        for k in K_CASES:
+            for metric in range(4):
+                p = L.global_plan(nv, k, metric, precision, sym)
+                mfma = precision == L.BF16X3 and metric != L.ABSDIFF
+                is_sym = bool(p["flags"] & 2)
+                assert is_sym == (mfma and sym == 1)
+                assert p["nv_pad"] >= nv and p["nv_pad"] - nv < max(p["rows"], 256)
+                assert p["partial_elems"] == p["nchunks"] * p["nv_pad"]
+                if is_sym:
+                    npan = p["nv_pad"] // p["rows"]
+                    assert p["nv_pad"] % p["rows"] == 0 and p["rows"] % 256 == 0 and p["rows"] <= 2048
+                    assert p["nchunks"] == npan and p["grid"] == npan * (npan + 1) // 2
+                    assert 2 * p["pairs"] >= p["nv_pad"] ** 2
+                else:
+                    assert p["nv_pad"] % p["rows"] == 0 and p["cols_per_chunk"] % 64 == 0
+                    assert p["nchunks"] * p["cols_per_chunk"] >= p["nv_pad"]
+                    assert (p["nchunks"] - 1) * p["cols_per_chunk"] < p["nv_pad"]
+                    assert p["grid"] == (p["nv_pad"] // p["rows"]) * p["nchunks"]
+                    if mfma:
+                        assert p["cols_per_chunk"] <= 2048
+
+
+def _decode_panel_pair(q):
+    """The symmetric kernel's (p, c) decode of a workgroup index."""
+    c = int((np.sqrt(np.float32(8.0 * q + 1.0)) - 1.0) * 0.5)
+    while (c + 1) * (c + 2) // 2 <= q:
+        c += 1
+    while c * (c + 1) // 2 > q:
+        c -= 1
+    return q - c * (c + 1) // 2, c
+
+
+@pytest.mark.parametrize("nv,k,metric", [(49, 5, 0), (435, 5, 1), (3876, 13, 0), (8320, 5, 2), (27936, 29, 1),
+                                         (63504, 13, 0), (254016, 49, 0), (1028196, 81, 0)])
+def test_contraction_plan_covers_each_pair_once(nv, k, metric):
+    for sym in (1, 0):
+        p = L.global_plan(nv, k, metric, L.BF16X3, sym)
+        npad, slots = p["nv_pad"], np.zeros((p["nchunks"], p["nv_pad"]), np.int32)
+        if p["flags"] & 2:
+            P = p["rows"]
+            npan = npad // P
+            seen = np.zeros((npan, npan), np.int32)
+            for q in range(p["grid"]):
+                a, c = _decode_panel_pair(q)
+                assert 0 <= a <= c < npan
+                seen[a, c] += 1
+                slots[c, a * P:(a + 1) * P] += 1           # row sums of panel a
+                if a != c:
+                    slots[a, c * P:(c + 1) * P] += 1       # column sums of panel c
+            assert (np.triu(seen) == np.triu(np.ones_like(seen))).all()
+        else:
+            rows, cpc, nrt = p["rows"], p["cols_per_chunk"], npad // p["rows"]
+            for q in range(p["grid"]):
+                rt, ch = q % nrt, q // nrt
+                assert ch < p["nchunks"] and (rt + 1) * rows <= npad
+                assert ch * cpc < npad
+                slots[ch, rt * rows:(rt + 1) * rows] += 1
+        assert (slots == 1).all()  # every partial slot has exactly one writer

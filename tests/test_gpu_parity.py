@@ -256,3 +256,27 @@ def test_block_full_size_2048(ctx, metric):
     out = clutils.calculate_block_repetition_map_2d(BM[metric], inp, rb, 0.0, True, ctx).reshape(img.shape)
     ref = O.block_map(metric, inp.image_array, rb.pixels, 15, 15, rb.mean, rb.std, 0.0, True)
     np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+@pytest.mark.parametrize("shape,b", [((61, 72), 5), ((150, 97), 7), ((300, 260), 11), ((33, 41), 3)])
+def test_global_symmetric_matches_full_pairs(ctx, metric, shape, b):
+    img = synthetic.spots(shape[1], shape[0], 30, seed=shape[0] + metric)
+    img[5:15, 5:15] = 0.3  # flat patch: special cases on both sides of the diagonal
+    inp = utils.get_input_image_2d(img)
+    m, s = O.local_stats_f32(inp.image_array, b, b)
+    nv = O.noise_var(inp.image_array)
+    st = clutils.LocalStatistics(m.ravel(), s.ravel())
+    res = {}
+    try:
+        for sym in (1, 0):
+            ctx.set_option(L.OPT_SYMMETRIC, sym)
+            res[sym] = clutils.get_global_accumulators_2d(ctx, metric, inp, st, b, b, nv, 0.0, "bf16x3")
+    finally:
+        ctx.set_option(L.OPT_SYMMETRIC, 1)
+    S64, W64 = O.global_accum_f64(metric, inp.image_array, s, b, b, nv, 0.0)
+    for sym in (1, 0):
+        S, W = (a.reshape(shape) for a in res[sym])
+        np.testing.assert_allclose(W, W64, rtol=2e-6, atol=1e-6)
+        np.testing.assert_allclose(S, S64, rtol=2e-4, atol=2e-4 * np.abs(S64).max())
+    np.testing.assert_allclose(res[1][1], res[0][1], rtol=2e-6)
