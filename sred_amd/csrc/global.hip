@@ -236,7 +236,8 @@ __global__ __launch_bounds__(256) void k_gather_f32(int metric, const float *__r
 __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict__ img, const float *__restrict__ means,
                                              const float *__restrict__ stds, const int *__restrict__ idx, int Nv,
                                              int Nv_pad, int K, const int *__restrict__ offs, float tscale,
-                                             float inv_sqrt_km1, float inv_sqrt_k, float4 *__restrict__ aux)
+                                             float inv_sqrt_km1, float inv_sqrt_k, int split_bf16,
+                                             float4 *__restrict__ aux)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= Nv_pad) return;
@@ -257,10 +258,18 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
         a.y = mu;
         a.z = s == 0.0f ? INFINITY : s * s;
     } else if (metric == SRED_NRMSE) {
+        // |u|^2 as the contraction sees it: with split BF16 the products are
+        // hi*hi + hi*lo + lo*hi, so the norm omits lo*lo too and identical
+        // blocks give mse = 0 up to FP32 rounding
         float sc = row_scale(metric, s, inv_sqrt_km1, inv_sqrt_k), n = 0.0f;
         for (int k = 0; k < K; k++) {
             float u = (img[p + offs[k]] - mu) * sc;
-            n += u * u;
+            if (split_bf16) {
+                float hi = bf2f(f2bf(u)), lo = bf2f(f2bf(u - hi));
+                n += hi * hi + 2.0f * hi * lo;
+            } else {
+                n += u * u;
+            }
         }
         a.y = n;
     }
@@ -1007,7 +1016,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     SRED_TRY(ctx->aux.ensure(sizeof(float4) * (size_t)Nv_pad));
     float4 *d_aux = ctx->aux.as<float4>();
     hipLaunchKernelGGL(k_aux, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
-                       d_idx, Nv, Nv_pad, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, d_aux);
+                       d_idx, Nv, Nv_pad, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, mfma ? 1 : 0, d_aux);
     const int ngroups = Nv_pad / 32;
     SRED_TRY(ctx->tflags.ensure(sizeof(int) * (size_t)ngroups));
     const int *d_tsm = ctx->tflags.as<int>();
