@@ -142,6 +142,19 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
                           int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
                           int normalize, float *d_out);
 
+/* ---- differentiation (no counterpart in the reference) ---- */
+/* Gradient of L = sum_i dL_dmap[i] * map[i] of the global Pearson map
+ * (relevance constant 0, normalised output; the forward of
+ * sred_global_map_2d_dev with the FP32 contraction) with respect to the
+ * normalised input pixels, by a hand-written adjoint.  Held fixed: the noise
+ * variance, nPixels, the relevance mask and the output min/max; flat blocks
+ * (std == 0) pass no gradient through their statistics.  K <= 160. */
+int sred_global_pearson_adjoint_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh,
+                                   const float *dL_dmap, float *dL_dimg);
+/* Same on device buffers; d_map (optional) receives the forward map. */
+int sred_global_pearson_adjoint_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh,
+                                       const float *d_dmap, float *d_grad, float *d_map);
+
 /* ---- introspection for measurement ---- */
 /* Launch plan of the global contraction for n_valid valid blocks of
  * block_pixels (K) pixels (no counterpart in the reference; host only).

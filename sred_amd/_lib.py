@@ -31,7 +31,8 @@ EXPORTS = [
     "sred_normalize_input_2d", "sred_reference_block_2d", "sred_noise_var_2d", "sred_relevance_mask_2d",
     "sred_apply_mask_2d", "sred_normalize_map_2d", "sred_local_stats_2d", "sred_global_accum_2d",
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
-    "sred_ctx_last_contraction", "sred_global_plan_2d",
+    "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
+    "sred_global_pearson_adjoint_2d_dev",
 ]
 
 
@@ -62,6 +63,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not built: run `python -m sred_amd.build` (hipcc, gfx950)")
+    # One HIP runtime per process: PyTorch wheels bundle their own HIP/HSA
+    # runtime, loaded by the unversioned name libamdhip64.so.  Loading torch
+    # first lets this library bind to that runtime through its soname
+    # (libamdhip64.so.7); loading it first would put a second runtime into the
+    # process and torch's device initialisation would then find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "sred_version": ([], _I),
@@ -89,6 +99,8 @@ def load():
         "sred_block_map_2d_dev": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_ctx_last_contraction": ([_VP, _F, ctypes.POINTER(_I), ctypes.POINTER(_I64)], _I),
         "sred_global_plan_2d": ([_I, _I, _I, _I, _I, _I, ctypes.POINTER(_I64)], _I),
+        "sred_global_pearson_adjoint_2d": ([_VP, _F, _I, _I, _I, _I, _F, _F], _I),
+        "sred_global_pearson_adjoint_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

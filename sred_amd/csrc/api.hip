@@ -234,7 +234,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->img, &ctx->means, &ctx->stds, &ctx->mask, &ctx->S, &ctx->W, &ctx->out,
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
-                      &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl};
+                      &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad};
     for (DevBuf *b : bufs) b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -641,6 +641,40 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
         return SRED_ERR_UNKNOWN_METRIC;
     }
     return block_pipeline(ctx, g, metric, d_img, tmpl, tmpl_mean, tmpl_std, relevance_constant, normalize, d_out);
+}
+
+int sred_global_pearson_adjoint_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh,
+                                   const float *dL_dmap, float *dL_dimg)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !dL_dmap || !dL_dimg) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(upload(ctx, ctx->dmap, dL_dmap, g.n()));
+    SRED_TRY(ctx->grad.ensure(sizeof(float) * g.n()));
+    SRED_TRY(global_pearson_adjoint_dev(ctx, g, ctx->img.as<float>(), ctx->dmap.as<float>(), ctx->grad.as<float>()));
+    SRED_TRY(download(ctx, dL_dimg, ctx->grad, g.n()));
+    return SRED_OK;
+}
+
+int sred_global_pearson_adjoint_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh,
+                                       const float *d_dmap, float *d_grad, float *d_map)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!d_img || !d_dmap || !d_grad) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(global_pearson_adjoint_dev(ctx, g, d_img, d_dmap, d_grad));
+    if (d_map)
+        SRED_HIP_CHECK(hipMemcpyAsync(d_map, ctx->out.ptr, sizeof(float) * g.n(), hipMemcpyDeviceToDevice, ctx->stream));
+    return SRED_OK;
 }
 
 int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision, int symmetric, int chunk_cols,

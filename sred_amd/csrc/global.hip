@@ -907,6 +907,15 @@ __global__ void k_zero(unsigned int *p, int64_t nwords)
         p[i] = 0u;
 }
 
+void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
+                       const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,
+                       float inv_sqrt_k, float *UT)
+{
+    size_t nu = (size_t)Nv_pad * K4;
+    hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                       d_means, d_stds, d_idx, Nv, Nv_pad, K, K4, d_offs, inv_sqrt_km1, inv_sqrt_k, UT);
+}
+
 int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes)
 {
     int64_t words = (int64_t)((bytes + 3) / 4);
@@ -997,6 +1006,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     SRED_HIP_CHECK(hipMemcpyAsync(&tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     const int Nv = tot.x;
+    ctx->last_nv = Nv;
     const float flat_count = (float)std::min(tot.y, 16777216);
 
     SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));

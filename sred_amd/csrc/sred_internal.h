@@ -61,7 +61,7 @@ struct sred_ctx {
     int chunk_cols = 0;
     int symmetric = 1;  // MFMA contraction over upper-triangle panel pairs
     // device scratch
-    sred::DevBuf img, means, stds, mask, S, W, out;          // pixel maps (host-pointer paths)
+    sred::DevBuf img, means, stds, mask, S, W, out, dmap, grad;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
@@ -70,6 +70,7 @@ struct sred_ctx {
     float last_ms = 0.0f;
     int last_launches = 0;
     int64_t last_pairs = 0;
+    int last_nv = 0;  // valid blocks of the last global accumulation
 };
 
 namespace sred {
@@ -106,6 +107,14 @@ struct GlobalParams {
 // Accumulators S, Wsum (float maps, w*h) of the global kernels.
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W);
+// FP32 gather of the valid blocks, k-major (UT[k][v]).
+void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
+                       const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,
+                       float inv_sqrt_k, float *UT);
+// Gradient of sum(dmap * map) of the global Pearson map (relevance 0,
+// normalised) with respect to the pixels (device buffers).
+int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_dmap,
+                               float *d_grad);
 // Host-finalize of CLUtils.getGlobal*2D on device maps.
 int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_S, const float *d_W,
                         const float *d_mask, float n_pixels, float c, int normalize, float *d_out);
