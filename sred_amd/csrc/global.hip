@@ -357,9 +357,9 @@ constexpr int kBN = 64;
 // 2 beyond; every B fragment read from LDS then serves 64 (or 32) rows.
 // (SSIM carries four row constants per row, so it keeps 4 row groups only up to K_pad = 96).
 constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
-// the symmetric kernel also carries column sums: 4 row groups for Pearson and
-// NRMSE up to K_pad = 96, 2 otherwise
-constexpr int rg_sym(int metric, int ks) { return (metric != SRED_SSIM && ks <= 3) ? 4 : 2; }
+// the symmetric kernel also carries column sums and a second accumulator set
+// (software pipelining): 2 row groups
+constexpr int rg_sym(int metric, int ks) { return ks <= 6 ? 2 : 1; }
 
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
@@ -478,6 +478,78 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 // across the 4 k-groups by shuffles, across the 4 waves through an LDS staging
 // buffer folded into an LDS accumulator for the panel's P columns.
 // ---------------------------------------------------------------------------
+// One 64-column tile of one wave (16*RG rows): the MFMAs of column group
+// cg+1 are issued before the epilogue of group cg, so the matrix core works
+// on the next group while the vector ALU finishes the current one; the
+// scheduling hints interleave one MFMA with a few VALU instructions.  COLS
+// adds the column sums (off-diagonal panel pairs).
+template <int M, int KS, int RG, bool EXACT, bool COLS>
+__device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
+                                         const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
+                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float (&accW)[RG][4],
+                                         float2 *stage_w, int li, int lk, const EpiConst &ec)
+{
+    constexpr int PITCH = KS * 4 + 1;
+    floatx4 acc[2][RG];
+#pragma unroll
+    for (int cg = -1; cg < 4; cg++) {
+        if (cg + 1 < 4) {  // issue group cg+1
+            const int col = (cg + 1) * 16 + li;
+            floatx4(&a)[RG] = acc[(cg + 1) & 1];
+#pragma unroll
+            for (int rg = 0; rg < RG; rg++) a[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
+                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
+#pragma unroll
+                for (int rg = 0; rg < RG; rg++) {
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, a[rg], 0, 0, 0);
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, a[rg], 0, 0, 0);
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, a[rg], 0, 0, 0);
+                }
+            }
+        }
+        if (cg >= 0) {  // epilogue of group cg
+            const float4 C = ccon[cg * 16 + li];
+            float cs = 0.0f, cw = 0.0f;
+#pragma unroll
+            for (int rg = 0; rg < RG; rg++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    epilogue<M, EXACT, COLS>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
+                                             &cw);
+            if (COLS) {
+                cs += __shfl_xor(cs, 16);
+                cw += __shfl_xor(cw, 16);
+                cs += __shfl_xor(cs, 32);
+                cw += __shfl_xor(cw, 32);
+                if (lk == 0) stage_w[cg * 16 + li] = make_float2(cs, cw);
+            }
+        }
+        if (cg >= 0 && cg + 1 < 4) {
+            // per MFMA of the next group, a few VALU/transcendental ops of this one
+#pragma unroll
+            for (int i = 0; i < 3 * KS * RG; i++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x402, 4, 0);  // VALU | TRANS
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3a'. symmetric split-BF16 MFMA contraction.  Every per-pair term is
+// symmetric in (i, j), so only panel pairs (p, c) with c >= p are evaluated.
+// Panels are P rows = P columns of the valid set.  Workgroup (p, c), c > p:
+// the P x P rectangle's row sums go to partial[c][rows of p] and its column
+// sums to partial[p][columns of c]; workgroup (p, p) evaluates its whole
+// diagonal square and writes row sums only.  Every partial slot therefore has
+// exactly one writer (deterministic, no atomics), and the FP64 reduction over
+// all slots gives the full row sums.  Column sums: per lane over its 16*RG
+// rows, across the 4 k-groups by shuffles, across the 4 waves through an LDS
+// staging buffer folded into an LDS accumulator for the panel's P columns.
+// ---------------------------------------------------------------------------
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__restrict__ Uhi,
                                                               const uint4 *__restrict__ Ulo,
@@ -492,6 +564,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     constexpr int PITCH = ROWU4 + 1;
     __shared__ uint4 lds_hi[kBN * PITCH];
     __shared__ uint4 lds_lo[kBN * PITCH];
+    __shared__ float4 ccon[kBN];
     __shared__ float2 stage[4][kBN];
     extern __shared__ float2 colacc[];  // P entries
 
@@ -551,64 +624,17 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
                 lds_hi[col * PITCH + qq] = Uhi[o];
                 lds_lo[col * PITCH + qq] = Ulo[o];
             }
+            if (tid >= 192) ccon[tid - 192] = col_const<M>(aux[c0 + tid - 192], ec);
             __syncthreads();
             const bool exact =
                 row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
-#pragma unroll 1
-            for (int cg = 0; cg < 4; cg++) {
-                const int col = cg * 16 + li;
-                const float4 C = col_const<M>(aux[c0 + col], ec);
-                floatx4 acc[RG];
-#pragma unroll
-                for (int rg = 0; rg < RG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int ks = 0; ks < KS; ks++) {
-                    bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
-                    bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
-#pragma unroll
-                    for (int rg = 0; rg < RG; rg++) {
-                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, acc[rg], 0, 0, 0);
-                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, acc[rg], 0, 0, 0);
-                        acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, acc[rg], 0, 0, 0);
-                    }
-                }
-                if (diag) {
-                    if (exact) {
-#pragma unroll
-                        for (int rg = 0; rg < RG; rg++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                epilogue<M, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
-                    } else {
-#pragma unroll
-                        for (int rg = 0; rg < RG; rg++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                epilogue<M, false>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
-                    }
-                } else {
-                    float cs = 0.0f, cw = 0.0f;
-                    if (exact) {
-#pragma unroll
-                        for (int rg = 0; rg < RG; rg++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                epilogue<M, true, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
-                                                        &cw);
-                    } else {
-#pragma unroll
-                        for (int rg = 0; rg < RG; rg++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                epilogue<M, false, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
-                                                         &cw);
-                    }
-                    cs += __shfl_xor(cs, 16);
-                    cw += __shfl_xor(cw, 16);
-                    cs += __shfl_xor(cs, 32);
-                    cw += __shfl_xor(cw, 32);
-                    if (lk == 0) stage[wave][col] = make_float2(cs, cw);
-                }
+            float2 *sw = stage[wave];
+            if (diag) {
+                if (exact) sym_tile<M, KS, RG, true, false>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
+                else sym_tile<M, KS, RG, false, false>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
+            } else {
+                if (exact) sym_tile<M, KS, RG, true, true>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
+                else sym_tile<M, KS, RG, false, true>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
             }
             pending = !diag;
             pending_col = c0 - col0;

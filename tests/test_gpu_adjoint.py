@@ -1,0 +1,110 @@
+"""GPU checks of the hand-written adjoint of the global Pearson map.
+
+* against the FP64 autograd restatement (oracle/torch_ref.py, itself pinned
+  by FP64 central differences at h=1e-6 in tests/test_oracle_adjoint.py) on
+  small crops;
+* exact linearity of the gradient in the loss weights at the full 512x512,
+  9x9 workload;
+* central finite differences (h = 1e-3) of the GPU forward with the noise
+  variance, mask, nPixels and output min/max held fixed, on sampled pixels.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import sred_amd
+from oracle import torch_ref as T
+from sred_amd import _lib as L
+from sred_amd import synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_grad(ctx, img, b, weights):
+    h, w = img.shape
+    g = np.zeros_like(img)
+    L.check(L.load().sred_global_pearson_adjoint_2d(ctx.handle, L.fptr(np.ascontiguousarray(img, np.float32)), w, h,
+                                                   b, b, L.fptr(np.ascontiguousarray(weights, np.float32)), L.fptr(g)))
+    return g
+
+
+@pytest.mark.parametrize("shape,b,seed", [((26, 28), 5, 4), ((36, 40), 9, 5), ((33, 30), 7, 6)])
+def test_adjoint_matches_fp64_autograd(ctx, shape, b, seed):
+    img = O.normalize_input(synthetic.spots(shape[1], shape[0], 10, seed=seed))
+    u = np.random.default_rng(seed + 36).random(shape).astype(np.float32)
+    _, g64, _ = T.pearson_map_grad(img, b, u.astype(np.float64))
+    g = gpu_grad(ctx, img, b, u)
+    scale = np.abs(g64).max()
+    err = np.abs(g - g64).max() / scale
+    assert err < 2e-3, err
+    # most pixels far tighter (kinks of max(0, r) aside)
+    assert np.median(np.abs(g - g64)) / scale < 2e-5
+
+
+def test_adjoint_linear_in_weights_full_size(ctx):
+    img = O.normalize_input(synthetic.spots(512, 512, 800, seed=4))
+    rng = np.random.default_rng(40)
+    u1 = rng.random(img.shape).astype(np.float32)
+    u2 = rng.random(img.shape).astype(np.float32)
+    g1 = gpu_grad(ctx, img, 9, u1).astype(np.float64)
+    g2 = gpu_grad(ctx, img, 9, u2).astype(np.float64)
+    g12 = gpu_grad(ctx, img, 9, u1 + 2 * u2).astype(np.float64)
+    assert np.abs(g12 - (g1 + 2 * g2)).max() <= 1e-4 * np.abs(g12).max()
+    assert np.isfinite(g1).all() and np.abs(g1).max() > 0
+
+
+def fixed_loss(ctx, img, b, u, fixed):
+    """L = sum(u * map) of the GPU forward (FP32 contraction) with the noise
+    variance, mask, nPixels and min/max of `fixed` held fixed."""
+    h, w = img.shape
+    inp = sred_amd.InputImage2D(np.ascontiguousarray(img, np.float32), w, h, w * h)
+    st = sred_amd.get_local_statistics_2d(ctx, inp, b, b)
+    rm, npx, mn, mx = fixed
+    rep = sred_amd.calculate_global_repetition_map_2d("Pearson's R", inp, b, b, sred_amd.block_size_2d(b, b), st, rm,
+                                                      npx, False, ctx, "fp32").reshape(h, w).astype(np.float64)
+    m = rm.relevance_mask.reshape(h, w) > 0
+    rep[m] = (rep[m] - mn) / (mx - mn + 1e-7)
+    return float((u.astype(np.float64) * rep).sum())
+
+
+def test_adjoint_central_differences(ctx):
+    b = 9
+    img = O.normalize_input(synthetic.spots(128, 128, 50, seed=4))
+    h_, w_ = img.shape
+    u = np.random.default_rng(40).random(img.shape).astype(np.float32)
+    g = gpu_grad(ctx, img, b, u)
+    inp = sred_amd.InputImage2D(img, w_, h_, w_ * h_)
+    st = sred_amd.get_local_statistics_2d(ctx, inp, b, b)
+    rm = sred_amd.get_relevance_mask(img, w_, h_, b // 2, b // 2, st.local_stds, 0.0)
+    npx = sred_amd.utils.count_relevant_pixels(rm.relevance_mask, w_, h_, b // 2, b // 2)
+    rep = sred_amd.calculate_global_repetition_map_2d("Pearson's R", inp, b, b, sred_amd.block_size_2d(b, b), st, rm,
+                                                      npx, False, ctx, "fp32").reshape(h_, w_)
+    sel = rep[rm.relevance_mask.reshape(h_, w_) > 0]
+    fixed = (rm, npx, float(sel.min()), float(sel.max()))
+    rng = np.random.default_rng(41)
+    pts = [(int(y), int(x)) for y, x in zip(rng.integers(6, h_ - 6, 24), rng.integers(6, w_ - 6, 24))]
+    step = 1e-3
+    fd, ad = [], []
+    for y, x in pts:
+        p = img.copy(); p[y, x] += step
+        q = img.copy(); q[y, x] -= step
+        fd.append((fixed_loss(ctx, p, b, u, fixed) - fixed_loss(ctx, q, b, u, fixed)) / (2 * step))
+        ad.append(g[y, x])
+    fd, ad = np.array(fd), np.array(ad)
+    # FP32 forward differences at h=1e-3 on a piecewise-smooth map: loose bar
+    assert np.abs(fd - ad).max() <= 0.05 * np.abs(ad).max() + 1e-3, (fd, ad)
+    assert np.corrcoef(fd, ad)[0, 1] > 0.99
+
+
+def test_autograd_function(ctx):
+    torch = pytest.importorskip("torch")
+    from sred_amd.autograd import PearsonMap
+    img = O.normalize_input(synthetic.spots(40, 36, 12, seed=7))
+    u = np.random.default_rng(3).random(img.shape).astype(np.float32)
+    x = torch.from_numpy(img).cuda().requires_grad_(True)
+    m = PearsonMap.apply(x, 5)
+    (m * torch.from_numpy(u).cuda()).sum().backward()
+    _, g64, _ = T.pearson_map_grad(img, 5, u.astype(np.float64))
+    assert np.abs(x.grad.cpu().numpy() - g64).max() < 2e-3 * np.abs(g64).max()
+    ref = O.global_map(O.PEARSON, img, 5, 5, 0.0, True, fp64=True)
+    assert np.abs(m.detach().cpu().numpy() - ref).max() < 1e-4
