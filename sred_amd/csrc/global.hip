@@ -11,7 +11,8 @@
 //                     order, so rows and columns are the same valid set;
 //   2. gather      -- each valid block, mean-subtracted and scaled, becomes
 //                     one row of U (N x K_pad): FP32 (k-major) or split BF16
-//                     hi/lo (row-major), plus a float4 of per-block scalars;
+//                     hi/lo (fragment-major: MFMA operand order), plus a
+//                     float4 of per-block scalars;
 //   3. contraction -- U * U^T tile by tile (FP32 VALU or bf16x3 MFMA); each
 //                     dot product goes straight into the metric's epilogue
 //                     (exp weight, normalisation, truncation, row sums), so
@@ -178,7 +179,13 @@ __device__ __forceinline__ float row_scale(int metric, float sigma, float inv_sq
     return inv_sqrt_k;  // NRMSE
 }
 
-// Split-BF16 rows: U_hi/U_lo[v][K_pad], 8 bf16 (one uint4) per thread.
+// Split-BF16 blocks in fragment-major order: U[g][ks][lane] (one uint4 = 8
+// bf16 each), g = block / 16, lane = 16 * kq + block % 16, holding elements
+// k = 32 ks + 8 kq .. 32 ks + 8 kq + 7.  That is exactly the operand fragment
+// of lane `lane` of v_mfma_f32_16x16x32_bf16 for k-step ks, for rows (A) and
+// columns (B) alike, so every fragment load -- global or LDS -- is 64
+// consecutive uint4 per wave, and a 64-column tile is one contiguous 4*KS KiB
+// run that can be copied straight into LDS.
 __global__ __launch_bounds__(256) void k_gather_bf16(int metric, const float *__restrict__ img,
                                                      const float *__restrict__ means, const float *__restrict__ stds,
                                                      const int *__restrict__ idx, int Nv, int Nv_pad, int K, int KS,
@@ -186,10 +193,12 @@ __global__ __launch_bounds__(256) void k_gather_bf16(int metric, const float *__
                                                      float inv_sqrt_k, uint4 *__restrict__ Uhi,
                                                      uint4 *__restrict__ Ulo)
 {
-    const int q4 = KS * 4;  // uint4 per row
+    const int per_group = KS * 64;  // uint4 per 16 blocks
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)Nv_pad * q4) return;
-    int v = (int)(t / q4), q = (int)(t - (int64_t)v * q4);
+    if (t >= (int64_t)Nv_pad * KS * 4) return;
+    const int g = (int)(t / per_group), rem = (int)(t - (int64_t)g * per_group);
+    const int lane = rem & 63;
+    const int v = g * 16 + (lane & 15), q = (rem >> 6) * 4 + (lane >> 4);
     unsigned short hi[8], lo[8];
     if (v < Nv) {
         int p = idx[v];
@@ -256,7 +265,8 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
         a.z = (s > 0.0f && s < kFastSigma) ? 1.0f : 0.0f;
     } else if (metric == SRED_SSIM) {
         a.y = mu;
-        a.z = s == 0.0f ? INFINITY : s * s;
+        a.z = mu * mu;
+        a.w = s == 0.0f ? INFINITY : s * s;
     } else if (metric == SRED_NRMSE) {
         // |u|^2 as the contraction sees it: with split BF16 the products are
         // hi*hi + hi*lo + lo*hi, so the norm omits lo*lo too and identical
@@ -292,7 +302,9 @@ __global__ __launch_bounds__(256) void k_tile_small(const float4 *__restrict__ a
 // ---------------------------------------------------------------------------
 // 3. epilogue
 // ---------------------------------------------------------------------------
-// Row constants R and column constants C from the per-block scalars:
+// Row constants R and column constants C from the per-block scalars.  The
+// per-block float4 is already the column form C (so a column tile's constants
+// can be copied into LDS as they are):
 //   Pearson: R = (t, eps/s),          C = (t, 1/s)
 //   SSIM:    R = (t, 2mu, mu^2, s^2+c2), C = (t, mu, mu^2, s^2)   (inf for flat)
 //   NRMSE:   R = (t, |u|^2),          C = (t, |u|^2)              (-inf for padding)
@@ -300,14 +312,10 @@ __global__ __launch_bounds__(256) void k_tile_small(const float4 *__restrict__ a
 template <int M> __device__ __forceinline__ float4 row_const(float4 a, const EpiConst &e)
 {
     if (M == SRED_PEARSON) return make_float4(a.x, e.eps * a.y, 0.0f, 0.0f);
-    if (M == SRED_SSIM) return make_float4(a.x, 2.0f * a.y, a.y * a.y, a.z + e.c2);
+    if (M == SRED_SSIM) return make_float4(a.x, 2.0f * a.y, a.z, a.w + e.c2);
     return a;
 }
-template <int M> __device__ __forceinline__ float4 col_const(float4 a, const EpiConst &)
-{
-    if (M == SRED_SSIM) return make_float4(a.x, a.y, a.y * a.y, a.z);
-    return a;
-}
+template <int M> __device__ __forceinline__ float4 col_const(float4 a, const EpiConst &) { return a; }
 
 template <int M, bool EXACT = true, bool COLS = false>
 __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, float &W,
@@ -359,7 +367,7 @@ constexpr int kBN = 64;
 constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
 // the symmetric kernel also carries column sums and a second accumulator set
 // (software pipelining): 2 row groups
-constexpr int rg_sym(int metric, int ks) { return ks <= 6 ? 2 : 1; }
+constexpr int rg_sym(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 5 : 6) ? 2 : 1; }
 
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
@@ -370,10 +378,9 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 {
     constexpr int RG = rg_for(M, KS);
     constexpr int BM = 64 * RG;
-    constexpr int ROWU4 = KS * 4;
-    constexpr int PITCH = ROWU4 + 1;  // odd pitch (in 16-byte units): conflict-free fragment reads
-    __shared__ uint4 lds_hi[kBN * PITCH];
-    __shared__ uint4 lds_lo[kBN * PITCH];
+    constexpr int TILE = 4 * KS * 64;  // uint4 of one 64-column tile (fragment-major)
+    __shared__ uint4 lds_hi[TILE];
+    __shared__ uint4 lds_lo[TILE];
 
     const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -385,7 +392,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
     for (int rg = 0; rg < RG; rg++)
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            size_t o = (size_t)(rowbase + rg * 16 + li) * ROWU4 + ks * 4 + lk;
+            size_t o = ((size_t)(rowbase / 16 + rg) * KS + ks) * 64 + lane;
             uint4 h = Uhi[o], l = Ulo[o];
             ahi[rg][ks] = __builtin_bit_cast(bf16x8, h);
             alo[rg][ks] = __builtin_bit_cast(bf16x8, l);
@@ -411,11 +418,10 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
     const int c_end = min(c_begin + cols_per_chunk, ncols);
     for (int c0 = c_begin; c0 < c_end; c0 += kBN) {
         __syncthreads();
-        for (int i = tid; i < kBN * ROWU4; i += 256) {
-            int col = i / ROWU4, q = i - col * ROWU4;
-            size_t o = (size_t)(c0 + col) * ROWU4 + q;
-            lds_hi[col * PITCH + q] = Uhi[o];
-            lds_lo[col * PITCH + q] = Ulo[o];
+        const size_t t0 = (size_t)(c0 / 16) * KS * 64;
+        for (int i = tid; i < TILE; i += 256) {
+            lds_hi[i] = Uhi[t0 + i];
+            lds_lo[i] = Ulo[t0 + i];
         }
         __syncthreads();
         const bool exact = row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
@@ -428,8 +434,8 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
             for (int rg = 0; rg < RG; rg++) acc[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
-                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
+                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[(cg * KS + ks) * 64 + lane]);
+                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[(cg * KS + ks) * 64 + lane]);
 #pragma unroll
                 for (int rg = 0; rg < RG; rg++) {
                     acc[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, acc[rg], 0, 0, 0);
@@ -467,41 +473,30 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 }
 
 // ---------------------------------------------------------------------------
-// 3a'. symmetric split-BF16 MFMA contraction.  Every per-pair term is
-// symmetric in (i, j), so only panel pairs (p, c) with c >= p are evaluated.
-// Panels are P rows = P columns of the valid set.  Workgroup (p, c), c > p:
-// the P x P rectangle's row sums go to partial[c][rows of p] and its column
-// sums to partial[p][columns of c]; workgroup (p, p) evaluates its whole
-// diagonal square and writes row sums only.  Every partial slot therefore has
-// exactly one writer (deterministic, no atomics), and the FP64 reduction over
-// all slots gives the full row sums.  Column sums: per lane over its 16 rows,
-// across the 4 k-groups by shuffles, across the 4 waves through an LDS staging
-// buffer folded into an LDS accumulator for the panel's P columns.
-// ---------------------------------------------------------------------------
 // One 64-column tile of one wave (16*RG rows): the MFMAs of column group
 // cg+1 are issued before the epilogue of group cg, so the matrix core works
 // on the next group while the vector ALU finishes the current one; the
 // scheduling hints interleave one MFMA with a few VALU instructions.  COLS
-// adds the column sums (off-diagonal panel pairs).
+// adds the column sums (off-diagonal panel pairs).  B fragments are read from
+// the fragment-major LDS tile: 64 consecutive uint4 per (group, k-step).
 template <int M, int KS, int RG, bool EXACT, bool COLS>
 __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
                                          const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
                                          const float4 (&R)[RG][4], float (&accS)[RG][4], float (&accW)[RG][4],
-                                         float2 *stage_w, int li, int lk, const EpiConst &ec)
+                                         float2 *stage_w, int lane, const EpiConst &ec)
 {
-    constexpr int PITCH = KS * 4 + 1;
+    const int li = lane & 15, lk = lane >> 4;
     floatx4 acc[2][RG];
 #pragma unroll
     for (int cg = -1; cg < 4; cg++) {
         if (cg + 1 < 4) {  // issue group cg+1
-            const int col = (cg + 1) * 16 + li;
             floatx4(&a)[RG] = acc[(cg + 1) & 1];
 #pragma unroll
             for (int rg = 0; rg < RG; rg++) a[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[col * PITCH + ks * 4 + lk]);
-                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[col * PITCH + ks * 4 + lk]);
+                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[((cg + 1) * KS + ks) * 64 + lane]);
+                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[((cg + 1) * KS + ks) * 64 + lane]);
 #pragma unroll
                 for (int rg = 0; rg < RG; rg++) {
                     a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, a[rg], 0, 0, 0);
@@ -538,6 +533,36 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
     }
 }
 
+// Asynchronous global -> LDS copy of one 64-column tile: 4*KS runs of 64
+// uint4 for hi and for lo (contiguous in the fragment-major layout), plus the
+// tile's 64 column constants.  Wave w copies runs w, w+4, ...; with
+// global_load_lds_dwordx4 the LDS destination is wave-uniform and lane l
+// lands 16*l bytes after it, so LDS keeps the global order.  Completion is
+// tracked by the issuing wave's vmcnt.
+template <int KS>
+__device__ __forceinline__ void tile_to_lds(const uint4 *Uhi, const uint4 *Ulo, const float4 *aux, int c0, uint4 *dhi,
+                                            uint4 *dlo, float4 *dcon, int wave, int lane)
+{
+    typedef __attribute__((address_space(3))) void *lds_ptr;
+    const size_t t0 = (size_t)(c0 / 16) * KS * 64;
+#pragma unroll
+    for (int i = 0; i < KS; i++) {
+        const int run = wave + 4 * i;
+        __builtin_amdgcn_global_load_lds((const void *)(Uhi + t0 + run * 64 + lane), (lds_ptr)(dhi + run * 64), 16, 0,
+                                         0);
+        __builtin_amdgcn_global_load_lds((const void *)(Ulo + t0 + run * 64 + lane), (lds_ptr)(dlo + run * 64), 16, 0,
+                                         0);
+    }
+    if (wave == 0) __builtin_amdgcn_global_load_lds((const void *)(aux + c0 + lane), (lds_ptr)dcon, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// two 64-column tiles in flight (copy of the next overlaps the current one)
+// while their LDS fits twice beside the column accumulator at 2 workgroups/CU
+constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
+
 // ---------------------------------------------------------------------------
 // 3a'. symmetric split-BF16 MFMA contraction.  Every per-pair term is
 // symmetric in (i, j), so only panel pairs (p, c) with c >= p are evaluated.
@@ -549,6 +574,13 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
 // all slots gives the full row sums.  Column sums: per lane over its 16*RG
 // rows, across the 4 k-groups by shuffles, across the 4 waves through an LDS
 // staging buffer folded into an LDS accumulator for the panel's P columns.
+//
+// The workgroup walks its rectangle as P/BM sweeps of P/64 column tiles
+// (step T = tile T % ntiles of sweep T / ntiles).  With double buffering the
+// copy of tile T+1 is issued before tile T is computed, into the other of two
+// LDS buffers (distinct arrays, so the compiler sees that the reads of one do
+// not depend on the copy into the other); one barrier per step then both
+// publishes tile T and retires every read of the buffer the next copy reuses.
 // ---------------------------------------------------------------------------
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__restrict__ Uhi,
@@ -560,12 +592,13 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
 {
     constexpr int RG = rg_sym(M, KS);
     constexpr int BM = 64 * RG;
-    constexpr int ROWU4 = KS * 4;
-    constexpr int PITCH = ROWU4 + 1;
-    __shared__ uint4 lds_hi[kBN * PITCH];
-    __shared__ uint4 lds_lo[kBN * PITCH];
-    __shared__ float4 ccon[kBN];
-    __shared__ float2 stage[4][kBN];
+    constexpr int TILE = 4 * KS * 64;
+    constexpr bool DB = sym_double_buffer(KS);
+    __shared__ uint4 hi0[TILE], lo0[TILE];
+    __shared__ float4 con0[kBN];
+    __shared__ uint4 hi1[DB ? TILE : 1], lo1[DB ? TILE : 1];
+    __shared__ float4 con1[DB ? kBN : 1];
+    __shared__ float2 stage[2][4][kBN];
     extern __shared__ float2 colacc[];  // P entries
 
     // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
@@ -577,68 +610,55 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     if (c >= npanels) return;  // grid larger than the panel-pair count: nothing to do
     const bool diag = (p == c);
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int li = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (!diag)
         for (int i = tid; i < P; i += 256) colacc[i] = make_float2(0.0f, 0.0f);
 
-    bool pending = false;  // stage[] holds the column sums of the previous tile
-    int pending_col = 0;
-    const int col0 = c * P;
-    for (int r0 = p * P; r0 < (p + 1) * P; r0 += BM) {
-        const int rowbase = r0 + wave * 16 * RG;
-        bf16x8 ahi[RG][KS], alo[RG][KS];
+    const int ntiles = P / kBN;  // multiple of 4: the step count below is even
+    const int NT = ntiles * (P / BM);
+    const int col0 = c * P, row0 = p * P;
+
+    bf16x8 ahi[RG][KS], alo[RG][KS];
+    float4 R[RG][4];
+    float accS[RG][4], accW[RG][4];
+    bool row_small = false;
+    int rowbase = 0;
+
+    auto fold = [&](int T) {  // column sums of step T's tile into colacc (after a barrier)
+        if (tid < kBN) {
+            const float2 a = stage[T & 1][0][tid], b = stage[T & 1][1][tid];
+            const float2 d = stage[T & 1][2][tid], e = stage[T & 1][3][tid];
+            float2 &t = colacc[(T % ntiles) * kBN + tid];
+            t.x += (a.x + b.x) + (d.x + e.x);
+            t.y += (a.y + b.y) + (d.y + e.y);
+        }
+    };
+    auto begin_sweep = [&](int T) {  // A fragments and row constants of sweep T / ntiles
+        rowbase = row0 + (T / ntiles) * BM + wave * 16 * RG;
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                size_t o = (size_t)(rowbase + rg * 16 + li) * ROWU4 + ks * 4 + lk;
+                const size_t o = ((size_t)(rowbase / 16 + rg) * KS + ks) * 64 + lane;
                 ahi[rg][ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
                 alo[rg][ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
             }
-        float4 R[RG][4];
+        const int lk = lane >> 4;
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) R[rg][r] = row_const<M>(aux[rowbase + rg * 16 + lk * 4 + r], ec);
-        bool row_small = false;
+        row_small = false;
         if (M == SRED_PEARSON)
             for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
-        float accS[RG][4], accW[RG][4];
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0f;
-
-        for (int c0 = col0; c0 < col0 + P; c0 += kBN) {
-            __syncthreads();
-            if (pending && tid < kBN) {
-                float2 a = stage[0][tid], b = stage[1][tid], d = stage[2][tid], e = stage[3][tid];
-                float2 &t = colacc[pending_col + tid];
-                t.x += (a.x + b.x) + (d.x + e.x);
-                t.y += (a.y + b.y) + (d.y + e.y);
-            }
-            for (int i = tid; i < kBN * ROWU4; i += 256) {
-                int col = i / ROWU4, qq = i - col * ROWU4;
-                size_t o = (size_t)(c0 + col) * ROWU4 + qq;
-                lds_hi[col * PITCH + qq] = Uhi[o];
-                lds_lo[col * PITCH + qq] = Ulo[o];
-            }
-            if (tid >= 192) ccon[tid - 192] = col_const<M>(aux[c0 + tid - 192], ec);
-            __syncthreads();
-            const bool exact =
-                row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
-            float2 *sw = stage[wave];
-            if (diag) {
-                if (exact) sym_tile<M, KS, RG, true, false>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
-                else sym_tile<M, KS, RG, false, false>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
-            } else {
-                if (exact) sym_tile<M, KS, RG, true, true>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
-                else sym_tile<M, KS, RG, false, true>(lds_hi, lds_lo, ccon, ahi, alo, R, accS, accW, sw, li, lk, ec);
-            }
-            pending = !diag;
-            pending_col = c0 - col0;
-        }
+    };
+    auto end_sweep = [&]() {  // row sums of the sweep -> partial[c][rows]
+        const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
@@ -651,15 +671,58 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
                 }
                 if (li == 0) partial[(size_t)c * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
             }
+    };
+    auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
+        const int c0 = col0 + (T % ntiles) * kBN;
+        const bool exact =
+            row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
+        float2 *sw = stage[T & 1][wave];
+        if (diag) {
+            if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+        } else {
+            if (exact) sym_tile<M, KS, RG, true, true>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, true>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+        }
+        if (T % ntiles == ntiles - 1) end_sweep();
+    };
+
+    if constexpr (DB) {
+        // step T: wait for tile T's copy, barrier, fold tile T-1's column sums,
+        // start sweep if needed, issue tile T+1's copy, compute tile T
+        auto step = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon, uint4 *nhi, uint4 *nlo,
+                        float4 *ncon) {
+            wait_vm_all();
+            __syncthreads();
+            if (!diag && T > 0) fold(T - 1);
+            if (T % ntiles == 0) {
+                // the A fragments land before the next copy is issued, so no
+                // later wait for them has to count (and drain) that copy
+                begin_sweep(T);
+                wait_vm_all();
+            }
+            if (T + 1 < NT) tile_to_lds<KS>(Uhi, Ulo, aux, col0 + ((T + 1) % ntiles) * kBN, nhi, nlo, ncon, wave, lane);
+            compute(T, chi, clo, ccon);
+        };
+        tile_to_lds<KS>(Uhi, Ulo, aux, col0, hi0, lo0, con0, wave, lane);
+        for (int T = 0; T < NT; T += 2) {
+            step(T, hi0, lo0, con0, hi1, lo1, con1);
+            step(T + 1, hi1, lo1, con1, hi0, lo0, con0);
+        }
+    } else {
+        for (int T = 0; T < NT; T++) {
+            __syncthreads();
+            if (!diag && T > 0) fold(T - 1);
+            if (T % ntiles == 0) begin_sweep(T);
+            tile_to_lds<KS>(Uhi, Ulo, aux, col0 + (T % ntiles) * kBN, hi0, lo0, con0, wave, lane);
+            wait_vm_all();
+            __syncthreads();
+            compute(T, hi0, lo0, con0);
+        }
     }
     if (!diag) {
         __syncthreads();
-        if (tid < kBN) {
-            float2 a = stage[0][tid], b = stage[1][tid], d = stage[2][tid], e = stage[3][tid];
-            float2 &t = colacc[pending_col + tid];
-            t.x += (a.x + b.x) + (d.x + e.x);
-            t.y += (a.y + b.y) + (d.y + e.y);
-        }
+        fold(NT - 1);
         __syncthreads();
         for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
     }

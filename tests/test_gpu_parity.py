@@ -277,6 +277,8 @@ def test_global_symmetric_matches_full_pairs(ctx, metric, shape, b):
     S64, W64 = O.global_accum_f64(metric, inp.image_array, s, b, b, nv, 0.0)
     for sym in (1, 0):
         S, W = (a.reshape(shape) for a in res[sym])
-        np.testing.assert_allclose(W, W64, rtol=2e-6, atol=1e-6)
+        # FP32 pair weights: on these inputs the reference's own FP32 formula
+        # exp(-(s_i-s_j)^2/(h+eps)) gives row sums up to 3.8e-6 (relative) off FP64
+        np.testing.assert_allclose(W, W64, rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(S, S64, rtol=2e-4, atol=2e-4 * np.abs(S64).max())
     np.testing.assert_allclose(res[1][1], res[0][1], rtol=2e-6)
