@@ -22,18 +22,24 @@
 //
 // Epilogue algebra (exact rewrites of the reference's per-pair formulas):
 //   w_ij = exp(-(s_i-s_j)^2 / (h+eps)) = exp2(-(t_i - t_j)^2), t = s*sqrt(log2 e/(h+eps))
-//   Pearson: u = (x-mu)/(s*sqrt(K-1)), r = u_i.u_j / (1 + eps/(s_i s_j))
-//            (= cov/(s_i s_j + eps)); flat blocks have u = 0, so pairs with one
-//            flat block give 0 as in the reference; a flat row gets the count
-//            of flat columns (both-flat pairs contribute weight 1).
+//   Pearson: u = (x-mu)/(s*sqrt(K-1)), r = u_i.u_j / (1 + delta),
+//            delta = eps/(s_i s_j) (= cov/(s_i s_j + eps)); the term
+//            max(r, 0) w = max(u_i.u_j, 0) exp2(-(t_i-t_j)^2 - log2(1+delta));
+//            flat blocks have u = 0, so pairs with one flat block give 0 as in
+//            the reference; a flat row gets the count of flat columns
+//            (both-flat pairs contribute weight 1).
 //   SSIM:    u = (x-mu)/sqrt(K-1) (u_i.u_j = cov); flat blocks carry an
 //            infinite variance so their term is 0; flat rows as Pearson.
 //   NRMSE:   u = (x-mu)/sqrt(K), mse = |u_i|^2 + |u_j|^2 - 2 u_i.u_j.
 //   AbsDiff: |s_i - s_j| w_ij = |t_i - t_j| w_ij / sqrt(log2 e/(h+eps)).
+// The weight sums Wsum_i = sum_j w_ij depend on the sigmas only and are
+// computed apart (weights.hip); the contraction accumulates the similarity
+// sums S only.
 #include <float.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "sred_internal.h"
 
@@ -44,14 +50,18 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr float kPadT = 1e30f;  // t of padding columns: their weight is exp2(-inf) = 0
 // Pearson fast path: with every sigma of a tile >= kFastSigma (or 0),
-// delta = eps/(s_i s_j) <= 3.1e-4 and r = dot*(1 - delta) differs from
-// dot/(1 + delta) by delta^2 <= 1e-7 relative; other tiles use the exact form.
+// delta = eps/(s_i s_j) <= 3.1e-4 and exp(-delta) differs from 1/(1 + delta)
+// by delta^2/2 <= 5e-8 relative, so log2(1 + delta) in the exponent becomes
+// delta log2(e), one product of a row and a column constant; other tiles use
+// the exact form.
 constexpr float kFastSigma = 0.018f;
+constexpr float kLn2f = 0.693147180559945309f;
 
 struct EpiConst {
     float eps;  // 1e-7
     float c1;   // SSIM 1e-4
     float c2;   // SSIM 9e-4
+    float pk;   // Pearson: eps * log2(e)
 };
 
 // ---------------------------------------------------------------------------
@@ -77,35 +87,47 @@ __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int6
     *flat = (s == 0.0f);
 }
 
+// counts[b] = (valid, flat) pixels of tile b; smax[b] = largest sigma of its valid pixels
 __global__ __launch_bounds__(kCompactBlock) void k_compact_count(const float *__restrict__ stds, int w, int h, int brw,
-                                                                 int brh, double thr_d, int2 *__restrict__ counts)
+                                                                 int brh, double thr_d, int2 *__restrict__ counts,
+                                                                 float *__restrict__ smax)
 {
     const int64_t n = (int64_t)w * h;
     const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
     int cv = 0, cf = 0;
+    float mx = 0.0f;
     for (int i = 0; i < kCompactItems; i++) {
         int v, f;
         valid_flags(stds, base + i, n, w, h, brw, brh, thr_d, &v, &f);
         cv += v;
         cf += f;
+        if (v) mx = fmaxf(mx, stds[base + i]);
     }
     __shared__ int sv[kCompactBlock], sf[kCompactBlock];
+    __shared__ float sm[kCompactBlock];
     sv[threadIdx.x] = cv;
     sf[threadIdx.x] = cf;
+    sm[threadIdx.x] = mx;
     __syncthreads();
     for (int s = kCompactBlock / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             sv[threadIdx.x] += sv[threadIdx.x + s];
             sf[threadIdx.x] += sf[threadIdx.x + s];
+            sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) counts[blockIdx.x] = make_int2(sv[0], sf[0]);
+    if (threadIdx.x == 0) {
+        counts[blockIdx.x] = make_int2(sv[0], sf[0]);
+        smax[blockIdx.x] = sm[0];
+    }
 }
 
 // Exclusive scan of block counts (one workgroup); totals at index nblocks.
-__global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ counts, int nblocks,
-                                                       int2 *__restrict__ offsets)
+// offsets[b] = exclusive prefix of counts; offsets[nblocks] = totals;
+// offsets[nblocks + 1].x = bits of the largest valid sigma
+__global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ counts, const float *__restrict__ smax,
+                                                       int nblocks, int2 *__restrict__ offsets)
 {
     __shared__ int sv[1024], sf[1024];
     __shared__ int carry_v, carry_f;
@@ -131,6 +153,17 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) offsets[nblocks] = make_int2(carry_v, carry_f);
+    // largest sigma (a strided pass, then a tree over the 1024 threads)
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < nblocks; i += 1024) mx = fmaxf(mx, smax[i]);
+    __shared__ float sm[1024];
+    sm[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offsets[nblocks + 1] = make_int2(__float_as_int(sm[0]), 0);
 }
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *__restrict__ stds, int w, int h,
@@ -305,48 +338,50 @@ __global__ __launch_bounds__(256) void k_tile_small(const float4 *__restrict__ a
 // Row constants R and column constants C from the per-block scalars.  The
 // per-block float4 is already the column form C (so a column tile's constants
 // can be copied into LDS as they are):
-//   Pearson: R = (t, eps/s),          C = (t, 1/s)
+//   Pearson: R = (t, eps log2(e)/s),  C = (t, 1/s)
 //   SSIM:    R = (t, 2mu, mu^2, s^2+c2), C = (t, mu, mu^2, s^2)   (inf for flat)
 //   NRMSE:   R = (t, |u|^2),          C = (t, |u|^2)              (-inf for padding)
 //   AbsDiff: R = (t),                 C = (t)
 template <int M> __device__ __forceinline__ float4 row_const(float4 a, const EpiConst &e)
 {
-    if (M == SRED_PEARSON) return make_float4(a.x, e.eps * a.y, 0.0f, 0.0f);
+    if (M == SRED_PEARSON) return make_float4(a.x, e.pk * a.y, 0.0f, 0.0f);
     if (M == SRED_SSIM) return make_float4(a.x, 2.0f * a.y, a.z, a.w + e.c2);
     return a;
 }
 template <int M> __device__ __forceinline__ float4 col_const(float4 a, const EpiConst &) { return a; }
 
 template <int M, bool EXACT = true, bool COLS = false>
-__device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, float &W,
-                                         const EpiConst &e, float *cS = nullptr, float *cW = nullptr)
+__device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, const EpiConst &e,
+                                         float *cS = nullptr)
 {
     const float d = R.x - C.x;
-    const float w = __builtin_amdgcn_exp2f(-(d * d));
     if (M == SRED_PEARSON) {
-        const float delta = R.y * C.y;
-        float r = EXACT ? dot * __builtin_amdgcn_rcpf(1.0f + delta) : fmaf(-dot, delta, dot);
-        r = fmaxf(r, 0.0f);
-        S = fmaf(r, w, S);
-        if (COLS) *cS = fmaf(r, w, *cS);
+        const float p = R.y * C.y;  // delta log2(e)
+        // exact: log2(1 + delta); fast: delta log2(e)
+        const float x = EXACT ? fmaf(d, d, __builtin_amdgcn_logf(fmaf(p, kLn2f, 1.0f))) : fmaf(d, d, p);
+        const float wt = __builtin_amdgcn_exp2f(-x);
+        // max(dot, 0) as one v_med3 (fmaxf would first canonicalise the MFMA result)
+        const float r = __builtin_amdgcn_fmed3f(dot, 0.0f, FLT_MAX);
+        S = fmaf(r, wt, S);
+        if (COLS) *cS = fmaf(r, wt, *cS);
     } else if (M == SRED_SSIM) {
+        const float w = __builtin_amdgcn_exp2f(-(d * d));
         float a = fmaf(R.y, C.y, e.c1);
         float b = fmaf(2.0f, dot, e.c2);
         float c = fmaf(R.z, C.z, e.c1);
         float t = fmaxf((a * b) * __builtin_amdgcn_rcpf(c * (R.w + C.w)) * w, 0.0f);
         S += t;
         if (COLS) *cS += t;
-    } else if (M == SRED_NRMSE) {
+    } else if (M == SRED_NRMSE) {  // unweighted (kernelGetGlobalNrmse2D.cl)
         float mse = fmaf(-2.0f, dot, R.y + C.y);
         float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
         S += t;
         if (COLS) *cS += t;
     } else {  // ABSDIFF
+        const float w = __builtin_amdgcn_exp2f(-(d * d));
         S = fmaf(fabsf(d), w, S);
         if (COLS) *cS = fmaf(fabsf(d), w, *cS);
     }
-    W += w;
-    if (COLS) *cW += w;
 }
 
 // ---------------------------------------------------------------------------
@@ -373,7 +408,7 @@ template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                        const float4 *__restrict__ aux,
                                                        const int *__restrict__ tile_small, int nrowtiles,
-                                                       int cols_per_chunk, int ncols, float2 *__restrict__ partial,
+                                                       int cols_per_chunk, int ncols, float *__restrict__ partial,
                                                        int Nv_pad, EpiConst ec)
 {
     constexpr int RG = rg_for(M, KS);
@@ -408,11 +443,11 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
         for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
 
     // per-lane FP32 row sums over the chunk (at most cols_per_chunk/16 terms each)
-    float accS[RG][4], accW[RG][4];
+    float accS[RG][4];
 #pragma unroll
     for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0f;
+        for (int r = 0; r < 4; r++) accS[rg][r] = 0.0f;
 
     const int c_begin = ch * cols_per_chunk;
     const int c_end = min(c_begin + cols_per_chunk, ncols);
@@ -447,14 +482,12 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 #pragma unroll
                 for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        epilogue<M, true>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
+                    for (int r = 0; r < 4; r++) epilogue<M, true>(acc[rg][r], R[rg][r], C, accS[rg][r], ec);
             } else {
 #pragma unroll
                 for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        epilogue<M, false>(acc[rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec);
+                    for (int r = 0; r < 4; r++) epilogue<M, false>(acc[rg][r], R[rg][r], C, accS[rg][r], ec);
             }
         }
     }
@@ -462,31 +495,41 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
     for (int rg = 0; rg < RG; rg++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            float s = accS[rg][r], w = accW[rg][r];
+            float s = accS[rg][r];
 #pragma unroll
-            for (int m = 1; m < 16; m <<= 1) {
-                s += __shfl_xor(s, m);
-                w += __shfl_xor(w, m);
-            }
-            if (li == 0) partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
+            for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
+            if (li == 0) partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = s;
         }
 }
 
 // ---------------------------------------------------------------------------
+// Sum over the 4 k-groups of a wave (lanes l, l^16, l^32, l^48) with lane
+// swaps on the vector ALU (v_permlane16/32_swap) instead of LDS permutes.
+__device__ __forceinline__ float sum_kgroups(float x)
+{
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    x = __uint_as_float(a[0]) + __uint_as_float(a[1]);  // rows 0+1 | 2+3
+    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
 // One 64-column tile of one wave (16*RG rows): the MFMAs of column group
 // cg+1 are issued before the epilogue of group cg, so the matrix core works
 // on the next group while the vector ALU finishes the current one; the
 // scheduling hints interleave one MFMA with a few VALU instructions.  COLS
-// adds the column sums (off-diagonal panel pairs).  B fragments are read from
+// adds the column sums (off-diagonal panel pairs): per lane over its rows,
+// over the k-groups at the end of the tile, then lane (kq, c) stores group
+// kq's column c, so the store needs no lane mask.  B fragments are read from
 // the fragment-major LDS tile: 64 consecutive uint4 per (group, k-step).
 template <int M, int KS, int RG, bool EXACT, bool COLS>
 __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
                                          const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
-                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float (&accW)[RG][4],
-                                         float2 *stage_w, int lane, const EpiConst &ec)
+                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float *stage_w, int lane,
+                                         const EpiConst &ec)
 {
     const int li = lane & 15, lk = lane >> 4;
     floatx4 acc[2][RG];
+    float colsum[4];
 #pragma unroll
     for (int cg = -1; cg < 4; cg++) {
         if (cg + 1 < 4) {  // issue group cg+1
@@ -507,20 +550,12 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
         }
         if (cg >= 0) {  // epilogue of group cg
             const float4 C = ccon[cg * 16 + li];
-            float cs = 0.0f, cw = 0.0f;
+            float cs = 0.0f;
 #pragma unroll
             for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    epilogue<M, EXACT, COLS>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], accW[rg][r], ec, &cs,
-                                             &cw);
-            if (COLS) {
-                cs += __shfl_xor(cs, 16);
-                cw += __shfl_xor(cw, 16);
-                cs += __shfl_xor(cs, 32);
-                cw += __shfl_xor(cw, 32);
-                if (lk == 0) stage_w[cg * 16 + li] = make_float2(cs, cw);
-            }
+                for (int r = 0; r < 4; r++) epilogue<M, EXACT, COLS>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], ec, &cs);
+            colsum[cg] = cs;
         }
         if (cg >= 0 && cg + 1 < 4) {
             // per MFMA of the next group, a few VALU/transcendental ops of this one
@@ -530,6 +565,15 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
                 __builtin_amdgcn_sched_group_barrier(0x402, 4, 0);  // VALU | TRANS
             }
         }
+    }
+    if (COLS) {
+        float mine = 0.0f;
+#pragma unroll
+        for (int cg = 0; cg < 4; cg++) {
+            const float v = sum_kgroups(colsum[cg]);
+            mine = (lk == cg) ? v : mine;
+        }
+        stage_w[lk * 16 + li] = mine;
     }
 }
 
@@ -587,7 +631,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
                                                               const uint4 *__restrict__ Ulo,
                                                               const float4 *__restrict__ aux,
                                                               const int *__restrict__ tile_small, int P,
-                                                              int npanels, float2 *__restrict__ partial, int Nv_pad,
+                                                              int npanels, float *__restrict__ partial, int Nv_pad,
                                                               EpiConst ec)
 {
     constexpr int RG = rg_sym(M, KS);
@@ -598,8 +642,8 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     __shared__ float4 con0[kBN];
     __shared__ uint4 hi1[DB ? TILE : 1], lo1[DB ? TILE : 1];
     __shared__ float4 con1[DB ? kBN : 1];
-    __shared__ float2 stage[2][4][kBN];
-    extern __shared__ float2 colacc[];  // P entries
+    __shared__ float stage[2][4][kBN];
+    extern __shared__ float colacc[];  // P entries
 
     // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
     const int q = blockIdx.x;
@@ -613,7 +657,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (!diag)
-        for (int i = tid; i < P; i += 256) colacc[i] = make_float2(0.0f, 0.0f);
+        for (int i = tid; i < P; i += 256) colacc[i] = 0.0f;
 
     const int ntiles = P / kBN;  // multiple of 4: the step count below is even
     const int NT = ntiles * (P / BM);
@@ -621,18 +665,14 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
 
     bf16x8 ahi[RG][KS], alo[RG][KS];
     float4 R[RG][4];
-    float accS[RG][4], accW[RG][4];
+    float accS[RG][4];
     bool row_small = false;
     int rowbase = 0;
 
     auto fold = [&](int T) {  // column sums of step T's tile into colacc (after a barrier)
-        if (tid < kBN) {
-            const float2 a = stage[T & 1][0][tid], b = stage[T & 1][1][tid];
-            const float2 d = stage[T & 1][2][tid], e = stage[T & 1][3][tid];
-            float2 &t = colacc[(T % ntiles) * kBN + tid];
-            t.x += (a.x + b.x) + (d.x + e.x);
-            t.y += (a.y + b.y) + (d.y + e.y);
-        }
+        if (tid < kBN)
+            colacc[(T % ntiles) * kBN + tid] +=
+                (stage[T & 1][0][tid] + stage[T & 1][1][tid]) + (stage[T & 1][2][tid] + stage[T & 1][3][tid]);
     };
     auto begin_sweep = [&](int T) {  // A fragments and row constants of sweep T / ntiles
         rowbase = row0 + (T / ntiles) * BM + wave * 16 * RG;
@@ -655,7 +695,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) accS[rg][r] = accW[rg][r] = 0.0f;
+            for (int r = 0; r < 4; r++) accS[rg][r] = 0.0f;
     };
     auto end_sweep = [&]() {  // row sums of the sweep -> partial[c][rows]
         const int li = lane & 15, lk = lane >> 4;
@@ -663,26 +703,23 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                float s = accS[rg][r], w = accW[rg][r];
+                float s = accS[rg][r];
 #pragma unroll
-                for (int m = 1; m < 16; m <<= 1) {
-                    s += __shfl_xor(s, m);
-                    w += __shfl_xor(w, m);
-                }
-                if (li == 0) partial[(size_t)c * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = make_float2(s, w);
+                for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
+                if (li == 0) partial[(size_t)c * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = s;
             }
     };
     auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
         const int c0 = col0 + (T % ntiles) * kBN;
         const bool exact =
             row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
-        float2 *sw = stage[T & 1][wave];
+        float *sw = stage[T & 1][wave];
         if (diag) {
-            if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
-            else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+            if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
         } else {
-            if (exact) sym_tile<M, KS, RG, true, true>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
-            else sym_tile<M, KS, RG, false, true>(chi, clo, ccon, ahi, alo, R, accS, accW, sw, lane, ec);
+            if (exact) sym_tile<M, KS, RG, true, true>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, true>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
         }
         if (T % ntiles == ntiles - 1) end_sweep();
     };
@@ -737,7 +774,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
 template <int M>
 __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ UT, const float4 *__restrict__ aux,
                                                       const int *__restrict__ tile_small, int K4, int Nv_pad,
-                                                      int nrowtiles, int cols_per_chunk, float2 *__restrict__ partial,
+                                                      int nrowtiles, int cols_per_chunk, float *__restrict__ partial,
                                                       EpiConst ec)
 {
     extern __shared__ float sm[];
@@ -755,7 +792,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
 #pragma unroll
     for (int i = 0; i < 4; i++) R[i] = row_const<M>(aux[row0 + ty * 4 + i], ec);
     const bool row_small = (M == SRED_PEARSON) && (tile_small[row0 / 32] | tile_small[row0 / 32 + 1]) != 0;
-    double accS[4] = {0, 0, 0, 0}, accW[4] = {0, 0, 0, 0};
+    double accS[4] = {0, 0, 0, 0};
     const int c_begin = ch * cols_per_chunk;
     const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
@@ -787,27 +824,23 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
         for (int j = 0; j < 4; j++) C[j] = col_const<M>(aux[c0 + tx * 4 + j], ec);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            float s = 0.0f, w = 0.0f;
+            float s = 0.0f;
             if (exact) {
 #pragma unroll
-                for (int j = 0; j < 4; j++) epilogue<M, true>(d[i][j], R[i], C[j], s, w, ec);
+                for (int j = 0; j < 4; j++) epilogue<M, true>(d[i][j], R[i], C[j], s, ec);
             } else {
 #pragma unroll
-                for (int j = 0; j < 4; j++) epilogue<M, false>(d[i][j], R[i], C[j], s, w, ec);
+                for (int j = 0; j < 4; j++) epilogue<M, false>(d[i][j], R[i], C[j], s, ec);
             }
             accS[i] += (double)s;
-            accW[i] += (double)w;
         }
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        double s = accS[i], w = accW[i];
+        double s = accS[i];
 #pragma unroll
-        for (int m = 1; m < 16; m <<= 1) {
-            s += __shfl_xor(s, m);
-            w += __shfl_xor(w, m);
-        }
-        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = make_float2((float)s, (float)w);
+        for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
+        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = (float)s;
     }
 }
 
@@ -816,7 +849,7 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
 // only the weight and |s_i - s_j| per pair.  64x64 pair tiles, 4x4 per thread.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_contract_absdiff(const float4 *__restrict__ aux, int Nv_pad, int nrowtiles,
-                                                          int cols_per_chunk, float2 *__restrict__ partial,
+                                                          int cols_per_chunk, float *__restrict__ partial,
                                                           EpiConst ec)
 {
     const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
@@ -826,7 +859,7 @@ __global__ __launch_bounds__(256) void k_contract_absdiff(const float4 *__restri
     float4 R[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) R[i] = aux[row0 + ty * 4 + i];
-    double accS[4] = {0, 0, 0, 0}, accW[4] = {0, 0, 0, 0};
+    double accS[4] = {0, 0, 0, 0};
     const int c_begin = ch * cols_per_chunk;
     const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
@@ -838,47 +871,38 @@ __global__ __launch_bounds__(256) void k_contract_absdiff(const float4 *__restri
         for (int j = 0; j < 4; j++) C[j] = make_float4(tcol[tx * 4 + j], 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            float s = 0.0f, w = 0.0f;
+            float s = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 4; j++) epilogue<SRED_ABSDIFF>(0.0f, R[i], C[j], s, w, ec);
+            for (int j = 0; j < 4; j++) epilogue<SRED_ABSDIFF>(0.0f, R[i], C[j], s, ec);
             accS[i] += (double)s;
-            accW[i] += (double)w;
         }
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        double s = accS[i], w = accW[i];
+        double s = accS[i];
 #pragma unroll
-        for (int m = 1; m < 16; m <<= 1) {
-            s += __shfl_xor(s, m);
-            w += __shfl_xor(w, m);
-        }
-        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = make_float2((float)s, (float)w);
+        for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
+        if (tx == 0) partial[(size_t)ch * Nv_pad + row0 + ty * 4 + i] = (float)s;
     }
 }
 
 // ---------------------------------------------------------------------------
 // 4. reduce: FP64 sum of the per-chunk partials, scattered to pixel maps.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce(const float2 *__restrict__ partial, int nchunks, int Nv, int Nv_pad,
+__global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nchunks, int Nv, int Nv_pad,
                                                 const int *__restrict__ idx, const float *__restrict__ stds,
                                                 int metric, float flat_count, double s_scale,
-                                                float *__restrict__ S_map, float *__restrict__ W_map)
+                                                float *__restrict__ S_map)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= Nv) return;
-    double s = 0.0, w = 0.0;
-    for (int c = 0; c < nchunks; c++) {
-        float2 p = partial[(size_t)c * Nv_pad + v];
-        s += (double)p.x;
-        w += (double)p.y;
-    }
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s += (double)partial[(size_t)c * Nv_pad + v];
     int p = idx[v];
     // both-flat pairs contribute 1*weight with weight exp(-0) = 1
     // (kernelGetGlobalPearson2D.cl:101-102, kernelGetGlobalSsim2D.cl:108-109)
     if ((metric == SRED_PEARSON || metric == SRED_SSIM) && stds[p] == 0.0f) s = (double)flat_count;
     S_map[p] = (float)(s * s_scale);
-    W_map[p] = (float)w;
 }
 
 // ---------------------------------------------------------------------------
@@ -910,7 +934,7 @@ __global__ __launch_bounds__(256) void k_finalize(const float *__restrict__ S, c
 // ---------------------------------------------------------------------------
 template <int M, int KS>
 static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
-                           const int *tsm, int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+                           const int *tsm, int nrt, int cpc, int ncols, float *part, int Nv_pad, EpiConst ec)
 {
     hipLaunchKernelGGL((k_contract_mfma<M, KS>), dim3(grid), dim3(256), 0, st, hi, lo, aux, tsm, nrt, cpc, ncols,
                        part, Nv_pad, ec);
@@ -918,7 +942,7 @@ static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint
 
 template <int M, int KS>
 static void launch_sym_ks(int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
-                          const int *tsm, int P, float2 *part, int Nv_pad, EpiConst ec)
+                          const int *tsm, int P, float *part, int Nv_pad, EpiConst ec)
 {
     hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, hi, lo, aux, tsm, P, Nv_pad / P,
                        part, Nv_pad, ec);
@@ -926,7 +950,7 @@ static void launch_sym_ks(int grid, size_t lds, hipStream_t st, const uint4 *hi,
 
 template <int M>
 static int launch_sym(int KS, int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo,
-                      const float4 *aux, const int *tsm, int P, float2 *part, int Nv_pad, EpiConst ec)
+                      const float4 *aux, const int *tsm, int P, float *part, int Nv_pad, EpiConst ec)
 {
     switch (KS) {
     case 1: launch_sym_ks<M, 1>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
@@ -946,7 +970,7 @@ static int launch_sym(int KS, int grid, size_t lds, hipStream_t st, const uint4 
 
 template <int M>
 static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
-                       const int *tsm, int nrt, int cpc, int ncols, float2 *part, int Nv_pad, EpiConst ec)
+                       const int *tsm, int nrt, int cpc, int ncols, float *part, int Nv_pad, EpiConst ec)
 {
     switch (KS) {
     case 1: launch_mfma_ks<M, 1>(grid, st, hi, lo, aux, tsm, nrt, cpc, ncols, part, Nv_pad, ec); break;
@@ -966,7 +990,7 @@ static int launch_mfma(int KS, int grid, hipStream_t st, const uint4 *hi, const 
 
 template <int M>
 static void launch_f32(int grid, size_t lds, hipStream_t st, const float *UT, const float4 *aux, const int *tsm,
-                       int K4, int Nv_pad, int nrt, int cpc, float2 *part, EpiConst ec)
+                       int K4, int Nv_pad, int nrt, int cpc, float *part, EpiConst ec)
 {
     hipLaunchKernelGGL((k_contract_f32<M>), dim3(grid), dim3(256), lds, st, UT, aux, tsm, K4, Nv_pad, nrt, cpc, part,
                        ec);
@@ -1072,31 +1096,35 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     const double thr_d = java_float_literal(p.threshold);
     const double fp_d = java_float_literal(p.filter_param);
     const double eps_d = java_float_literal(kEps);
-    const float tscale = (float)std::sqrt(1.4426950408889634 / (fp_d + eps_d));
+    const double tscale_d = std::sqrt(1.4426950408889634 / (fp_d + eps_d));
+    const float tscale = (float)tscale_d;
     const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(g.K - 1)));
     const float inv_sqrt_k = (float)(1.0 / std::sqrt((double)g.K));
-    EpiConst ec{kEps, 0.0001f, 0.0009f};
+    EpiConst ec{kEps, 0.0001f, 0.0009f, (float)((double)kEps * 1.4426950408889634)};
 
     // 1. compaction
     const int64_t n = g.n();
     const int nblk = (int)((n + kCompactTile - 1) / kCompactTile);
-    SRED_TRY(ctx->counts.ensure(sizeof(int2) * (2 * (size_t)nblk + 2)));
+    SRED_TRY(ctx->counts.ensure(sizeof(int2) * (3 * (size_t)nblk + 4)));
     int2 *d_counts = ctx->counts.as<int2>();
-    int2 *d_offsets = d_counts + nblk;
+    int2 *d_offsets = d_counts + nblk;                                   // nblk + 2 entries
+    float *d_smax = reinterpret_cast<float *>(d_offsets + nblk + 2);  // nblk entries
     SRED_TRY(ctx->valid_idx.ensure(sizeof(int) * (size_t)n + 256));
     int *d_idx = ctx->valid_idx.as<int>();
     hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
-                       g.brh, thr_d, d_counts);
-    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, nblk, d_offsets);
+                       g.brh, thr_d, d_counts, d_smax);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, d_smax, nblk, d_offsets);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
                        g.brh, thr_d, d_offsets, d_idx);
     SRED_HIP_CHECK(hipGetLastError());
-    int2 tot;
-    SRED_HIP_CHECK(hipMemcpyAsync(&tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
+    int2 tot[2];
+    SRED_HIP_CHECK(hipMemcpyAsync(tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    const int Nv = tot.x;
+    const int Nv = tot[0].x;
+    float sigma_max;
+    std::memcpy(&sigma_max, &tot[1].x, sizeof sigma_max);
     ctx->last_nv = Nv;
-    const float flat_count = (float)std::min(tot.y, 16777216);
+    const float flat_count = (float)std::min(tot[0].y, 16777216);
 
     SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));
     SRED_TRY(zero_dev(ctx, d_W, sizeof(float) * n));
@@ -1104,6 +1132,8 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     ctx->last_launches = 0;
     ctx->last_pairs = (int64_t)Nv * Nv;
     if (Nv == 0) return SRED_OK;
+    // weight sums (independent of the contraction)
+    SRED_TRY(weight_sums_dev(ctx, d_stds, d_idx, Nv, tscale_d, sigma_max, d_W));
     ContractionPlan pl;
     SRED_TRY(plan_contraction(Nv, g.K, metric, p.precision, ctx->symmetric, ctx->chunk_cols, &pl));
     const bool mfma = pl.mfma, sym = pl.sym;
@@ -1125,8 +1155,8 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     const int KS = pl.KS, K4 = pl.K4;
     const int nrowtiles = pl.nrowtiles, cpc = pl.cols_per_chunk, nchunks = pl.nchunks;
     const int grid = (int)pl.grid;
-    SRED_TRY(ctx->partial.ensure(sizeof(float2) * (size_t)pl.partial_elems));
-    float2 *d_part = ctx->partial.as<float2>();
+    SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)pl.partial_elems));
+    float *d_part = ctx->partial.as<float>();
 
     if (mfma) {
         size_t u4 = (size_t)Nv_pad * KS * 4;
@@ -1152,7 +1182,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
     if (sym) {
         const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
-        const size_t lds = sizeof(float2) * (size_t)P;
+        const size_t lds = sizeof(float) * (size_t)P;
         int rc;
         if (metric == SRED_PEARSON)
             rc = launch_sym<SRED_PEARSON>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
@@ -1204,7 +1234,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     // 4. reduce
     const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)tscale : 1.0;
     hipLaunchKernelGGL(k_reduce, dim3((Nv + 255) / 256), dim3(256), 0, ctx->stream, d_part, nchunks, Nv, Nv_pad, d_idx,
-                       d_stds, metric, flat_count, s_scale, d_S, d_W);
+                       d_stds, metric, flat_count, s_scale, d_S);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }

@@ -63,6 +63,7 @@ struct sred_ctx {
     // device scratch
     sred::DevBuf img, means, stds, mask, S, W, out, dmap, grad;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
+    sred::DevBuf wmom, wpart;  // weight-sum moments
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
     // measurement of the last global contraction
@@ -107,6 +108,11 @@ struct GlobalParams {
 // Accumulators S, Wsum (float maps, w*h) of the global kernels.
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W);
+// Weight sums W (float map entries idx[v], v < Nv) of the global kernels:
+// sum over valid j of exp2(-(t_i - t_j)^2), t = sigma * tscale, by a binned
+// Hermite expansion in FP64; sigma_max bounds the valid blocks' sigmas.
+int weight_sums_dev(sred_ctx *ctx, const float *d_stds, const int *d_idx, int Nv, double tscale, float sigma_max,
+                    float *d_W);
 // FP32 gather of the valid blocks, k-major (UT[k][v]).
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
                        const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,

@@ -1,0 +1,138 @@
+// weights.hip -- weight sums W of the global repetition maps.
+//
+// Every global kernel of the reference accumulates, per reference pixel,
+// Wsum_i = sum over valid test blocks j of w_ij, w_ij = exp(-(s_i - s_j)^2 /
+// (h + eps)) (kernelGetGlobalPearson2D.cl:92-95; the SSIM, NRMSE and
+// AbsDiffStds kernels use the same weight).  It depends on the block standard
+// deviations only: with t = s sqrt(log2(e) / (h + eps)), w_ij =
+// exp2(-(t_i - t_j)^2), a one-dimensional Gaussian sum, so it does not have to
+// ride along in the N x N contraction.  It is evaluated here in FP64 by a
+// binned Hermite expansion (the fast Gauss transform):
+//
+//   * blocks are binned by t, bin b centred at c_b = b * kBinW; a bin keeps
+//     the moments M_n = sum_j u_j^n, u_j = (t_j - c_b) / (kBinW / 2) in [-1, 1);
+//   * with a = t_i - c_b, x = sqrt(ln 2) a and z = sqrt(ln 2) kBinW / 2, the
+//     generating function of the Hermite polynomials, exp(2xy - y^2) =
+//     sum_n H_n(x) y^n / n!, gives
+//       sum_{j in b} exp2(-(a - u_j kBinW / 2)^2) = exp2(-a^2) sum_n c_n M_n,
+//       c_n = H_n(x) z^n / n!,  c_{n+1} = (2 x z c_n - 2 z^2 c_{n-1}) / (n + 1);
+//   * with kBinW = 1/2 and kMoments = 16 terms the truncation is below 1e-15
+//     of a bin's weight (Cramer's bound on H_n); bins with |a| > kRadius hold
+//     weights below 2^-49 each and are skipped (< 2e-9 relative over 10^6
+//     blocks, against W_i >= 1 from the block itself).
+//
+// Moments are accumulated in 2^-42 fixed point with 64-bit integer atomics
+// (per-workgroup LDS tables, then an ordered sum over workgroups): integer
+// sums do not depend on the order of the additions, so W is deterministic.
+#include <cmath>
+
+#include "sred_internal.h"
+
+namespace sred {
+
+constexpr int kMoments = 16;
+constexpr double kBinW = 0.5;
+constexpr double kRadius = 7.25;
+constexpr double kFix = 4398046511104.0;  // 2^42: |u^n| 2^42 per block, 2^62 for 2^20 blocks
+constexpr int kChunkBins = 256;            // bins per moments pass (LDS table 32 KiB)
+constexpr int kMomentGroups = 512;         // workgroups of a moments pass
+constexpr int kMaxBins = 1 << 20;
+constexpr double kSqrtLn2 = 0.83255461115769775635;
+
+__global__ __launch_bounds__(256) void k_wmoments(const float *__restrict__ stds, const int *__restrict__ idx, int Nv,
+                                                  double tscale, int bin0, int nb,
+                                                  unsigned long long *__restrict__ part)
+{
+    __shared__ unsigned long long tab[kChunkBins * kMoments];
+    for (int i = threadIdx.x; i < kChunkBins * kMoments; i += 256) tab[i] = 0ull;
+    __syncthreads();
+    for (int v = blockIdx.x * 256 + threadIdx.x; v < Nv; v += gridDim.x * 256) {
+        const double t = (double)stds[idx[v]] * tscale;
+        const double q = floor(t / kBinW + 0.5);
+        const int b = (int)q - bin0;
+        if (b < 0 || b >= nb) continue;
+        const double u = (t - q * kBinW) * (2.0 / kBinW);
+        double pw = 1.0;
+#pragma unroll
+        for (int n = 0; n < kMoments; n++) {
+            atomicAdd(&tab[b * kMoments + n], (unsigned long long)llrint(pw * kFix));
+            pw *= u;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kChunkBins * kMoments; i += 256)
+        part[(size_t)blockIdx.x * kChunkBins * kMoments + i] = tab[i];
+}
+
+__global__ __launch_bounds__(256) void k_wmoments_sum(const unsigned long long *__restrict__ part, int ngroups,
+                                                      int bin0, int nb, double *__restrict__ M)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nb * kMoments) return;
+    unsigned long long s = 0ull;
+    for (int g = 0; g < ngroups; g++) s += part[(size_t)g * kChunkBins * kMoments + i];
+    M[(size_t)bin0 * kMoments + i] = (double)(long long)s / kFix;
+}
+
+__global__ __launch_bounds__(256) void k_weval(const float *__restrict__ stds, const int *__restrict__ idx, int Nv,
+                                               double tscale, const double *__restrict__ M, int nbins,
+                                               float *__restrict__ W_map)
+{
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= Nv) return;
+    const int p = idx[v];
+    const double t = (double)stds[p] * tscale;
+    const double z = kSqrtLn2 * (kBinW / 2.0);
+    const int blo = max((int)floor((t - kRadius) / kBinW + 0.5), 0);
+    const int bhi = min((int)floor((t + kRadius) / kBinW + 0.5), nbins - 1);
+    double W = 0.0;
+    for (int b = blo; b <= bhi; b++) {
+        const double *m = M + (size_t)b * kMoments;
+        if (m[0] == 0.0) continue;
+        const double a = t - (double)b * kBinW;
+        const double xz2 = 2.0 * kSqrtLn2 * a * z, zz2 = 2.0 * z * z;
+        double cm1 = 1.0, c = xz2;
+        double acc = fma(c, m[1], m[0]);
+#pragma unroll
+        for (int n = 1; n + 1 < kMoments; n++) {
+            const double cn1 = (xz2 * c - zz2 * cm1) / (double)(n + 1);
+            acc = fma(cn1, m[n + 1], acc);
+            cm1 = c;
+            c = cn1;
+        }
+        W = fma(exp2(-a * a), acc, W);
+    }
+    W_map[p] = (float)W;
+}
+
+// W_map[idx[v]] for the Nv valid blocks; sigma_max bounds their standard
+// deviations (it sizes the bin table).
+int weight_sums_dev(sred_ctx *ctx, const float *d_stds, const int *d_idx, int Nv, double tscale, float sigma_max,
+                    float *d_W)
+{
+    if (Nv <= 0) return SRED_OK;
+    const double tmax = (double)sigma_max * tscale;
+    if (!(tmax < (double)(kMaxBins - 2) * kBinW)) {
+        set_error("block standard deviations span too wide a range of weights (normalise the image)");
+        return SRED_ERR_ARG;
+    }
+    const int nbins = (int)std::floor(tmax / kBinW + 0.5) + 1;
+    SRED_TRY(ctx->wmom.ensure(sizeof(double) * (size_t)nbins * kMoments));
+    SRED_TRY(ctx->wpart.ensure(sizeof(unsigned long long) * (size_t)kMomentGroups * kChunkBins * kMoments));
+    double *d_M = ctx->wmom.as<double>();
+    unsigned long long *d_part = ctx->wpart.as<unsigned long long>();
+    const int groups = std::min(kMomentGroups, (Nv + 255) / 256);
+    for (int bin0 = 0; bin0 < nbins; bin0 += kChunkBins) {
+        const int nb = std::min(kChunkBins, nbins - bin0);
+        hipLaunchKernelGGL(k_wmoments, dim3(groups), dim3(256), 0, ctx->stream, d_stds, d_idx, Nv, tscale, bin0, nb,
+                           d_part);
+        hipLaunchKernelGGL(k_wmoments_sum, dim3((nb * kMoments + 255) / 256), dim3(256), 0, ctx->stream, d_part,
+                           groups, bin0, nb, d_M);
+    }
+    hipLaunchKernelGGL(k_weval, dim3((Nv + 255) / 256), dim3(256), 0, ctx->stream, d_stds, d_idx, Nv, tscale, d_M,
+                       nbins, d_W);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
+}  // namespace sred
