@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
     if (v >= Nv_pad) return;
     float4 a = make_float4(kPadT, 0.0f, 0.0f, 0.0f);
     if (v >= Nv) {
-        if (metric == SRED_SSIM) a.z = INFINITY;
+        if (metric == SRED_SSIM) a.w = INFINITY;
         if (metric == SRED_NRMSE) a.y = -INFINITY;
         aux[v] = a;
         return;
@@ -503,28 +503,19 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 }
 
 // ---------------------------------------------------------------------------
-// Sum over the 4 k-groups of a wave (lanes l, l^16, l^32, l^48) with lane
-// swaps on the vector ALU (v_permlane16/32_swap) instead of LDS permutes.
-__device__ __forceinline__ float sum_kgroups(float x)
-{
-    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-    x = __uint_as_float(a[0]) + __uint_as_float(a[1]);  // rows 0+1 | 2+3
-    const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-    return __uint_as_float(b[0]) + __uint_as_float(b[1]);
-}
-
 // One 64-column tile of one wave (16*RG rows): the MFMAs of column group
 // cg+1 are issued before the epilogue of group cg, so the matrix core works
 // on the next group while the vector ALU finishes the current one; the
 // scheduling hints interleave one MFMA with a few VALU instructions.  COLS
-// adds the column sums (off-diagonal panel pairs): per lane over its rows,
-// over the k-groups at the end of the tile, then lane (kq, c) stores group
-// kq's column c, so the store needs no lane mask.  B fragments are read from
-// the fragment-major LDS tile: 64 consecutive uint4 per (group, k-step).
+// adds the column sums (off-diagonal panel pairs): per lane over its rows for
+// each of the 4 column groups, stored as one float4 per lane at the end of
+// the tile (the k-groups and waves are summed when the tile is folded, so no
+// cross-lane operation is needed).  B fragments are read from the
+// fragment-major LDS tile: 64 consecutive uint4 per (group, k-step).
 template <int M, int KS, int RG, bool EXACT, bool COLS>
 __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
                                          const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
-                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float *stage_w, int lane,
+                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float4 *stage_w, int lane,
                                          const EpiConst &ec)
 {
     const int li = lane & 15, lk = lane >> 4;
@@ -566,15 +557,7 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
             }
         }
     }
-    if (COLS) {
-        float mine = 0.0f;
-#pragma unroll
-        for (int cg = 0; cg < 4; cg++) {
-            const float v = sum_kgroups(colsum[cg]);
-            mine = (lk == cg) ? v : mine;
-        }
-        stage_w[lk * 16 + li] = mine;
-    }
+    if (COLS) stage_w[lane] = make_float4(colsum[0], colsum[1], colsum[2], colsum[3]);
 }
 
 // Asynchronous global -> LDS copy of one 64-column tile: 4*KS runs of 64
@@ -642,7 +625,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     __shared__ float4 con0[kBN];
     __shared__ uint4 hi1[DB ? TILE : 1], lo1[DB ? TILE : 1];
     __shared__ float4 con1[DB ? kBN : 1];
-    __shared__ float stage[2][4][kBN];
+    __shared__ float4 stage[2][4][64];  // [parity][wave][lane]: column-group sums
     extern __shared__ float colacc[];  // P entries
 
     // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
@@ -670,9 +653,18 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     int rowbase = 0;
 
     auto fold = [&](int T) {  // column sums of step T's tile into colacc (after a barrier)
-        if (tid < kBN)
-            colacc[(T % ntiles) * kBN + tid] +=
-                (stage[T & 1][0][tid] + stage[T & 1][1][tid]) + (stage[T & 1][2][tid] + stage[T & 1][3][tid]);
+        if (tid < kBN) {  // column tid = 16 * group + c: lanes c, c+16, c+32, c+48 of every wave
+            const int g = tid >> 4, c = tid & 15;
+            float t = 0.0f;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+#pragma unroll
+                for (int kq = 0; kq < 4; kq++) {
+                    const float4 v = stage[T & 1][w][kq * 16 + c];
+                    t += g == 0 ? v.x : g == 1 ? v.y : g == 2 ? v.z : v.w;
+                }
+            colacc[(T % ntiles) * kBN + tid] += t;
+        }
     };
     auto begin_sweep = [&](int T) {  // A fragments and row constants of sweep T / ntiles
         rowbase = row0 + (T / ntiles) * BM + wave * 16 * RG;
@@ -713,7 +705,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
         const int c0 = col0 + (T % ntiles) * kBN;
         const bool exact =
             row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
-        float *sw = stage[T & 1][wave];
+        float4 *sw = stage[T & 1][wave];
         if (diag) {
             if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
             else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);

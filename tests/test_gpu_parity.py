@@ -282,3 +282,19 @@ def test_global_symmetric_matches_full_pairs(ctx, metric, shape, b):
         np.testing.assert_allclose(W, W64, rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(S, S64, rtol=2e-4, atol=2e-4 * np.abs(S64).max())
     np.testing.assert_allclose(res[1][1], res[0][1], rtol=2e-6)
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_symmetric_contraction_deterministic(ctx, metric):
+    """Repeated runs of the double-buffered symmetric kernel (several panels,
+    column sums staged through LDS) give bit-identical accumulators."""
+    shape, b = (300, 260), 11
+    img = synthetic.spots(shape[1], shape[0], 30, seed=shape[0] + metric)
+    img[5:15, 5:15] = 0.3
+    inp = utils.get_input_image_2d(img)
+    m, s = O.local_stats_f32(inp.image_array, b, b)
+    nv = O.noise_var(inp.image_array)
+    st = clutils.LocalStatistics(m.ravel(), s.ravel())
+    runs = [clutils.get_global_accumulators_2d(ctx, metric, inp, st, b, b, nv, 0.0, "bf16x3")[0] for _ in range(4)]
+    for r in runs[1:]:
+        np.testing.assert_array_equal(r, runs[0])
