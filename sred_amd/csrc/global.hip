@@ -652,19 +652,17 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     bool row_small = false;
     int rowbase = 0;
 
-    auto fold = [&](int T) {  // column sums of step T's tile into colacc (after a barrier)
-        if (tid < kBN) {  // column tid = 16 * group + c: lanes c, c+16, c+32, c+48 of every wave
-            const int g = tid >> 4, c = tid & 15;
-            float t = 0.0f;
-#pragma unroll
-            for (int w = 0; w < 4; w++)
-#pragma unroll
-                for (int kq = 0; kq < 4; kq++) {
-                    const float4 v = stage[T & 1][w][kq * 16 + c];
-                    t += g == 0 ? v.x : g == 1 ? v.y : g == 2 ? v.z : v.w;
-                }
-            colacc[(T % ntiles) * kBN + tid] += t;
-        }
+    auto fold = [&](int T) {
+        // column sums of step T's tile into colacc (after a barrier), spread
+        // over all threads: wave g takes column group g; thread (c, kq) sums
+        // column 16g+c of k-group kq over the 4 waves, then the 4 k-groups
+        // (adjacent lanes) are added by DPP quad permutes
+        const int c = (tid >> 2) & 15, kq = tid & 3;
+        const float *st = reinterpret_cast<const float *>(&stage[T & 1][0][kq * 16 + c]) + wave;
+        float v = ((st[0] + st[256]) + st[512]) + st[768];  // waves 0..3 (64 float4 apart)
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));  // kq ^ 1
+        v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));  // kq ^ 2
+        if (kq == 0) colacc[(T % ntiles) * kBN + wave * 16 + c] += v;
     };
     auto begin_sweep = [&](int T) {  // A fragments and row constants of sweep T / ntiles
         rowbase = row0 + (T / ntiles) * BM + wave * 16 * RG;
