@@ -583,8 +583,13 @@ __device__ __forceinline__ void tile_to_lds(const uint4 *Uhi, const uint4 *Ulo, 
     if (wave == 0) __builtin_amdgcn_global_load_lds((const void *)(aux + c0 + lane), (lds_ptr)dcon, 16, 0, 0);
 }
 
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima)
-__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// s_waitcnt vmcnt(0) expcnt(0) lgkmcnt(0): this wave's global->LDS copies
+// (vmcnt) and its own LDS stores (lgkmcnt) are complete.  The LDS count is
+// waited for explicitly too: an explicit wait for the vector-memory count
+// alone right before a barrier has been seen to take the place of the
+// LDS-count wait the barrier needs, leaving a tile's column-sum stores in
+// flight while other waves read them.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
 // two 64-column tiles in flight (copy of the next overlaps the current one)
 // while their LDS fits twice beside the column accumulator at 2 workgroups/CU
@@ -738,6 +743,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
         }
     } else {
         for (int T = 0; T < NT; T++) {
+            wait_vm_all();
             __syncthreads();
             if (!diag && T > 0) fold(T - 1);
             if (T % ntiles == 0) begin_sweep(T);
@@ -748,8 +754,10 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
         }
     }
     if (!diag) {
+        wait_vm_all();
         __syncthreads();
         fold(NT - 1);
+        wait_vm_all();
         __syncthreads();
         for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
     }
