@@ -142,6 +142,23 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
                           int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
                           int normalize, float *d_out);
 
+/* ---- Hu template metric (kernelGetPatchHu2D.cl:11-282; the reference has
+ * the kernel but no Java caller, so these entries define its host side) ---- */
+/* Map of sqrt(hu1(template) * hu1(window)) over every interior window, FP64
+ * moments as the kernel computes them (window: mean over the square minus its
+ * corners / K, min-max over the float ellipse, 1-based raw and 0-based central
+ * moments).  tmpl is the bw x bh template RECTANGLE (row-major, host), used as
+ * given inside the ellipse (the kernel's patch_pixels); border pixels are 0. */
+int sred_block_hu_2d(sred_ctx *ctx, const float *img, int w, int h, const float *tmpl, int bw, int bh, float *out);
+/* Same on device buffers (tmpl stays a host array). */
+int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const float *tmpl, int bw, int bh,
+                         float *d_out);
+/* Template preparation for the Hu map (the kernel expects a mean-subtracted,
+ * normalised template and nothing in the reference prepares one): the
+ * window's own steps (kernelGetPatchHu2D.cl:127-182) applied to the bw x bh
+ * block rectangle; cells outside the ellipse are 0.  Host only. */
+int sred_hu_prepare_template_2d(const float *blk, int bw, int bh, float *tmpl_out);
+
 /* ---- differentiation (no counterpart in the reference) ---- */
 /* Gradient of L = sum_i dL_dmap[i] * map[i] of the global Pearson map
  * (relevance constant 0, normalised output; the forward of

@@ -68,6 +68,12 @@ def lib():
         L.or_global_map.argtypes = [c_int, c_int, _F, c_int, c_int, c_int, c_int, c_float, c_int, vp, vp, c_int]
         L.or_block_map.argtypes = [c_int, _F, c_int, c_int, _F, c_int, c_int, c_float, c_float, c_float, c_int, _F]
         L.or_block_raw.argtypes = [c_int, _F, _F, _F, c_int, c_int, _F, c_int, c_int, c_float, c_float, _F]
+        L.or_hu1_template.argtypes = [_F, c_int, c_int, c_double]
+        L.or_hu1_template.restype = c_double
+        L.or_block_hu.argtypes = [_F, c_int, c_int, _F, c_int, c_int, c_double, _F]
+        L.or_block_hu.restype = None
+        L.or_hu_prepare_template.argtypes = [_F, c_int, c_int, c_double, _F]
+        L.or_hu_prepare_template.restype = None
         _lib = L
     return _lib
 
@@ -255,4 +261,37 @@ def block_map(metric, img, tmpl, bw, bh, tmpl_mean, tmpl_std, c=0.0, normalize=T
                             int(normalize), out.reshape(-1))
     if rc:
         raise ValueError("block map failed")
+    return out
+
+
+def hu_eps():
+    """The double the Hu kernel's EPSILON macro denotes (Java ""+1e-7f)."""
+    return java_float_literal(np.float32(1e-7))
+
+
+def hu1_template(tmpl, eps=None):
+    """hu1 of a template rectangle (bh, bw), kernelGetPatchHu2D.cl:36-101."""
+    t = _f32(tmpl)
+    bh, bw = t.shape
+    return lib().or_hu1_template(t.reshape(-1), bw, bh, hu_eps() if eps is None else eps)
+
+
+def block_hu(img, tmpl, eps=None):
+    """Hu template map, kernelGetPatchHu2D.cl (border 0)."""
+    img = _f32(img)
+    t = _f32(tmpl)
+    h, w = img.shape
+    bh, bw = t.shape
+    out = np.zeros_like(img)
+    lib().or_block_hu(img.reshape(-1), w, h, t.reshape(-1), bw, bh, hu_eps() if eps is None else eps,
+                      out.reshape(-1))
+    return out
+
+
+def hu_prepare_template(blk, eps=None):
+    """The build's Hu template preparation (window steps on the template)."""
+    b = _f32(blk)
+    bh, bw = b.shape
+    out = np.zeros_like(b)
+    lib().or_hu_prepare_template(b.reshape(-1), bw, bh, hu_eps() if eps is None else eps, out.reshape(-1))
     return out

@@ -645,3 +645,140 @@ done:
     free(means); free(stds); free(mask);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Hu template metric, kernelGetPatchHu2D.cl                                 */
+/* ------------------------------------------------------------------------ */
+
+/* float ellipse predicate of the kernel, e.g. kernelGetPatchHu2D.cl:38-40 */
+static int hu_in_f(int i, int j, int brw, int brh)
+{
+    float dx = (float)(i - brw) / (float)brw;
+    float dy = (float)(j - brh) / (float)brh;
+    float a = dx * dx;
+    float b = dy * dy;
+    return a + b <= 1.0f;
+}
+/* double ellipse predicate, kernelGetPatchHu2D.cl:58-60 */
+static int hu_in_d(int i, int j, int brw, int brh)
+{
+    double dx = (double)(i - brw) / brw;
+    double dy = (double)(j - brh) / brh;
+    double a = dx * dx;
+    double b = dy * dy;
+    return a + b <= 1.0;
+}
+
+/* hu1 of a patch held in p[bw*bh] over the cells where in[] is set,
+ * kernelGetPatchHu2D.cl:52-101 and :190-239 */
+static double hu1_of(const double *p, const unsigned char *in, int bw, int bh, double eps)
+{
+    double M00 = 0.0, M10 = 0.0, M01 = 0.0;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++)
+            if (in[j * bw + i]) {
+                double v = p[j * bw + i];
+                M00 += v;
+                M10 += (double)(i + 1) * v;
+                M01 += (double)(j + 1) * v;
+            }
+    double cx = M10 / (M00 + eps);
+    double cy = M01 / (M00 + eps);
+    double mu20 = 0.0, mu02 = 0.0;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++)
+            if (in[j * bw + i]) {
+                double v = p[j * bw + i];
+                double ddx = (double)i - cx;
+                double ddy = (double)j - cy;
+                mu20 += (ddx * ddx) * v;
+                mu02 += (ddy * ddy) * v;
+            }
+    double m2 = M00 * M00;
+    double nu20 = mu20 / (m2 + eps);
+    double nu02 = mu02 / (m2 + eps);
+    return nu20 + nu02;
+}
+
+double or_hu1_template(const float *tmpl, int bw, int bh, double eps)
+{
+    int brw = bw / 2, brh = bh / 2;
+    double *p = (double *)calloc((size_t)bw * bh, sizeof(double));
+    unsigned char *in = (unsigned char *)calloc((size_t)bw * bh, 1);
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) {
+            if (hu_in_f(i, j, brw, brh)) p[j * bw + i] = (double)tmpl[j * bw + i];  /* :36-44 */
+            in[j * bw + i] = (unsigned char)hu_in_d(i, j, brw, brh);              /* :58-60 */
+        }
+    double r = hu1_of(p, in, bw, bh, eps);
+    free(p);
+    free(in);
+    return r;
+}
+
+/* the window steps of kernelGetPatchHu2D.cl:127-182 on a patch whose
+ * rectangle pixel (i, j) is get(i, j); fills p (FP64) and in (float ellipse) */
+static void hu_window(const float *src, int pitch, int bw, int bh, int K, double eps, double *p, unsigned char *in)
+{
+    int brw = bw / 2, brh = bh / 2;
+    double mean = 0.0;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) {
+            /* (float)((i-gx)/bRW): integer division, so only the 4 corners fail */
+            float dx = (float)((i - brw) / brw);
+            float dy = (float)((j - brh) / brh);
+            if (dx * dx + dy * dy <= 1.0f) {
+                double v = (double)src[j * pitch + i];
+                p[j * bw + i] = v;
+                mean += v;
+            }
+        }
+    mean /= (double)K;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) in[j * bw + i] = (unsigned char)hu_in_f(i, j, brw, brh);
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (int k = 0; k < bw * bh; k++)
+        if (in[k]) {
+            p[k] -= mean;
+            mn = p[k] < mn ? p[k] : mn;
+            mx = p[k] > mx ? p[k] : mx;
+        }
+    for (int k = 0; k < bw * bh; k++)
+        if (in[k]) p[k] = (p[k] - mn) / (mx - mn + eps);
+}
+
+void or_block_hu(const float *img, int w, int h, const float *tmpl, int bw, int bh, double eps, float *out)
+{
+    int brw = bw / 2, brh = bh / 2;
+    int K = or_footprint(bw, bh, NULL, NULL);
+    double ref = or_hu1_template(tmpl, bw, bh, eps);
+#pragma omp parallel
+    {
+        double *p = (double *)calloc((size_t)bw * bh, sizeof(double));
+        unsigned char *in = (unsigned char *)calloc((size_t)bw * bh, 1);
+#pragma omp for schedule(static)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                if (x < brw || x >= w - brw || y < brh || y >= h - brh) {
+                    out[(int64_t)y * w + x] = 0.0f;
+                    continue;
+                }
+                hu_window(img + (int64_t)(y - brh) * w + (x - brw), w, bw, bh, K, eps, p, in);
+                double c = hu1_of(p, in, bw, bh, eps);
+                out[(int64_t)y * w + x] = (float)sqrt(ref * c);
+            }
+        free(p);
+        free(in);
+    }
+}
+
+void or_hu_prepare_template(const float *blk, int bw, int bh, double eps, float *out)
+{
+    int K = or_footprint(bw, bh, NULL, NULL);
+    double *p = (double *)calloc((size_t)bw * bh, sizeof(double));
+    unsigned char *in = (unsigned char *)calloc((size_t)bw * bh, 1);
+    hu_window(blk, bw, bw, bh, K, eps, p, in);
+    for (int k = 0; k < bw * bh; k++) out[k] = in[k] ? (float)p[k] : 0.0f;
+    free(p);
+    free(in);
+}

@@ -110,6 +110,28 @@ int or_block_map(int metric, const float *img, int w, int h, const float *tmpl, 
 int or_block_raw(int metric, const float *img, const float *means, const float *stds, int w, int h,
                  const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std, float *out);
 
+/* Hu template metric, kernelGetPatchHu2D.cl:11-282 (no Java caller in the
+ * reference).  FP64 throughout, following the kernel's loops:
+ *   - template: the bw x bh rectangle `tmpl` is used as given where the float
+ *     ellipse predicate holds (:36-44); its moments use the double predicate
+ *     (:56-67, :80-96);
+ *   - window: mean over the square minus its four corners (integer division
+ *     in the predicate, :129-140) divided by K; mean subtraction, min/max and
+ *     min-max normalisation over the float ellipse (:143-182);
+ *   - raw moments with 1-based coordinates (i+1, j+1), central moments with
+ *     0-based ones (:190-234); hu1 = nu20 + nu02, nu = mu / (M00^2 + eps);
+ *   - out = (float) sqrt(hu1_template * hu1_window) (:267); border pixels 0.
+ * pow(x, 2.0) is x*x (the correctly rounded square).  eps is the double the
+ * kernel's EPSILON macro denotes.  Only hu1 reaches the output. */
+double or_hu1_template(const float *tmpl, int bw, int bh, double eps);
+void or_block_hu(const float *img, int w, int h, const float *tmpl, int bw, int bh, double eps, float *out);
+/* The template preparation this build defines for the Hu map (the kernel
+ * expects "mean-subtracted and normalised" template pixels and has no
+ * caller): the window steps of :129-182 applied to the template rectangle
+ * (mean over the square minus corners / K, subtraction and min-max over the
+ * float ellipse), in FP64, rounded to float; pixels outside the ellipse 0. */
+void or_hu_prepare_template(const float *blk, int bw, int bh, double eps, float *out);
+
 #ifdef __cplusplus
 }
 #endif

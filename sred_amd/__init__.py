@@ -7,17 +7,18 @@ Utils interfaces for that path (ctypes), plus the two 2D commands.
 from . import _lib
 from ._lib import (ABSDIFF, BF16X3, FP32, NRMSE, PEARSON, SSIM, Context, NonBinaryMaskError, SredError,
                    load)
-from .clutils import (BLOCK_METRICS, GLOBAL_METRICS, calculate_block_repetition_map_2d,
+from .clutils import (BLOCK_METRICS, GLOBAL_METRICS, calculate_block_hu_map_2d, calculate_block_repetition_map_2d,
                       calculate_global_repetition_map_2d, get_local_statistics_2d, get_opencl_resources)
 from .commands import block_repetition_2d, global_repetition_2d
 from .utils import (InputImage2D, ReferenceBlock2D, RelevanceMask, apply_mask_2d, block_size_2d,
-                    get_input_image_2d, get_mean_noise_var_2d, get_reference_block_2d, get_relevance_mask,
+                    get_hu_template_2d, get_input_image_2d, get_mean_noise_var_2d, get_reference_block_2d, get_relevance_mask,
                     normalize_image_2d)
 
 __all__ = [
     "ABSDIFF", "BF16X3", "FP32", "NRMSE", "PEARSON", "SSIM", "Context", "NonBinaryMaskError", "SredError", "load",
-    "BLOCK_METRICS", "GLOBAL_METRICS", "calculate_block_repetition_map_2d", "calculate_global_repetition_map_2d",
+    "BLOCK_METRICS", "GLOBAL_METRICS", "calculate_block_hu_map_2d", "calculate_block_repetition_map_2d", "calculate_global_repetition_map_2d",
     "get_local_statistics_2d", "get_opencl_resources", "block_repetition_2d", "global_repetition_2d",
-    "InputImage2D", "ReferenceBlock2D", "RelevanceMask", "apply_mask_2d", "block_size_2d", "get_input_image_2d",
+    "InputImage2D", "ReferenceBlock2D", "RelevanceMask", "apply_mask_2d", "block_size_2d", "get_hu_template_2d",
+    "get_input_image_2d",
     "get_mean_noise_var_2d", "get_reference_block_2d", "get_relevance_mask", "normalize_image_2d",
 ]

@@ -32,7 +32,7 @@ EXPORTS = [
     "sred_apply_mask_2d", "sred_normalize_map_2d", "sred_local_stats_2d", "sred_global_accum_2d",
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
-    "sred_global_pearson_adjoint_2d_dev",
+    "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
 ]
 
 
@@ -101,6 +101,9 @@ def load():
         "sred_global_plan_2d": ([_I, _I, _I, _I, _I, _I, ctypes.POINTER(_I64)], _I),
         "sred_global_pearson_adjoint_2d": ([_VP, _F, _I, _I, _I, _I, _F, _F], _I),
         "sred_global_pearson_adjoint_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
+        "sred_block_hu_2d": ([_VP, _F, _I, _I, _F, _I, _I, _F], _I),
+        "sred_block_hu_2d_dev": ([_VP, _VP, _I, _I, _F, _I, _I, _VP], _I),
+        "sred_hu_prepare_template_2d": ([_F, _I, _I, _F], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -97,3 +97,19 @@ def calculate_block_repetition_map_2d(metric, input_image, reference_block, rele
         L.fptr(L.f32(reference_block.pixels)), reference_block.width, reference_block.height, reference_block.mean,
         reference_block.std, relevance_constant, int(bool(normalize_output)), L.fptr(out)))
     return out
+
+
+def calculate_block_hu_map_2d(input_image, template, resources):
+    """Hu template map (kernelGetPatchHu2D.cl:11-282; the reference has no Java
+    caller for it): sqrt(hu1(template) * hu1(window)) per interior pixel.
+
+    template is the (bh, bw) template rectangle as the kernel's patch_pixels
+    (for example from utils.get_hu_template_2d).  Returns float32 w*h.
+    """
+    t = L.f32(template)
+    bh, bw = t.shape
+    out = np.zeros(input_image.size, np.float32)
+    img = L.f32(input_image.image_array).reshape(-1)
+    L.check(L.load().sred_block_hu_2d(resources.handle, L.fptr(img), input_image.width, input_image.height,
+                                      L.fptr(t.reshape(-1)), bw, bh, L.fptr(out)))
+    return out

@@ -644,6 +644,46 @@ int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, 
     return block_pipeline(ctx, g, metric, d_img, tmpl, tmpl_mean, tmpl_std, relevance_constant, normalize, d_out);
 }
 
+int sred_block_hu_2d(sred_ctx *ctx, const float *img, int w, int h, const float *tmpl, int bw, int bh, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !tmpl || !out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * g.n()));
+    SRED_TRY(block_hu_dev(ctx, g, ctx->img.as<float>(), tmpl, ctx->out.as<float>()));
+    SRED_TRY(download(ctx, out, ctx->out, g.n()));
+    return SRED_OK;
+}
+
+int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const float *tmpl, int bw, int bh,
+                         float *d_out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!d_img || !tmpl || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return block_hu_dev(ctx, g, d_img, tmpl, d_out);
+}
+
+int sred_hu_prepare_template_2d(const float *blk, int bw, int bh, float *tmpl_out)
+{
+    Geometry g;
+    SRED_TRY(make_geometry(bw, bh, bw, bh, &g));
+    if (!blk || !tmpl_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return hu_prepare_template_host(blk, bw, bh, g.K, tmpl_out);
+}
+
 int sred_global_pearson_adjoint_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh,
                                    const float *dL_dmap, float *dL_dimg)
 {

@@ -128,6 +128,13 @@ int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const floa
 // means/stds are written when both output pointers are non-NULL.
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
                   float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out);
+// Hu template map (kernelGetPatchHu2D.cl) of a device image; h_tmpl is the
+// host template rectangle.  Synchronises.
+int block_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *h_tmpl, float *d_out);
+// hu1 of a template rectangle (kernelGetPatchHu2D.cl:36-101), host.
+double hu1_template_host(const float *tmpl, int bw, int bh, double eps);
+// Hu template preparation (the window steps of kernelGetPatchHu2D.cl:127-182), host.
+int hu_prepare_template_host(const float *blk, int bw, int bh, int K, float *out);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);

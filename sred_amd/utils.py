@@ -134,3 +134,15 @@ def normalize_image_2d(image_array, image_width, image_height, border_width, bor
     L.check(L.load().sred_normalize_map_2d(L.fptr(out.reshape(-1)), image_width, image_height, border_width,
                                            border_height, L.fptr(m)))
     return out
+
+
+def get_hu_template_2d(block):
+    """Template rectangle for the Hu map: the window steps of
+    kernelGetPatchHu2D.cl:127-182 (mean over the square minus corners / K,
+    subtraction and min-max over the ellipse) applied to a (bh, bw) block.
+    Even sizes raise ValueError (the reference's block commands reject them)."""
+    b = L.f32(block)
+    bh, bw = b.shape
+    out = np.zeros_like(b)
+    L.check(L.load().sred_hu_prepare_template_2d(L.fptr(b.reshape(-1)), bw, bh, L.fptr(out.reshape(-1))))
+    return out
