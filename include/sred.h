@@ -180,8 +180,11 @@ int sred_global_pearson_adjoint_2d_dev(sred_ctx *ctx, const float *d_img, int w,
  * evaluated, flags (1 = MFMA, 2 = symmetric). */
 int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision, int symmetric, int chunk_cols,
                         int64_t *out);
-/* Average duration (ms, HIP events) and launch count of the dominant
- * contraction kernel in the last global call on this context. */
+/* Duration (ms, HIP events on the context's stream) and launch count of the
+ * dominant kernel of the last call on this context -- the contraction of a
+ * global call, the stencil of a block or Hu call, the pair kernel of an
+ * adjoint call -- and its work units (block pairs evaluated, or pixels).
+ * No counterpart in the reference (measurement). */
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);
 
 #ifdef __cplusplus

@@ -316,9 +316,11 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
                       inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), ctx->u32.as<float>());
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)Nv_pad * K16));
     float *d_Gc = ctx->partial.as<float>();
+    SRED_TRY(time_begin(ctx));
     SRED_TRY(launch_adjoint(K16 / 16, dim3(Nv_pad / 64), lds, ctx->stream, ctx->u32.as<float>(), d_info, K, K4, Nv,
                             Nv_pad, kEps, (float)(-2.0 / H), inv_sqrt_km1, d_Gc));
     SRED_HIP_CHECK(hipGetLastError());
+    SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
     // pixel -> block index, then the transpose stencil
     SRED_TRY(ctx->counts.ensure(sizeof(int) * (size_t)n));
     int *d_pix2v = ctx->counts.as<int>();

@@ -126,6 +126,25 @@ int upload_offsets(sred_ctx *ctx, const Geometry &g, int pitch, DevBuf *buf, int
     return SRED_OK;
 }
 
+int time_begin(sred_ctx *ctx)
+{
+    if (!ctx->ev0) {
+        SRED_HIP_CHECK(hipEventCreate(&ctx->ev0));
+        SRED_HIP_CHECK(hipEventCreate(&ctx->ev1));
+    }
+    ctx->last_launches = 0;
+    SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    return SRED_OK;
+}
+
+int time_end(sred_ctx *ctx, int64_t units)
+{
+    SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->last_launches = 1;
+    ctx->last_pairs = units;
+    return SRED_OK;
+}
+
 static int check_ctx(sred_ctx *ctx)
 {
     if (!ctx) {

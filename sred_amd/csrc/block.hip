@@ -106,11 +106,13 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
     SRED_HIP_CHECK(hipMemcpy(ctx->tmpl.ptr, h_tmpl, sizeof(float) * g.K, hipMemcpyHostToDevice));
     dim3 grid((g.w + kBlkTile - 1) / kBlkTile, (g.h + kBlkTile - 1) / kBlkTile);
     size_t lds = sizeof(float) * ((size_t)tw * th + g.K);
+    SRED_TRY(time_begin(ctx));
     hipLaunchKernelGGL(k_block, grid, dim3(256), lds, ctx->stream, metric, d_img, g.w, g.h, g.brw, g.brh,
                        ctx->offs.as<int>(), g.K, ctx->tmpl.as<float>(), java_float_literal(tmpl_mean),
                        java_float_literal(tmpl_std), tmpl_std, java_float_literal(kEps), d_out,
                        d_stds_out ? d_means_out : nullptr, d_means_out ? d_stds_out : nullptr);
     SRED_HIP_CHECK(hipGetLastError());
+    SRED_TRY(time_end(ctx, g.n()));
     return SRED_OK;
 }
 

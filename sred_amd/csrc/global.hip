@@ -1173,11 +1173,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     SRED_HIP_CHECK(hipGetLastError());
 
     // 3. contraction (timed with events on the launch stream)
-    if (!ctx->ev0) {
-        SRED_HIP_CHECK(hipEventCreate(&ctx->ev0));
-        SRED_HIP_CHECK(hipEventCreate(&ctx->ev1));
-    }
-    SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+    SRED_TRY(time_begin(ctx));
     if (sym) {
         const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
         const size_t lds = sizeof(float) * (size_t)P;
@@ -1225,9 +1221,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
             launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
     }
     SRED_HIP_CHECK(hipGetLastError());
-    SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
-    ctx->last_launches = 1;
-    ctx->last_pairs = pl.pairs_evaluated;
+    SRED_TRY(time_end(ctx, pl.pairs_evaluated));
 
     // 4. reduce
     const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)tscale : 1.0;

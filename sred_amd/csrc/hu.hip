@@ -182,9 +182,11 @@ int block_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const flo
     const int *d_l = ctx->offs.as<int>();
     const size_t lds = sizeof(float) * (size_t)tw * th;
     dim3 grid((g.w + kHuTile - 1) / kHuTile, (g.h + kHuTile - 1) / kHuTile);
+    SRED_TRY(time_begin(ctx));
     hipLaunchKernelGGL(k_block_hu, grid, dim3(kHuTile * kHuTile), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
                        d_l, nsq, d_l + nsq, d_l + nsq + nell, g.K, hu_ref, eps, d_out);
     SRED_HIP_CHECK(hipGetLastError());
+    SRED_TRY(time_end(ctx, g.n()));
     // the host list must outlive the asynchronous copy
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return SRED_OK;

@@ -76,6 +76,10 @@ struct sred_ctx {
 
 namespace sred {
 
+// Bracket the dominant kernel of a call with the context's HIP events on its
+// stream (read back by sred_ctx_last_contraction); units = work items of it.
+int time_begin(sred_ctx *ctx);
+int time_end(sred_ctx *ctx, int64_t units);
 // Upload the footprint offsets dy*pitch+dx (cached per (bw, bh, pitch)).
 int upload_offsets(sred_ctx *ctx, const Geometry &g, int pitch, DevBuf *buf, int *key);
 // Zero device memory with a kernel (graph-capture safe, unlike memset nodes).
