@@ -31,6 +31,10 @@ void set_error(const std::string &msg);
 
 constexpr float kEps = 1e-7f;           // Utils.EPSILON, Utils.java:20
 constexpr int kMaxK = 3200;             // footprint size cap (63x63 ellipse has 3109 px)
+// Context scratch `misc`: [0, 1024) relevance-mask count, [1024, 2048)
+// normalisation min/max, [2048, 4096) mask check, [4096, ...) min/max
+// partials of up to 1024 workgroups (3 words each).
+constexpr size_t kMiscBytes = 4096 + 3 * 4 * 1024;
 
 // Growable device scratch buffer owned by a context.
 struct DevBuf {

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_relevance_mask(const float *__restrict_
 int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
                        float *n_pixels)
 {
-    SRED_TRY(ctx->misc.ensure(4096));
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));
     unsigned long long *d_count = ctx->misc.as<unsigned long long>();
     SRED_TRY(zero_dev(ctx, d_count, sizeof(unsigned long long)));
     int blocks = (int)std::min<int64_t>((g.n() + 255) / 256, 4096);
@@ -217,8 +217,11 @@ __global__ void k_minmax_init(unsigned int *mm, float init_max)
 
 // Java Math.min/Math.max over the interior (mask > 0 where a mask is given);
 // order-independent, so a parallel reduction gives the sequential result.
+// Two stages without atomics: each workgroup writes its (min, max, non-binary)
+// to part[3 * blockIdx.x ..], then k_minmax_final folds them into mm (which
+// holds the initial values).
 __global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ map, const float *__restrict__ mask, int w,
-                                                int h, int brw, int brh, unsigned int *__restrict__ mm)
+                                                int h, int brw, int brh, unsigned int *__restrict__ part)
 {
     int64_t n = (int64_t)w * h;
     unsigned int lo = 0xffffffffu, hi = 0u;
@@ -249,9 +252,38 @@ __global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ map, c
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (slo[0] != 0xffffffffu) atomicMin(&mm[0], slo[0]);
-        if (shi[0] != 0u) atomicMax(&mm[1], shi[0]);
-        if (snb) atomicOr(&mm[2], 1u);
+        part[3 * blockIdx.x] = slo[0];
+        part[3 * blockIdx.x + 1] = shi[0];
+        part[3 * blockIdx.x + 2] = snb ? 1u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_minmax_final(const unsigned int *__restrict__ part, int nparts,
+                                                      unsigned int *__restrict__ mm)
+{
+    unsigned int lo = 0xffffffffu, hi = 0u, nb = 0u;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        lo = min(lo, part[3 * i]);
+        hi = max(hi, part[3 * i + 1]);
+        nb |= part[3 * i + 2];
+    }
+    __shared__ unsigned int slo[256], shi[256], snb[256];
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    snb[threadIdx.x] = nb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + s]);
+            shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + s]);
+            snb[threadIdx.x] |= snb[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        mm[0] = min(mm[0], slo[0]);
+        mm[1] = max(mm[1], shi[0]);
+        mm[2] |= snb[0];
     }
 }
 
@@ -272,15 +304,28 @@ __global__ __launch_bounds__(256) void k_remap(float *__restrict__ map, const fl
 }
 
 static int grid_for(const Geometry &g) { return (int)std::min<int64_t>((g.n() + 255) / 256, 8192); }
+constexpr int kMinmaxParts = 1024;  // workgroups of k_minmax (grid-stride); kMiscBytes holds their partials
+constexpr size_t kMiscPartOffset = 4096;
+static int grid_minmax(const Geometry &g) { return (int)std::min<int64_t>((g.n() + 255) / 256, kMinmaxParts); }
+
+// min/max (and the non-binary flag) of map over the selected pixels into mm[0..2]
+static int minmax_dev(sred_ctx *ctx, const Geometry &g, const float *d_map, const float *d_mask, unsigned int *mm)
+{
+    unsigned int *part = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + kMiscPartOffset);
+    const int grid = grid_minmax(g);
+    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh, part);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, ctx->stream, part, grid, mm);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
 
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value)
 {
-    SRED_TRY(ctx->misc.ensure(4096));
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));
     unsigned int *mm = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + 1024);
     float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, mm, init_max);
-    hipLaunchKernelGGL(k_minmax, dim3(grid_for(g)), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh,
-                       mm);
+    SRED_TRY(minmax_dev(ctx, g, d_map, d_mask, mm));
     hipLaunchKernelGGL(k_remap, dim3(grid_for(g)), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh,
                        mm);
     SRED_HIP_CHECK(hipGetLastError());
@@ -289,11 +334,10 @@ int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const floa
 
 int mask_is_binary_dev(sred_ctx *ctx, const Geometry &g, const float *d_mask, int *is_binary)
 {
-    SRED_TRY(ctx->misc.ensure(4096));
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));
     unsigned int *mm = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + 2048);
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, mm, -FLT_MAX);
-    hipLaunchKernelGGL(k_minmax, dim3(grid_for(g)), dim3(256), 0, ctx->stream, d_mask, d_mask, g.w, g.h, g.brw, g.brh,
-                       mm);
+    SRED_TRY(minmax_dev(ctx, g, d_mask, d_mask, mm));
     SRED_HIP_CHECK(hipGetLastError());
     unsigned int flag = 0;
     SRED_HIP_CHECK(hipMemcpyAsync(&flag, mm + 2, sizeof flag, hipMemcpyDeviceToHost, ctx->stream));
