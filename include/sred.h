@@ -63,11 +63,18 @@ int sred_ctx_synchronize(sred_ctx *ctx);
 /* Column-chunk override for the global contraction (0 = automatic). */
 int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols);
 /* Tuning options of the global contraction (no counterpart in the
- * reference): SRED_OPT_CHUNK_COLS (as above) and SRED_OPT_SYMMETRIC (1, the
+ * reference): SRED_OPT_CHUNK_COLS (as above); SRED_OPT_SYMMETRIC (1, the
  * default: the split-BF16 contraction evaluates each unordered block pair
- * once and adds it to both rows; 0: every ordered pair). */
+ * once and adds it to both rows; 0: every ordered pair); for batched calls
+ * SRED_OPT_AUTOTUNE (1, the default: time the panel sizes of a new class;
+ * 0: the largest that fits), SRED_OPT_BATCH_PANEL (force a panel size: a
+ * multiple of 256 up to 2048; 0 = off) and SRED_OPT_BATCH_MEMORY_MB (partial
+ * sums per wave, default 16384). */
 #define SRED_OPT_CHUNK_COLS 1
 #define SRED_OPT_SYMMETRIC 2
+#define SRED_OPT_AUTOTUNE 3
+#define SRED_OPT_BATCH_PANEL 4
+#define SRED_OPT_BATCH_MEMORY_MB 5
 int sred_ctx_set_option(sred_ctx *ctx, int option, int value);
 
 /* Metric from the reference's dialog string; block != 0 uses the block
@@ -141,6 +148,28 @@ int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float
 int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl,
                           int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
                           int normalize, float *d_out);
+
+/* ---- batched global repetition (no counterpart in the reference: SReD
+ * runs one image per command and loops stacks in Java) ---- */
+typedef struct sred_global_problem {
+    const float *d_img; /* device, w*h floats, input already normalised */
+    int w, h, bw, bh;
+    float *d_out;       /* device, w*h floats: the map of sred_global_map_2d_dev */
+} sred_global_problem;
+/* The maps of n images of any sizes and block sizes, each as
+ * sred_global_map_2d_dev computes it with the split-BF16 contraction (same
+ * relevance constant and normalisation for all).  The front steps run per
+ * image; the contractions of all images run as persistent kernels (one per
+ * block-size class and memory wave) inside ONE HIP graph launch together with
+ * every image's reduction, finalize and normalisation.  Panel sizes come from
+ * an autotuner (timed on the first batch holding a (metric, K_pad) class and
+ * cached in the context) unless SRED_OPT_BATCH_PANEL forces one.  Metrics:
+ * Pearson, SSIM, NRMSE (K <= 256).  Synchronises. */
+int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
+                         float relevance_constant, int normalize);
+/* Panel size the autotuner chose on this context for (metric, k_steps =
+ * ceil(K/32)), 0 if none yet. */
+int sred_ctx_tuned_panel(sred_ctx *ctx, int metric, int k_steps, int *panel);
 
 /* ---- Hu template metric (kernelGetPatchHu2D.cl:11-282; the reference has
  * the kernel but no Java caller, so these entries define its host side) ---- */

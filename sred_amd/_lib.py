@@ -22,7 +22,7 @@ SRED_ERR_NO_DEVICE = -6
 
 PEARSON, SSIM, NRMSE, ABSDIFF = 0, 1, 2, 3
 FP32, BF16X3 = 0, 1
-OPT_CHUNK_COLS, OPT_SYMMETRIC = 1, 2
+OPT_CHUNK_COLS, OPT_SYMMETRIC, OPT_AUTOTUNE, OPT_BATCH_PANEL, OPT_BATCH_MEMORY_MB = 1, 2, 3, 4, 5
 
 # every symbol include/sred.h declares
 EXPORTS = [
@@ -33,6 +33,7 @@ EXPORTS = [
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
+    "sred_global_batch_2d", "sred_ctx_tuned_panel",
 ]
 
 
@@ -50,6 +51,13 @@ class NonBinaryMaskError(SredError, ValueError):
 
 _lib = None
 _F = ctypes.POINTER(ctypes.c_float)
+
+
+class GlobalProblem(ctypes.Structure):
+    """sred_global_problem: one image of a batched global call (device pointers)."""
+
+    _fields_ = [("d_img", ctypes.c_void_p), ("w", ctypes.c_int), ("h", ctypes.c_int), ("bw", ctypes.c_int),
+                ("bh", ctypes.c_int), ("d_out", ctypes.c_void_p)]
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _FL = ctypes.c_float
@@ -104,6 +112,8 @@ def load():
         "sred_block_hu_2d": ([_VP, _F, _I, _I, _F, _I, _I, _F], _I),
         "sred_block_hu_2d_dev": ([_VP, _VP, _I, _I, _F, _I, _I, _VP], _I),
         "sred_hu_prepare_template_2d": ([_F, _I, _I, _F], _I),
+        "sred_global_batch_2d": ([_VP, _I, _I, ctypes.POINTER(GlobalProblem), _FL, _I], _I),
+        "sred_ctx_tuned_panel": ([_VP, _I, _I, ctypes.POINTER(_I)], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -181,6 +191,12 @@ class Context:
 
     def set_option(self, option, value):
         check(self._lib.sred_ctx_set_option(self.handle, int(option), int(value)))
+
+    def tuned_panel(self, metric, k_steps):
+        """Panel size the autotuner chose for (metric, ceil(K/32)), 0 if none yet."""
+        p = ctypes.c_int()
+        check(self._lib.sred_ctx_tuned_panel(self.handle, int(metric), int(k_steps), ctypes.byref(p)))
+        return p.value
 
     def last_contraction(self):
         """(ms, launches, pairs_evaluated) of the last global contraction."""

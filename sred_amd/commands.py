@@ -68,3 +68,13 @@ def block_map_device(ctx, metric_id, d_img, width, height, reference_block, rele
                                            L.fptr(L.f32(reference_block.pixels)), reference_block.width,
                                            reference_block.height, reference_block.mean, reference_block.std,
                                            relevance_constant, int(bool(normalize_output)), d_out))
+
+
+def global_batch_device(ctx, metric_id, problems, relevance_constant=0.0, normalize_output=True):
+    """sred_global_batch_2d: problems is a list of (d_img, width, height, block_width, block_height, d_out)
+    device-pointer tuples; every map is written into its d_out."""
+    arr = (L.GlobalProblem * len(problems))()
+    for i, (d_img, w, h, bw, bh, d_out) in enumerate(problems):
+        arr[i] = L.GlobalProblem(d_img, w, h, bw, bh, d_out)
+    L.check(L.load().sred_global_batch_2d(ctx.handle, metric_id, len(problems), arr, relevance_constant,
+                                          int(bool(normalize_output))))

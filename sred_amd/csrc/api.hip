@@ -254,8 +254,10 @@ int sred_ctx_destroy(sred_ctx *ctx)
     DevBuf *bufs[] = {&ctx->img, &ctx->means, &ctx->stds, &ctx->mask, &ctx->S, &ctx->W, &ctx->out,
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
-                      &ctx->wmom, &ctx->wpart};
+                      &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab};
     for (DevBuf *b : bufs) b->release();
+    for (BatchStore *b : ctx->batch_store) delete b;
+    ctx->batch_store.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     delete ctx;
@@ -299,6 +301,17 @@ int sred_ctx_set_option(sred_ctx *ctx, int option, int value)
         return SRED_OK;
     case SRED_OPT_SYMMETRIC:
         ctx->symmetric = value != 0;
+        return SRED_OK;
+    case SRED_OPT_AUTOTUNE:
+        ctx->autotune = value != 0;
+        return SRED_OK;
+    case SRED_OPT_BATCH_PANEL:
+        if (value < 0 || value > 2048 || value % 256 != 0) break;
+        ctx->batch_panel = value;
+        return SRED_OK;
+    case SRED_OPT_BATCH_MEMORY_MB:
+        if (value < 1) break;
+        ctx->batch_mb = value;
         return SRED_OK;
     default:
         break;
@@ -690,6 +703,30 @@ int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const 
         return SRED_ERR_ARG;
     }
     return block_hu_dev(ctx, g, d_img, tmpl, d_out);
+}
+
+int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
+                         float relevance_constant, int normalize)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (n < 0 || (n > 0 && !problems)) {
+        set_error("invalid problem list");
+        return SRED_ERR_ARG;
+    }
+    if (n == 0) return SRED_OK;
+    return global_batch_dev(ctx, metric, n, problems, relevance_constant, normalize);
+}
+
+int sred_ctx_tuned_panel(sred_ctx *ctx, int metric, int k_steps, int *panel)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!panel) {
+        set_error("null output");
+        return SRED_ERR_ARG;
+    }
+    auto f = ctx->tuned_panel.find(std::make_pair(metric, k_steps));
+    *panel = f == ctx->tuned_panel.end() ? 0 : f->second;
+    return SRED_OK;
 }
 
 int sred_hu_prepare_template_2d(const float *blk, int bw, int bh, float *tmpl_out)

@@ -1,0 +1,317 @@
+// batch.hip -- batched global repetition: many images of uneven sizes at
+// several block sizes in one HIP graph launch.
+//
+// No counterpart in the reference (SReD runs one image per command; stacks
+// loop in Java, BlockRepetition2D_.java:95-137).  Per image the pipeline is
+// that of sred_global_map_2d_dev (GlobalRepetition2D_.java:124-145) with the
+// symmetric split-BF16 contraction:
+//
+//   front   (per image, host-driven, small synchronisations): statistics,
+//           noise variance, relevance mask and nPixels, compaction, weight
+//           sums, per-block scalars and the gathered block matrix (padded to
+//           a multiple of 2048 rows so any panel size up to 2048 applies);
+//   graph   (one hipGraphLaunch): per block-size class (K_pad = 32*KS) and
+//           memory wave, ONE persistent contraction kernel whose workgroups
+//           walk the panel pairs of every image of the class (SymProblem
+//           table); then each image's FP64 reduction, finalize, mask and
+//           normalisation.
+//
+// Panel size per (metric, KS): SRED_OPT_BATCH_PANEL if set, else the
+// autotuner's choice (cached in the context): on the first batch holding a
+// class it times one persistent launch per candidate (256..2048, limited by
+// the partial-sum memory of the largest image) over a subset of the class's
+// images and keeps the fastest.  Waves bound the partial-sum memory
+// (npanels x Nv_pad floats per image) by SRED_OPT_BATCH_MEMORY_MB; the waves
+// run one after the other inside the graph and share one buffer.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "sred_internal.h"
+
+namespace sred {
+
+constexpr int kPadRows = 2048;
+constexpr int kPanels[] = {256, 512, 1024, 2048};
+
+BatchStore::~BatchStore() { release(); }
+void BatchStore::release()
+{
+    DevBuf *b[] = {&means, &stds, &mask, &S, &W, &counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2,
+                   &wmom, &wpart};
+    for (DevBuf *x : b) x->release();
+    offs2_key[0] = offs2_key[1] = offs2_key[2] = -1;
+}
+GlobalBufs BatchStore::bufs()
+{
+    return GlobalBufs{&counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2, offs2_key, &wmom, &wpart};
+}
+
+namespace {
+
+struct Item {
+    Geometry g;
+    const float *d_img = nullptr;
+    float *d_out = nullptr;
+    BatchStore *st = nullptr;
+    GlobalPrepared r;
+    float n_pixels = 0.0f;
+    // with the class panel size
+    int Nv_pad = 0, npanels = 0;
+    int64_t part_off = 0;  // floats into the wave's partial buffer
+};
+
+int64_t partial_floats(int Nv, int P)
+{
+    const int64_t Nv_pad = (int64_t)(Nv + P - 1) / P * P;
+    return Nv_pad / P * Nv_pad;
+}
+
+struct Launch {
+    int KS, P, nprob;
+    int64_t tab_off, nitems;
+    std::vector<int> items;  // indices into the item list
+};
+
+// SymProblem entries of `ids` with panel P, partial offsets from part_off
+// (updated), appended to tab.
+int64_t fill_table(std::vector<Item> &it, const std::vector<int> &ids, int P, float *d_part,
+                   std::vector<SymProblem> &tab)
+{
+    int64_t item0 = 0, off = 0;
+    for (int i : ids) {
+        Item &x = it[i];
+        x.Nv_pad = (x.r.Nv + P - 1) / P * P;
+        x.npanels = x.Nv_pad / P;
+        x.part_off = off;
+        tab.push_back(SymProblem{x.r.uhi, x.r.ulo, x.r.d_aux, x.r.d_tsm, d_part + off, x.Nv_pad, x.npanels, item0});
+        item0 += (int64_t)x.npanels * (x.npanels + 1) / 2;
+        off += (int64_t)x.npanels * x.Nv_pad;
+    }
+    return item0;
+}
+
+// Panel size of a class (see the file comment).
+int class_panel(sred_ctx *ctx, int metric, int KS, std::vector<Item> &it, const std::vector<int> &ids, int *P_out)
+{
+    if (ctx->batch_panel > 0) {
+        *P_out = ctx->batch_panel;
+        return SRED_OK;
+    }
+    const auto key = std::make_pair(metric, KS);
+    auto f = ctx->tuned_panel.find(key);
+    if (f != ctx->tuned_panel.end()) {
+        *P_out = f->second;
+        return SRED_OK;
+    }
+    int max_nv = 0;
+    for (int i : ids) max_nv = std::max(max_nv, it[i].r.Nv);
+    const int64_t budget = (int64_t)ctx->batch_mb << 18;  // floats
+    std::vector<int> cand;
+    for (int P : kPanels)
+        if (partial_floats(max_nv, P) <= budget) cand.push_back(P);
+    if (cand.empty()) cand.push_back(2048);
+    if (!ctx->autotune || cand.size() == 1) {
+        *P_out = cand.back();
+        ctx->tuned_panel[key] = *P_out;
+        return SRED_OK;
+    }
+    // tuning subset: images in order until ~1e11 pairs, within the budget at the smallest candidate
+    std::vector<int> sub;
+    double pairs = 0.0;
+    int64_t mem = 0;
+    for (int i : ids) {
+        const int64_t m = partial_floats(it[i].r.Nv, cand.front());
+        if (!sub.empty() && (pairs >= 1e11 || mem + m > budget)) break;
+        sub.push_back(i);
+        pairs += (double)it[i].r.Nv * it[i].r.Nv;
+        mem += m;
+    }
+    SRED_TRY(ctx->batch_part.ensure(sizeof(float) * (size_t)std::max<int64_t>(mem, 1)));
+    const EpiConst ec = epi_const();
+    float best_ms = 1e30f;
+    int best = cand.back();
+    for (int P : cand) {
+        std::vector<SymProblem> tab;
+        const int64_t nitems = fill_table(it, sub, P, ctx->batch_part.as<float>(), tab);
+        if ((int64_t)tab.size() * (int64_t)sizeof(SymProblem) > 0) {
+            SRED_TRY(ctx->batch_tab.ensure(sizeof(SymProblem) * tab.size()));
+            SRED_HIP_CHECK(hipMemcpy(ctx->batch_tab.ptr, tab.data(), sizeof(SymProblem) * tab.size(),
+                                     hipMemcpyHostToDevice));
+        }
+        SRED_TRY(time_begin(ctx));
+        SRED_TRY(launch_sym_metric(metric, KS, nitems, ctx->stream, ctx->batch_tab.as<SymProblem>(), (int)tab.size(),
+                                   P, ec));
+        SRED_TRY(time_end(ctx, nitems));
+        float ms = 0.0f;
+        SRED_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+        SRED_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = P;
+        }
+    }
+    ctx->tuned_panel[key] = best;
+    *P_out = best;
+    return SRED_OK;
+}
+
+}  // namespace
+
+int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem *probs, float c, int normalize)
+{
+    if (metric < 0 || metric > 2) {
+        set_error("the batched global map takes Pearson, SSIM or NRMSE");
+        return metric == SRED_ABSDIFF ? SRED_ERR_ARG : SRED_ERR_UNKNOWN_METRIC;
+    }
+    std::vector<Item> it(n);
+    while ((int)ctx->batch_store.size() < n) ctx->batch_store.push_back(new BatchStore());
+    // ---- front, per image
+    for (int i = 0; i < n; i++) {
+        Item &x = it[i];
+        SRED_TRY(make_geometry(probs[i].w, probs[i].h, probs[i].bw, probs[i].bh, &x.g));
+        if (!probs[i].d_img || !probs[i].d_out) {
+            set_error("null buffer in batch problem " + std::to_string(i));
+            return SRED_ERR_ARG;
+        }
+        if ((x.g.K + 31) / 32 > 8) {
+            set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+            return SRED_ERR_ARG;
+        }
+        x.d_img = probs[i].d_img;
+        x.d_out = probs[i].d_out;
+        x.st = ctx->batch_store[i];
+        const int64_t np = x.g.n();
+        BatchStore &st = *x.st;
+        SRED_TRY(st.means.ensure(sizeof(float) * np));
+        SRED_TRY(st.stds.ensure(sizeof(float) * np));
+        SRED_TRY(st.mask.ensure(sizeof(float) * np));
+        SRED_TRY(st.S.ensure(sizeof(float) * np));
+        SRED_TRY(st.W.ensure(sizeof(float) * np));
+        SRED_TRY(launch_local_stats(ctx, x.g, x.d_img, st.means.as<float>(), st.stds.as<float>()));
+        float nv;
+        SRED_TRY(noise_var_dev(ctx, x.d_img, x.g.w, x.g.h, &nv));
+        const float thr = nv * c;
+        SRED_TRY(relevance_mask_dev(ctx, x.g, st.stds.as<float>(), thr, st.mask.as<float>(), &x.n_pixels));
+        GlobalParams p;
+        p.metric = metric;
+        p.precision = SRED_BF16X3;
+        p.filter_param = nv;
+        p.threshold = thr;
+        SRED_TRY(global_prepare(ctx, st.bufs(), x.g, p, x.d_img, st.means.as<float>(), st.stds.as<float>(),
+                                st.S.as<float>(), st.W.as<float>(), 1, 0, kPadRows, &x.r));
+    }
+    // ---- classes, panel sizes, waves
+    std::map<int, std::vector<int>> cls;  // KS -> images with Nv > 0
+    for (int i = 0; i < n; i++)
+        if (it[i].r.Nv > 0) cls[it[i].r.pl.KS].push_back(i);
+    const int64_t budget = (int64_t)ctx->batch_mb << 18;
+    std::vector<Launch> launches;
+    int64_t max_wave = 1;
+    for (auto &kv : cls) {
+        int P;
+        SRED_TRY(class_panel(ctx, metric, kv.first, it, kv.second, &P));
+        std::vector<int> wave;
+        int64_t mem = 0;
+        auto flush = [&]() {
+            if (wave.empty()) return;
+            launches.push_back(Launch{kv.first, P, (int)wave.size(), 0, 0, wave});
+            max_wave = std::max(max_wave, mem);
+            wave.clear();
+            mem = 0;
+        };
+        for (int i : kv.second) {
+            const int64_t m = partial_floats(it[i].r.Nv, P);
+            if (!wave.empty() && mem + m > budget) flush();
+            wave.push_back(i);
+            mem += m;
+        }
+        flush();
+    }
+    // every wave's partial sums share one buffer; the tables point into it
+    SRED_TRY(ctx->batch_part.ensure(sizeof(float) * (size_t)max_wave));
+    float *d_part = ctx->batch_part.as<float>();
+    std::vector<SymProblem> tab;
+    for (Launch &L : launches) {
+        L.tab_off = (int64_t)tab.size();
+        L.nitems = fill_table(it, L.items, L.P, d_part, tab);
+    }
+    if (!tab.empty()) {
+        SRED_TRY(ctx->batch_tab.ensure(sizeof(SymProblem) * tab.size()));
+        SRED_HIP_CHECK(hipMemcpy(ctx->batch_tab.ptr, tab.data(), sizeof(SymProblem) * tab.size(),
+                                 hipMemcpyHostToDevice));
+    }
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));  // normalisation scratch, before capture
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+
+    // ---- one graph: contractions, reductions, finalize
+    hipStream_t cs = nullptr;
+    SRED_HIP_CHECK(hipStreamCreate(&cs));
+    hipStream_t saved = ctx->stream;
+    ctx->stream = cs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = SRED_OK;
+    auto fail = [&](int code) {
+        if (rc == SRED_OK) rc = code;
+    };
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        set_error("hipStreamBeginCapture failed");
+        fail(SRED_ERR_HIP);
+    }
+    const EpiConst ec = epi_const();
+    int64_t pairs = 0;
+    for (const Launch &L : launches) {
+        if (rc) break;
+        int e = launch_sym_metric(metric, L.KS, L.nitems, cs, ctx->batch_tab.as<SymProblem>() + L.tab_off, L.nprob,
+                                  L.P, ec);
+        if (e) { fail(e); break; }
+        pairs += L.nitems * (int64_t)L.P * L.P;
+        for (int i : L.items) {
+            Item &x = it[i];
+            e = global_reduce_dev(ctx, x.r, metric, d_part + x.part_off, x.npanels, x.Nv_pad, x.st->stds.as<float>(),
+                                  x.st->S.as<float>());
+            if (!e)
+                e = global_finalize_dev(ctx, x.g, metric, x.st->S.as<float>(), x.st->W.as<float>(),
+                                        x.st->mask.as<float>(), x.n_pixels, c, normalize, x.d_out);
+            if (e) { fail(e); break; }
+        }
+    }
+    for (int i = 0; i < n && !rc; i++)
+        if (it[i].r.Nv == 0) {
+            Item &x = it[i];
+            int e = global_finalize_dev(ctx, x.g, metric, x.st->S.as<float>(), x.st->W.as<float>(),
+                                        x.st->mask.as<float>(), x.n_pixels, c, normalize, x.d_out);
+            if (e) fail(e);
+        }
+    hipError_t ce = hipStreamEndCapture(cs, &graph);
+    if (ce != hipSuccess && !rc) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        fail(SRED_ERR_HIP);
+    }
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_error("hipGraphInstantiate failed");
+        fail(SRED_ERR_HIP);
+    }
+    if (!rc) {
+        int e = time_begin(ctx);
+        if (!e && hipGraphLaunch(exec, cs) != hipSuccess) {
+            set_error("hipGraphLaunch failed");
+            e = SRED_ERR_HIP;
+        }
+        if (!e) e = time_end(ctx, pairs);
+        if (!e && hipStreamSynchronize(cs) != hipSuccess) {
+            set_error("batch graph execution failed");
+            e = SRED_ERR_HIP;
+        }
+        if (e) fail(e);
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    ctx->stream = saved;
+    (void)hipStreamDestroy(cs);
+    return rc;
+}
+
+}  // namespace sred

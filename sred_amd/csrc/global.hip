@@ -57,12 +57,7 @@ constexpr float kPadT = 1e30f;  // t of padding columns: their weight is exp2(-i
 constexpr float kFastSigma = 0.018f;
 constexpr float kLn2f = 0.693147180559945309f;
 
-struct EpiConst {
-    float eps;  // 1e-7
-    float c1;   // SSIM 1e-4
-    float c2;   // SSIM 9e-4
-    float pk;   // Pearson: eps * log2(e)
-};
+
 
 // ---------------------------------------------------------------------------
 // 1. compaction (deterministic, row-major order preserved)
@@ -607,6 +602,12 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
 // rows, across the 4 k-groups by shuffles, across the 4 waves through an LDS
 // staging buffer folded into an LDS accumulator for the panel's P columns.
 //
+// The kernel is persistent: workgroup g takes work items g, g + gridDim, ...
+// of a list that may span several problems (SymProblem, one per image of a
+// batch); an item is one panel pair of one problem.  Items of one problem are
+// contiguous and chunk-major, so the workgroups resident together still share
+// a column panel.
+//
 // The workgroup walks its rectangle as P/BM sweeps of P/64 column tiles
 // (step T = tile T % ntiles of sweep T / ntiles).  With double buffering the
 // copy of tile T+1 is issued before tile T is computed, into the other of two
@@ -615,12 +616,8 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
 // publishes tile T and retires every read of the buffer the next copy reuses.
 // ---------------------------------------------------------------------------
 template <int M, int KS>
-__global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__restrict__ Uhi,
-                                                              const uint4 *__restrict__ Ulo,
-                                                              const float4 *__restrict__ aux,
-                                                              const int *__restrict__ tile_small, int P,
-                                                              int npanels, float *__restrict__ partial, int Nv_pad,
-                                                              EpiConst ec)
+__global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *__restrict__ probs, int nprob,
+                                                              long long nitems, int P, EpiConst ec)
 {
     constexpr int RG = rg_sym(M, KS);
     constexpr int BM = 64 * RG;
@@ -633,17 +630,33 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
     __shared__ float4 stage[2][4][64];  // [parity][wave][lane]: column-group sums
     extern __shared__ float colacc[];  // P entries
 
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    // the problem holding this item: the last one whose first item is <= item
+    int qlo = 0, qhi = nprob - 1;
+    while (qlo < qhi) {
+        const int mid = (qlo + qhi + 1) >> 1;
+        if (probs[mid].item0 <= item) qlo = mid;
+        else qhi = mid - 1;
+    }
+    const uint4 *__restrict__ Uhi = probs[qlo].uhi;
+    const uint4 *__restrict__ Ulo = probs[qlo].ulo;
+    const float4 *__restrict__ aux = probs[qlo].aux;
+    const int *__restrict__ tile_small = probs[qlo].tsm;
+    float *__restrict__ partial = probs[qlo].partial;
+    const int Nv_pad = probs[qlo].Nv_pad;
     // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
-    const int q = blockIdx.x;
+    const int q = (int)(item - probs[qlo].item0);
     int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
     while ((c + 1) * (c + 2) / 2 <= q) c++;
     while (c * (c + 1) / 2 > q) c--;
     const int p = q - c * (c + 1) / 2;
-    if (c >= npanels) return;  // grid larger than the panel-pair count: nothing to do
     const bool diag = (p == c);
 
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // the previous item's last LDS reads (column sums) and copies are done
+    wait_vm_all();
+    __syncthreads();
     if (!diag)
         for (int i = tid; i < P; i += 256) colacc[i] = 0.0f;
 
@@ -761,6 +774,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const uint4 *__res
         __syncthreads();
         for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
     }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -938,32 +952,51 @@ static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint
                        part, Nv_pad, ec);
 }
 
+// Persistent launch over nitems panel pairs of the problems in d_probs: as
+// many workgroups as can be resident at once (never more than the items).
 template <int M, int KS>
-static void launch_sym_ks(int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo, const float4 *aux,
-                          const int *tsm, int P, float *part, int Nv_pad, EpiConst ec)
+static int launch_sym_ks(long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob, int P,
+                         EpiConst ec)
 {
-    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, hi, lo, aux, tsm, P, Nv_pad / P,
-                       part, Nv_pad, ec);
+    int dev = 0, cus = 0, per_cu = 0;
+    SRED_HIP_CHECK(hipGetDevice(&dev));
+    SRED_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    SRED_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_contract_mfma_sym<M, KS>, 256, lds));
+    const long long slots = (long long)std::max(per_cu, 1) * cus;
+    const int grid = (int)std::min<long long>(nitems, slots);
+    if (grid <= 0) return SRED_OK;
+    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems, P, ec);
+    return SRED_OK;
 }
 
 template <int M>
-static int launch_sym(int KS, int grid, size_t lds, hipStream_t st, const uint4 *hi, const uint4 *lo,
-                      const float4 *aux, const int *tsm, int P, float *part, int Nv_pad, EpiConst ec)
+static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob,
+                      int P, EpiConst ec)
 {
     switch (KS) {
-    case 1: launch_sym_ks<M, 1>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 2: launch_sym_ks<M, 2>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 3: launch_sym_ks<M, 3>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 4: launch_sym_ks<M, 4>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 5: launch_sym_ks<M, 5>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 6: launch_sym_ks<M, 6>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 7: launch_sym_ks<M, 7>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
-    case 8: launch_sym_ks<M, 8>(grid, lds, st, hi, lo, aux, tsm, P, part, Nv_pad, ec); break;
+    case 1: return launch_sym_ks<M, 1>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 2: return launch_sym_ks<M, 2>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 3: return launch_sym_ks<M, 3>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 4: return launch_sym_ks<M, 4>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 5: return launch_sym_ks<M, 5>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, P, ec);
     default:
         set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
         return SRED_ERR_ARG;
     }
-    return SRED_OK;
+}
+
+int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
+                      int P, EpiConst ec)
+{
+    const size_t lds = sizeof(float) * (size_t)P;
+    if (metric == SRED_PEARSON) return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, P, ec);
+    if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, P, ec);
+    if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, P, ec);
+    set_error("the symmetric contraction takes Pearson, SSIM or NRMSE");
+    return SRED_ERR_ARG;
 }
 
 template <int M>
@@ -1040,6 +1073,12 @@ int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes)
 // Launch geometry of the contraction for Nv valid blocks of K pixels.
 int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, ContractionPlan *pl)
 {
+    return plan_contraction_p(Nv, K, metric, precision, symmetric, chunk_cols, 0, pl);
+}
+
+int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, int panel,
+                       ContractionPlan *pl)
+{
     ContractionPlan q{};
     if (Nv < 0 || K < 2 || metric < 0 || metric > 3) {
         set_error("invalid contraction plan arguments");
@@ -1056,6 +1095,13 @@ int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, in
     // symmetric panels: P rows = P columns, a multiple of 256, up to 2048,
     // small enough for at least ~2000 panel pairs
     q.P = q.sym ? std::min(2048, std::max(256, ((Nv + 63) / 64 + 255) / 256 * 256)) : 0;
+    if (q.sym && panel > 0) {
+        if (panel % 256 != 0 || panel > 2048) {
+            set_error("panel size must be a multiple of 256, at most 2048");
+            return SRED_ERR_ARG;
+        }
+        q.P = panel;
+    }
     q.Nv_pad = q.sym ? (Nv + q.P - 1) / q.P * q.P : (Nv + 255) / 256 * 256;
     int bytes_per_col;
     if (q.sym) { q.bm = 64 * rg_sym(metric, q.KS); bytes_per_col = q.KS * 32 * 4; }
@@ -1083,32 +1129,47 @@ int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, in
     return SRED_OK;
 }
 
-int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
-                     const float *d_means, const float *d_stds, float *d_S, float *d_W)
+// Buffers of one global problem: the context's own for single calls, one set
+// per image for a batch.
+GlobalBufs ctx_global_bufs(sred_ctx *ctx)
+{
+    return GlobalBufs{&ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi, &ctx->ulo,
+                      &ctx->offs2, ctx->offs2_key, &ctx->wmom, &ctx->wpart};
+}
+
+// Steps 1-2 of a global problem: compaction (one synchronisation, for Nv),
+// weight sums into d_W (zero elsewhere, as d_S), the launch plan (panel size
+// `panel` for the symmetric contraction, 0 = automatic), per-block scalars,
+// small-sigma flags and the gathered block matrix.  pad_rows > 0 pads the
+// split-BF16 block matrix and scalars to a multiple of pad_rows rows, so any
+// panel size dividing it can be used afterwards.
+int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p, const float *d_img,
+                   const float *d_means, const float *d_stds, float *d_S, float *d_W, int symmetric, int panel,
+                   int pad_rows, GlobalPrepared *out)
 {
     const int metric = p.metric;
     if (metric < 0 || metric > 3) {
         set_error("unknown metric");
         return SRED_ERR_UNKNOWN_METRIC;
     }
+    GlobalPrepared r{};
     const double thr_d = java_float_literal(p.threshold);
     const double fp_d = java_float_literal(p.filter_param);
     const double eps_d = java_float_literal(kEps);
-    const double tscale_d = std::sqrt(1.4426950408889634 / (fp_d + eps_d));
-    const float tscale = (float)tscale_d;
+    r.tscale_d = std::sqrt(1.4426950408889634 / (fp_d + eps_d));
+    const float tscale = (float)r.tscale_d;
     const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(g.K - 1)));
     const float inv_sqrt_k = (float)(1.0 / std::sqrt((double)g.K));
-    EpiConst ec{kEps, 0.0001f, 0.0009f, (float)((double)kEps * 1.4426950408889634)};
 
     // 1. compaction
     const int64_t n = g.n();
     const int nblk = (int)((n + kCompactTile - 1) / kCompactTile);
-    SRED_TRY(ctx->counts.ensure(sizeof(int2) * (3 * (size_t)nblk + 4)));
-    int2 *d_counts = ctx->counts.as<int2>();
+    SRED_TRY(b.counts->ensure(sizeof(int2) * (3 * (size_t)nblk + 4)));
+    int2 *d_counts = b.counts->as<int2>();
     int2 *d_offsets = d_counts + nblk;                                   // nblk + 2 entries
     float *d_smax = reinterpret_cast<float *>(d_offsets + nblk + 2);  // nblk entries
-    SRED_TRY(ctx->valid_idx.ensure(sizeof(int) * (size_t)n + 256));
-    int *d_idx = ctx->valid_idx.as<int>();
+    SRED_TRY(b.valid_idx->ensure(sizeof(int) * (size_t)n + 256));
+    int *d_idx = b.valid_idx->as<int>();
     hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
                        g.brh, thr_d, d_counts, d_smax);
     hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, d_smax, nblk, d_offsets);
@@ -1121,90 +1182,124 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     const int Nv = tot[0].x;
     float sigma_max;
     std::memcpy(&sigma_max, &tot[1].x, sizeof sigma_max);
-    ctx->last_nv = Nv;
-    const float flat_count = (float)std::min(tot[0].y, 16777216);
+    r.Nv = Nv;
+    r.flat_count = (float)std::min(tot[0].y, 16777216);
+    r.d_idx = d_idx;
 
     SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));
     SRED_TRY(zero_dev(ctx, d_W, sizeof(float) * n));
-    ctx->last_ms = 0.0f;
-    ctx->last_launches = 0;
-    ctx->last_pairs = (int64_t)Nv * Nv;
-    if (Nv == 0) return SRED_OK;
+    if (Nv == 0) {
+        *out = r;
+        return SRED_OK;
+    }
     // weight sums (independent of the contraction)
-    SRED_TRY(weight_sums_dev(ctx, d_stds, d_idx, Nv, tscale_d, sigma_max, d_W));
-    ContractionPlan pl;
-    SRED_TRY(plan_contraction(Nv, g.K, metric, p.precision, ctx->symmetric, ctx->chunk_cols, &pl));
-    const bool mfma = pl.mfma, sym = pl.sym;
-    const int P = pl.P, Nv_pad = pl.Nv_pad;
+    SRED_TRY(weight_sums_dev(ctx, *b.wmom, *b.wpart, d_stds, d_idx, Nv, r.tscale_d, sigma_max, d_W));
+    SRED_TRY(plan_contraction_p(Nv, g.K, metric, p.precision, symmetric, ctx->chunk_cols, panel, &r.pl));
+    const ContractionPlan &pl = r.pl;
+    const bool mfma = pl.mfma;
+    const int Nv_rows = (pad_rows > 0 && mfma) ? std::max(pl.Nv_pad, (Nv + pad_rows - 1) / pad_rows * pad_rows)
+                                               : pl.Nv_pad;
+    r.Nv_rows = Nv_rows;
 
     // 2. offsets relative to the image and per-block scalars
-    SRED_TRY(upload_offsets(ctx, g, g.w, &ctx->offs2, ctx->offs2_key));
-    const int *d_offs = ctx->offs2.as<int>();
-    SRED_TRY(ctx->aux.ensure(sizeof(float4) * (size_t)Nv_pad));
-    float4 *d_aux = ctx->aux.as<float4>();
-    hipLaunchKernelGGL(k_aux, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
-                       d_idx, Nv, Nv_pad, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, mfma ? 1 : 0, d_aux);
-    const int ngroups = Nv_pad / 32;
-    SRED_TRY(ctx->tflags.ensure(sizeof(int) * (size_t)ngroups));
-    const int *d_tsm = ctx->tflags.as<int>();
+    SRED_TRY(upload_offsets(ctx, g, g.w, b.offs2, b.offs2_key));
+    const int *d_offs = b.offs2->as<int>();
+    SRED_TRY(b.aux->ensure(sizeof(float4) * (size_t)Nv_rows));
+    float4 *d_aux = b.aux->as<float4>();
+    hipLaunchKernelGGL(k_aux, dim3((Nv_rows + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
+                       d_idx, Nv, Nv_rows, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, mfma ? 1 : 0, d_aux);
+    const int ngroups = Nv_rows / 32;
+    SRED_TRY(b.tflags->ensure(sizeof(int) * (size_t)std::max(ngroups, 1)));
     hipLaunchKernelGGL(k_tile_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_aux, ngroups,
-                       ctx->tflags.as<int>());
+                       b.tflags->as<int>());
+    r.d_aux = d_aux;
+    r.d_tsm = b.tflags->as<int>();
+    if (mfma) {
+        size_t u4 = (size_t)Nv_rows * pl.KS * 4;
+        SRED_TRY(b.uhi->ensure(sizeof(uint4) * u4));
+        SRED_TRY(b.ulo->ensure(sizeof(uint4) * u4));
+        hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                           d_means, d_stds, d_idx, Nv, Nv_rows, g.K, pl.KS, d_offs, inv_sqrt_km1, inv_sqrt_k,
+                           b.uhi->as<uint4>(), b.ulo->as<uint4>());
+        r.uhi = b.uhi->as<uint4>();
+        r.ulo = b.ulo->as<uint4>();
+    } else if (metric != SRED_ABSDIFF) {
+        size_t nu = (size_t)pl.Nv_pad * pl.K4;
+        SRED_TRY(b.u32->ensure(sizeof(float) * nu));
+        hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                           d_means, d_stds, d_idx, Nv, pl.Nv_pad, g.K, pl.K4, d_offs, inv_sqrt_km1, inv_sqrt_k,
+                           b.u32->as<float>());
+        r.u32 = b.u32->as<float>();
+    }
+    SRED_HIP_CHECK(hipGetLastError());
+    *out = r;
+    return SRED_OK;
+}
 
-    const int KS = pl.KS, K4 = pl.K4;
+// Step 4: FP64 sum of a problem's partials (nchunks slots of stride Nv_pad)
+// into the S map; flat rows of Pearson/SSIM get the flat-column count.
+int global_reduce_dev(sred_ctx *ctx, const GlobalPrepared &r, int metric, const float *d_part, int nchunks,
+                      int Nv_pad, const float *d_stds, float *d_S)
+{
+    if (r.Nv == 0) return SRED_OK;
+    const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)(float)r.tscale_d : 1.0;
+    hipLaunchKernelGGL(k_reduce, dim3((r.Nv + 255) / 256), dim3(256), 0, ctx->stream, d_part, nchunks, r.Nv, Nv_pad,
+                       r.d_idx, d_stds, metric, r.flat_count, s_scale, d_S);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
+EpiConst epi_const() { return EpiConst{kEps, 0.0001f, 0.0009f, (float)((double)kEps * 1.4426950408889634)}; }
+
+int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
+                     const float *d_means, const float *d_stds, float *d_S, float *d_W)
+{
+    const int metric = p.metric;
+    GlobalPrepared r;
+    ctx->last_ms = 0.0f;
+    ctx->last_launches = 0;
+    SRED_TRY(global_prepare(ctx, ctx_global_bufs(ctx), g, p, d_img, d_means, d_stds, d_S, d_W, ctx->symmetric, 0, 0,
+                            &r));
+    ctx->last_nv = r.Nv;
+    ctx->last_pairs = (int64_t)r.Nv * r.Nv;
+    if (r.Nv == 0) return SRED_OK;
+    const ContractionPlan &pl = r.pl;
+    const int KS = pl.KS, K4 = pl.K4, P = pl.P, Nv_pad = pl.Nv_pad;
     const int nrowtiles = pl.nrowtiles, cpc = pl.cols_per_chunk, nchunks = pl.nchunks;
     const int grid = (int)pl.grid;
+    const EpiConst ec = epi_const();
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)pl.partial_elems));
     float *d_part = ctx->partial.as<float>();
 
-    if (mfma) {
-        size_t u4 = (size_t)Nv_pad * KS * 4;
-        SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
-        SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
-        hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
-                           d_means, d_stds, d_idx, Nv, Nv_pad, g.K, KS, d_offs, inv_sqrt_km1, inv_sqrt_k,
-                           ctx->uhi.as<uint4>(), ctx->ulo.as<uint4>());
-    } else if (metric != SRED_ABSDIFF) {
-        size_t nu = (size_t)Nv_pad * K4;
-        SRED_TRY(ctx->u32.ensure(sizeof(float) * nu));
-        hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
-                           d_means, d_stds, d_idx, Nv, Nv_pad, g.K, K4, d_offs, inv_sqrt_km1, inv_sqrt_k,
-                           ctx->u32.as<float>());
-    }
-    SRED_HIP_CHECK(hipGetLastError());
-
     // 3. contraction (timed with events on the launch stream)
     SRED_TRY(time_begin(ctx));
-    if (sym) {
-        const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
-        const size_t lds = sizeof(float) * (size_t)P;
-        int rc;
-        if (metric == SRED_PEARSON)
-            rc = launch_sym<SRED_PEARSON>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
-        else if (metric == SRED_SSIM)
-            rc = launch_sym<SRED_SSIM>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
-        else
-            rc = launch_sym<SRED_NRMSE>(KS, grid, lds, ctx->stream, hi, lo, d_aux, d_tsm, P, d_part, Nv_pad, ec);
-        SRED_TRY(rc);
-    } else if (mfma) {
-        const uint4 *hi = ctx->uhi.as<uint4>(), *lo = ctx->ulo.as<uint4>();
+    if (pl.sym) {
+        // one problem: its table entry lives in the context (the copy reads it
+        // before this call returns: the stream was synchronised above)
+        ctx->host_symtab.assign(1, SymProblem{r.uhi, r.ulo, r.d_aux, r.d_tsm, d_part, Nv_pad, Nv_pad / P, 0});
+        SRED_TRY(ctx->symtab.ensure(sizeof(SymProblem)));
+        SRED_HIP_CHECK(hipMemcpyAsync(ctx->symtab.ptr, ctx->host_symtab.data(), sizeof(SymProblem),
+                                      hipMemcpyHostToDevice, ctx->stream));
+        SRED_TRY(launch_sym_metric(metric, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, P, ec));
+    } else if (pl.mfma) {
         int rc = SRED_OK;
         switch (metric) {
         case SRED_PEARSON:
-            rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad,
-                                           d_part, Nv_pad, ec);
+            rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc,
+                                           Nv_pad, d_part, Nv_pad, ec);
             break;
         case SRED_SSIM:
-            rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad, d_part,
-                                        Nv_pad, ec);
+            rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc, Nv_pad,
+                                        d_part, Nv_pad, ec);
             break;
         default:
-            rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, hi, lo, d_aux, d_tsm, nrowtiles, cpc, Nv_pad,
-                                         d_part, Nv_pad, ec);
+            rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc,
+                                         Nv_pad, d_part, Nv_pad, ec);
             break;
         }
         SRED_TRY(rc);
     } else if (metric == SRED_ABSDIFF) {
-        hipLaunchKernelGGL(k_contract_absdiff, dim3(grid), dim3(256), 0, ctx->stream, d_aux, Nv_pad, nrowtiles, cpc,
+        hipLaunchKernelGGL(k_contract_absdiff, dim3(grid), dim3(256), 0, ctx->stream, r.d_aux, Nv_pad, nrowtiles, cpc,
                            d_part, ec);
     } else {
         size_t lds = sizeof(float) * 2 * (size_t)K4 * 64;
@@ -1212,23 +1307,21 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
             set_error("block too large for the FP32 contraction tile");
             return SRED_ERR_ARG;
         }
-        const float *UT = ctx->u32.as<float>();
         if (metric == SRED_PEARSON)
-            launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc,
+                                     d_part, ec);
         else if (metric == SRED_SSIM)
-            launch_f32<SRED_SSIM>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_SSIM>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part,
+                                  ec);
         else
-            launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, UT, d_aux, d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part, ec);
+            launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc,
+                                   d_part, ec);
     }
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_end(ctx, pl.pairs_evaluated));
 
     // 4. reduce
-    const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)tscale : 1.0;
-    hipLaunchKernelGGL(k_reduce, dim3((Nv + 255) / 256), dim3(256), 0, ctx->stream, d_part, nchunks, Nv, Nv_pad, d_idx,
-                       d_stds, metric, flat_count, s_scale, d_S);
-    SRED_HIP_CHECK(hipGetLastError());
-    return SRED_OK;
+    return global_reduce_dev(ctx, r, metric, d_part, nchunks, Nv_pad, d_stds, d_S);
 }
 
 int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_S, const float *d_W,

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/sred.h"
@@ -59,6 +61,38 @@ double java_float_literal(float f);
 
 }  // namespace sred
 
+namespace sred {
+// Epilogue constants of the global contractions.
+struct EpiConst {
+    float eps;  // 1e-7
+    float c1;   // SSIM 1e-4
+    float c2;   // SSIM 9e-4
+    float pk;   // Pearson: eps * log2(e)
+};
+// One problem of a symmetric split-BF16 contraction launch (a persistent
+// launch may carry several): its fragment-major block matrix, per-block
+// constants and small-sigma flags, and its partial-sum array [npanels][Nv_pad].
+struct SymProblem {
+    const uint4 *uhi, *ulo;
+    const float4 *aux;
+    const int *tsm;
+    float *partial;
+    int Nv_pad, npanels;
+    long long item0;  // index of this problem's first panel pair in the launch's work list
+};
+}  // namespace sred
+
+namespace sred {
+// Per-image scratch of a batched global call (owned by the context).
+struct BatchStore {
+    DevBuf means, stds, mask, S, W, counts, valid_idx, aux, tflags, u32, uhi, ulo, offs2, wmom, wpart;
+    int offs2_key[3] = {-1, -1, -1};
+    ~BatchStore();
+    void release();
+    struct GlobalBufs bufs();
+};
+}  // namespace sred
+
 struct sred_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -68,6 +102,15 @@ struct sred_ctx {
     sred::DevBuf img, means, stds, mask, S, W, out, dmap, grad;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     sred::DevBuf wmom, wpart;  // weight-sum moments
+    sred::DevBuf symtab;       // problem table of the symmetric contraction
+    std::vector<sred::SymProblem> host_symtab;
+    // batched global calls
+    std::vector<sred::BatchStore *> batch_store;
+    sred::DevBuf batch_part, batch_tab;
+    std::map<std::pair<int, int>, int> tuned_panel;  // (metric, KS) -> panel size
+    int autotune = 1;
+    int batch_panel = 0;       // forced panel size (0: autotuner)
+    int64_t batch_mb = 16384;  // partial-sum memory per wave
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
     // measurement of the last global contraction
@@ -106,6 +149,9 @@ struct ContractionPlan {
     int64_t grid, partial_elems, pairs_evaluated;
 };
 int plan_contraction(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, ContractionPlan *pl);
+// Same with the symmetric panel size forced (panel > 0: multiple of 256, <= 2048).
+int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, int chunk_cols, int panel,
+                       ContractionPlan *pl);
 
 struct GlobalParams {
     int metric = SRED_PEARSON;
@@ -113,14 +159,44 @@ struct GlobalParams {
     float filter_param = 0.0f;  // noise variance
     float threshold = 0.0f;     // noise variance * relevance constant
 };
+// Scratch of one global problem (pointers to buffers owned by a context).
+struct GlobalBufs {
+    DevBuf *counts, *valid_idx, *aux, *tflags, *u32, *uhi, *ulo, *offs2;
+    int *offs2_key;
+    DevBuf *wmom, *wpart;
+};
+GlobalBufs ctx_global_bufs(sred_ctx *ctx);
+// A prepared global problem (see global_prepare).
+struct GlobalPrepared {
+    int Nv = 0, Nv_rows = 0;
+    float flat_count = 0.0f;
+    double tscale_d = 0.0;
+    ContractionPlan pl{};
+    const int *d_idx = nullptr;
+    const float4 *d_aux = nullptr;
+    const int *d_tsm = nullptr;
+    const uint4 *uhi = nullptr, *ulo = nullptr;
+    const float *u32 = nullptr;
+};
+int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p, const float *d_img,
+                   const float *d_means, const float *d_stds, float *d_S, float *d_W, int symmetric, int panel,
+                   int pad_rows, GlobalPrepared *out);
+int global_reduce_dev(sred_ctx *ctx, const GlobalPrepared &r, int metric, const float *d_part, int nchunks,
+                      int Nv_pad, const float *d_stds, float *d_S);
+EpiConst epi_const();
 // Accumulators S, Wsum (float maps, w*h) of the global kernels.
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W);
 // Weight sums W (float map entries idx[v], v < Nv) of the global kernels:
 // sum over valid j of exp2(-(t_i - t_j)^2), t = sigma * tscale, by a binned
 // Hermite expansion in FP64; sigma_max bounds the valid blocks' sigmas.
-int weight_sums_dev(sred_ctx *ctx, const float *d_stds, const int *d_idx, int Nv, double tscale, float sigma_max,
-                    float *d_W);
+int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_stds, const int *d_idx, int Nv,
+                    double tscale, float sigma_max, float *d_W);
+// Persistent symmetric split-BF16 contraction over nitems panel pairs of the
+// nprob problems in d_probs (device table), all with K_pad = 32*KS and panel
+// size P (metric Pearson, SSIM or NRMSE).
+int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
+                      int P, EpiConst ec);
 // FP32 gather of the valid blocks, k-major (UT[k][v]).
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
                        const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,
@@ -136,6 +212,8 @@ int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const floa
 // means/stds are written when both output pointers are non-NULL.
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
                   float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out);
+// Batched global maps (see batch.hip).
+int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem *probs, float c, int normalize);
 // Hu template map (kernelGetPatchHu2D.cl) of a device image; h_tmpl is the
 // host template rectangle.  Synchronises.
 int block_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *h_tmpl, float *d_out);

@@ -107,8 +107,8 @@ __global__ __launch_bounds__(256) void k_weval(const float *__restrict__ stds, c
 
 // W_map[idx[v]] for the Nv valid blocks; sigma_max bounds their standard
 // deviations (it sizes the bin table).
-int weight_sums_dev(sred_ctx *ctx, const float *d_stds, const int *d_idx, int Nv, double tscale, float sigma_max,
-                    float *d_W)
+int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_stds, const int *d_idx, int Nv,
+                    double tscale, float sigma_max, float *d_W)
 {
     if (Nv <= 0) return SRED_OK;
     const double tmax = (double)sigma_max * tscale;
@@ -117,10 +117,10 @@ int weight_sums_dev(sred_ctx *ctx, const float *d_stds, const int *d_idx, int Nv
         return SRED_ERR_ARG;
     }
     const int nbins = (int)std::floor(tmax / kBinW + 0.5) + 1;
-    SRED_TRY(ctx->wmom.ensure(sizeof(double) * (size_t)nbins * kMoments));
-    SRED_TRY(ctx->wpart.ensure(sizeof(unsigned long long) * (size_t)kMomentGroups * kChunkBins * kMoments));
-    double *d_M = ctx->wmom.as<double>();
-    unsigned long long *d_part = ctx->wpart.as<unsigned long long>();
+    SRED_TRY(wmom.ensure(sizeof(double) * (size_t)nbins * kMoments));
+    SRED_TRY(wpart.ensure(sizeof(unsigned long long) * (size_t)kMomentGroups * kChunkBins * kMoments));
+    double *d_M = wmom.as<double>();
+    unsigned long long *d_part = wpart.as<unsigned long long>();
     const int groups = std::min(kMomentGroups, (Nv + 255) / 256);
     for (int bin0 = 0; bin0 < nbins; bin0 += kChunkBins) {
         const int nb = std::min(kChunkBins, nbins - bin0);

@@ -1,0 +1,96 @@
+"""Batched global repetition (sred_global_batch_2d): images of uneven sizes
+and block sizes in one graph launch, against the single-image path and the
+FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import sred_amd
+from sred_amd import _lib as L
+from sred_amd import synthetic
+from sred_amd.commands import global_batch_device, global_map_device
+
+pytestmark = pytest.mark.gpu
+MAP_TOL = 1e-4
+
+SHAPES = [((61, 72), 5), ((97, 150), 9), ((64, 64), 5), ((80, 70), 7), ((120, 100), 13), ((40, 45), 3),
+          ((130, 90), 9)]
+
+
+def images(seed0=20):
+    return [O.normalize_input(synthetic.spots(s[1], s[0], 25, seed=seed0 + i)) for i, (s, _) in enumerate(SHAPES)]
+
+
+def run_batch(ctx, metric, imgs, c=0.0):
+    torch = pytest.importorskip("torch")
+    d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+    d_outs = [torch.full_like(d, 7.0) for d in d_imgs]
+    probs = [(d.data_ptr(), im.shape[1], im.shape[0], b, b, o.data_ptr())
+             for d, im, o, (_, b) in zip(d_imgs, imgs, d_outs, SHAPES)]
+    global_batch_device(ctx, metric, probs, c, True)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in d_outs]
+
+
+def run_single(ctx, metric, im, b, c=0.0):
+    torch = pytest.importorskip("torch")
+    d = torch.from_numpy(im).cuda()
+    o = torch.zeros_like(d)
+    global_map_device(ctx, metric, L.BF16X3, d.data_ptr(), im.shape[1], im.shape[0], b, b, c, True, o.data_ptr())
+    torch.cuda.synchronize()
+    return o.cpu().numpy()
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_batch_matches_single_and_oracle(metric):
+    imgs = images()
+    with sred_amd.Context(0) as ctx:
+        outs = run_batch(ctx, metric, imgs)
+        for (shape, b), im, out in zip(SHAPES, imgs, outs):
+            single = run_single(ctx, metric, im, b)
+            assert np.abs(out - single).max() <= 2e-6, (shape, b)
+            ref = O.global_map(metric, im, b, b, 0.0, True, fp64=True)
+            assert np.abs(out - ref).max() <= MAP_TOL, (shape, b)
+        # every class got a panel size from the autotuner
+        for ks in {(sred_amd.block_size_2d(b, b) + 31) // 32 for _, b in SHAPES}:
+            assert ctx.tuned_panel(metric, ks) in (256, 512, 1024, 2048)
+
+
+def test_batch_forced_panel_is_the_single_path_bit_for_bit():
+    im = images()[1]
+    b = SHAPES[1][1]
+    with sred_amd.Context(0) as ctx:
+        nv = (im.shape[0] - b + 1) * (im.shape[1] - b + 1)
+        P = L.global_plan(nv, sred_amd.block_size_2d(b, b), 0, L.BF16X3)["rows"]
+        ctx.set_option(L.OPT_BATCH_PANEL, P)
+        torch = pytest.importorskip("torch")
+        d = torch.from_numpy(im).cuda()
+        o = torch.zeros_like(d)
+        global_batch_device(ctx, 0, [(d.data_ptr(), im.shape[1], im.shape[0], b, b, o.data_ptr())])
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(o.cpu().numpy(), run_single(ctx, 0, im, b))
+
+
+def test_batch_waves_repeat_and_relevance():
+    imgs = images(40)
+    with sred_amd.Context(0) as ctx:
+        ctx.set_option(L.OPT_BATCH_PANEL, 256)
+        one = run_batch(ctx, 0, imgs, c=0.5)
+        ctx.set_option(L.OPT_BATCH_MEMORY_MB, 1)  # several memory waves inside the graph
+        waves = run_batch(ctx, 0, imgs, c=0.5)
+        again = run_batch(ctx, 0, imgs, c=0.5)
+        for a, b_, c_ in zip(one, waves, again):
+            np.testing.assert_array_equal(a, b_)
+            np.testing.assert_array_equal(b_, c_)
+        for (shape, b), im, out in zip(SHAPES, imgs, one):
+            ref = O.global_map(0, im, b, b, 0.5, True, fp64=True)
+            assert np.abs(out - ref).max() <= MAP_TOL, (shape, b)
+
+
+def test_batch_errors():
+    with sred_amd.Context(0) as ctx:
+        with pytest.raises(sred_amd.SredError):
+            global_batch_device(ctx, 3, [(0, 10, 10, 3, 3, 0)])  # AbsDiff is not batched
+        with pytest.raises(sred_amd.SredError):
+            ctx.set_option(L.OPT_BATCH_PANEL, 300)
+        global_batch_device(ctx, 0, [])
