@@ -92,6 +92,22 @@ int64_t fill_table(std::vector<Item> &it, const std::vector<int> &ids, int P, fl
     return item0;
 }
 
+// The table in ctx->batch_tab, followed by the launches' item counter.
+size_t counter_offset(size_t entries) { return (sizeof(SymProblem) * entries + 255) / 256 * 256; }
+unsigned long long *batch_counter(sred_ctx *ctx)
+{
+    return reinterpret_cast<unsigned long long *>(ctx->batch_tab.as<char>() + ctx->batch_ctr_off);
+}
+int upload_table(sred_ctx *ctx, const std::vector<SymProblem> &tab)
+{
+    ctx->batch_ctr_off = counter_offset(tab.size());
+    SRED_TRY(ctx->batch_tab.ensure(ctx->batch_ctr_off + 256));
+    if (!tab.empty())
+        SRED_HIP_CHECK(hipMemcpy(ctx->batch_tab.ptr, tab.data(), sizeof(SymProblem) * tab.size(),
+                                 hipMemcpyHostToDevice));
+    return SRED_OK;
+}
+
 // Panel size of a class (see the file comment).
 int class_panel(sred_ctx *ctx, int metric, int KS, std::vector<Item> &it, const std::vector<int> &ids, int *P_out)
 {
@@ -135,14 +151,10 @@ int class_panel(sred_ctx *ctx, int metric, int KS, std::vector<Item> &it, const 
     for (int P : cand) {
         std::vector<SymProblem> tab;
         const int64_t nitems = fill_table(it, sub, P, ctx->batch_part.as<float>(), tab);
-        if ((int64_t)tab.size() * (int64_t)sizeof(SymProblem) > 0) {
-            SRED_TRY(ctx->batch_tab.ensure(sizeof(SymProblem) * tab.size()));
-            SRED_HIP_CHECK(hipMemcpy(ctx->batch_tab.ptr, tab.data(), sizeof(SymProblem) * tab.size(),
-                                     hipMemcpyHostToDevice));
-        }
+        SRED_TRY(upload_table(ctx, tab));
         SRED_TRY(time_begin(ctx));
         SRED_TRY(launch_sym_metric(metric, KS, nitems, ctx->stream, ctx->batch_tab.as<SymProblem>(), (int)tab.size(),
-                                   P, ec));
+                                   batch_counter(ctx), P, ec));
         SRED_TRY(time_end(ctx, nitems));
         float ms = 0.0f;
         SRED_HIP_CHECK(hipEventSynchronize(ctx->ev1));
@@ -237,11 +249,7 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
         L.tab_off = (int64_t)tab.size();
         L.nitems = fill_table(it, L.items, L.P, d_part, tab);
     }
-    if (!tab.empty()) {
-        SRED_TRY(ctx->batch_tab.ensure(sizeof(SymProblem) * tab.size()));
-        SRED_HIP_CHECK(hipMemcpy(ctx->batch_tab.ptr, tab.data(), sizeof(SymProblem) * tab.size(),
-                                 hipMemcpyHostToDevice));
-    }
+    SRED_TRY(upload_table(ctx, tab));
     SRED_TRY(ctx->misc.ensure(kMiscBytes));  // normalisation scratch, before capture
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
 
@@ -265,7 +273,7 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
     for (const Launch &L : launches) {
         if (rc) break;
         int e = launch_sym_metric(metric, L.KS, L.nitems, cs, ctx->batch_tab.as<SymProblem>() + L.tab_off, L.nprob,
-                                  L.P, ec);
+                                  batch_counter(ctx), L.P, ec);
         if (e) { fail(e); break; }
         pairs += L.nitems * (int64_t)L.P * L.P;
         for (int i : L.items) {

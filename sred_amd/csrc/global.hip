@@ -59,6 +59,13 @@ constexpr float kLn2f = 0.693147180559945309f;
 
 
 
+// Zero device words with a kernel (a graph can hold it, unlike a memset node).
+__global__ void k_zero(unsigned int *p, int64_t nwords)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // 1. compaction (deterministic, row-major order preserved)
 // ---------------------------------------------------------------------------
@@ -397,7 +404,7 @@ constexpr int kBN = 64;
 constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
 // the symmetric kernel also carries column sums and a second accumulator set
 // (software pipelining): 2 row groups
-constexpr int rg_sym(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 5 : 6) ? 2 : 1; }
+constexpr int rg_sym(int metric, int ks) { return ks <= 5 ? 2 : 1; }
 
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
@@ -617,7 +624,8 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
 // ---------------------------------------------------------------------------
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *__restrict__ probs, int nprob,
-                                                              long long nitems, int P, EpiConst ec)
+                                                              long long nitems, unsigned long long *__restrict__ next,
+                                                              int P, EpiConst ec)
 {
     constexpr int RG = rg_sym(M, KS);
     constexpr int BM = 64 * RG;
@@ -632,7 +640,17 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (long long item = blockIdx.x; item < nitems; item += gridDim.x) {
+    __shared__ long long s_item;
+  for (;;) {
+    // next item from the launch's counter, in order (as the hardware would
+    // dispatch one workgroup per item); every thread has read the previous
+    // s_item and every LDS access of the previous item is done
+    wait_vm_all();
+    __syncthreads();
+    if (tid == 0) s_item = (long long)atomicAdd(next, 1ull);
+    __syncthreads();
+    const long long item = s_item;
+    if (item >= nitems) break;
     // the problem holding this item: the last one whose first item is <= item
     int qlo = 0, qhi = nprob - 1;
     while (qlo < qhi) {
@@ -654,9 +672,6 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     const int p = q - c * (c + 1) / 2;
     const bool diag = (p == c);
 
-    // the previous item's last LDS reads (column sums) and copies are done
-    wait_vm_all();
-    __syncthreads();
     if (!diag)
         for (int i = tid; i < P; i += 256) colacc[i] = 0.0f;
 
@@ -952,36 +967,38 @@ static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint
                        part, Nv_pad, ec);
 }
 
-// Persistent launch over nitems panel pairs of the problems in d_probs: as
-// many workgroups as can be resident at once (never more than the items).
+// Persistent launch over nitems panel pairs of the problems in d_probs:
+// a few workgroups per CU claim items in order from a counter (zeroed here
+// by a kernel, so the launch can sit in a graph); workgroups beyond the
+// resident ones find the counter exhausted and exit.
 template <int M, int KS>
-static int launch_sym_ks(long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob, int P,
-                         EpiConst ec)
+static int launch_sym_ks(long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob,
+                         unsigned long long *d_next, int P, EpiConst ec)
 {
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0;
     SRED_HIP_CHECK(hipGetDevice(&dev));
     SRED_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    SRED_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_contract_mfma_sym<M, KS>, 256, lds));
-    const long long slots = (long long)std::max(per_cu, 1) * cus;
-    const int grid = (int)std::min<long long>(nitems, slots);
+    const int grid = (int)std::min<long long>(nitems, 8LL * std::max(cus, 1));
     if (grid <= 0) return SRED_OK;
-    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems, P, ec);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned int *>(d_next), (int64_t)2);
+    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems, d_next,
+                       P, ec);
     return SRED_OK;
 }
 
 template <int M>
 static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      int P, EpiConst ec)
+                      unsigned long long *d_next, int P, EpiConst ec)
 {
     switch (KS) {
-    case 1: return launch_sym_ks<M, 1>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 2: return launch_sym_ks<M, 2>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 3: return launch_sym_ks<M, 3>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 4: return launch_sym_ks<M, 4>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 5: return launch_sym_ks<M, 5>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, P, ec);
-    case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, P, ec);
+    case 1: return launch_sym_ks<M, 1>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 2: return launch_sym_ks<M, 2>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 3: return launch_sym_ks<M, 3>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 4: return launch_sym_ks<M, 4>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 5: return launch_sym_ks<M, 5>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
     default:
         set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
         return SRED_ERR_ARG;
@@ -989,12 +1006,12 @@ static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, cons
 }
 
 int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      int P, EpiConst ec)
+                      unsigned long long *d_next, int P, EpiConst ec)
 {
     const size_t lds = sizeof(float) * (size_t)P;
-    if (metric == SRED_PEARSON) return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, P, ec);
-    if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, P, ec);
-    if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, P, ec);
+    if (metric == SRED_PEARSON) return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
     set_error("the symmetric contraction takes Pearson, SSIM or NRMSE");
     return SRED_ERR_ARG;
 }
@@ -1043,12 +1060,6 @@ static int choose_chunk_cols(int Nv_pad, int bytes_per_col, int nrowtiles, int o
         cols = std::max(64, std::min(std::min(cols, cols_fill), max_cols));
     }
     return std::min(cols, Nv_pad);
-}
-
-__global__ void k_zero(unsigned int *p, int64_t nwords)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nwords; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0u;
 }
 
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
@@ -1277,10 +1288,12 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         // one problem: its table entry lives in the context (the copy reads it
         // before this call returns: the stream was synchronised above)
         ctx->host_symtab.assign(1, SymProblem{r.uhi, r.ulo, r.d_aux, r.d_tsm, d_part, Nv_pad, Nv_pad / P, 0});
-        SRED_TRY(ctx->symtab.ensure(sizeof(SymProblem)));
+        SRED_TRY(ctx->symtab.ensure(sizeof(SymProblem) + 256));
         SRED_HIP_CHECK(hipMemcpyAsync(ctx->symtab.ptr, ctx->host_symtab.data(), sizeof(SymProblem),
                                       hipMemcpyHostToDevice, ctx->stream));
-        SRED_TRY(launch_sym_metric(metric, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, P, ec));
+        unsigned long long *d_next =
+            reinterpret_cast<unsigned long long *>(ctx->symtab.as<char>() + (sizeof(SymProblem) + 127) / 128 * 128);
+        SRED_TRY(launch_sym_metric(metric, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, d_next, P, ec));
     } else if (pl.mfma) {
         int rc = SRED_OK;
         switch (metric) {

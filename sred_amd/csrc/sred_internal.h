@@ -106,7 +106,8 @@ struct sred_ctx {
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
     std::vector<sred::BatchStore *> batch_store;
-    sred::DevBuf batch_part, batch_tab;
+    sred::DevBuf batch_part, batch_tab;  // partial sums of a wave; problem tables + item counter
+    size_t batch_ctr_off = 0;
     std::map<std::pair<int, int>, int> tuned_panel;  // (metric, KS) -> panel size
     int autotune = 1;
     int batch_panel = 0;       // forced panel size (0: autotuner)
@@ -195,8 +196,9 @@ int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_s
 // Persistent symmetric split-BF16 contraction over nitems panel pairs of the
 // nprob problems in d_probs (device table), all with K_pad = 32*KS and panel
 // size P (metric Pearson, SSIM or NRMSE).
+// d_next: a 64-bit device counter the launch zeroes and its workgroups claim items from.
 int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      int P, EpiConst ec);
+                      unsigned long long *d_next, int P, EpiConst ec);
 // FP32 gather of the valid blocks, k-major (UT[k][v]).
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
                        const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,
