@@ -10,7 +10,10 @@ array (height, width).  Shapes and contents:
                                  background 0.1, Poisson noise at 200 photons peak;
 * lattice_and_blobs(1024, ...)   left half: ring lattice (pitch 24 px, rotated
                                  U[0,30] degrees); right half: ~900 blobs with
-                                 sigma U[1,4]; background 0.1 plus N(0, 0.05^2).
+                                 sigma U[1,4]; background 0.1 plus N(0, 0.05^2);
+* spot_ring_mixture(w, h, seed)  spots at the density of spots(256, 256, 200)
+                                 and rings at that of rings_and_blobs(2048, 3000),
+                                 background 0.1 plus N(0, 0.05^2) (any w x h).
 """
 import numpy as np
 
@@ -87,5 +90,20 @@ def lattice_and_blobs(size=1024, pitch=24.0, n_blobs=900, seed=2, background=0.1
         x, y = rng.uniform(0, size - half), rng.uniform(0, size)
         s = rng.uniform(1.0, 4.0)
         _splat(right, x, y, 4 * s, _gauss(s, rng.uniform(0.5, 1.0)))
+    img += rng.normal(0.0, noise, img.shape)
+    return img.astype(np.float32)
+
+
+def spot_ring_mixture(width, height, seed=3, background=0.1, noise=0.05):
+    rng = np.random.default_rng(seed)
+    img = np.full((height, width), background, np.float64)
+    n_spots = int(round(200.0 * width * height / (256 * 256)))
+    n_rings = int(round(3000.0 * width * height / (2048 * 2048)))
+    for _ in range(n_spots):
+        x, y = rng.uniform(0, width), rng.uniform(0, height)
+        _splat(img, x, y, 8.0, _gauss(2.0, rng.uniform(0.5, 1.0)))
+    for _ in range(n_rings):
+        x, y = rng.uniform(0, width), rng.uniform(0, height)
+        _splat(img, x, y, 10.0, _ring(5.0, 1.5, rng.uniform(0.5, 1.0)))
     img += rng.normal(0.0, noise, img.shape)
     return img.astype(np.float32)
