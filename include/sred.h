@@ -188,6 +188,41 @@ int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const 
  * block rectangle; cells outside the ellipse are 0.  Host only. */
 int sred_hu_prepare_template_2d(const float *blk, int bw, int bh, float *tmpl_out);
 
+/* ---- all-pairs vector maps (OpenCL kernels without a Java caller) ---- */
+/* Per-pixel Hu invariant hu = nu20 + nu02 of kernelGetHuMap.cl:146-172
+ * (getInvariant on the bw x bh rectangle, patch = pixel - means[p]), in the
+ * kernel's float order; border pixels 0.  means: local means of
+ * sred_local_stats_2d (host).  Synchronous. */
+int sred_hu_invariants_2d(sred_ctx *ctx, const float *img, const float *means, int w, int h, int bw, int bh,
+                          float *out);
+/* Global Hu map, kernelGetHuMap.cl:15-172: for every reference pixel i,
+ * sum over all interior pixels j of |hu_j - hu_i| * (100^sim - 1) / 99,
+ * sim = 1 - |s_i - s_j| / (|s_i| + |s_j| + 1e-7) (getExpDecayWeight,
+ * :136-144).  hu as sred_hu_invariants_2d (bit-exact), pair terms in FP32,
+ * row sums in FP64.  means/stds: sred_local_stats_2d outputs.  rows/nrows:
+ * the reference pixels y*w+x to compute (the kernel's uniqueStdCoords list,
+ * :27-39; border and out-of-range entries are skipped); NULL computes every
+ * interior pixel.  out (w*h) is 0 at pixels not computed (the kernel adds
+ * into a buffer the caller zeroes).  rows is a host array in both forms. */
+int sred_global_hu_2d(sred_ctx *ctx, const float *img, const float *means, const float *stds, int w, int h, int bw,
+                      int bh, const int32_t *rows, int64_t nrows, float *out);
+int sred_global_hu_2d_dev(sred_ctx *ctx, const float *d_img, const float *d_means, const float *d_stds, int w, int h,
+                          int bw, int bh, const int32_t *rows, int64_t nrows, float *d_out);
+/* Phase correlation map, kernelGetPhaseCorrelationMap.cl:13-179.  That
+ * kernel does not compile (nUnique undefined, :27) and its DFT uses integer
+ * phases (:66-67); this is textbook phase correlation on the same blocks,
+ * FP64: b = (pixel - mean) / (std + 1e-7) on the bw x bh rectangle, unit
+ * spectra U = F / (|F| + 1e-7), R = U_i conj(U_j), r = Re IDFT(R); the first
+ * strict positive maximum of r in raster order gives the displacement d,
+ * wrapped to [-bw/2, bw/2] x [-bh/2, bh/2] (0 if r has no positive value);
+ * map[i] = sum over interior j of |d_ij| * w_ij with the kernel's weight
+ * (:167-178): 1 if s_i == s_j, else 1 - 2 |s_i - s_j| / (|s_i| + |s_j|).
+ * Block sides odd, 3..11.  rows/out as sred_global_hu_2d. */
+int sred_phase_correlation_2d(sred_ctx *ctx, const float *img, const float *means, const float *stds, int w, int h,
+                              int bw, int bh, const int32_t *rows, int64_t nrows, float *out);
+int sred_phase_correlation_2d_dev(sred_ctx *ctx, const float *d_img, const float *d_means, const float *d_stds,
+                                  int w, int h, int bw, int bh, const int32_t *rows, int64_t nrows, float *d_out);
+
 /* ---- differentiation (no counterpart in the reference) ---- */
 /* Gradient of L = sum_i dL_dmap[i] * map[i] of the global Pearson map
  * (relevance constant 0, normalised output; the forward of

@@ -74,6 +74,21 @@ def lib():
         L.or_block_hu.restype = None
         L.or_hu_prepare_template.argtypes = [_F, c_int, c_int, c_double, _F]
         L.or_hu_prepare_template.restype = None
+        L.or_hu_invariant.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_float, c_double]
+        L.or_hu_invariant.restype = c_float
+        L.or_hu_invariants.argtypes = [_F, _F, c_int, c_int, c_int, c_int, c_double, _F]
+        L.or_hu_invariants.restype = None
+        L.or_hu_weight.argtypes = [c_float, c_float, c_double]
+        L.or_hu_weight.restype = c_double
+        L.or_global_hu_sums.argtypes = [_F, _F, c_int, c_int, c_int, c_int, c_double, vp, c_i64, _D, c_int]
+        L.or_global_hu_sums.restype = None
+        L.or_phase_peak_blocks.argtypes = [_D, _D, c_int, c_int, c_double, vp, ctypes.POINTER(c_int),
+                                           ctypes.POINTER(c_int)]
+        L.or_phase_peak_blocks.restype = c_int
+        L.or_phase_weight.argtypes = [c_float, c_float]
+        L.or_phase_weight.restype = c_double
+        L.or_phase_map.argtypes = [_F, _F, _F, c_int, c_int, c_int, c_int, c_double, vp, c_i64, _D, c_int]
+        L.or_phase_map.restype = None
         _lib = L
     return _lib
 
@@ -294,4 +309,65 @@ def hu_prepare_template(blk, eps=None):
     bh, bw = b.shape
     out = np.zeros_like(b)
     lib().or_hu_prepare_template(b.reshape(-1), bw, bh, hu_eps() if eps is None else eps, out.reshape(-1))
+    return out
+
+
+def hu_invariants(img, means, bw, bh, eps=None):
+    """hu = inv20 + inv02 of kernelGetHuMap.cl:146-172 at every interior pixel (border 0)."""
+    img, means = _f32(img), _f32(means)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    lib().or_hu_invariants(img.reshape(-1), means.reshape(-1), w, h, bw, bh, hu_eps() if eps is None else eps,
+                           out.reshape(-1))
+    return out
+
+
+def hu_weight(a, b, eps=None):
+    """getExpDecayWeight of kernelGetHuMap.cl:136-144 in FP64."""
+    return lib().or_hu_weight(float(a), float(b), hu_eps() if eps is None else eps)
+
+
+def global_hu_sums(hu, stds, bw, bh, rows=None, eps=None, nthreads=0):
+    """FP64 global Hu map sums (kernelGetHuMap.cl:87-133) of given invariants."""
+    hu, stds = _f32(hu), _f32(stds)
+    h, w = hu.shape
+    out = np.zeros((h, w), np.float64)
+    rp, nr, _keep = _rows(rows)
+    lib().or_global_hu_sums(hu.reshape(-1), stds.reshape(-1), w, h, bw, bh, hu_eps() if eps is None else eps,
+                            rp, nr, out.reshape(-1), nthreads)
+    return out
+
+
+def global_hu_map(img, means, stds, bw, bh, rows=None, eps=None, nthreads=0):
+    """Global Hu map (kernelGetHuMap.cl) from local statistics, FP64 sums."""
+    return global_hu_sums(hu_invariants(img, means, bw, bh, eps), stds, bw, bh, rows, eps, nthreads)
+
+
+def phase_peak_blocks(bi, bj, eps=None):
+    """r = Re IDFT(normalised cross power) of two normalised blocks (bh, bw) and
+    the wrapped peak (dx, dy); found is False when no value of r is positive."""
+    bi = np.ascontiguousarray(bi, np.float64)
+    bj = np.ascontiguousarray(bj, np.float64)
+    bh, bw = bi.shape
+    r = np.zeros((bh, bw), np.float64)
+    dx, dy = ctypes.c_int(), ctypes.c_int()
+    f = lib().or_phase_peak_blocks(bi.reshape(-1), bj.reshape(-1), bw, bh, hu_eps() if eps is None else eps,
+                                   r.ctypes.data, ctypes.byref(dx), ctypes.byref(dy))
+    return r, (dx.value, dy.value), bool(f)
+
+
+def phase_weight(a, b):
+    """getExpDecayWeight of kernelGetPhaseCorrelationMap.cl:167-178 (float)."""
+    return lib().or_phase_weight(float(a), float(b))
+
+
+def phase_map(img, means, stds, bw, bh, rows=None, eps=None, nthreads=0):
+    """Phase correlation map (textbook specification of
+    kernelGetPhaseCorrelationMap.cl), FP64."""
+    img, means, stds = _f32(img), _f32(means), _f32(stds)
+    h, w = img.shape
+    out = np.zeros((h, w), np.float64)
+    rp, nr, _keep = _rows(rows)
+    lib().or_phase_map(img.reshape(-1), means.reshape(-1), stds.reshape(-1), w, h, bw, bh,
+                       hu_eps() if eps is None else eps, rp, nr, out.reshape(-1), nthreads)
     return out

@@ -782,3 +782,261 @@ void or_hu_prepare_template(const float *blk, int bw, int bh, double eps, float 
     free(p);
     free(in);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Global Hu map, kernelGetHuMap.cl:15-172                                   */
+/* ------------------------------------------------------------------------ */
+
+/* getInvariant(p, q) of kernelGetHuMap.cl:146-172 for (2, 0) and (0, 2) on
+ * the bw x bh rectangle around (x0, y0), patch = pixel - mean (:62-72), in
+ * the kernel's float order.  Both calls sum mu_00 identically.  x_avg is the
+ * float sum 1..bw over bw (:147-151); pow(d, 2) is d*d (d is an integer or
+ * half-integer, so exact), pow(d, 0) is 1; pow(mu_00, 1 + 2/2) = mu_00^2
+ * rounded to float; "+ EPSILON" promotes to double (the macro is an
+ * unsuffixed literal), and the quotient is rounded back to float (:169-171). */
+float or_hu_invariant(const float *img, int w, int bw, int bh, int x0, int y0, float mean, double eps)
+{
+    int brw = bw / 2, brh = bh / 2;
+    float xavg = 0.0f, yavg = 0.0f;
+    for (int i = 1; i <= bw; i++) xavg += (float)i;
+    xavg /= (float)bw;
+    for (int i = 1; i <= bh; i++) yavg += (float)i;
+    yavg /= (float)bh;
+    float mu20 = 0.0f, mu02 = 0.0f, mu00 = 0.0f;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) {
+            float v = img[(int64_t)(y0 - brh + j) * w + (x0 - brw + i)] - mean;
+            float dx = (float)i + 1.0f - xavg;
+            float dy = (float)j + 1.0f - yavg;
+            mu20 += ((dx * dx) * 1.0f) * v;
+            mu02 += (1.0f * (dy * dy)) * v;
+            mu00 += v;
+        }
+    float m2 = mu00 * mu00;
+    float inv20 = (float)((double)mu20 / ((double)m2 + eps));
+    float inv02 = (float)((double)mu02 / ((double)m2 + eps));
+    return inv20 + inv02;
+}
+
+void or_hu_invariants(const float *img, const float *means, int w, int h, int bw, int bh, double eps, float *hu)
+{
+    int brw = bw / 2, brh = bh / 2;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int64_t p = (int64_t)y * w + x;
+            if (x < brw || x >= w - brw || y < brh || y >= h - brh)
+                hu[p] = 0.0f;
+            else
+                hu[p] = or_hu_invariant(img, w, bw, bh, x, y, means[p], eps);
+        }
+}
+
+/* getExpDecayWeight of kernelGetHuMap.cl:136-144 in FP64:
+ * (100^(1 - |a-b| / (|a| + |b| + eps)) - 1) / 99. */
+double or_hu_weight(float a, float b, double eps)
+{
+    double sim = 1.0 - fabs((double)a - (double)b) / (fabs((double)a) + fabs((double)b) + eps);
+    return (pow(100.0, sim) - 1.0) / 99.0;
+}
+
+/* hu_map[i] = sum over interior j (raster order) of |hu_j - hu_i| w(s_i, s_j)
+ * (kernelGetHuMap.cl:87-133), FP64 with Neumaier compensation, for the
+ * interior reference pixels in rows (NULL: all interior pixels; the
+ * kernel's uniqueStdCoords list, :27-39).  Other pixels of out are 0. */
+void or_global_hu_sums(const float *hu, const float *stds, int w, int h, int bw, int bh, double eps,
+                       const int32_t *rows, int64_t nrows, double *out, int nthreads)
+{
+    int brw = bw / 2, brh = bh / 2;
+    int64_t n = (int64_t)w * h;
+    for (int64_t p = 0; p < n; p++) out[p] = 0.0;
+    int64_t R = rows ? nrows : n;
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t r = 0; r < R; r++) {
+        int64_t p = rows ? rows[r] : r;
+        if (p < 0 || p >= n) continue;
+        int x0 = (int)(p % w), y0 = (int)(p / w);
+        if (x0 < brw || x0 >= w - brw || y0 < brh || y0 >= h - brh) continue;
+        ksum s = {0.0, 0.0};
+        for (int y1 = brh; y1 < h - brh; y1++)
+            for (int x1 = brw; x1 < w - brw; x1++) {
+                int64_t q = (int64_t)y1 * w + x1;
+                kadd(&s, fabs((double)hu[q] - (double)hu[p]) * or_hu_weight(stds[p], stds[q], eps));
+            }
+        out[p] = s.s + s.c;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Phase correlation map, kernelGetPhaseCorrelationMap.cl:13-179             */
+/* ------------------------------------------------------------------------ */
+/* The reference kernel does not compile (nUnique undefined at :27) and its
+ * DFT phases use integer division (:66-67); the specification restated here
+ * is textbook phase correlation on the same blocks:
+ *   b(x, y)  = (pixel - mean) / (std + eps) on the bw x bh rectangle (:50-57);
+ *   F        = 2D DFT of b, F(k) = sum b(x) exp(-2 pi i (kx x / bw + ky y / bh));
+ *   U        = F / (|F| + eps), the block's unit spectrum (0 where F is 0);
+ *   R(k)     = U_i(k) conj(U_j(k)), the normalised cross power spectrum
+ *              (F_i conj F_j / |F_i conj F_j| wherever F_i, F_j >> eps);
+ *   r(x, y)  = Re IDFT(R) = (1/n) Re sum_k R(k) exp(+2 pi i (...));
+ *   peak     = first strict maximum of r over (y, x) in raster order, counted
+ *              only if positive (:145-157, "zero ... no correlation");
+ *   d        = peak coordinates wrapped to [-bw/2, bw/2] x [-bh/2, bh/2]
+ *              (a circular shift s of block j against block i peaks at -s);
+ *   term     = |d| * w(s_i, s_j), w the kernel's getExpDecayWeight
+ *              (:167-178): 1 if s_i == s_j, else 1 - 2 |s_i - s_j| / (|s_i| + |s_j|),
+ *              in float;
+ *   map[i]   = sum over interior j of term (:75-164), FP64.
+ * Angles are 2 pi q / n with the integer q = ((kx x mod bw) bh + (ky y mod bh)
+ * bw) mod n, n = bw bh, so the twiddles are a table of n cosines and sines. */
+
+#define OR_PI 3.14159265358979323846
+
+static void phase_twiddles(int bw, int bh, double *c, double *s)
+{
+    int n = bw * bh;
+    for (int q = 0; q < n; q++) {
+        c[q] = cos(2.0 * OR_PI * (double)q / (double)n);
+        s[q] = sin(2.0 * OR_PI * (double)q / (double)n);
+    }
+}
+
+static int phase_q(int kx, int ky, int x, int y, int bw, int bh)
+{
+    return (((kx * x) % bw) * bh + ((ky * y) % bh) * bw) % (bw * bh);
+}
+
+/* unit spectrum U = F / (|F| + eps) of a bw x bh real block (row-major),
+ * F its full 2D DFT summed in raster order */
+static void phase_dft(const double *b, int bw, int bh, double eps, const double *c, const double *s, double *re,
+                      double *im)
+{
+    for (int ky = 0; ky < bh; ky++)
+        for (int kx = 0; kx < bw; kx++) {
+            double a = 0.0, bi = 0.0;
+            for (int y = 0; y < bh; y++)
+                for (int x = 0; x < bw; x++) {
+                    int q = phase_q(kx, ky, x, y, bw, bh);
+                    a += b[y * bw + x] * c[q];
+                    bi -= b[y * bw + x] * s[q];
+                }
+            double inv = 1.0 / (sqrt(a * a + bi * bi) + eps);
+            re[ky * bw + kx] = a * inv;
+            im[ky * bw + kx] = bi * inv;
+        }
+}
+
+/* r of two unit spectra; returns the wrapped peak (dx, dy) and 1, or 0 when
+ * no value of r is positive (then dx = dy = 0).  r_out (n) may be NULL. */
+static int phase_peak(const double *ri, const double *ii, const double *rj, const double *ij, int bw, int bh,
+                      const double *c, const double *s, double *Rr, double *Ri, double *r_out, int *dx, int *dy)
+{
+    int n = bw * bh;
+    for (int k = 0; k < n; k++) {
+        Rr[k] = ri[k] * rj[k] + ii[k] * ij[k];   /* U_i conj(U_j) */
+        Ri[k] = ii[k] * rj[k] - ri[k] * ij[k];
+    }
+    double best = 0.0;
+    int px = 0, py = 0, found = 0;
+    for (int y = 0; y < bh; y++)
+        for (int x = 0; x < bw; x++) {
+            double v = 0.0;
+            for (int ky = 0; ky < bh; ky++)
+                for (int kx = 0; kx < bw; kx++) {
+                    int q = phase_q(kx, ky, x, y, bw, bh);
+                    v += Rr[ky * bw + kx] * c[q] - Ri[ky * bw + kx] * s[q];
+                }
+            v /= (double)n;
+            if (r_out) r_out[y * bw + x] = v;
+            if (v > best) {
+                best = v;
+                px = x;
+                py = y;
+                found = 1;
+            }
+        }
+    *dx = px <= bw / 2 ? px : px - bw;
+    *dy = py <= bh / 2 ? py : py - bh;
+    return found;
+}
+
+int or_phase_peak_blocks(const double *bi, const double *bj, int bw, int bh, double eps, double *r_out, int *dx,
+                         int *dy)
+{
+    int n = bw * bh;
+    double *buf = (double *)malloc(sizeof(double) * 8 * (size_t)n);
+    double *c = buf, *s = buf + n, *ri = buf + 2 * n, *ii = buf + 3 * n, *rj = buf + 4 * n, *ij = buf + 5 * n;
+    double *Rr = buf + 6 * n, *Ri = buf + 7 * n;
+    phase_twiddles(bw, bh, c, s);
+    phase_dft(bi, bw, bh, eps, c, s, ri, ii);
+    phase_dft(bj, bw, bh, eps, c, s, rj, ij);
+    int f = phase_peak(ri, ii, rj, ij, bw, bh, c, s, Rr, Ri, r_out, dx, dy);
+    free(buf);
+    return f;
+}
+
+double or_phase_weight(float a, float b)
+{
+    float wgt;
+    if (a == b)
+        wgt = 1.0f;
+    else
+        wgt = 1.0f - (2.0f * (fabsf(a - b) / (fabsf(a) + fabsf(b))));
+    return (double)wgt;
+}
+
+void or_phase_map(const float *img, const float *means, const float *stds, int w, int h, int bw, int bh,
+                  double eps, const int32_t *rows, int64_t nrows, double *out, int nthreads)
+{
+    int brw = bw / 2, brh = bh / 2, n = bw * bh;
+    int iw = w - 2 * brw, ih = h - 2 * brh;
+    int64_t npix = (int64_t)w * h;
+    for (int64_t p = 0; p < npix; p++) out[p] = 0.0;
+    if (iw <= 0 || ih <= 0) return;
+    int64_t ni = (int64_t)iw * ih;
+    double *c = (double *)malloc(sizeof(double) * n), *s = (double *)malloc(sizeof(double) * n);
+    phase_twiddles(bw, bh, c, s);
+    /* spectra of every interior block */
+    double *sre = (double *)malloc(sizeof(double) * n * ni), *sim = (double *)malloc(sizeof(double) * n * ni);
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        double *b = (double *)malloc(sizeof(double) * n);
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < ni; j++) {
+            int x0 = brw + (int)(j % iw), y0 = brh + (int)(j / iw);
+            int64_t p = (int64_t)y0 * w + x0;
+            for (int y = 0; y < bh; y++)
+                for (int x = 0; x < bw; x++)
+                    b[y * bw + x] = ((double)img[(int64_t)(y0 - brh + y) * w + (x0 - brw + x)] - (double)means[p]) /
+                                    ((double)stds[p] + eps);
+            phase_dft(b, bw, bh, eps, c, s, sre + j * n, sim + j * n);
+        }
+        free(b);
+    }
+    int64_t R = rows ? nrows : npix;
+#pragma omp parallel num_threads(nt)
+    {
+        double *Rr = (double *)malloc(sizeof(double) * n), *Ri = (double *)malloc(sizeof(double) * n);
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t r = 0; r < R; r++) {
+            int64_t p = rows ? rows[r] : r;
+            if (p < 0 || p >= npix) continue;
+            int x0 = (int)(p % w), y0 = (int)(p / w);
+            if (x0 < brw || x0 >= w - brw || y0 < brh || y0 >= h - brh) continue;
+            int64_t i = (int64_t)(y0 - brh) * iw + (x0 - brw);
+            double acc = 0.0;
+            for (int64_t j = 0; j < ni; j++) {
+                int dx, dy;
+                phase_peak(sre + i * n, sim + i * n, sre + j * n, sim + j * n, bw, bh, c, s, Rr, Ri, NULL, &dx, &dy);
+                int64_t q = (int64_t)(brh + j / iw) * w + (brw + j % iw);
+                acc += sqrt((double)(dx * dx + dy * dy)) * or_phase_weight(stds[p], stds[q]);
+            }
+            out[p] = acc;
+        }
+        free(Rr);
+        free(Ri);
+    }
+    free(sre); free(sim); free(c); free(s);
+}

@@ -132,6 +132,37 @@ void or_block_hu(const float *img, int w, int h, const float *tmpl, int bw, int 
  * float ellipse), in FP64, rounded to float; pixels outside the ellipse 0. */
 void or_hu_prepare_template(const float *blk, int bw, int bh, double eps, float *out);
 
+/* Global Hu map, kernelGetHuMap.cl:15-172 (no caller in the reference).
+ * or_hu_invariant: hu = inv20 + inv02 of getInvariant (:146-172) on the
+ * bw x bh rectangle around (x0, y0) with patch = pixel - mean, in the
+ * kernel's float order (see the definition for the promotions).
+ * or_hu_invariants: that value at every interior pixel (border 0).
+ * or_hu_weight: getExpDecayWeight (:136-144) in FP64.
+ * or_global_hu_sums: map[i] = sum over interior j of |hu_j - hu_i| w(s_i, s_j)
+ * (:87-133) in FP64 for the interior pixels of rows (NULL: all), others 0. */
+float or_hu_invariant(const float *img, int w, int bw, int bh, int x0, int y0, float mean, double eps);
+void or_hu_invariants(const float *img, const float *means, int w, int h, int bw, int bh, double eps, float *hu);
+double or_hu_weight(float a, float b, double eps);
+void or_global_hu_sums(const float *hu, const float *stds, int w, int h, int bw, int bh, double eps,
+                       const int32_t *rows, int64_t nrows, double *out, int nthreads);
+
+/* Phase correlation map, kernelGetPhaseCorrelationMap.cl:13-179.  The
+ * reference kernel does not compile; this is the textbook specification on
+ * the same blocks and weight (full statement in sred_oracle.c):
+ * normalised blocks (pixel - mean) / (std + eps) on the bw x bh rectangle,
+ * 2D DFTs F, unit spectra U = F / (|F| + eps), R = U_i conj(U_j),
+ * r = Re IDFT(R), the
+ * first positive strict maximum in raster order, its coordinates wrapped to
+ * signed displacements, map[i] = sum_j |d_ij| w(s_i, s_j) in FP64.
+ * or_phase_peak_blocks: r (n values, may be NULL) and the wrapped peak of two
+ * normalised blocks; returns 1, or 0 if no value of r is positive.
+ * or_phase_weight: getExpDecayWeight (:167-178), float, as double. */
+int or_phase_peak_blocks(const double *bi, const double *bj, int bw, int bh, double eps, double *r_out, int *dx,
+                         int *dy);
+double or_phase_weight(float a, float b);
+void or_phase_map(const float *img, const float *means, const float *stds, int w, int h, int bw, int bh,
+                  double eps, const int32_t *rows, int64_t nrows, double *out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

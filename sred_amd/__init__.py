@@ -8,7 +8,9 @@ from . import _lib
 from ._lib import (ABSDIFF, BF16X3, FP32, NRMSE, PEARSON, SSIM, Context, NonBinaryMaskError, SredError,
                    load)
 from .clutils import (BLOCK_METRICS, GLOBAL_METRICS, calculate_block_hu_map_2d, calculate_block_repetition_map_2d,
-                      calculate_global_repetition_map_2d, get_local_statistics_2d, get_opencl_resources)
+                      calculate_global_hu_map_2d, calculate_global_repetition_map_2d,
+                      calculate_phase_correlation_map_2d, get_hu_invariants_2d, get_local_statistics_2d,
+                      get_opencl_resources)
 from .commands import block_repetition_2d, global_repetition_2d
 from .utils import (InputImage2D, ReferenceBlock2D, RelevanceMask, apply_mask_2d, block_size_2d,
                     get_hu_template_2d, get_input_image_2d, get_mean_noise_var_2d, get_reference_block_2d, get_relevance_mask,
@@ -16,7 +18,9 @@ from .utils import (InputImage2D, ReferenceBlock2D, RelevanceMask, apply_mask_2d
 
 __all__ = [
     "ABSDIFF", "BF16X3", "FP32", "NRMSE", "PEARSON", "SSIM", "Context", "NonBinaryMaskError", "SredError", "load",
-    "BLOCK_METRICS", "GLOBAL_METRICS", "calculate_block_hu_map_2d", "calculate_block_repetition_map_2d", "calculate_global_repetition_map_2d",
+    "BLOCK_METRICS", "GLOBAL_METRICS", "calculate_block_hu_map_2d", "calculate_block_repetition_map_2d",
+    "calculate_global_hu_map_2d", "calculate_global_repetition_map_2d", "calculate_phase_correlation_map_2d",
+    "get_hu_invariants_2d",
     "get_local_statistics_2d", "get_opencl_resources", "block_repetition_2d", "global_repetition_2d",
     "InputImage2D", "ReferenceBlock2D", "RelevanceMask", "apply_mask_2d", "block_size_2d", "get_hu_template_2d",
     "get_input_image_2d",

@@ -33,7 +33,8 @@ EXPORTS = [
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
-    "sred_global_batch_2d", "sred_ctx_tuned_panel",
+    "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_hu_invariants_2d", "sred_global_hu_2d",
+    "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev",
 ]
 
 
@@ -114,6 +115,11 @@ def load():
         "sred_hu_prepare_template_2d": ([_F, _I, _I, _F], _I),
         "sred_global_batch_2d": ([_VP, _I, _I, ctypes.POINTER(GlobalProblem), _FL, _I], _I),
         "sred_ctx_tuned_panel": ([_VP, _I, _I, ctypes.POINTER(_I)], _I),
+        "sred_hu_invariants_2d": ([_VP, _F, _F, _I, _I, _I, _I, _F], _I),
+        "sred_global_hu_2d": ([_VP, _F, _F, _F, _I, _I, _I, _I, _VP, _I64, _F], _I),
+        "sred_global_hu_2d_dev": ([_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP, _I64, _VP], _I),
+        "sred_phase_correlation_2d": ([_VP, _F, _F, _F, _I, _I, _I, _I, _VP, _I64, _F], _I),
+        "sred_phase_correlation_2d_dev": ([_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP, _I64, _VP], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

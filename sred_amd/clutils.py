@@ -113,3 +113,54 @@ def calculate_block_hu_map_2d(input_image, template, resources):
     L.check(L.load().sred_block_hu_2d(resources.handle, L.fptr(img), input_image.width, input_image.height,
                                       L.fptr(t.reshape(-1)), bw, bh, L.fptr(out)))
     return out
+
+
+def _rows_arg(unique_coords):
+    if unique_coords is None:
+        return None, 0, None
+    r = np.ascontiguousarray(unique_coords, dtype=np.int32)
+    return r.ctypes.data, r.size, r
+
+
+def get_hu_invariants_2d(input_image, local_statistics, block_width, block_height, resources):
+    """Per-pixel Hu invariant nu20 + nu02 of kernelGetHuMap.cl:146-172 (border 0)."""
+    img = L.f32(input_image.image_array).reshape(-1)
+    out = np.zeros(input_image.size, np.float32)
+    L.check(L.load().sred_hu_invariants_2d(resources.handle, L.fptr(img), L.fptr(L.f32(local_statistics.local_means)),
+                                           input_image.width, input_image.height, block_width, block_height,
+                                           L.fptr(out)))
+    return out
+
+
+def calculate_global_hu_map_2d(input_image, local_statistics, block_width, block_height, resources,
+                               unique_coords=None):
+    """Global Hu map (kernelGetHuMap.cl:15-172; no Java caller in the reference):
+    sum over interior j of |hu_j - hu_i| * (100^sim - 1) / 99.
+
+    unique_coords: optional pixel indices y*w+x to compute (the kernel's
+    uniqueStdCoords list); other pixels are 0.  Returns float32 w*h.
+    """
+    img = L.f32(input_image.image_array).reshape(-1)
+    out = np.zeros(input_image.size, np.float32)
+    rp, nr, _keep = _rows_arg(unique_coords)
+    L.check(L.load().sred_global_hu_2d(resources.handle, L.fptr(img), L.fptr(L.f32(local_statistics.local_means)),
+                                       L.fptr(L.f32(local_statistics.local_stds)), input_image.width,
+                                       input_image.height, block_width, block_height, rp, nr, L.fptr(out)))
+    return out
+
+
+def calculate_phase_correlation_map_2d(input_image, local_statistics, block_width, block_height, resources,
+                                       unique_coords=None):
+    """Phase correlation map (kernelGetPhaseCorrelationMap.cl:13-179, which does
+    not compile in the reference; textbook phase correlation on its blocks):
+    sum over interior j of |peak displacement_ij| * w_ij.  Odd block sides
+    3..11.  unique_coords as calculate_global_hu_map_2d.  Returns float32 w*h.
+    """
+    img = L.f32(input_image.image_array).reshape(-1)
+    out = np.zeros(input_image.size, np.float32)
+    rp, nr, _keep = _rows_arg(unique_coords)
+    L.check(L.load().sred_phase_correlation_2d(
+        resources.handle, L.fptr(img), L.fptr(L.f32(local_statistics.local_means)),
+        L.fptr(L.f32(local_statistics.local_stds)), input_image.width, input_image.height, block_width, block_height,
+        rp, nr, L.fptr(out)))
+    return out

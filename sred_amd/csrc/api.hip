@@ -254,7 +254,8 @@ int sred_ctx_destroy(sred_ctx *ctx)
     DevBuf *bufs[] = {&ctx->img, &ctx->means, &ctx->stds, &ctx->mask, &ctx->S, &ctx->W, &ctx->out,
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
-                      &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab};
+                      &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
+                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -703,6 +704,78 @@ int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const 
         return SRED_ERR_ARG;
     }
     return block_hu_dev(ctx, g, d_img, tmpl, d_out);
+}
+
+// All-pairs vector maps: host-buffer entries upload img/means/stds, run the
+// device path and download the map.
+static int pair_map_host(sred_ctx *ctx, int which, const float *img, const float *means, const float *stds, int w,
+                         int h, int bw, int bh, const int32_t *rows, int64_t nrows, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!img || !means || (which != 2 && !stds) || !out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, g.n()));
+    SRED_TRY(upload(ctx, ctx->means, means, g.n()));
+    if (which != 2) SRED_TRY(upload(ctx, ctx->stds, stds, g.n()));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * g.n()));
+    float *d_out = ctx->out.as<float>();
+    if (which == 0)
+        SRED_TRY(global_hu_dev(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>(), rows,
+                               nrows, d_out));
+    else if (which == 1)
+        SRED_TRY(phase_map_dev(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>(), rows,
+                               nrows, d_out));
+    else
+        SRED_TRY(hu_invariants_dev(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), d_out));
+    return download(ctx, out, ctx->out, g.n());
+}
+
+static int pair_map_dev(sred_ctx *ctx, int which, const float *d_img, const float *d_means, const float *d_stds,
+                        int w, int h, int bw, int bh, const int32_t *rows, int64_t nrows, float *d_out)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    if (!d_img || !d_means || !d_stds || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (which == 0) return global_hu_dev(ctx, g, d_img, d_means, d_stds, rows, nrows, d_out);
+    return phase_map_dev(ctx, g, d_img, d_means, d_stds, rows, nrows, d_out);
+}
+
+int sred_hu_invariants_2d(sred_ctx *ctx, const float *img, const float *means, int w, int h, int bw, int bh,
+                          float *out)
+{
+    return pair_map_host(ctx, 2, img, means, nullptr, w, h, bw, bh, nullptr, 0, out);
+}
+
+int sred_global_hu_2d(sred_ctx *ctx, const float *img, const float *means, const float *stds, int w, int h, int bw,
+                      int bh, const int32_t *rows, int64_t nrows, float *out)
+{
+    return pair_map_host(ctx, 0, img, means, stds, w, h, bw, bh, rows, nrows, out);
+}
+
+int sred_global_hu_2d_dev(sred_ctx *ctx, const float *d_img, const float *d_means, const float *d_stds, int w, int h,
+                          int bw, int bh, const int32_t *rows, int64_t nrows, float *d_out)
+{
+    return pair_map_dev(ctx, 0, d_img, d_means, d_stds, w, h, bw, bh, rows, nrows, d_out);
+}
+
+int sred_phase_correlation_2d(sred_ctx *ctx, const float *img, const float *means, const float *stds, int w, int h,
+                              int bw, int bh, const int32_t *rows, int64_t nrows, float *out)
+{
+    return pair_map_host(ctx, 1, img, means, stds, w, h, bw, bh, rows, nrows, out);
+}
+
+int sred_phase_correlation_2d_dev(sred_ctx *ctx, const float *d_img, const float *d_means, const float *d_stds,
+                                  int w, int h, int bw, int bh, const int32_t *rows, int64_t nrows, float *d_out)
+{
+    return pair_map_dev(ctx, 1, d_img, d_means, d_stds, w, h, bw, bh, rows, nrows, d_out);
 }
 
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,

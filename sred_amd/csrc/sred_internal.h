@@ -102,6 +102,7 @@ struct sred_ctx {
     sred::DevBuf img, means, stds, mask, S, W, out, dmap, grad;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     sred::DevBuf wmom, wpart;  // weight-sum moments
+    sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
@@ -223,6 +224,16 @@ int block_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const flo
 double hu1_template_host(const float *tmpl, int bw, int bh, double eps);
 // Hu template preparation (the window steps of kernelGetPatchHu2D.cl:127-182), host.
 int hu_prepare_template_host(const float *blk, int bw, int bh, int K, float *out);
+// Per-pixel Hu invariant nu20 + nu02 (kernelGetHuMap.cl:146-172), border 0.
+int hu_invariants_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means, float *d_out);
+// Global Hu map (kernelGetHuMap.cl) and phase correlation map (textbook
+// specification of kernelGetPhaseCorrelationMap.cl) of a device image with
+// its local statistics; rows: optional host list of reference pixels.
+// Synchronise.
+int global_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means, const float *d_stds,
+                  const int32_t *rows, int64_t nrows, float *d_out);
+int phase_map_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means, const float *d_stds,
+                  const int32_t *rows, int64_t nrows, float *d_out);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);
