@@ -338,14 +338,14 @@ __global__ __launch_bounds__(256) void k_phase_rows(const double2 *__restrict__ 
 
 // The pair kernel.  Lane = row; the column's unit spectrum and std are
 // wave-uniform.  UREG: the row spectrum is held in registers (small blocks)
-// or read per pair from rspec (coalesced over lanes).
+// or read per pair from rspec (coalesced over lanes).  Products and sums are
+// contracted to FMAs (r only decides the peak; the weight is exact either way).
 template <int BW, int BH, bool UREG>
 __global__ __launch_bounds__(64) void k_phase_pairs(const double2 *__restrict__ spec, const float *__restrict__ sig,
                                                     int ncols, const double2 *__restrict__ rspec,
                                                     const float *__restrict__ rsig, int R, int chunk, PhaseTw tw,
                                                     double *__restrict__ part)
 {
-#pragma clang fp contract(off)
     constexpr int BRW = BW / 2, BRH = BH / 2, NT = BW * (BRH + 1);
     const int r = blockIdx.x * 64 + threadIdx.x;
     if (r >= R) return;
