@@ -250,6 +250,12 @@ int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision
  * adjoint call -- and its work units (block pairs evaluated, or pixels).
  * No counterpart in the reference (measurement). */
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);
+/* Ordered block pairs whose evaluation the last global call (single or
+ * batched, split-BF16 symmetric contraction, Pearson or SSIM) skipped because
+ * their t = sigma * sqrt(log2(e) / (h + eps)) differ by more than sqrt(40),
+ * i.e. their weight exp(-(s_i - s_j)^2 / (h + eps)) is below 2^-40; 0 for
+ * other paths.  Synchronises.  No counterpart in the reference (measurement). */
+int sred_ctx_last_skipped(sred_ctx *ctx, int64_t *pairs_skipped);
 
 #ifdef __cplusplus
 }

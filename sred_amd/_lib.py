@@ -34,7 +34,7 @@ EXPORTS = [
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
     "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_hu_invariants_2d", "sred_global_hu_2d",
-    "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev",
+    "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev", "sred_ctx_last_skipped",
 ]
 
 
@@ -107,6 +107,7 @@ def load():
         "sred_global_map_2d_dev": ([_VP, _I, _I, _VP, _I, _I, _I, _I, _FL, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
         "sred_block_map_2d_dev": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_ctx_last_contraction": ([_VP, _F, ctypes.POINTER(_I), ctypes.POINTER(_I64)], _I),
+        "sred_ctx_last_skipped": ([_VP, ctypes.POINTER(_I64)], _I),
         "sred_global_plan_2d": ([_I, _I, _I, _I, _I, _I, ctypes.POINTER(_I64)], _I),
         "sred_global_pearson_adjoint_2d": ([_VP, _F, _I, _I, _I, _I, _F, _F], _I),
         "sred_global_pearson_adjoint_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
@@ -211,3 +212,9 @@ class Context:
         p = ctypes.c_int64()
         check(self._lib.sred_ctx_last_contraction(self.handle, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(p)))
         return ms.value, n.value, p.value
+
+    def last_skipped(self):
+        """Ordered block pairs the last global call skipped (weight < 2^-40)."""
+        p = ctypes.c_int64()
+        check(self._lib.sred_ctx_last_skipped(self.handle, ctypes.byref(p)))
+        return p.value

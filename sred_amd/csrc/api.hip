@@ -255,7 +255,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
-                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec};
+                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -844,6 +844,18 @@ int sred_global_pearson_adjoint_2d_dev(sred_ctx *ctx, const float *d_img, int w,
     SRED_TRY(global_pearson_adjoint_dev(ctx, g, d_img, d_dmap, d_grad));
     if (d_map)
         SRED_HIP_CHECK(hipMemcpyAsync(d_map, ctx->out.ptr, sizeof(float) * g.n(), hipMemcpyDeviceToDevice, ctx->stream));
+    return SRED_OK;
+}
+
+int sred_ctx_last_skipped(sred_ctx *ctx, int64_t *pairs_skipped)
+{
+    SRED_TRY(check_ctx(ctx));
+    unsigned long long v = 0;
+    if (ctx->last_skip) {
+        SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        SRED_HIP_CHECK(hipMemcpy(&v, ctx->last_skip, sizeof v, hipMemcpyDeviceToHost));
+    }
+    if (pairs_skipped) *pairs_skipped = (int64_t)v;
     return SRED_OK;
 }
 

@@ -39,13 +39,14 @@ BatchStore::~BatchStore() { release(); }
 void BatchStore::release()
 {
     DevBuf *b[] = {&means, &stds, &mask, &S, &W, &counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2,
-                   &wmom, &wpart};
+                   &wmom, &wpart, &sortbuf};
     for (DevBuf *x : b) x->release();
     offs2_key[0] = offs2_key[1] = offs2_key[2] = -1;
 }
 GlobalBufs BatchStore::bufs()
 {
-    return GlobalBufs{&counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2, offs2_key, &wmom, &wpart};
+    return GlobalBufs{&counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2, offs2_key, &wmom, &wpart,
+                      &sortbuf};
 }
 
 namespace {
@@ -178,6 +179,7 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
         return metric == SRED_ABSDIFF ? SRED_ERR_ARG : SRED_ERR_UNKNOWN_METRIC;
     }
     std::vector<Item> it(n);
+    ctx->last_skip = nullptr;
     while ((int)ctx->batch_store.size() < n) ctx->batch_store.push_back(new BatchStore());
     // ---- front, per image
     for (int i = 0; i < n; i++) {
@@ -270,6 +272,10 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
     }
     const EpiConst ec = epi_const();
     int64_t pairs = 0;
+    if (!rc) {
+        int e = zero_dev(ctx, batch_counter(ctx) + 1, sizeof(unsigned long long));  // pairs skipped, all launches
+        if (e) fail(e);
+    }
     for (const Launch &L : launches) {
         if (rc) break;
         int e = launch_sym_metric(metric, L.KS, L.nitems, cs, ctx->batch_tab.as<SymProblem>() + L.tab_off, L.nprob,
@@ -309,6 +315,7 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
             e = SRED_ERR_HIP;
         }
         if (!e) e = time_end(ctx, pairs);
+        if (!e) ctx->last_skip = batch_counter(ctx) + 1;
         if (!e && hipStreamSynchronize(cs) != hipSuccess) {
             set_error("batch graph execution failed");
             e = SRED_ERR_HIP;

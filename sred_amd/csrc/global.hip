@@ -593,6 +593,13 @@ __device__ __forceinline__ void tile_to_lds(const uint4 *Uhi, const uint4 *Ulo, 
 // flight while other waves read them.
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
+// Weighted metrics (Pearson, SSIM) skip pairs whose t values are more than
+// kSkipGap apart: such a pair's weight exp2(-(t_i - t_j)^2) is below 2^-40
+// and its similarity term at most 1 in magnitude, so a row sum over N blocks
+// changes by less than N * 2^-40 (< 1e-6 for N = 10^6, against a self-pair
+// term of about 1).
+constexpr float kSkipGap = 6.3245553f;  // sqrt(40)
+
 // two 64-column tiles in flight (copy of the next overlaps the current one)
 // while their LDS fits twice beside the column accumulator at 2 workgroups/CU
 constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
@@ -614,6 +621,12 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
 // batch); an item is one panel pair of one problem.  Items of one problem are
 // contiguous and chunk-major, so the workgroups resident together still share
 // a column panel.
+//
+// Valid blocks are ordered by ascending sigma (so ascending t): for Pearson
+// and SSIM a panel pair whose t ranges are more than kSkipGap apart writes
+// zero partial sums without being evaluated, and inside an evaluated panel
+// pair a wave skips each column tile whose t range is that far from its
+// rows' range.  next[1] counts the ordered pairs skipped.
 //
 // The workgroup walks its rectangle as P/BM sweeps of P/64 column tiles
 // (step T = tile T % ntiles of sweep T / ntiles).  With double buffering the
@@ -671,19 +684,30 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     while (c * (c + 1) / 2 > q) c--;
     const int p = q - c * (c + 1) / 2;
     const bool diag = (p == c);
+    const int col0 = c * P, row0 = p * P;
+    if (M != SRED_NRMSE && !diag && aux[col0].x - aux[row0 + P - 1].x > kSkipGap) {
+        // every pair of the panel pair has a vanishing weight
+        for (int i = tid; i < P; i += 256) {
+            partial[(size_t)c * Nv_pad + row0 + i] = 0.0f;
+            partial[(size_t)p * Nv_pad + col0 + i] = 0.0f;
+        }
+        if (tid == 0) atomicAdd(next + 1, 2ull * (unsigned long long)P * (unsigned long long)P);
+        continue;
+    }
 
     if (!diag)
         for (int i = tid; i < P; i += 256) colacc[i] = 0.0f;
 
     const int ntiles = P / kBN;  // multiple of 4: the step count below is even
     const int NT = ntiles * (P / BM);
-    const int col0 = c * P, row0 = p * P;
 
     bf16x8 ahi[RG][KS], alo[RG][KS];
     float4 R[RG][4];
     float accS[RG][4];
     bool row_small = false;
     int rowbase = 0;
+    float wt_lo = 0.0f, wt_hi = 0.0f;  // t range of this wave's rows
+    unsigned long long nskip = 0;      // ordered pairs skipped by this wave
 
     auto fold = [&](int T) {
         // column sums of step T's tile into colacc (after a barrier), spread
@@ -715,6 +739,8 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
         row_small = false;
         if (M == SRED_PEARSON)
             for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
+        wt_lo = aux[rowbase].x;
+        wt_hi = aux[rowbase + 16 * RG - 1].x;
 #pragma unroll
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
@@ -734,6 +760,16 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     };
     auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
         const int c0 = col0 + (T % ntiles) * kBN;
+        if (M != SRED_NRMSE) {
+            const float tlo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[0].x)));
+            const float thi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[kBN - 1].x)));
+            if (tlo - wt_hi > kSkipGap || wt_lo - thi > kSkipGap) {  // the tile's weights vanish for these rows
+                if (!diag) stage[T & 1][wave][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                nskip += (diag ? 1ull : 2ull) * (unsigned long long)(16 * RG * kBN);
+                if (T % ntiles == ntiles - 1) end_sweep();
+                return;
+            }
+        }
         const bool exact =
             row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
         float4 *sw = stage[T & 1][wave];
@@ -789,6 +825,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
         __syncthreads();
         for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
     }
+    if (nskip && lane == 0) atomicAdd(next + 1, nskip);
   }
 }
 
@@ -1145,7 +1182,7 @@ int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, 
 GlobalBufs ctx_global_bufs(sred_ctx *ctx)
 {
     return GlobalBufs{&ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi, &ctx->ulo,
-                      &ctx->offs2, ctx->offs2_key, &ctx->wmom, &ctx->wpart};
+                      &ctx->offs2, ctx->offs2_key, &ctx->wmom, &ctx->wpart, &ctx->sortbuf};
 }
 
 // Steps 1-2 of a global problem: compaction (one synchronisation, for Nv),
@@ -1211,6 +1248,14 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     const int Nv_rows = (pad_rows > 0 && mfma) ? std::max(pl.Nv_pad, (Nv + pad_rows - 1) / pad_rows * pad_rows)
                                                : pl.Nv_pad;
     r.Nv_rows = Nv_rows;
+    if (mfma && Nv > 1) {
+        // contraction order: ascending sigma (lets the symmetric kernel skip
+        // panel pairs and tiles whose weights vanish)
+        int *sorted = nullptr;
+        SRED_TRY(sort_valid_by_sigma(ctx, *b.sortbuf, d_stds, d_idx, Nv, &sorted));
+        d_idx = sorted;
+        r.d_idx = d_idx;
+    }
 
     // 2. offsets relative to the image and per-block scalars
     SRED_TRY(upload_offsets(ctx, g, g.w, b.offs2, b.offs2_key));
@@ -1269,6 +1314,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     GlobalPrepared r;
     ctx->last_ms = 0.0f;
     ctx->last_launches = 0;
+    ctx->last_skip = nullptr;
     SRED_TRY(global_prepare(ctx, ctx_global_bufs(ctx), g, p, d_img, d_means, d_stds, d_S, d_W, ctx->symmetric, 0, 0,
                             &r));
     ctx->last_nv = r.Nv;
@@ -1293,6 +1339,8 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
                                       hipMemcpyHostToDevice, ctx->stream));
         unsigned long long *d_next =
             reinterpret_cast<unsigned long long *>(ctx->symtab.as<char>() + (sizeof(SymProblem) + 127) / 128 * 128);
+        SRED_TRY(zero_dev(ctx, d_next + 1, sizeof(unsigned long long)));
+        ctx->last_skip = d_next + 1;
         SRED_TRY(launch_sym_metric(metric, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, d_next, P, ec));
     } else if (pl.mfma) {
         int rc = SRED_OK;

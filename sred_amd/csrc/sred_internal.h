@@ -85,7 +85,7 @@ struct SymProblem {
 namespace sred {
 // Per-image scratch of a batched global call (owned by the context).
 struct BatchStore {
-    DevBuf means, stds, mask, S, W, counts, valid_idx, aux, tflags, u32, uhi, ulo, offs2, wmom, wpart;
+    DevBuf means, stds, mask, S, W, counts, valid_idx, aux, tflags, u32, uhi, ulo, offs2, wmom, wpart, sortbuf;
     int offs2_key[3] = {-1, -1, -1};
     ~BatchStore();
     void release();
@@ -102,6 +102,7 @@ struct sred_ctx {
     sred::DevBuf img, means, stds, mask, S, W, out, dmap, grad;          // pixel maps (host-pointer paths)
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     sred::DevBuf wmom, wpart;  // weight-sum moments
+    sred::DevBuf sortbuf;      // valid blocks sorted by sigma
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
@@ -121,6 +122,7 @@ struct sred_ctx {
     int last_launches = 0;
     int64_t last_pairs = 0;
     int last_nv = 0;  // valid blocks of the last global accumulation
+    unsigned long long *last_skip = nullptr;  // device counter of pairs skipped by the last symmetric contraction
 };
 
 namespace sred {
@@ -165,7 +167,7 @@ struct GlobalParams {
 struct GlobalBufs {
     DevBuf *counts, *valid_idx, *aux, *tflags, *u32, *uhi, *ulo, *offs2;
     int *offs2_key;
-    DevBuf *wmom, *wpart;
+    DevBuf *wmom, *wpart, *sortbuf;
 };
 GlobalBufs ctx_global_bufs(sred_ctx *ctx);
 // A prepared global problem (see global_prepare).
@@ -189,6 +191,9 @@ EpiConst epi_const();
 // Accumulators S, Wsum (float maps, w*h) of the global kernels.
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W);
+// The valid-block list d_idx[Nv] reordered by ascending sigma (stable), in
+// buf; *d_sorted points into buf.
+int sort_valid_by_sigma(sred_ctx *ctx, DevBuf &buf, const float *d_stds, const int *d_idx, int Nv, int **d_sorted);
 // Weight sums W (float map entries idx[v], v < Nv) of the global kernels:
 // sum over valid j of exp2(-(t_i - t_j)^2), t = sigma * tscale, by a binned
 // Hermite expansion in FP64; sigma_max bounds the valid blocks' sigmas.

@@ -298,3 +298,30 @@ def test_symmetric_contraction_deterministic(ctx, metric):
     runs = [clutils.get_global_accumulators_2d(ctx, metric, inp, st, b, b, nv, 0.0, "bf16x3")[0] for _ in range(4)]
     for r in runs[1:]:
         np.testing.assert_array_equal(r, runs[0])
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_global_skips_vanishing_weights(ctx, metric):
+    """Two textures of very different block sigma (low-noise background |
+    uniform random texture): the symmetric kernel skips the cross pairs,
+    whose weights are below 2^-40, and the accumulators and the normalised
+    map still match FP64."""
+    rng = np.random.default_rng(17)
+    h, w, b = 160, 160, 5
+    img = (0.1 + 0.02 * rng.standard_normal((h, w))).astype(np.float32)
+    img[:, w // 2:] = rng.random((h, w - w // 2)).astype(np.float32)
+    inp = utils.get_input_image_2d(img)
+    m, s = O.local_stats_f32(inp.image_array, b, b)
+    nv = O.noise_var(inp.image_array)
+    st = clutils.LocalStatistics(m.ravel(), s.ravel())
+    S, W = clutils.get_global_accumulators_2d(ctx, metric, inp, st, b, b, nv, 0.0, "bf16x3")
+    n = (h - b + 1) * (w - b + 1)
+    skipped = ctx.last_skipped()
+    assert skipped > 0.3 * n * n, skipped
+    S64, W64 = O.global_accum_f64(metric, inp.image_array, s, b, b, nv, 0.0)
+    np.testing.assert_allclose(W.reshape(h, w), W64, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(S.reshape(h, w), S64, rtol=2e-4, atol=2e-4 * np.abs(S64).max())
+    out = sred_amd.global_repetition_2d(img, b, b, 0.0, GM[metric], resources=ctx)
+    assert ctx.last_skipped() > 0.3 * n * n
+    ref = O.global_map(metric, inp.image_array, b, b, 0.0, True, fp64=True)
+    assert np.abs(out - ref).max() <= MAP_TOL
