@@ -330,6 +330,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     std::vector<int> dx, dy, dxy(2 * K);
     footprint(g.bw, g.bh, &dx, &dy);
     for (int k = 0; k < K; k++) { dxy[2 * k] = dx[k]; dxy[2 * k + 1] = dy[k]; }
+    ctx->host_tmpl.clear();  // the buffer no longer holds a block template
     SRED_TRY(ctx->tmpl.ensure(sizeof(int) * 2 * K));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     SRED_HIP_CHECK(hipMemcpy(ctx->tmpl.ptr, dxy.data(), sizeof(int) * 2 * K, hipMemcpyHostToDevice));

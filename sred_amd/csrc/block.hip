@@ -12,85 +12,180 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "sred_internal.h"
 
 namespace sred {
 
-constexpr int kBlkTile = 16;
+// Workgroup: 16 x 16 threads, each computing 4 horizontally adjacent windows,
+// so one workgroup covers 64 x 16 output pixels.
+constexpr int kBx = 64, kBy = 16;
+constexpr int kMaxRows = SRED_MAX_BLOCK_SIDE + 1;
 
-__global__ __launch_bounds__(256) void k_block(int metric, const float *__restrict__ img, int w, int h, int brw,
-                                               int brh, const int *__restrict__ offs, int K,
-                                               const float *__restrict__ tmpl, double tmpl_mean_d, double tmpl_std_d,
-                                               float tmpl_std_f, double eps_d, float *__restrict__ out,
-                                               float *__restrict__ means_out, float *__restrict__ stds_out)
+// Ellipse rows: half-width hw[j] and first footprint index kb[j] of row j
+// (dy = j - brh); the footprint is row-major and every row is the contiguous
+// run dx = -hw..hw (the float predicate is monotone in |dx|).
+struct RowTab {
+    int hw[kMaxRows];
+    int kb[kMaxRows];
+};
+
+// One ellipse row of the four windows of a thread, as a sliding run: element
+// e (0 <= e < L + 3) of the union of the four runs is loaded once (load(e))
+// and is element e - w of window w's run when 0 <= e - w < L (f(w)).  Every
+// window sees its elements in its own footprint order, so each sum keeps the
+// reference's sequential float order.  L is odd: a single-element row gives
+// window w element e = w; otherwise (L >= 3) the first and last three
+// elements have fixed window sets and the rest feed all four windows.
+template <class LD, class F>
+__device__ __forceinline__ void row_run(int L, LD &&load, F &&f)
+{
+    if (L == 1) {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            load(w);
+            f(w);
+        }
+        return;
+    }
+    load(0);
+    f(0);
+    load(1);
+    f(0);
+    f(1);
+    load(2);
+    f(0);
+    f(1);
+    f(2);
+    for (int e = 3; e < L; e++) {
+        load(e);
+#pragma unroll
+        for (int w = 0; w < 4; w++) f(w);
+    }
+    load(L);
+    f(1);
+    f(2);
+    f(3);
+    load(L + 1);
+    f(2);
+    f(3);
+    load(L + 2);
+    f(3);
+}
+
+// LDS tile: row stride RS = 1 mod 64, so the 64 lanes of a wave (16 column
+// groups 4 words apart, 4 rows) address 64 distinct banks.
+// tp4[i] = (t[i], t[i-1], t[i-2], t[i-3]) (t = template, 0 outside 0..K-1):
+// element e of row j is template entry kb[j] + e - w for window w.
+template <int M>
+__global__ __launch_bounds__(256) void k_block(const float *__restrict__ img, int w, int h, int brw, int brh, int K,
+                                               RowTab rt, const float4 *__restrict__ tp4, double tmpl_mean_d,
+                                               double tmpl_std_d, float tmpl_std_f, double eps_d,
+                                               float *__restrict__ out, float *__restrict__ means_out,
+                                               float *__restrict__ stds_out)
 {
 #pragma clang fp contract(off)
     extern __shared__ float sm[];
-    const int tw = kBlkTile + 2 * brw, th = kBlkTile + 2 * brh;
+    const int tw = kBx + 2 * brw, th = kBy + 2 * brh;
+    const int RS = (tw + 62) / 64 * 64 + 1;
     float *tile = sm;
-    float *tp = sm + tw * th;
-    const int x0 = blockIdx.x * kBlkTile, y0 = blockIdx.y * kBlkTile;
+    float4 *tp = reinterpret_cast<float4 *>(sm + ((th * RS + 3) & ~3));
+    const int x0 = blockIdx.x * kBx, y0 = blockIdx.y * kBy;
     for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
-        int ly = i / tw, lx = i - ly * tw;
-        int gy = y0 - brh + ly, gx = x0 - brw + lx;
-        tile[i] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? img[(int64_t)gy * w + gx] : 0.0f;
+        const int ly = i / tw, lx = i - ly * tw;
+        const int gy = y0 - brh + ly, gx = x0 - brw + lx;
+        tile[ly * RS + lx] =
+            (gx >= 0 && gx < w && gy >= 0 && gy < h) ? img[(int64_t)gy * w + gx] : 0.0f;
     }
-    for (int i = threadIdx.x; i < K; i += blockDim.x) tp[i] = tmpl[i];
+    if (M != SRED_ABSDIFF)
+        for (int i = threadIdx.x; i < K + 3; i += blockDim.x) tp[i] = tp4[i];
     __syncthreads();
-    const int tx = threadIdx.x % kBlkTile, ty = threadIdx.x / kBlkTile;
-    const int gx = x0 + tx, gy = y0 + ty;
-    if (gx >= w || gy >= h) return;
-    const int64_t o = (int64_t)gy * w + gx;
-    if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
-        out[o] = 0.0f;
-        if (means_out) { means_out[o] = 0.0f; stds_out[o] = 0.0f; }
-        return;
-    }
-    const float *c = tile + (ty + brh) * tw + (tx + brw);
-    // window statistics, kernelGetLocalStatistics2D.cl:40-55
-    float m = 0.0f;
-    for (int k = 0; k < K; k++) m += c[offs[k]];
-    m /= (float)K;
-    float v = 0.0f;
-    for (int k = 0; k < K; k++) {
-        float d = c[offs[k]] - m;
-        v += d * d;
-    }
-    v /= (float)(K - 1);
-    const float cs = sqrtf(v);
-    if (means_out) { means_out[o] = m; stds_out[o] = cs; }
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int gx0 = x0 + 4 * tx, gy = y0 + ty;
+    if (gx0 >= w || gy >= h) return;
+    const int nrows = 2 * brh + 1;
+    const float *trow = tile + ty * RS + 4 * tx;  // the windows' top row, this thread's first window
 
-    float r;
-    if (metric == SRED_ABSDIFF) {  // kernelGetBlockAbsDiffStds2D.cl:103-105
-        r = fabsf(tmpl_std_f - cs);
-    } else if (metric == SRED_NRMSE) {  // kernelGetBlockNrmse2D.cl:69-74
-        float mse = 0.0f;
-        for (int k = 0; k < K; k++) {
-            float e = tp[k] - (c[offs[k]] - m);
-            mse += e * e;
-        }
-        mse /= (float)K;
-        r = sqrtf(mse);
-    } else {
-        float cov = 0.0f;  // kernelGetBlockPearson2D.cl:70-74
-        for (int k = 0; k < K; k++) cov += tp[k] * (c[offs[k]] - m);
-        cov /= (float)(K - 1);
-        if (tmpl_std_d == 0.0 && cs == 0.0f) {
-            r = 1.0f;
-        } else if (tmpl_std_d == 0.0 || cs == 0.0f) {
-            r = 0.0f;
-        } else if (metric == SRED_PEARSON) {  // kernelGetBlockPearson2D.cl:88 (double literals)
-            r = fmaxf(0.0f, (float)((double)cov / ((tmpl_std_d * (double)cs) + eps_d)));
-        } else {  // kernelGetBlockSsim2D.cl:169 (sum denominator, double literals)
-            const float c1 = 0.0001f, c2 = 0.0009f;
-            double num = (2.0 * tmpl_mean_d * (double)m + (double)c1) * (double)(2.0f * cov + c2);
-            double den = ((tmpl_mean_d * tmpl_mean_d) + (double)(m * m) + (double)c1) *
-                         ((tmpl_std_d * tmpl_std_d) + (double)(cs * cs) + (double)c2);
-            r = fmaxf(0.0f, (float)(num / den));
-        }
+    // window means, kernelGetLocalStatistics2D.cl:40-47
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < nrows; j++) {
+        const int hw = rt.hw[j], c0 = brw - hw;
+        const float *rp = trow + j * RS;
+        float v = 0.0f;
+        row_run(
+            2 * hw + 1,
+            [&](int e) { v = rp[c0 + e]; },
+            [&](int wi) { m[wi] += v; });
     }
-    out[o] = r;
+#pragma unroll
+    for (int wi = 0; wi < 4; wi++) m[wi] /= (float)K;
+
+    // variance (kernelGetLocalStatistics2D.cl:48-55) and the template sum
+    // (covariance, or squared error for NRMSE) in one sweep: the two sums are
+    // independent sequences, each in footprint order
+    float var[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < nrows; j++) {
+        const int hw = rt.hw[j], c0 = brw - hw, kb = rt.kb[j];
+        const float *rp = trow + j * RS;
+        float v = 0.0f;
+        float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        row_run(
+            2 * hw + 1,
+            [&](int e) {
+                v = rp[c0 + e];
+                if (M != SRED_ABSDIFF) t = tp[kb + e];
+            },
+            [&](int wi) {
+                const float d = v - m[wi];
+                var[wi] += d * d;
+                if (M == SRED_PEARSON || M == SRED_SSIM) {
+                    const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
+                    acc[wi] += tv * d;  // kernelGetBlockPearson2D.cl:70-74
+                } else if (M == SRED_NRMSE) {
+                    const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
+                    const float q = tv - d;  // kernelGetBlockNrmse2D.cl:69-72
+                    acc[wi] += q * q;
+                }
+            });
+    }
+
+#pragma unroll
+    for (int wi = 0; wi < 4; wi++) {
+        const int gx = gx0 + wi;
+        if (gx >= w) break;
+        const int64_t o = (int64_t)gy * w + gx;
+        if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
+            out[o] = 0.0f;
+            if (means_out) { means_out[o] = 0.0f; stds_out[o] = 0.0f; }
+            continue;
+        }
+        const float cs = sqrtf(var[wi] / (float)(K - 1));
+        if (means_out) { means_out[o] = m[wi]; stds_out[o] = cs; }
+        float r;
+        if (M == SRED_ABSDIFF) {  // kernelGetBlockAbsDiffStds2D.cl:103-105
+            r = fabsf(tmpl_std_f - cs);
+        } else if (M == SRED_NRMSE) {  // kernelGetBlockNrmse2D.cl:73-74
+            r = sqrtf(acc[wi] / (float)K);
+        } else {
+            const float cov = acc[wi] / (float)(K - 1);
+            if (tmpl_std_d == 0.0 && cs == 0.0f) {
+                r = 1.0f;
+            } else if (tmpl_std_d == 0.0 || cs == 0.0f) {
+                r = 0.0f;
+            } else if (M == SRED_PEARSON) {  // kernelGetBlockPearson2D.cl:88 (double literals)
+                r = fmaxf(0.0f, (float)((double)cov / ((tmpl_std_d * (double)cs) + eps_d)));
+            } else {  // kernelGetBlockSsim2D.cl:169 (sum denominator, double literals)
+                const float c1 = 0.0001f, c2 = 0.0009f;
+                const float mm = m[wi];
+                double num = (2.0 * tmpl_mean_d * (double)mm + (double)c1) * (double)(2.0f * cov + c2);
+                double den = ((tmpl_mean_d * tmpl_mean_d) + (double)(mm * mm) + (double)c1) *
+                             ((tmpl_std_d * tmpl_std_d) + (double)(cs * cs) + (double)c2);
+                r = fmaxf(0.0f, (float)(num / den));
+            }
+        }
+        out[o] = r;
+    }
 }
 
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
@@ -100,17 +195,61 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
         set_error("unknown metric");
         return SRED_ERR_UNKNOWN_METRIC;
     }
-    const int tw = kBlkTile + 2 * g.brw, th = kBlkTile + 2 * g.brh;
-    SRED_TRY(upload_offsets(ctx, g, tw, &ctx->offs, ctx->offs_key));
-    SRED_TRY(ctx->tmpl.ensure(sizeof(float) * g.K));
-    SRED_HIP_CHECK(hipMemcpy(ctx->tmpl.ptr, h_tmpl, sizeof(float) * g.K, hipMemcpyHostToDevice));
-    dim3 grid((g.w + kBlkTile - 1) / kBlkTile, (g.h + kBlkTile - 1) / kBlkTile);
-    size_t lds = sizeof(float) * ((size_t)tw * th + g.K);
+    // ellipse rows as runs (row-major footprint)
+    std::vector<int> dx, dy;
+    footprint(g.bw, g.bh, &dx, &dy);
+    RowTab rt{};
+    int k = 0;
+    for (int j = 0; j < g.bh; j++) {
+        const int y = j - g.brh;
+        rt.kb[j] = k;
+        int n = 0;
+        while (k + n < g.K && dy[k + n] == y) n++;
+        if (n == 0 || n % 2 == 0 || dx[k] != -(n / 2) || dx[k + n - 1] != n / 2) {
+            set_error("unsupported block footprint");
+            return SRED_ERR_ARG;
+        }
+        rt.hw[j] = n / 2;
+        k += n;
+    }
+    // expanded template (t[i], t[i-1], t[i-2], t[i-3])
+    std::vector<float> tp4(4 * (size_t)(g.K + 3), 0.0f);
+    for (int i = 0; i < g.K + 3; i++)
+        for (int w = 0; w < 4; w++) {
+            const int q = i - w;
+            tp4[4 * i + w] = (q >= 0 && q < g.K) ? h_tmpl[q] : 0.0f;
+        }
+    if (tp4 != ctx->host_tmpl || !ctx->tmpl.ptr) {
+        ctx->host_tmpl.clear();
+        SRED_TRY(ctx->tmpl.ensure(sizeof(float) * tp4.size()));
+        SRED_HIP_CHECK(hipMemcpy(ctx->tmpl.ptr, tp4.data(), sizeof(float) * tp4.size(), hipMemcpyHostToDevice));
+        ctx->host_tmpl = tp4;
+    }
+    const int tw = kBx + 2 * g.brw, th = kBy + 2 * g.brh;
+    const int RS = (tw + 62) / 64 * 64 + 1;
+    const size_t lds = sizeof(float) * (size_t)((th * RS + 3) & ~3) + sizeof(float4) * (size_t)(g.K + 3);
+    dim3 grid((g.w + kBx - 1) / kBx, (g.h + kBy - 1) / kBy);
+    float *mo = d_stds_out ? d_means_out : nullptr, *so = d_means_out ? d_stds_out : nullptr;
+    const float4 *d_tp = ctx->tmpl.as<float4>();
+    const double tm = java_float_literal(tmpl_mean), ts = java_float_literal(tmpl_std), ed = java_float_literal(kEps);
     SRED_TRY(time_begin(ctx));
-    hipLaunchKernelGGL(k_block, grid, dim3(256), lds, ctx->stream, metric, d_img, g.w, g.h, g.brw, g.brh,
-                       ctx->offs.as<int>(), g.K, ctx->tmpl.as<float>(), java_float_literal(tmpl_mean),
-                       java_float_literal(tmpl_std), tmpl_std, java_float_literal(kEps), d_out,
-                       d_stds_out ? d_means_out : nullptr, d_means_out ? d_stds_out : nullptr);
+    switch (metric) {
+    case SRED_PEARSON:
+        hipLaunchKernelGGL(k_block<SRED_PEARSON>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+        break;
+    case SRED_SSIM:
+        hipLaunchKernelGGL(k_block<SRED_SSIM>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh, g.K,
+                           rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+        break;
+    case SRED_NRMSE:
+        hipLaunchKernelGGL(k_block<SRED_NRMSE>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+        break;
+    default:
+        hipLaunchKernelGGL(k_block<SRED_ABSDIFF>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+    }
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_end(ctx, g.n()));
     return SRED_OK;

@@ -116,6 +116,7 @@ struct sred_ctx {
     int64_t batch_mb = 16384;  // partial-sum memory per wave
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
+    std::vector<float> host_tmpl;  // expanded template last copied into `tmpl` (skips repeated uploads)
     // measurement of the last global contraction
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;

@@ -202,9 +202,10 @@ __device__ __forceinline__ float ord2f(unsigned int u)
 {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
-__device__ __forceinline__ bool interior(int64_t i, int w, int h, int brw, int brh, int *x_out = nullptr)
+__device__ __forceinline__ bool interior(int64_t i, int w, int h, int brw, int brh)
 {
-    int x = (int)(i % w), y = (int)(i / w);
+    const int ii = (int)i;  // images hold fewer than 2^31 pixels
+    const int y = ii / w, x = ii - y * w;
     return x >= brw && x < w - brw && y >= brh && y < h - brh;
 }
 
@@ -223,19 +224,22 @@ __global__ void k_minmax_init(unsigned int *mm, float init_max)
 __global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ map, const float *__restrict__ mask, int w,
                                                 int h, int brw, int brh, unsigned int *__restrict__ part)
 {
-    int64_t n = (int64_t)w * h;
     unsigned int lo = 0xffffffffu, hi = 0u;
     int nonbinary = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (!interior(i, w, h, brw, brh)) continue;
-        if (mask) {
-            float m = mask[i];
-            if (m != 0.0f && m != 1.0f) nonbinary = 1;
-            if (!(m > 0.0f)) continue;
+    // whole interior rows per workgroup (no per-pixel index division)
+    for (int y = brh + blockIdx.x; y < h - brh; y += gridDim.x) {
+        const int64_t row = (int64_t)y * w;
+        for (int x = brw + threadIdx.x; x < w - brw; x += blockDim.x) {
+            const int64_t i = row + x;
+            if (mask) {
+                float m = mask[i];
+                if (m != 0.0f && m != 1.0f) nonbinary = 1;
+                if (!(m > 0.0f)) continue;
+            }
+            unsigned int o = f2ord(map[i]);
+            lo = min(lo, o);
+            hi = max(hi, o);
         }
-        unsigned int o = f2ord(map[i]);
-        lo = min(lo, o);
-        hi = max(hi, o);
     }
     __shared__ unsigned int slo[256], shi[256];
     __shared__ int snb;
@@ -295,18 +299,21 @@ __global__ __launch_bounds__(256) void k_remap(float *__restrict__ map, const fl
     if (mm[2]) return;  // non-binary mask: the reference throws before remapping
     const float mn = ord2f(mm[0]), mx = ord2f(mm[1]);
     const float den = mx - mn + kEps;
-    int64_t n = (int64_t)w * h;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (!interior(i, w, h, brw, brh)) continue;
-        if (mask && !(mask[i] > 0.0f)) continue;
-        map[i] = (map[i] - mn) / den;
+    for (int y = brh + blockIdx.x; y < h - brh; y += gridDim.x) {
+        const int64_t row = (int64_t)y * w;
+        for (int x = brw + threadIdx.x; x < w - brw; x += blockDim.x) {
+            const int64_t i = row + x;
+            if (mask && !(mask[i] > 0.0f)) continue;
+            map[i] = (map[i] - mn) / den;
+        }
     }
 }
 
 static int grid_for(const Geometry &g) { return (int)std::min<int64_t>((g.n() + 255) / 256, 8192); }
 constexpr int kMinmaxParts = 1024;  // workgroups of k_minmax (grid-stride); kMiscBytes holds their partials
 constexpr size_t kMiscPartOffset = 4096;
-static int grid_minmax(const Geometry &g) { return (int)std::min<int64_t>((g.n() + 255) / 256, kMinmaxParts); }
+// one workgroup per interior row, at most kMinmaxParts (partial slots)
+static int grid_minmax(const Geometry &g) { return std::max(1, std::min(g.h - 2 * g.brh, kMinmaxParts)); }
 
 // min/max (and the non-binary flag) of map over the selected pixels into mm[0..2]
 static int minmax_dev(sred_ctx *ctx, const Geometry &g, const float *d_map, const float *d_mask, unsigned int *mm)
@@ -326,7 +333,8 @@ int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const floa
     float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, mm, init_max);
     SRED_TRY(minmax_dev(ctx, g, d_map, d_mask, mm));
-    hipLaunchKernelGGL(k_remap, dim3(grid_for(g)), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh,
+    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min(g.h - 2 * g.brh, 8192))), dim3(256), 0, ctx->stream, d_map,
+                       d_mask, g.w, g.h, g.brw, g.brh,
                        mm);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
