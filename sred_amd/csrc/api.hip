@@ -180,8 +180,19 @@ static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const fl
         SRED_TRY(ctx->means.ensure(sizeof(float) * n));
         SRED_TRY(ctx->stds.ensure(sizeof(float) * n));
     }
+    // the first normalisation directly follows the stencil (unmasked): the
+    // stencil's workgroups supply its min/max partials
+    const bool fused = (metric == SRED_ABSDIFF) || ((metric == SRED_PEARSON || metric == SRED_SSIM) && c == 0.0f &&
+                                                    normalize);
+    int nparts = 0;
+    unsigned int *d_parts = nullptr;
+    if (fused) {
+        SRED_TRY(ctx->mmpart.ensure(sizeof(unsigned int) * 3 * (size_t)block_workgroups(g)));
+        d_parts = ctx->mmpart.as<unsigned int>();
+    }
     SRED_TRY(block_raw_dev(ctx, g, metric, d_img, need_stats ? ctx->means.as<float>() : nullptr,
-                           need_stats ? ctx->stds.as<float>() : nullptr, tmpl, tmpl_mean, tmpl_std, d_out));
+                           need_stats ? ctx->stds.as<float>() : nullptr, tmpl, tmpl_mean, tmpl_std, d_out, d_parts,
+                           &nparts));
     float *d_mask = nullptr;
     if (need_stats) {
         float nv, np;
@@ -191,8 +202,12 @@ static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const fl
         SRED_TRY(relevance_mask_dev(ctx, g, ctx->stds.as<float>(), nv * c, d_mask, &np));
     }
     if (metric == SRED_PEARSON || metric == SRED_SSIM) {
-        if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
-        if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, c > 0.0f ? d_mask : nullptr, 0));
+        if (fused)
+            SRED_TRY(normalize_map_from_parts_dev(ctx, g, d_out, d_parts, nparts, 0));
+        else {
+            if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
+            if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, c > 0.0f ? d_mask : nullptr, 0));
+        }
     } else if (metric == SRED_NRMSE) {
         if (c == 0.0f) {
             SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
@@ -204,7 +219,7 @@ static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const fl
         SRED_TRY(invert_masked_dev(ctx, g, d_out, d_mask));
         if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, d_mask, 0));
     } else {
-        SRED_TRY(normalize_map_dev(ctx, g, d_out, nullptr, 1));  // Float.MIN_VALUE max initialiser
+        SRED_TRY(normalize_map_from_parts_dev(ctx, g, d_out, d_parts, nparts, 1));  // Float.MIN_VALUE max initialiser
         SRED_TRY(one_minus_dev(ctx, g, d_out));
         if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
         if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, c > 0.0f ? d_mask : nullptr, 0));
@@ -255,7 +270,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
-                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf};
+                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_block(const float *__restrict__ img, in
                                                RowTab rt, const float4 *__restrict__ tp4, double tmpl_mean_d,
                                                double tmpl_std_d, float tmpl_std_f, double eps_d,
                                                float *__restrict__ out, float *__restrict__ means_out,
-                                               float *__restrict__ stds_out)
+                                               float *__restrict__ stds_out, unsigned int *__restrict__ mmpart)
 {
 #pragma clang fp contract(off)
     extern __shared__ float sm[];
@@ -103,93 +103,119 @@ __global__ __launch_bounds__(256) void k_block(const float *__restrict__ img, in
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int gx0 = x0 + 4 * tx, gy = y0 + ty;
-    if (gx0 >= w || gy >= h) return;
-    const int nrows = 2 * brh + 1;
-    const float *trow = tile + ty * RS + 4 * tx;  // the windows' top row, this thread's first window
+    unsigned int lo = 0xffffffffu, hi = 0u;  // interior min/max of this thread's outputs (ordered ints)
+    if (gx0 < w && gy < h) {
+        const int nrows = 2 * brh + 1;
+        const float *trow = tile + ty * RS + 4 * tx;  // the windows' top row, this thread's first window
 
-    // window means, kernelGetLocalStatistics2D.cl:40-47
-    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int j = 0; j < nrows; j++) {
-        const int hw = rt.hw[j], c0 = brw - hw;
-        const float *rp = trow + j * RS;
-        float v = 0.0f;
-        row_run(
-            2 * hw + 1,
-            [&](int e) { v = rp[c0 + e]; },
-            [&](int wi) { m[wi] += v; });
-    }
-#pragma unroll
-    for (int wi = 0; wi < 4; wi++) m[wi] /= (float)K;
-
-    // variance (kernelGetLocalStatistics2D.cl:48-55) and the template sum
-    // (covariance, or squared error for NRMSE) in one sweep: the two sums are
-    // independent sequences, each in footprint order
-    float var[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int j = 0; j < nrows; j++) {
-        const int hw = rt.hw[j], c0 = brw - hw, kb = rt.kb[j];
-        const float *rp = trow + j * RS;
-        float v = 0.0f;
-        float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        row_run(
-            2 * hw + 1,
-            [&](int e) {
-                v = rp[c0 + e];
-                if (M != SRED_ABSDIFF) t = tp[kb + e];
-            },
-            [&](int wi) {
-                const float d = v - m[wi];
-                var[wi] += d * d;
-                if (M == SRED_PEARSON || M == SRED_SSIM) {
-                    const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
-                    acc[wi] += tv * d;  // kernelGetBlockPearson2D.cl:70-74
-                } else if (M == SRED_NRMSE) {
-                    const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
-                    const float q = tv - d;  // kernelGetBlockNrmse2D.cl:69-72
-                    acc[wi] += q * q;
-                }
-            });
-    }
-
-#pragma unroll
-    for (int wi = 0; wi < 4; wi++) {
-        const int gx = gx0 + wi;
-        if (gx >= w) break;
-        const int64_t o = (int64_t)gy * w + gx;
-        if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
-            out[o] = 0.0f;
-            if (means_out) { means_out[o] = 0.0f; stds_out[o] = 0.0f; }
-            continue;
+        // window means, kernelGetLocalStatistics2D.cl:40-47
+        float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < nrows; j++) {
+            const int hw = rt.hw[j], c0 = brw - hw;
+            const float *rp = trow + j * RS;
+            float v = 0.0f;
+            row_run(
+                2 * hw + 1,
+                [&](int e) { v = rp[c0 + e]; },
+                [&](int wi) { m[wi] += v; });
         }
-        const float cs = sqrtf(var[wi] / (float)(K - 1));
-        if (means_out) { means_out[o] = m[wi]; stds_out[o] = cs; }
-        float r;
-        if (M == SRED_ABSDIFF) {  // kernelGetBlockAbsDiffStds2D.cl:103-105
-            r = fabsf(tmpl_std_f - cs);
-        } else if (M == SRED_NRMSE) {  // kernelGetBlockNrmse2D.cl:73-74
-            r = sqrtf(acc[wi] / (float)K);
-        } else {
-            const float cov = acc[wi] / (float)(K - 1);
-            if (tmpl_std_d == 0.0 && cs == 0.0f) {
-                r = 1.0f;
-            } else if (tmpl_std_d == 0.0 || cs == 0.0f) {
-                r = 0.0f;
-            } else if (M == SRED_PEARSON) {  // kernelGetBlockPearson2D.cl:88 (double literals)
-                r = fmaxf(0.0f, (float)((double)cov / ((tmpl_std_d * (double)cs) + eps_d)));
-            } else {  // kernelGetBlockSsim2D.cl:169 (sum denominator, double literals)
-                const float c1 = 0.0001f, c2 = 0.0009f;
-                const float mm = m[wi];
-                double num = (2.0 * tmpl_mean_d * (double)mm + (double)c1) * (double)(2.0f * cov + c2);
-                double den = ((tmpl_mean_d * tmpl_mean_d) + (double)(mm * mm) + (double)c1) *
-                             ((tmpl_std_d * tmpl_std_d) + (double)(cs * cs) + (double)c2);
-                r = fmaxf(0.0f, (float)(num / den));
+    #pragma unroll
+        for (int wi = 0; wi < 4; wi++) m[wi] /= (float)K;
+
+        // variance (kernelGetLocalStatistics2D.cl:48-55) and the template sum
+        // (covariance, or squared error for NRMSE) in one sweep: the two sums are
+        // independent sequences, each in footprint order
+        float var[4] = {0.0f, 0.0f, 0.0f, 0.0f}, acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < nrows; j++) {
+            const int hw = rt.hw[j], c0 = brw - hw, kb = rt.kb[j];
+            const float *rp = trow + j * RS;
+            float v = 0.0f;
+            float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            row_run(
+                2 * hw + 1,
+                [&](int e) {
+                    v = rp[c0 + e];
+                    if (M != SRED_ABSDIFF) t = tp[kb + e];
+                },
+                [&](int wi) {
+                    const float d = v - m[wi];
+                    var[wi] += d * d;
+                    if (M == SRED_PEARSON || M == SRED_SSIM) {
+                        const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
+                        acc[wi] += tv * d;  // kernelGetBlockPearson2D.cl:70-74
+                    } else if (M == SRED_NRMSE) {
+                        const float tv = wi == 0 ? t.x : wi == 1 ? t.y : wi == 2 ? t.z : t.w;
+                        const float q = tv - d;  // kernelGetBlockNrmse2D.cl:69-72
+                        acc[wi] += q * q;
+                    }
+                });
+        }
+
+    #pragma unroll
+        for (int wi = 0; wi < 4; wi++) {
+            const int gx = gx0 + wi;
+            if (gx >= w) break;
+            const int64_t o = (int64_t)gy * w + gx;
+            if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
+                out[o] = 0.0f;
+                if (means_out) { means_out[o] = 0.0f; stds_out[o] = 0.0f; }
+                continue;
             }
+            const float cs = sqrtf(var[wi] / (float)(K - 1));
+            if (means_out) { means_out[o] = m[wi]; stds_out[o] = cs; }
+            float r;
+            if (M == SRED_ABSDIFF) {  // kernelGetBlockAbsDiffStds2D.cl:103-105
+                r = fabsf(tmpl_std_f - cs);
+            } else if (M == SRED_NRMSE) {  // kernelGetBlockNrmse2D.cl:73-74
+                r = sqrtf(acc[wi] / (float)K);
+            } else {
+                const float cov = acc[wi] / (float)(K - 1);
+                if (tmpl_std_d == 0.0 && cs == 0.0f) {
+                    r = 1.0f;
+                } else if (tmpl_std_d == 0.0 || cs == 0.0f) {
+                    r = 0.0f;
+                } else if (M == SRED_PEARSON) {  // kernelGetBlockPearson2D.cl:88 (double literals)
+                    r = fmaxf(0.0f, (float)((double)cov / ((tmpl_std_d * (double)cs) + eps_d)));
+                } else {  // kernelGetBlockSsim2D.cl:169 (sum denominator, double literals)
+                    const float c1 = 0.0001f, c2 = 0.0009f;
+                    const float mm = m[wi];
+                    double num = (2.0 * tmpl_mean_d * (double)mm + (double)c1) * (double)(2.0f * cov + c2);
+                    double den = ((tmpl_mean_d * tmpl_mean_d) + (double)(mm * mm) + (double)c1) *
+                                 ((tmpl_std_d * tmpl_std_d) + (double)(cs * cs) + (double)c2);
+                    r = fmaxf(0.0f, (float)(num / den));
+                }
+            }
+            out[o] = r;
+            lo = min(lo, f2ord(r));
+            hi = max(hi, f2ord(r));
         }
-        out[o] = r;
+    }
+    if (mmpart) {  // workgroup partial of the normalisation's min/max (Utils.java:991-1012)
+        __shared__ unsigned int slo[256], shi[256];
+        slo[threadIdx.x] = lo;
+        shi[threadIdx.x] = hi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                slo[threadIdx.x] = min(slo[threadIdx.x], slo[threadIdx.x + s]);
+                shi[threadIdx.x] = max(shi[threadIdx.x], shi[threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const int b = blockIdx.y * gridDim.x + blockIdx.x;
+            mmpart[3 * b] = slo[0];
+            mmpart[3 * b + 1] = shi[0];
+            mmpart[3 * b + 2] = 0u;
+        }
     }
 }
 
+int64_t block_workgroups(const Geometry &g) { return (int64_t)((g.w + kBx - 1) / kBx) * ((g.h + kBy - 1) / kBy); }
+
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
-                  float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out)
+                  float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out,
+                  unsigned int *d_mmpart, int *nparts)
 {
     if (metric < 0 || metric > 3) {
         set_error("unknown metric");
@@ -229,6 +255,7 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
     const int RS = (tw + 62) / 64 * 64 + 1;
     const size_t lds = sizeof(float) * (size_t)((th * RS + 3) & ~3) + sizeof(float4) * (size_t)(g.K + 3);
     dim3 grid((g.w + kBx - 1) / kBx, (g.h + kBy - 1) / kBy);
+    if (nparts) *nparts = (int)(grid.x * grid.y);
     float *mo = d_stds_out ? d_means_out : nullptr, *so = d_means_out ? d_stds_out : nullptr;
     const float4 *d_tp = ctx->tmpl.as<float4>();
     const double tm = java_float_literal(tmpl_mean), ts = java_float_literal(tmpl_std), ed = java_float_literal(kEps);
@@ -236,19 +263,19 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
     switch (metric) {
     case SRED_PEARSON:
         hipLaunchKernelGGL(k_block<SRED_PEARSON>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
-                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so, d_mmpart);
         break;
     case SRED_SSIM:
         hipLaunchKernelGGL(k_block<SRED_SSIM>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh, g.K,
-                           rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+                           rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so, d_mmpart);
         break;
     case SRED_NRMSE:
         hipLaunchKernelGGL(k_block<SRED_NRMSE>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
-                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so, d_mmpart);
         break;
     default:
         hipLaunchKernelGGL(k_block<SRED_ABSDIFF>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
-                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so);
+                           g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so, d_mmpart);
     }
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_end(ctx, g.n()));

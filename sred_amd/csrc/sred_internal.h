@@ -13,6 +13,17 @@
 
 namespace sred {
 
+// Order-preserving map of floats to unsigned ints (min/max by integer compare).
+__device__ __forceinline__ unsigned int f2ord(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned int u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
 // Per-thread error text behind sred_last_error().
 void set_error(const std::string &msg);
 
@@ -103,6 +114,7 @@ struct sred_ctx {
     sred::DevBuf tile_var, counts, valid_idx, aux, tflags, u32, uhi, ulo, partial, misc, offs, offs2, tmpl;
     sred::DevBuf wmom, wpart;  // weight-sum moments
     sred::DevBuf sortbuf;      // valid blocks sorted by sigma
+    sred::DevBuf mmpart;       // per-workgroup min/max partials of a block map
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
@@ -219,8 +231,17 @@ int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const floa
                         const float *d_mask, float n_pixels, float c, int normalize, float *d_out);
 // Block-repetition raw kernel output (before host post-processing); window
 // means/stds are written when both output pointers are non-NULL.
+// Normalisation without mask of a map whose interior min/max partials (3
+// words each: min, max, 0 as ordered ints) are given; init_max_min_value as
+// normalize_map_dev.
+int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const unsigned int *d_parts,
+                                 int nparts, int init_max_min_value);
+// d_mmpart (optional): per-workgroup interior min/max partials of the output
+// (block_workgroups(g) of them), *nparts receives their count.
+int64_t block_workgroups(const Geometry &g);
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
-                  float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out);
+                  float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out,
+                  unsigned int *d_mmpart = nullptr, int *nparts = nullptr);
 // Batched global maps (see batch.hip).
 int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem *probs, float c, int normalize);
 // Hu template map (kernelGetPatchHu2D.cl) of a device image; h_tmpl is the

@@ -193,15 +193,6 @@ int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, fl
 // ---------------------------------------------------------------------------
 // Map operations of the host finalize.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned int f2ord(float f)
-{
-    unsigned int u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float ord2f(unsigned int u)
-{
-    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
-}
 __device__ __forceinline__ bool interior(int64_t i, int w, int h, int brw, int brh)
 {
     const int ii = (int)i;  // images hold fewer than 2^31 pixels
@@ -262,8 +253,15 @@ __global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ map, c
     }
 }
 
+__device__ __forceinline__ void k_minmax_final_body(const unsigned int *__restrict__ part, int nparts,
+                                                    unsigned int *__restrict__ mm);
 __global__ __launch_bounds__(256) void k_minmax_final(const unsigned int *__restrict__ part, int nparts,
                                                       unsigned int *__restrict__ mm)
+{
+    k_minmax_final_body(part, nparts, mm);
+}
+__device__ __forceinline__ void k_minmax_final_body(const unsigned int *__restrict__ part, int nparts,
+                                                    unsigned int *__restrict__ mm)
 {
     unsigned int lo = 0xffffffffu, hi = 0u, nb = 0u;
     for (int i = threadIdx.x; i < nparts; i += 256) {
@@ -289,6 +287,20 @@ __global__ __launch_bounds__(256) void k_minmax_final(const unsigned int *__rest
         mm[1] = max(mm[1], shi[0]);
         mm[2] |= snb[0];
     }
+}
+
+// mm = (min, max, non-binary) of the initial values folded with nparts
+// partials (one launch: initialisation and final reduction together).
+__global__ __launch_bounds__(256) void k_minmax_from_parts(const unsigned int *__restrict__ part, int nparts,
+                                                           float init_max, unsigned int *__restrict__ mm)
+{
+    if (threadIdx.x == 0) {
+        mm[0] = f2ord(FLT_MAX);
+        mm[1] = f2ord(init_max);
+        mm[2] = 0;
+    }
+    __syncthreads();
+    k_minmax_final_body(part, nparts, mm);
 }
 
 // Utils.java:1014-1028: (x - min) / (max - min + EPSILON) on the selected pixels.
@@ -336,6 +348,19 @@ int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const floa
     hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min(g.h - 2 * g.brh, 8192))), dim3(256), 0, ctx->stream, d_map,
                        d_mask, g.w, g.h, g.brw, g.brh,
                        mm);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
+int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const unsigned int *d_parts,
+                                 int nparts, int init_max_min_value)
+{
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));
+    unsigned int *mm = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + 1024);
+    const float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
+    hipLaunchKernelGGL(k_minmax_from_parts, dim3(1), dim3(256), 0, ctx->stream, d_parts, nparts, init_max, mm);
+    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min(g.h - 2 * g.brh, 8192))), dim3(256), 0, ctx->stream, d_map,
+                       nullptr, g.w, g.h, g.brw, g.brh, mm);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
