@@ -213,6 +213,230 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float *__restrict__ U
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-BF16 MFMA form of the adjoint pass (blocks up to K = 96).
+//
+// Per 64-column tile a wave (32 rows i) computes the transposed dot-product
+// tile D[j][i] = u_j . u_i with v_mfma_f32_16x16x32_bf16 (A = the column
+// blocks' fragments from LDS, B = the wave's row fragments in registers;
+// hi*hi + hi*lo + lo*hi), then per pair the coefficient P_ij and the
+// dL/ds_i term on the vector ALU.  In the MFMA output layout lane l holds
+// i = l % 16 and j = 4(l / 16) + r, which is already the B-operand layout of
+// the second product G^T[k][i] = sum_j u_jk P_ij when the column blocks'
+// transposed fragments use the same j assignment: V[jg][kg][lane] holds
+// u_{j,k}, k = 16kg + l % 16, j = 32jg + jmap(l / 16, e), jmap(q, e) = 4q + e
+// for e < 4 and 16 + 4q + e - 4 beyond.  P is split into hi + lo BF16 and the
+// second product uses three MFMAs as well; its FP32 accumulators are flushed
+// into FP32 totals every 32 tiles, dL/ds_i into FP64 every tile.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// hi = RNE bf16 pair of (a, b) (one v_cvt_pk_bf16_f32), lo = the same of the remainders
+__device__ __forceinline__ void split_bf16x2(float a, float b, unsigned int &hi, unsigned int &lo)
+{
+    const bf16x2 h = __builtin_convertvector(floatx2{a, b}, bf16x2);
+    const floatx2 hf = __builtin_convertvector(h, floatx2);
+    const bf16x2 l = __builtin_convertvector(floatx2{a - hf.x, b - hf.y}, bf16x2);
+    hi = __builtin_bit_cast(unsigned int, h);
+    lo = __builtin_bit_cast(unsigned int, l);
+}
+
+__global__ __launch_bounds__(256) void k_transpose_frag(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+                                                        int Nv_pad, int KS, uint4 *__restrict__ Vhi,
+                                                        uint4 *__restrict__ Vlo)
+{
+    const int KG = 2 * KS;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(Nv_pad / 32) * KG * 64) return;
+    const int lane = (int)(t & 63);
+    const int64_t rr = t >> 6;
+    const int kg = (int)(rr % KG), jg = (int)(rr / KG);
+    const int k = 16 * kg + (lane & 15), q = lane >> 4;
+    // u_{j,k} sits in U[j/16][k/32][(j%16) + 16((k%32)/8)], element k%8
+    const int ks = k >> 5, lq = (k & 31) >> 3, e8 = k & 7;
+    unsigned int h[4] = {0u, 0u, 0u, 0u}, l[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int j = 32 * jg + (e < 4 ? 4 * q + e : 16 + 4 * q + (e - 4));
+        const size_t o = ((size_t)(j >> 4) * KS + ks) * 64 + (j & 15) + 16 * lq;
+        const unsigned int hw = reinterpret_cast<const unsigned short *>(&Uhi[o])[e8];
+        const unsigned int lw = reinterpret_cast<const unsigned short *>(&Ulo[o])[e8];
+        h[e >> 1] |= (e & 1) ? hw << 16 : hw;
+        l[e >> 1] |= (e & 1) ? lw << 16 : lw;
+    }
+    Vhi[t] = make_uint4(h[0], h[1], h[2], h[3]);
+    Vlo[t] = make_uint4(l[0], l[1], l[2], l[3]);
+}
+
+constexpr int kAdjWaves = 8;                         // waves per workgroup, 16 rows each
+constexpr int kAdjRows = kAdjWaves * 16;             // rows per workgroup
+
+// s_waitcnt vmcnt(0) expcnt(0) lgkmcnt(0): this wave's global->LDS copies and
+// LDS accesses are complete (issued before every barrier of the tile loop).
+__device__ __forceinline__ void adj_wait_all() { __builtin_amdgcn_s_waitcnt(0); }
+
+template <int KS>
+__global__ __launch_bounds__(512, 1) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+                                                         const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
+                                                         const float4 *__restrict__ info, const float *__restrict__ UT,
+                                                         int K, int Nv, int Nv_pad, float eps, float m2_over_H,
+                                                         float inv_sqrt_km1, int K16, float *__restrict__ Gc)
+{
+    constexpr int KG = 2 * KS;
+    constexpr int URUN = 4 * KS * 64, VRUN = 2 * KG * 64;  // uint4 of one 64-column tile
+    // two tile buffers (distinct arrays: reading one does not wait for the copy into the other)
+    __shared__ uint4 uh0[URUN], ul0[URUN], vh0[VRUN], vl0[VRUN];
+    __shared__ uint4 uh1[URUN], ul1[URUN], vh1[VRUN], vl1[VRUN];
+    __shared__ float4 ci0[64], ci1[64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int li = lane & 15, lq = lane >> 4;
+    const int rowbase = blockIdx.x * kAdjRows + wave * 16;
+
+    bf16x8 bh[KS], bl[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const size_t o = ((size_t)(rowbase / 16) * KS + ks) * 64 + lane;
+        bh[ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
+        bl[ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
+    }
+    const float4 R = info[rowbase + li];
+    const float rm = m2_over_H * R.x;  // (-2/H) s_i
+    floatx4 acc[KG];
+    float tot[KG][4];
+#pragma unroll
+    for (int kg = 0; kg < KG; kg++) {
+        acc[kg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; r++) tot[kg][r] = 0.0f;
+    }
+    double gs = 0.0;
+    const int ntiles = Nv_pad / 64;
+
+    // asynchronous copy of tile T: 16 KS runs of 64 uint4 (the column blocks'
+    // fragments hi/lo and their transposed fragments hi/lo) and the info run
+    auto copy = [&](int T, uint4 *duh, uint4 *dul, uint4 *dvh, uint4 *dvl, float4 *dci) {
+        typedef __attribute__((address_space(3))) void *lds_ptr;
+        const int c0 = T * 64;
+        const size_t ou = (size_t)(c0 / 16) * KS * 64, ov = (size_t)(c0 / 32) * KG * 64;
+        for (int run = wave; run < 16 * KS; run += kAdjWaves) {
+            const int a = run / (4 * KS), b = run - a * (4 * KS);
+            const uint4 *src = a == 0 ? Uhi + ou : a == 1 ? Ulo + ou : a == 2 ? Vhi + ov : Vlo + ov;
+            uint4 *dst = a == 0 ? duh : a == 1 ? dul : a == 2 ? dvh : dvl;
+            __builtin_amdgcn_global_load_lds((const void *)(src + b * 64 + lane), (lds_ptr)(dst + b * 64), 16, 0, 0);
+        }
+        if (wave == kAdjWaves - 1)
+            __builtin_amdgcn_global_load_lds((const void *)(info + c0 + lane), (lds_ptr)dci, 16, 0, 0);
+    };
+
+    auto compute = [&](int T, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl, const float4 *ci) {
+        float gsum = 0.0f;
+        // two 32-column steps: the coefficients of a step feed its second-product MFMAs at once
+#pragma unroll
+        for (int st = 0; st < 2; st++) {
+            float pv[8];  // coefficients P of the step, B-operand element order
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                const int cg = 2 * st + half;
+                floatx4 d = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, uh[(cg * KS + ks) * 64 + lane]);
+                    const bf16x8 al = __builtin_bit_cast(bf16x8, ul[(cg * KS + ks) * 64 + lane]);
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ks], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ks], d, 0, 0, 0);
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ks], d, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const float4 C = ci[cg * 16 + 4 * lq + r];  // column j = c0 + 16cg + 4(l/16) + r
+                    const float si = R.x, sj = C.x;
+                    const float dt = R.w - C.w;
+                    const float w = __builtin_amdgcn_exp2f(-(dt * dt));
+                    const float A = R.y + C.y;
+                    const float ss = si * sj;
+                    const float inv = __builtin_amdgcn_rcpf(ss + eps);
+                    const float rv = d[r] * (ss * inv);  // cov / (s_i s_j + eps); 0 for flat blocks
+                    const float rr = fmaxf(rv, 0.0f);
+                    const float M = rv > 0.0f ? w * inv : 0.0f;
+                    const float P = A * M * sj;
+                    // -A M r s_j + w (-2/H)(s_i - s_j)(A r + b_i + b_j)
+                    gsum = fmaf(w * (rm - m2_over_H * sj), fmaf(A, rr, R.z + C.z), fmaf(-P, rr, gsum));
+                    pv[half * 4 + r] = P;
+                }
+            }
+            unsigned int ph[4], pl[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) split_bf16x2(pv[2 * q], pv[2 * q + 1], ph[q], pl[q]);
+            const bf16x8 bH = __builtin_bit_cast(bf16x8, make_uint4(ph[0], ph[1], ph[2], ph[3]));
+            const bf16x8 bL = __builtin_bit_cast(bf16x8, make_uint4(pl[0], pl[1], pl[2], pl[3]));
+#pragma unroll
+            for (int kg = 0; kg < KG; kg++) {
+                const bf16x8 ah = __builtin_bit_cast(bf16x8, vh[(st * KG + kg) * 64 + lane]);
+                const bf16x8 al = __builtin_bit_cast(bf16x8, vl[(st * KG + kg) * 64 + lane]);
+                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bH, acc[kg], 0, 0, 0);
+                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bL, acc[kg], 0, 0, 0);
+                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bH, acc[kg], 0, 0, 0);
+            }
+        }
+        gs += (double)gsum;
+        if ((T & 31) == 31 || T == ntiles - 1) {
+#pragma unroll
+            for (int kg = 0; kg < KG; kg++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) tot[kg][r] += acc[kg][r];
+                acc[kg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+        }
+    };
+
+    // step T: tile T has landed (own copies waited, then the barrier), every
+    // wave is done with tile T-1, whose buffer now receives tile T+1
+    auto step = [&](int T, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl, const float4 *ci,
+                    uint4 *nuh, uint4 *nul, uint4 *nvh, uint4 *nvl, float4 *nci) {
+        adj_wait_all();
+        __syncthreads();
+        if (T + 1 < ntiles) copy(T + 1, nuh, nul, nvh, nvl, nci);
+        compute(T, uh, ul, vh, vl, ci);
+    };
+    copy(0, uh0, ul0, vh0, vl0, ci0);
+    for (int T = 0; T < ntiles; T += 2) {
+        step(T, uh0, ul0, vh0, vl0, ci0, uh1, ul1, vh1, vl1, ci1);
+        if (T + 1 < ntiles) step(T + 1, uh1, ul1, vh1, vl1, ci1, uh0, ul0, vh0, vl0, ci0);
+    }
+
+    // per row i: G_k = (sum_j P_ij u_jk + dL/ds_i u_ik) / sqrt(K-1), centred over k < K.
+    // Lane l holds i = l % 16 and k = 16kg + 4(l/16) + r.
+    const int v = rowbase + li;
+    double g = gs;
+    g += __shfl_xor(g, 16);
+    g += __shfl_xor(g, 32);
+    const bool live = v < Nv && R.x != 0.0f;
+    double gk[KG][4], part = 0.0;
+#pragma unroll
+    for (int kg = 0; kg < KG; kg++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k = 16 * kg + 4 * lq + r;
+            const double uik = (k < K) ? (double)UT[(size_t)k * Nv_pad + v] : 0.0;
+            gk[kg][r] = (k < K) ? ((double)tot[kg][r] + g * uik) * (double)inv_sqrt_km1 : 0.0;
+            part += gk[kg][r];
+        }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    const double mean = part / (double)K;
+#pragma unroll
+    for (int kg = 0; kg < KG; kg++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k = 16 * kg + 4 * lq + r;
+            if (k < K16) Gc[(size_t)v * K16 + k] = (live && k < K) ? (float)(gk[kg][r] - mean) : 0.0f;
+        }
+}
+
 // dL/dx_q = sum_k Gc[block centred at q - off_k][k] (transpose of the gather).
 __global__ __launch_bounds__(256) void k_adjoint_scatter(const float *__restrict__ Gc, const int *__restrict__ pix2v,
                                                          const int *__restrict__ dxy, int K, int K16, int w, int h,
@@ -274,7 +498,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
         set_error("block too large for the adjoint tile");
         return SRED_ERR_ARG;
     }
-    // forward: statistics, noise variance, mask, accumulators (FP32) and the normalised map
+    // forward: statistics, noise variance, mask, accumulators and the normalised map
     SRED_TRY(ctx->means.ensure(sizeof(float) * n));
     SRED_TRY(ctx->stds.ensure(sizeof(float) * n));
     SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
@@ -289,7 +513,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     SRED_TRY(relevance_mask_dev(ctx, g, d_stds, 0.0f, d_mask, &np));
     GlobalParams p;
     p.metric = SRED_PEARSON;
-    p.precision = SRED_FP32;
+    p.precision = SRED_BF16X3;  // the forward map: split-BF16 symmetric contraction
     p.filter_param = nv;
     p.threshold = 0.0f;
     SRED_TRY(global_accum_dev(ctx, g, p, d_img, d_means, d_stds, d_S, d_W));
@@ -298,7 +522,9 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     const int Nv = ctx->last_nv;
     SRED_TRY(zero_dev(ctx, d_grad, sizeof(float) * n));
     if (Nv == 0) return SRED_OK;
-    const int Nv_pad = (Nv + 63) / 64 * 64;
+    const int KS = (K + 31) / 32;
+    const bool mfma = KS <= 3;  // split-BF16 MFMA pass (its accumulators fit the registers up to K = 96)
+    const int Nv_pad = mfma ? (Nv + kAdjRows - 1) / kAdjRows * kAdjRows : (Nv + 63) / 64 * 64;
     const int *d_idx = ctx->valid_idx.as<int>();
 
     const double H = java_float_literal(nv) + java_float_literal(kEps);
@@ -316,11 +542,38 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
                       inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), ctx->u32.as<float>());
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)Nv_pad * K16));
     float *d_Gc = ctx->partial.as<float>();
-    SRED_TRY(time_begin(ctx));
-    SRED_TRY(launch_adjoint(K16 / 16, dim3(Nv_pad / 64), lds, ctx->stream, ctx->u32.as<float>(), d_info, K, K4, Nv,
-                            Nv_pad, kEps, (float)(-2.0 / H), inv_sqrt_km1, d_Gc));
-    SRED_HIP_CHECK(hipGetLastError());
-    SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
+    if (mfma) {
+        const size_t u4 = (size_t)Nv_pad * KS * 4;  // uint4 per array (fragment-major, and transposed)
+        SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
+        SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
+        SRED_TRY(ctx->adj_v.ensure(2 * sizeof(uint4) * u4));
+        uint4 *d_uhi = ctx->uhi.as<uint4>(), *d_ulo = ctx->ulo.as<uint4>();
+        uint4 *d_vhi = ctx->adj_v.as<uint4>(), *d_vlo = d_vhi + u4;
+        launch_gather_bf16(ctx, SRED_PEARSON, d_img, d_means, d_stds, d_idx, Nv, Nv_pad, K, KS,
+                           ctx->offs2.as<int>(), inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), d_uhi, d_ulo);
+        hipLaunchKernelGGL(k_transpose_frag, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, d_uhi,
+                           d_ulo, Nv_pad, KS, d_vhi, d_vlo);
+        SRED_TRY(time_begin(ctx));
+        const float m2H = (float)(-2.0 / H);
+        const float *UTp = ctx->u32.as<float>();
+        if (KS == 1)
+            hipLaunchKernelGGL(k_adjoint_mfma<1>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
+                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+        else if (KS == 2)
+            hipLaunchKernelGGL(k_adjoint_mfma<2>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
+                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+        else
+            hipLaunchKernelGGL(k_adjoint_mfma<3>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
+                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+        SRED_HIP_CHECK(hipGetLastError());
+        SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
+    } else {
+        SRED_TRY(time_begin(ctx));
+        SRED_TRY(launch_adjoint(K16 / 16, dim3(Nv_pad / 64), lds, ctx->stream, ctx->u32.as<float>(), d_info, K, K4,
+                                Nv, Nv_pad, kEps, (float)(-2.0 / H), inv_sqrt_km1, d_Gc));
+        SRED_HIP_CHECK(hipGetLastError());
+        SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
+    }
     // pixel -> block index, then the transpose stencil
     SRED_TRY(ctx->counts.ensure(sizeof(int) * (size_t)n));
     int *d_pix2v = ctx->counts.as<int>();

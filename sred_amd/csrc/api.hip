@@ -270,7 +270,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
-                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart};
+                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();

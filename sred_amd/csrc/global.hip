@@ -198,13 +198,6 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *
 // ---------------------------------------------------------------------------
 // 2. gather
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned short f2bf(float f)
-{
-    unsigned int u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
 
 // Per-block row scale of u for each metric.
 __device__ __forceinline__ float row_scale(int metric, float sigma, float inv_sqrt_km1, float inv_sqrt_k)
@@ -1097,6 +1090,15 @@ static int choose_chunk_cols(int Nv_pad, int bytes_per_col, int nrowtiles, int o
         cols = std::max(64, std::min(std::min(cols, cols_fill), max_cols));
     }
     return std::min(cols, Nv_pad);
+}
+
+void launch_gather_bf16(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
+                        const int *d_idx, int Nv, int Nv_pad, int K, int KS, const int *d_offs, float inv_sqrt_km1,
+                        float inv_sqrt_k, uint4 *Uhi, uint4 *Ulo)
+{
+    size_t u4 = (size_t)Nv_pad * KS * 4;
+    hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
+                       d_means, d_stds, d_idx, Nv, Nv_pad, K, KS, d_offs, inv_sqrt_km1, inv_sqrt_k, Uhi, Ulo);
 }
 
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,

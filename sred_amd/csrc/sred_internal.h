@@ -24,6 +24,15 @@ __device__ __forceinline__ float ord2f(unsigned int u)
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
+// bfloat16 by round-to-nearest-even (finite inputs), and back.
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned int)b << 16); }
+
 // Per-thread error text behind sred_last_error().
 void set_error(const std::string &msg);
 
@@ -115,6 +124,7 @@ struct sred_ctx {
     sred::DevBuf wmom, wpart;  // weight-sum moments
     sred::DevBuf sortbuf;      // valid blocks sorted by sigma
     sred::DevBuf mmpart;       // per-workgroup min/max partials of a block map
+    sred::DevBuf adj_v;        // transposed split-BF16 fragments of the adjoint
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
@@ -218,6 +228,10 @@ int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_s
 // d_next: a 64-bit device counter the launch zeroes and its workgroups claim items from.
 int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
                       unsigned long long *d_next, int P, EpiConst ec);
+// Split-BF16 gather of the valid blocks, fragment-major (Uhi/Ulo[Nv_pad/16][KS][64]).
+void launch_gather_bf16(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
+                        const int *d_idx, int Nv, int Nv_pad, int K, int KS, const int *d_offs, float inv_sqrt_km1,
+                        float inv_sqrt_k, uint4 *Uhi, uint4 *Ulo);
 // FP32 gather of the valid blocks, k-major (UT[k][v]).
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
                        const int *d_idx, int Nv, int Nv_pad, int K, int K4, const int *d_offs, float inv_sqrt_km1,
