@@ -188,6 +188,32 @@ int sred_block_hu_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, const 
  * block rectangle; cells outside the ellipse are 0.  Host only. */
 int sred_hu_prepare_template_2d(const float *blk, int bw, int bh, float *tmpl_out);
 
+/* ---- variance stabilisation (the optional step before the path) ---- */
+/* VarianceStabilisingTransform2D_.getGAT (VarianceStabilisingTransform2D_.java:136-151),
+ * in place: v <- (g v + ref < 0) ? 0 : (2/g) sqrt(g v + ref),
+ * ref = 3/8 g^2 + s^2 - g o, double arithmetic rounded to float (bit-exact).
+ * Utils.getInputImage2D (Utils.java:697-702) passes the fitted parameters
+ * rounded to float.  Synchronous; _dev enqueues on the context's stream. */
+int sred_gat_2d(sred_ctx *ctx, float *img, int64_t n, double gain, double sigma, double offset);
+int sred_gat_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset);
+/* GATMinimizer2D.userFunction (GATMinimizer2D.java:64-110): mean squared
+ * deviation from 1 of the variances of the (w/6) x (h/6)-pixel blocks after
+ * the fit's transform (special case v <= -ref/g).  Block sums are formed row
+ * by row (each row's sequential double sum, rows added in order), blocks in
+ * raster order.  Needs w, h >= 6. */
+int sred_gat_objective_2d(sred_ctx *ctx, const float *img, int w, int h, double gain, double sigma, double offset,
+                          double *error);
+/* GATMinimizer2D.run (GATMinimizer2D.java:33-61) with the library's own
+ * Nelder-Mead simplex in place of ImageJ's Minimizer (coefficients 1, 2,
+ * 1/2, 1/2; initial simplex (gain0, sigma0, offset0) + (1, 10, 100) steps;
+ * max_iter iterations or until every vertex has the same objective).
+ * params[3] = (gain, sigma, offset); *error (may be NULL) = the objective
+ * there.  The reference calls it with (1, 10, 100) and 5000 iterations. */
+int sred_gat_minimize_2d(sred_ctx *ctx, const float *img, int w, int h, double gain0, double sigma0, double offset0,
+                         int max_iter, double *params, double *error);
+int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0,
+                             double offset0, int max_iter, double *params, double *error);
+
 /* ---- all-pairs vector maps (OpenCL kernels without a Java caller) ---- */
 /* Per-pixel Hu invariant hu = nu20 + nu02 of kernelGetHuMap.cl:146-172
  * (getInvariant on the bw x bh rectangle, patch = pixel - means[p]), in the

@@ -28,6 +28,7 @@ PEARSON, SSIM, NRMSE, ABSDIFF = 0, 1, 2, 3
 _F = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _D = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _I = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_FN3 = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 def build():
@@ -89,6 +90,14 @@ def lib():
         L.or_phase_weight.restype = c_double
         L.or_phase_map.argtypes = [_F, _F, _F, c_int, c_int, c_int, c_int, c_double, vp, c_i64, _D, c_int]
         L.or_phase_map.restype = None
+        L.or_gat.argtypes = [_F, c_i64, c_double, c_double, c_double]
+        L.or_gat.restype = None
+        L.or_gat_objective.argtypes = [_F, c_int, c_int, c_double, c_double, c_double]
+        L.or_gat_objective.restype = c_double
+        L.or_nelder_mead3.argtypes = [_FN3, vp, _D, _D, c_int, _D]
+        L.or_nelder_mead3.restype = c_double
+        L.or_gat_minimize.argtypes = [_F, c_int, c_int, c_double, c_double, c_double, c_int, _D]
+        L.or_gat_minimize.restype = c_double
         _lib = L
     return _lib
 
@@ -371,3 +380,34 @@ def phase_map(img, means, stds, bw, bh, rows=None, eps=None, nthreads=0):
     lib().or_phase_map(img.reshape(-1), means.reshape(-1), stds.reshape(-1), w, h, bw, bh,
                        hu_eps() if eps is None else eps, rp, nr, out.reshape(-1), nthreads)
     return out
+
+
+def gat(img, gain, sigma, offset):
+    """VarianceStabilisingTransform2D_.getGAT (a copy)."""
+    out = _f32(img).copy()
+    lib().or_gat(out.reshape(-1), out.size, float(gain), float(sigma), float(offset))
+    return out
+
+
+def gat_objective(img, gain, sigma, offset):
+    """GATMinimizer2D.userFunction (block sums row by row)."""
+    img = _f32(img)
+    h, w = img.shape
+    return lib().or_gat_objective(img.reshape(-1), w, h, float(gain), float(sigma), float(offset))
+
+
+def nelder_mead3(f, x0, d, max_iter):
+    """The oracle's Nelder-Mead on a Python function of 3 parameters."""
+    cb = _FN3(lambda x, _c: float(f(np.array([x[0], x[1], x[2]]))))
+    x = np.zeros(3)
+    fx = lib().or_nelder_mead3(cb, None, np.asarray(x0, np.float64), np.asarray(d, np.float64), int(max_iter), x)
+    return x, fx
+
+
+def gat_minimize(img, gain0=1.0, sigma0=10.0, offset0=100.0, max_iter=5000):
+    """GATMinimizer2D.run with the oracle's simplex: (gain, sigma, offset), objective."""
+    img = _f32(img)
+    h, w = img.shape
+    x = np.zeros(3)
+    fx = lib().or_gat_minimize(img.reshape(-1), w, h, gain0, sigma0, offset0, int(max_iter), x)
+    return x, fx

@@ -1040,3 +1040,147 @@ void or_phase_map(const float *img, const float *means, const float *stds, int w
     }
     free(sre); free(sim); free(c); free(s);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Variance stabilisation: GAT and its simplex fit                          */
+/* ------------------------------------------------------------------------ */
+
+/* VarianceStabilisingTransform2D_.getGAT, VarianceStabilisingTransform2D_.java:136-151:
+ * ref = 3/8 g^2 + s^2 - g o; v <- (g v + ref < 0) ? 0 : (2/g) sqrt(g v + ref),
+ * in double, rounded to float, in place. */
+void or_gat(float *px, int64_t n, double gain, double sigma, double offset)
+{
+    double ref = (3.0 / 8.0) * gain * gain + sigma * sigma - gain * offset;
+    for (int64_t i = 0; i < n; i++) {
+        double v = (double)px[i];
+        if ((gain * v + ref) < 0.0)
+            v = 0.0;
+        else
+            v = (2.0 / gain) * sqrt(gain * v + ref);
+        px[i] = (float)v;
+    }
+}
+
+/* The per-pixel transform of GATMinimizer2D.applyGAT (GATMinimizer2D.java:140-155),
+ * whose special case is v <= -ref/gain. */
+static float gat_fit_px(float p, double gain, double ref)
+{
+    double v = (double)p;
+    if (v <= -ref / gain)
+        v = 0.0;
+    else
+        v = (2.0 / gain) * sqrt(gain * v + ref);
+    return (float)v;
+}
+
+/* GATMinimizer2D.userFunction (GATMinimizer2D.java:64-110): the image is cut
+ * into nbx x nby blocks of (w/6) x (h/6) pixels (nbx = w / (w/6)); each
+ * block's mean and variance of the transformed pixels (sum v and sum of the
+ * float squares v*v, in double; var = sq/n - mean^2, :115-137) gives
+ * error += (var - 1)^2 / nBlocks in block raster order.  The sums of a block
+ * are taken row by row: each row's sequential double sum over x, then the
+ * row sums added in row order (the reference adds all pixels of a block in
+ * one sequence; this order is the one the GPU reduction reproduces exactly).
+ * Returns NaN for images narrower or shorter than 6 pixels (the reference
+ * divides by zero there). */
+double or_gat_objective(const float *px, int w, int h, double gain, double sigma, double offset)
+{
+    int bw = w / 6, bh = h / 6;
+    if (bw <= 0 || bh <= 0) return NAN;
+    int nbx = w / bw, nby = h / bh, nb = nbx * nby;
+    double ref = (3.0 / 8.0) * gain * gain + sigma * sigma - gain * offset;
+    double error = 0.0;
+    for (int by = 0; by < nby; by++)
+        for (int bx = 0; bx < nbx; bx++) {
+            double mean = 0.0, sq = 0.0;
+            for (int j = by * bh; j < (by + 1) * bh; j++) {
+                double rm = 0.0, rs = 0.0;
+                for (int i = bx * bw; i < (bx + 1) * bw; i++) {
+                    float v = gat_fit_px(px[(int64_t)j * w + i], gain, ref);
+                    rm += v;
+                    rs += v * v;  /* float product, as the reference's sq_sum += v * v */
+                }
+                mean += rm;
+                sq += rs;
+            }
+            int n = bw * bh;
+            mean = mean / n;
+            double var = sq / n - mean * mean;
+            double delta = var - 1.0;
+            error += (delta * delta) / (double)nb;
+        }
+    return error;
+}
+
+/* Nelder-Mead simplex in 3 dimensions (the build's own minimiser in place of
+ * ImageJ's Minimizer used by GATMinimizer2D.run, GATMinimizer2D.java:33-61;
+ * ImageJ's restarts and random initialisation are not reproduced).
+ * Lagarias et al. (1998) coefficients: reflection 1, expansion 2,
+ * contraction 1/2, shrink 1/2.  Initial simplex x0, x0 + d_i e_i.  Each
+ * iteration orders the vertices by f (stable: ties keep their order), then
+ * tries the reflection, expansion, outside or inside contraction, or
+ * shrinks.  Stops after max_iter iterations or when every vertex has the
+ * same f (the reference's setMaxError(0)).  x receives the best vertex;
+ * returns its f.  f(params, ctx) is the objective. */
+typedef double (*or_fn3)(const double *x, void *ctx);
+
+double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, int max_iter, double *x)
+{
+    double v[4][3], fv[4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) v[i][k] = x0[k];
+        if (i > 0) v[i][i - 1] += d[i - 1];
+        fv[i] = f(v[i], ctx);
+    }
+    for (int it = 0; it < max_iter; it++) {
+        /* stable insertion sort by f */
+        for (int i = 1; i < 4; i++)
+            for (int j = i; j > 0 && fv[j] < fv[j - 1]; j--) {
+                double tf = fv[j]; fv[j] = fv[j - 1]; fv[j - 1] = tf;
+                for (int k = 0; k < 3; k++) { double t = v[j][k]; v[j][k] = v[j - 1][k]; v[j - 1][k] = t; }
+            }
+        if (fv[3] == fv[0]) break;
+        double c[3], xr[3], xe[3], xc[3];
+        for (int k = 0; k < 3; k++) c[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0;
+        for (int k = 0; k < 3; k++) xr[k] = c[k] + (c[k] - v[3][k]);
+        double fr = f(xr, ctx);
+        if (fr < fv[0]) {
+            for (int k = 0; k < 3; k++) xe[k] = c[k] + 2.0 * (c[k] - v[3][k]);
+            double fe = f(xe, ctx);
+            if (fe < fr) { for (int k = 0; k < 3; k++) v[3][k] = xe[k]; fv[3] = fe; }
+            else { for (int k = 0; k < 3; k++) v[3][k] = xr[k]; fv[3] = fr; }
+            continue;
+        }
+        if (fr < fv[2]) { for (int k = 0; k < 3; k++) v[3][k] = xr[k]; fv[3] = fr; continue; }
+        int outside = fr < fv[3];
+        for (int k = 0; k < 3; k++) xc[k] = outside ? c[k] + 0.5 * (xr[k] - c[k]) : c[k] + 0.5 * (v[3][k] - c[k]);
+        double fc = f(xc, ctx);
+        if (outside ? fc <= fr : fc < fv[3]) { for (int k = 0; k < 3; k++) v[3][k] = xc[k]; fv[3] = fc; continue; }
+        for (int i = 1; i < 4; i++) {  /* shrink towards the best vertex */
+            for (int k = 0; k < 3; k++) v[i][k] = v[0][k] + 0.5 * (v[i][k] - v[0][k]);
+            fv[i] = f(v[i], ctx);
+        }
+    }
+    int b = 0;
+    for (int i = 1; i < 4; i++) if (fv[i] < fv[b]) b = i;
+    for (int k = 0; k < 3; k++) x[k] = v[b][k];
+    return fv[b];
+}
+
+typedef struct { const float *px; int w, h; } gat_ctx;
+static double gat_obj3(const double *x, void *c)
+{
+    const gat_ctx *g = (const gat_ctx *)c;
+    return or_gat_objective(g->px, g->w, g->h, x[0], x[1], x[2]);
+}
+
+/* GATMinimizer2D.run with the simplex above: initial (gain, sigma, offset)
+ * and steps (1, 10, 100) (GATMinimizer2D.java:36-45).  params[3] receives
+ * the fit; returns the objective there. */
+double or_gat_minimize(const float *px, int w, int h, double gain0, double sigma0, double offset0, int max_iter,
+                       double *params)
+{
+    gat_ctx c = {px, w, h};
+    double x0[3] = {gain0, sigma0, offset0}, d[3] = {1.0, 10.0, 100.0};
+    return or_nelder_mead3(gat_obj3, &c, x0, d, max_iter, params);
+}

@@ -163,6 +163,20 @@ double or_phase_weight(float a, float b);
 void or_phase_map(const float *img, const float *means, const float *stds, int w, int h, int bw, int bh,
                   double eps, const int32_t *rows, int64_t nrows, double *out, int nthreads);
 
+/* Variance stabilisation (VarianceStabilisingTransform2D_.getGAT and
+ * GATMinimizer2D).  or_gat: getGAT in place (:136-151).  or_gat_objective:
+ * GATMinimizer2D.userFunction with block sums row by row (definition in
+ * sred_oracle.c).  or_nelder_mead3 / or_gat_minimize: the build's own
+ * Nelder-Mead simplex in place of ImageJ's Minimizer (parity of the fit
+ * with ImageJ unpinned; the library's simplex follows this one step for
+ * step). */
+void or_gat(float *px, int64_t n, double gain, double sigma, double offset);
+double or_gat_objective(const float *px, int w, int h, double gain, double sigma, double offset);
+typedef double (*or_fn3)(const double *x, void *ctx);
+double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, int max_iter, double *x);
+double or_gat_minimize(const float *px, int w, int h, double gain0, double sigma0, double offset0, int max_iter,
+                       double *params);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ EXPORTS = [
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
     "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_hu_invariants_2d", "sred_global_hu_2d",
     "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev", "sred_ctx_last_skipped",
+    "sred_gat_2d", "sred_gat_2d_dev", "sred_gat_objective_2d", "sred_gat_minimize_2d", "sred_gat_minimize_2d_dev",
 ]
 
 
@@ -63,6 +64,7 @@ _VP = ctypes.c_void_p
 _I = ctypes.c_int
 _FL = ctypes.c_float
 _I64 = ctypes.c_int64
+_D = ctypes.c_double
 
 
 def load():
@@ -108,6 +110,12 @@ def load():
         "sred_block_map_2d_dev": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_ctx_last_contraction": ([_VP, _F, ctypes.POINTER(_I), ctypes.POINTER(_I64)], _I),
         "sred_ctx_last_skipped": ([_VP, ctypes.POINTER(_I64)], _I),
+        "sred_gat_2d": ([_VP, _F, _I64, _D, _D, _D], _I),
+        "sred_gat_2d_dev": ([_VP, _VP, _I64, _D, _D, _D], _I),
+        "sred_gat_objective_2d": ([_VP, _F, _I, _I, _D, _D, _D, ctypes.POINTER(_D)], _I),
+        "sred_gat_minimize_2d": ([_VP, _F, _I, _I, _D, _D, _D, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)], _I),
+        "sred_gat_minimize_2d_dev": ([_VP, _VP, _I, _I, _D, _D, _D, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)],
+                                     _I),
         "sred_global_plan_2d": ([_I, _I, _I, _I, _I, _I, ctypes.POINTER(_I64)], _I),
         "sred_global_pearson_adjoint_2d": ([_VP, _F, _I, _I, _I, _I, _F, _F], _I),
         "sred_global_pearson_adjoint_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),

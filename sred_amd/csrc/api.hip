@@ -793,6 +793,64 @@ int sred_phase_correlation_2d_dev(sred_ctx *ctx, const float *d_img, const float
     return pair_map_dev(ctx, 1, d_img, d_means, d_stds, w, h, bw, bh, rows, nrows, d_out);
 }
 
+int sred_gat_2d(sred_ctx *ctx, float *img, int64_t n, double gain, double sigma, double offset)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    if (n == 0) return SRED_OK;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(gat_dev(ctx, ctx->img.as<float>(), n, gain, sigma, offset));
+    return download(ctx, img, ctx->img, n);
+}
+
+int sred_gat_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    return gat_dev(ctx, d_img, n, gain, sigma, offset);
+}
+
+int sred_gat_objective_2d(sred_ctx *ctx, const float *img, int w, int h, double gain, double sigma, double offset,
+                          double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !error || w <= 0 || h <= 0) {
+        set_error("null buffer or empty image");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, (int64_t)w * h));
+    return gat_objective_dev(ctx, ctx->img.as<float>(), w, h, gain, sigma, offset, error);
+}
+
+int sred_gat_minimize_2d(sred_ctx *ctx, const float *img, int w, int h, double gain0, double sigma0, double offset0,
+                         int max_iter, double *params, double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !params || w <= 0 || h <= 0 || max_iter < 0) {
+        set_error("null buffer, empty image or negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, (int64_t)w * h));
+    return gat_minimize_dev(ctx, ctx->img.as<float>(), w, h, gain0, sigma0, offset0, max_iter, params, error);
+}
+
+int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0,
+                             double offset0, int max_iter, double *params, double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !params || w <= 0 || h <= 0 || max_iter < 0) {
+        set_error("null buffer, empty image or negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    return gat_minimize_dev(ctx, d_img, w, h, gain0, sigma0, offset0, max_iter, params, error);
+}
+
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
                          float relevance_constant, int normalize)
 {

@@ -275,6 +275,13 @@ int global_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
                   const int32_t *rows, int64_t nrows, float *d_out);
 int phase_map_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means, const float *d_stds,
                   const int32_t *rows, int64_t nrows, float *d_out);
+// Variance stabilisation (vst.hip): getGAT in place on a device image; the
+// GATMinimizer2D objective of one parameter set; the simplex fit.
+int gat_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset);
+int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain, double sigma, double offset,
+                      double *error);
+int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0, double offset0,
+                     int max_iter, double *params, double *error);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);

@@ -2,8 +2,9 @@
 
 Same names (snake_case), argument meaning and error behaviour as the Java
 methods; the arithmetic runs in libsred_hip.so's host C++ functions with Java
-float semantics.  Variance stabilisation (GAT/VST) is not part of this path:
-asking for it raises NotImplementedError.
+float semantics.  Variance stabilisation by the generalised Anscombe
+transform runs on the GPU with the library's own simplex fit ("Simplex"
+method); the "Quad/Octree" method raises NotImplementedError.
 """
 import ctypes
 from dataclasses import dataclass
@@ -54,17 +55,89 @@ def block_size_2d(block_width, block_height):
     return L.check(L.load().sred_block_size_2d(block_width, block_height))
 
 
-def get_input_image_2d(image, stabilise_noise_variance=False, normalize_output=True):
+def get_gat_2d(pixels, gain, sigma, offset, resources=None):
+    """VarianceStabilisingTransform2D_.getGAT (VarianceStabilisingTransform2D_.java:136-151).
+
+    Returns a new float32 array (the Java method transforms in place).
+    """
+    arr = L.f32(np.array(pixels, dtype=np.float32, copy=True))
+    own = resources is None
+    res = L.Context(0) if own else resources
+    try:
+        L.check(L.load().sred_gat_2d(res.handle, L.fptr(arr.reshape(-1)), arr.size, float(gain), float(sigma),
+                                     float(offset)))
+    finally:
+        if own:
+            res.close()
+    return arr
+
+
+class GATMinimizer2D:
+    """GATMinimizer2D (GATMinimizer2D.java:15-156): fits (gain, sigma, offset)
+    so that the GAT-transformed image has unit variance in its (w/6) x (h/6)
+    blocks.  run() uses the library's Nelder-Mead simplex in place of
+    ImageJ's Minimizer (which is not restated)."""
+
+    def __init__(self, pixels, width, height, gain, sigma, offset, max_iter, resources=None):
+        self.pixels = L.f32(pixels).reshape(-1)
+        self.width, self.height = int(width), int(height)
+        self.gain, self.sigma, self.offset = float(gain), float(sigma), float(offset)
+        self.max_iter = int(max_iter)
+        self.error = None
+        self._res = resources
+
+    def _with_ctx(self, fn):
+        own = self._res is None
+        res = L.Context(0) if own else self._res
+        try:
+            return fn(res)
+        finally:
+            if own:
+                res.close()
+
+    def run(self):
+        params = (ctypes.c_double * 3)()
+        err = ctypes.c_double()
+
+        def go(res):
+            L.check(L.load().sred_gat_minimize_2d(res.handle, L.fptr(self.pixels), self.width, self.height, self.gain,
+                                                  self.sigma, self.offset, self.max_iter, params,
+                                                  ctypes.byref(err)))
+        self._with_ctx(go)
+        self.gain, self.sigma, self.offset = params[0], params[1], params[2]
+        self.error = err.value
+
+    def user_function(self, params, _v=0.0):
+        """The objective at params = (gain, sigma, offset) (GATMinimizer2D.java:64-110)."""
+        err = ctypes.c_double()
+
+        def go(res):
+            L.check(L.load().sred_gat_objective_2d(res.handle, L.fptr(self.pixels), self.width, self.height,
+                                                   float(params[0]), float(params[1]), float(params[2]),
+                                                   ctypes.byref(err)))
+        self._with_ctx(go)
+        return err.value
+
+
+def get_input_image_2d(image, stabilise_noise_variance=False, normalize_output=True, gat_method="Simplex",
+                       max_iter=5000, resources=None):
     """Utils.getInputImage2D (Utils.java:653-725, overload :746-775) for an array.
 
-    Converts to float32 and min-max normalises as (v-min)/(max-min+EPSILON).
+    Converts to float32; with stabilise_noise_variance, fits the GAT
+    parameters (GATMinimizer2D with initial (1, 10, 100), Utils.java:695) and
+    applies getGAT with them rounded to float (:697-701); then min-max
+    normalises as (v-min)/(max-min+EPSILON).
     """
-    if stabilise_noise_variance:
-        raise NotImplementedError("variance stabilisation (GAT) is outside this library's path")
     arr = L.f32(np.array(image, dtype=np.float32, copy=True))
     if arr.ndim != 2:
         raise ValueError("expected a 2D image")
     h, w = arr.shape
+    if stabilise_noise_variance:
+        if gat_method != "Simplex":
+            raise NotImplementedError("the Quad/Octree GAT method is not built")
+        m = GATMinimizer2D(arr, w, h, 1, 10, 100, max_iter, resources)
+        m.run()
+        arr = get_gat_2d(arr, np.float32(m.gain), np.float32(m.sigma), np.float32(m.offset), resources).reshape(h, w)
     if normalize_output:
         L.check(L.load().sred_normalize_input_2d(L.fptr(arr), arr.size))
     return InputImage2D(arr, w, h, w * h)

@@ -1,0 +1,69 @@
+"""GPU variance stabilisation against the oracle: getGAT bit-exact, the
+GATMinimizer2D objective bit-exact (same summation order), and the simplex
+fit step for step (identical parameters), then the stabilised input image
+of Utils.getInputImage2D."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sred_amd
+from sred_amd import _lib as L
+from sred_amd import utils
+
+pytestmark = pytest.mark.gpu
+
+
+def poisson_gauss(h, w, seed, gain=2.0, mean=50.0, sigma=3.0, offset=10.0):
+    rng = np.random.default_rng(seed)
+    lam = mean * (1.0 + 0.5 * np.sin(np.arange(w)[None, :] / 17.0) * np.cos(np.arange(h)[:, None] / 23.0))
+    return (gain * rng.poisson(lam) + rng.normal(offset, sigma, (h, w))).astype(np.float32)
+
+
+@pytest.mark.parametrize("params", [(1.0, 10.0, 100.0), (2.5, 3.0, 40.0), (0.7, 0.0, -5.0)])
+def test_gat_bitexact(ctx, params):
+    img = poisson_gauss(123, 97, 1)
+    img[:3] = -500.0  # special case rows
+    got = utils.get_gat_2d(img, *params, resources=ctx)
+    np.testing.assert_array_equal(got, O.gat(img, *params))
+
+
+@pytest.mark.parametrize("shape", [(6, 6), (13, 29), (100, 77), (256, 256), (1024, 1024)])
+def test_gat_objective_bitexact(ctx, shape):
+    img = poisson_gauss(*shape, seed=2)
+    m = utils.GATMinimizer2D(img, shape[1], shape[0], 1, 10, 100, 10, resources=ctx)
+    for p in [(1.0, 10.0, 100.0), (2.0, 3.0, 10.0), (0.5, 20.0, -30.0)]:
+        assert m.user_function(p) == O.gat_objective(img, *p)
+
+
+@pytest.mark.parametrize("shape,iters", [((48, 60), 120), ((128, 96), 250), ((256, 256), 300)])
+def test_gat_minimize_same_path_as_oracle(ctx, shape, iters):
+    img = poisson_gauss(*shape, seed=3)
+    m = utils.GATMinimizer2D(img, shape[1], shape[0], 1, 10, 100, iters, resources=ctx)
+    m.run()
+    p, f = O.gat_minimize(img, 1.0, 10.0, 100.0, iters)
+    assert (m.gain, m.sigma, m.offset) == tuple(p)
+    assert m.error == f
+
+
+def test_stabilised_input_image(ctx):
+    img = poisson_gauss(96, 120, 4)
+    got = utils.get_input_image_2d(img, stabilise_noise_variance=True, max_iter=200, resources=ctx).image_array
+    p, _ = O.gat_minimize(img, 1.0, 10.0, 100.0, 200)
+    want = O.normalize_input(O.gat(img, *(float(np.float32(v)) for v in p)))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_gat_minimize_device_path_and_errors(ctx):
+    torch = pytest.importorskip("torch")
+    img = poisson_gauss(64, 64, 5)
+    d = torch.from_numpy(img).cuda()
+    params = (ctypes.c_double * 3)()
+    err = ctypes.c_double()
+    L.check(L.load().sred_gat_minimize_2d_dev(ctx.handle, d.data_ptr(), 64, 64, 1.0, 10.0, 100.0, 80, params,
+                                              ctypes.byref(err)))
+    p, f = O.gat_minimize(img, 1.0, 10.0, 100.0, 80)
+    assert tuple(params) == tuple(p) and err.value == f
+    with pytest.raises(sred_amd.SredError):
+        utils.GATMinimizer2D(np.zeros((5, 40), np.float32), 40, 5, 1, 10, 100, 10, resources=ctx).run()
