@@ -52,12 +52,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: bf16x3 steps only, then a 256 MiB device copy (PMC calibration)")
-    ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase"], default="global",
+    ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase", "vst"],
+                    default="global",
                     help="global: the headline metric (default); block: block repetition 2048x2048 with a 15x15 "
                          "ring template (four metrics and the Hu map); adjoint: global Pearson map + adjoint, "
                          "512x512, 9x9; batch: 64 images of sides U{128..1024} x block sizes {5,9,13,17} "
                          "in one batched call; hu: global Hu map 1024x1024, 11x11; phase: phase correlation map "
-                         "256x256, 5x5")
+                         "256x256, 5x5; vst: variance-stabilised input image 1024x1024 (GAT fit, 5000 simplex "
+                         "iterations, and transform)")
     return ap.parse_args()
 
 
@@ -293,6 +295,82 @@ def pairmaps_run(args):
         "cpu_baseline": cpu}))
 
 
+def vst_run(args):
+    """Variance-stabilised input image (Utils.getInputImage2D with the Simplex
+    GAT, maxIter 5000 as GlobalRepetition2D_.java:114): 1024x1024 Poisson
+    counts (rings and blobs, 200 photons at peak, seed 1).  One step = the
+    simplex fit on the device image (each iteration one launch that evaluates
+    the reflection, expansion and both contractions and writes the four
+    values to pinned host memory) plus the transform and the min-max
+    normalisation.  Roofline of the objective's row kernel: the FP64 vector
+    ALU, 10 algorithmic flops per pixel and parameter set (g*v + ref, the
+    square root, the scale by 2/g, the float square, two double adds, the
+    comparison with -ref/g and the two conversions counted as one each)."""
+    import ctypes
+
+    import torch
+
+    import oracle as O
+    import sred_amd
+    from sred_amd import _lib as L
+    from sred_amd import synthetic
+
+    S, ITERS = 1024, 5000
+    img = synthetic.rings_and_blobs(S, seed=1)
+    d_img = torch.from_numpy(img).cuda()
+    d_work = torch.empty_like(d_img)
+    ctx = sred_amd.Context(0)
+    params = (ctypes.c_double * 3)()
+    err = ctypes.c_double()
+
+    def step():
+        L.check(L.load().sred_gat_minimize_2d_dev(ctx.handle, d_img.data_ptr(), S, S, 1.0, 10.0, 100.0, ITERS,
+                                                  params, ctypes.byref(err)))
+        d_work.copy_(d_img)
+        g, s, o = (float(np.float32(v)) for v in params)
+        L.check(L.load().sred_gat_2d_dev(ctx.handle, d_work.data_ptr(), S * S, g, s, o))
+        mn, mx = d_work.min(), d_work.max()
+        d_work.sub_(mn).div_(mx - mn + 1e-7)
+
+    t, _ = _timed(step, args.steps, args.warmup, lambda: 0.0)
+    # the row kernel's time: one more fit with HIP events around every launch
+    # (kept out of the timed steps, since the events add about 4 us a launch)
+    ctx.set_option(L.OPT_FIT_TIMING, 1)
+    step()
+    torch.cuda.synchronize()
+    kms, launches, evals = ctx.last_contraction()
+    ctx.set_option(L.OPT_FIT_TIMING, 0)
+    fit = tuple(params)
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        cores = host_cores()
+        n_it = 20
+        t0 = time.perf_counter()
+        O.gat_minimize(img, 1.0, 10.0, 100.0, n_it)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": n_it / t_cpu, "unit": "simplex iterations/s", "cores": 1, "kind": "port",
+               "sample": f"oracle restatement (same simplex, same objective), {n_it} iterations of the {S}x{S} "
+                         f"fit, {t_cpu:.2f} s"}
+    flops = 10.0 * evals  # per pixel and parameter set, as above
+    ach = flops / (kms * 1e-3) / 1e12
+    print(f"# vst: {t * 1e3:.1f} ms per stabilised image, {launches} objective launches, fit {fit}",
+          file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "ms per variance-stabilised 1024² input image (GAT fit, 5000 simplex iterations)",
+        "value": t * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Poisson counts of 3000 rings + 3000 blobs at 1024x1024 (200 photons peak), seed 1",
+        "config": {"workload": "Utils.getInputImage2D, Simplex GAT, maxIter 5000, 1024x1024"},
+        "objective_launches": launches, "ms_per_launch": t * 1e3 / max(launches, 1), "fit": fit,
+        "objective": err.value, "objective_pixel_evals_per_s": evals / t,
+        "rows_kernel_ms_per_launch": kms / max(launches, 1),
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP64_VEC_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP64_VEC_TFLOPS, "traffic": None,
+                     "flops_per_pixel_set": 10, "pixel_set_evals_per_launch": evals / max(launches, 1)},
+        "cpu_baseline": cpu}))
+
+
 def adjoint_run(args):
     """Global Pearson map + hand-written adjoint, 512x512 spots image (800
     spots), 9x9 blocks (K = 49), loss weights U[0,1] (seed 40): one step =
@@ -410,6 +488,9 @@ def main():
         return
     if args.workload in ("hu", "phase"):
         pairmaps_run(args)
+        return
+    if args.workload == "vst":
+        vst_run(args)
         return
     if args.profile:
         profile_run(args)

@@ -69,12 +69,15 @@ int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols);
  * SRED_OPT_AUTOTUNE (1, the default: time the panel sizes of a new class;
  * 0: the largest that fits), SRED_OPT_BATCH_PANEL (force a panel size: a
  * multiple of 256 up to 2048; 0 = off) and SRED_OPT_BATCH_MEMORY_MB (partial
- * sums per wave, default 16384). */
+ * sums per wave, default 16384); SRED_OPT_FIT_TIMING (0, the default; 1: a GAT
+ * fit brackets every objective launch's row kernel with HIP events, summed
+ * for sred_ctx_last_contraction, at about 4 us per launch). */
 #define SRED_OPT_CHUNK_COLS 1
 #define SRED_OPT_SYMMETRIC 2
 #define SRED_OPT_AUTOTUNE 3
 #define SRED_OPT_BATCH_PANEL 4
 #define SRED_OPT_BATCH_MEMORY_MB 5
+#define SRED_OPT_FIT_TIMING 6
 int sred_ctx_set_option(sred_ctx *ctx, int option, int value);
 
 /* Metric from the reference's dialog string; block != 0 uses the block
@@ -198,9 +201,10 @@ int sred_gat_2d(sred_ctx *ctx, float *img, int64_t n, double gain, double sigma,
 int sred_gat_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset);
 /* GATMinimizer2D.userFunction (GATMinimizer2D.java:64-110): mean squared
  * deviation from 1 of the variances of the (w/6) x (h/6)-pixel blocks after
- * the fit's transform (special case v <= -ref/g).  Block sums are formed row
- * by row (each row's sequential double sum, rows added in order), blocks in
- * raster order.  Needs w, h >= 6. */
+ * the fit's transform (special case v <= -ref/g).  Each block row's pixels,
+ * each block's row sums and the block terms are summed in double over 64
+ * lanes (element i to lane i mod 64, in order) combined by a fixed xor
+ * butterfly.  Needs w, h >= 6. */
 int sred_gat_objective_2d(sred_ctx *ctx, const float *img, int w, int h, double gain, double sigma, double offset,
                           double *error);
 /* GATMinimizer2D.run (GATMinimizer2D.java:33-61) with the library's own
@@ -273,7 +277,8 @@ int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision
 /* Duration (ms, HIP events on the context's stream) and launch count of the
  * dominant kernel of the last call on this context -- the contraction of a
  * global call, the stencil of a block or Hu call, the pair kernel of an
- * adjoint call -- and its work units (block pairs evaluated, or pixels).
+ * adjoint call, the objective's row kernel of a GAT fit (summed over its
+ * launches) -- and its work units (block pairs evaluated, or pixels).
  * No counterpart in the reference (measurement). */
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);
 /* Ordered block pairs whose evaluation the last global call (single or

@@ -1073,14 +1073,30 @@ static float gat_fit_px(float p, double gain, double ref)
     return (float)v;
 }
 
+/* Sum of a[0..63] as a 64-lane xor butterfly: for s = 32, 16, ..., 1 every
+ * lane l takes a[l] + a[l ^ s]; the result is lane 0's value. */
+static double lane_tree64(double *a)
+{
+    double t[64];
+    for (int s = 32; s >= 1; s >>= 1) {
+        for (int l = 0; l < 64; l++) t[l] = a[l] + a[l ^ s];
+        memcpy(a, t, sizeof t);
+    }
+    return a[0];
+}
+
 /* GATMinimizer2D.userFunction (GATMinimizer2D.java:64-110): the image is cut
  * into nbx x nby blocks of (w/6) x (h/6) pixels (nbx = w / (w/6)); each
  * block's mean and variance of the transformed pixels (sum v and sum of the
  * float squares v*v, in double; var = sq/n - mean^2, :115-137) gives
- * error += (var - 1)^2 / nBlocks in block raster order.  The sums of a block
- * are taken row by row: each row's sequential double sum over x, then the
- * row sums added in row order (the reference adds all pixels of a block in
- * one sequence; this order is the one the GPU reduction reproduces exactly).
+ * error = sum over blocks of (var - 1)^2 / nBlocks.
+ * Summation order (the reference adds all pixels of a block in one sequence
+ * and the blocks in raster order; this order is the one the GPU reduction
+ * reproduces exactly): a block row's pixels go to 64 lanes (pixel i to lane
+ * i mod 64, each lane adding its pixels in order) and the lanes are combined
+ * by lane_tree64; a block's row sums likewise (row r of the block to lane
+ * r mod 64); the block terms likewise (block b, raster index, to lane
+ * b mod 64).
  * Returns NaN for images narrower or shorter than 6 pixels (the reference
  * divides by zero there). */
 double or_gat_objective(const float *px, int w, int h, double gain, double sigma, double offset)
@@ -1089,27 +1105,31 @@ double or_gat_objective(const float *px, int w, int h, double gain, double sigma
     if (bw <= 0 || bh <= 0) return NAN;
     int nbx = w / bw, nby = h / bh, nb = nbx * nby;
     double ref = (3.0 / 8.0) * gain * gain + sigma * sigma - gain * offset;
-    double error = 0.0;
+    double be[64], bm[64], bs[64], lm[64], ls[64];
+    for (int l = 0; l < 64; l++) be[l] = 0.0;
     for (int by = 0; by < nby; by++)
         for (int bx = 0; bx < nbx; bx++) {
-            double mean = 0.0, sq = 0.0;
-            for (int j = by * bh; j < (by + 1) * bh; j++) {
-                double rm = 0.0, rs = 0.0;
-                for (int i = bx * bw; i < (bx + 1) * bw; i++) {
-                    float v = gat_fit_px(px[(int64_t)j * w + i], gain, ref);
-                    rm += v;
-                    rs += v * v;  /* float product, as the reference's sq_sum += v * v */
+            for (int l = 0; l < 64; l++) bm[l] = bs[l] = 0.0;
+            for (int r = 0; r < bh; r++) {
+                const float *row = px + (int64_t)(by * bh + r) * w + (int64_t)bx * bw;
+                for (int l = 0; l < 64; l++) lm[l] = ls[l] = 0.0;
+                for (int i = 0; i < bw; i++) {
+                    float v = gat_fit_px(row[i], gain, ref);
+                    lm[i % 64] += v;
+                    ls[i % 64] += v * v;  /* float product, as the reference's sq_sum += v * v */
                 }
-                mean += rm;
-                sq += rs;
+                bm[r % 64] += lane_tree64(lm);
+                bs[r % 64] += lane_tree64(ls);
             }
+            double mean = lane_tree64(bm), sq = lane_tree64(bs);
             int n = bw * bh;
             mean = mean / n;
             double var = sq / n - mean * mean;
             double delta = var - 1.0;
-            error += (delta * delta) / (double)nb;
+            int b = by * nbx + bx;
+            be[b % 64] += (delta * delta) / (double)nb;
         }
-    return error;
+    return lane_tree64(be);
 }
 
 /* Nelder-Mead simplex in 3 dimensions (the build's own minimiser in place of

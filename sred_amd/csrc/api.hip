@@ -133,6 +133,7 @@ int time_begin(sred_ctx *ctx)
         SRED_HIP_CHECK(hipEventCreate(&ctx->ev1));
     }
     ctx->last_launches = 0;
+    ctx->summed_ms = -1.0f;
     SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
     return SRED_OK;
 }
@@ -276,6 +277,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
     ctx->batch_store.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->h_fit) (void)hipHostFree(ctx->h_fit);
     delete ctx;
     return SRED_OK;
 }
@@ -328,6 +330,9 @@ int sred_ctx_set_option(sred_ctx *ctx, int option, int value)
     case SRED_OPT_BATCH_MEMORY_MB:
         if (value < 1) break;
         ctx->batch_mb = value;
+        return SRED_OK;
+    case SRED_OPT_FIT_TIMING:
+        ctx->fit_timing = value != 0;
         return SRED_OK;
     default:
         break;
@@ -956,7 +961,9 @@ int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *
 {
     SRED_TRY(check_ctx(ctx));
     float t = 0.0f;
-    if (ctx->last_launches > 0 && ctx->ev0) {
+    if (ctx->summed_ms >= 0.0f) {
+        t = ctx->summed_ms;
+    } else if (ctx->last_launches > 0 && ctx->ev0) {
         SRED_HIP_CHECK(hipEventSynchronize(ctx->ev1));
         SRED_HIP_CHECK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
     }

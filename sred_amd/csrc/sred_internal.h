@@ -136,16 +136,19 @@ struct sred_ctx {
     int autotune = 1;
     int batch_panel = 0;       // forced panel size (0: autotuner)
     int64_t batch_mb = 16384;  // partial-sum memory per wave
+    int fit_timing = 0;        // GAT fit: HIP events around every objective launch
     int offs_key[3] = {-1, -1, -1}, offs2_key[3] = {-1, -1, -1};
     std::vector<float> host_tile_var;
     std::vector<float> host_tmpl;  // expanded template last copied into `tmpl` (skips repeated uploads)
     // measurement of the last global contraction
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
+    float summed_ms = -1.0f;  // >= 0: the dominant kernel's time summed over the launches of the last call
     int last_launches = 0;
     int64_t last_pairs = 0;
     int last_nv = 0;  // valid blocks of the last global accumulation
     unsigned long long *last_skip = nullptr;  // device counter of pairs skipped by the last symmetric contraction
+    double *h_fit = nullptr;  // pinned host doubles the GAT objective kernel writes its values to
 };
 
 namespace sred {

@@ -13,12 +13,16 @@
 //   * userFunction: the image in nbx x nby blocks of (w/6) x (h/6) pixels;
 //     per block the mean and variance of the transformed pixels (the fit's
 //     own transform, special case v <= -ref/g); error = sum (var - 1)^2 / nb.
-//     k_gat_rows takes one (parameter set, image row, block) per thread: the
-//     sequential double sums of v and of the float squares v*v over the row's
-//     pixels of the block; k_gat_error adds the row sums of each block in row
-//     order and the block errors in raster order, so an evaluation is
-//     deterministic and equals the CPU restatement bit for bit (IEEE double
-//     arithmetic and correctly rounded square roots on both).
+//     k_gat_rows takes one (image row, block column) per wave for all
+//     parameter sets of the launch (one pixel load, up to four transforms):
+//     lane l adds the transformed pixels l, l + 64, ... of the block row (and
+//     their float squares) in double, then a fixed xor butterfly over the 64
+//     lanes gives the row sums.  k_gat_error (one workgroup per parameter
+//     set) combines the row sums of each block the same way (row r to lane
+//     r mod 64) and then the block terms (block b to lane b mod 64), and
+//     writes the value to pinned host memory.  An evaluation is deterministic
+//     and equals the CPU restatement bit for bit (IEEE double arithmetic,
+//     commutative adds and correctly rounded square roots on both).
 //   * The minimiser: ImageJ's Minimizer (restarts, random initial simplex)
 //     is third-party and not restated; this is a Nelder-Mead simplex with
 //     the Lagarias et al. (1998) coefficients, initial simplex x0 + d_i e_i
@@ -34,7 +38,8 @@
 
 namespace sred {
 
-constexpr int kGatSets = 4;  // parameter sets per objective launch
+constexpr int kGatSets = 4;        // parameter sets per objective launch
+constexpr int kGatErrThreads = 1024;  // k_gat_error workgroup (16 waves)
 
 struct GatSets {
     double gain[kGatSets], ref[kGatSets];
@@ -53,58 +58,87 @@ __global__ __launch_bounds__(256) void k_gat(float *__restrict__ px, int64_t n, 
     }
 }
 
-// rows[set][j][bx] = (sum v, sum v*v) over the row j of block column bx
+// Sum over the 64 lanes of a wave: for s = 32, ..., 1 each lane adds lane
+// (l ^ s)'s value; every lane ends with the same result.
+__device__ __forceinline__ double lane_tree64(double a)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) a = a + __shfl_xor(a, s, 64);
+    return a;
+}
+
+// rows[set][j][bx] = (sum v, sum v*v) over row j of block column bx; one
+// wave per (j, bx) evaluates every parameter set of the launch
 __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, int w, int bw, int nbx, int nrows,
                                                   int nsets, GatSets ps, double2 *__restrict__ rows)
 {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nsets * nrows * nbx) return;
-    const int set = t / (nrows * nbx), rem = t - set * (nrows * nbx);
-    const int j = rem / nbx, bx = rem - j * nbx;
-    const double gain = ps.gain[set], ref = ps.ref[set];
-    const double lim = -ref / gain;
-    const float *p = px + (int64_t)j * w + (int64_t)bx * bw;
-    double rm = 0.0, rs = 0.0;
-    for (int i = 0; i < bw; i++) {
-        double d = (double)p[i];
-        if (d <= lim)
-            d = 0.0;
-        else
-            d = (2.0 / gain) * sqrt(gain * d + ref);
-        const float v = (float)d;
-        rm += v;
-        rs += v * v;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform
+    const int per_set = nrows * nbx;
+    if (t >= per_set) return;
+    const int j = t / nbx, bx = t - j * nbx;
+    double lim[kGatSets], two_g[kGatSets], rm[kGatSets], rs[kGatSets];
+#pragma unroll
+    for (int s = 0; s < kGatSets; s++) {
+        lim[s] = -ps.ref[s] / ps.gain[s];
+        two_g[s] = 2.0 / ps.gain[s];
+        rm[s] = rs[s] = 0.0;
     }
-    rows[t] = make_double2(rm, rs);
+    const float *p = px + (int64_t)j * w + (int64_t)bx * bw;
+    for (int i = lane; i < bw; i += 64) {
+        const double d = (double)p[i];
+#pragma unroll
+        for (int s = 0; s < kGatSets; s++) {
+            if (s < nsets) {
+                const float v = d <= lim[s] ? 0.0f : (float)(two_g[s] * sqrt(ps.gain[s] * d + ps.ref[s]));
+                rm[s] += v;
+                rs[s] += v * v;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kGatSets; s++) {
+        if (s < nsets) {
+            const double a = lane_tree64(rm[s]), b = lane_tree64(rs[s]);
+            if (lane == 0) rows[(size_t)s * per_set + t] = make_double2(a, b);
+        }
+    }
 }
 
-// err[set] from the row sums: blocks in raster order, rows of a block in order
-__global__ __launch_bounds__(128) void k_gat_error(const double2 *__restrict__ rows, int bw, int bh, int nbx, int nby,
-                                                   int nrows, double *__restrict__ err)
+// err[set] from the row sums of the set (one workgroup per set): each wave
+// takes blocks wave, wave + 16, ...; the block terms are combined by wave 0.
+__global__ __launch_bounds__(kGatErrThreads) void k_gat_error(const double2 *__restrict__ rows, int bw, int bh,
+                                                              int nbx, int nby, int nrows, double *__restrict__ err)
 {
 #pragma clang fp contract(off)
     __shared__ double be[128];
-    const int set = blockIdx.x, b = threadIdx.x, nb = nbx * nby;
-    if (b < nb) {
+    const int set = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nb = nbx * nby;
+    const double2 *r = rows + (size_t)set * nrows * nbx;
+    for (int b = wave; b < nb; b += kGatErrThreads / 64) {
         const int by = b / nbx, bx = b - by * nbx;
-        const double2 *r = rows + (size_t)set * nrows * nbx;
         double mean = 0.0, sq = 0.0;
-        for (int j = by * bh; j < (by + 1) * bh; j++) {
-            mean += r[(size_t)j * nbx + bx].x;
-            sq += r[(size_t)j * nbx + bx].y;
+        for (int k = lane; k < bh; k += 64) {
+            const double2 q = r[(size_t)(by * bh + k) * nbx + bx];
+            mean += q.x;
+            sq += q.y;
         }
+        mean = lane_tree64(mean);
+        sq = lane_tree64(sq);
         const int n = bw * bh;
         mean = mean / n;
         const double var = sq / n - mean * mean;
         const double delta = var - 1.0;
-        be[b] = (delta * delta) / (double)nb;
+        if (lane == 0) be[b] = (delta * delta) / (double)nb;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wave == 0) {
         double e = 0.0;
-        for (int i = 0; i < nb; i++) e += be[i];
-        err[set] = e;
+        if (lane < nb) e += be[lane];
+        if (lane + 64 < nb) e += be[lane + 64];
+        e = lane_tree64(e);
+        if (lane == 0) err[set] = e;
     }
 }
 
@@ -131,8 +165,9 @@ struct GatEval {
     const float *d_img;
     int w, h, bw, bh, nbx, nby, nrows;
     double2 *d_rows;
-    double *d_err;
-    int64_t launches = 0;
+    double *h_err;  // pinned host memory written by k_gat_error
+    int64_t launches = 0, pixel_evals = 0;
+    double rows_ms = 0.0;  // k_gat_rows time summed over the launches (HIP events, when fit_timing is on)
 
     int run(const double (*x)[3], int nsets, double *f)
     {
@@ -141,15 +176,23 @@ struct GatEval {
             ps.gain[s] = x[s][0];
             ps.ref[s] = gat_ref(x[s][0], x[s][1], x[s][2]);
         }
-        const int threads = nsets * nrows * nbx;
-        hipLaunchKernelGGL(k_gat_rows, dim3((threads + 255) / 256), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
-                           nrows, nsets, ps, d_rows);
-        hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(128), 0, ctx->stream, d_rows, bw, bh, nbx, nby, nrows,
-                           d_err);
+        const int waves = nrows * nbx;
+        if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
+        hipLaunchKernelGGL(k_gat_rows, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx, nrows,
+                           nsets, ps, d_rows);
+        if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
+        hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw, bh, nbx, nby,
+                           nrows, h_err);
         SRED_HIP_CHECK(hipGetLastError());
-        SRED_HIP_CHECK(hipMemcpyAsync(f, d_err, sizeof(double) * nsets, hipMemcpyDeviceToHost, ctx->stream));
         SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (int s = 0; s < nsets; s++) f[s] = h_err[s];
+        if (ctx->fit_timing) {
+            float ms = 0.0f;
+            SRED_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            rows_ms += ms;
+        }
         launches++;
+        pixel_evals += (int64_t)nsets * nrows * nbx * bw;
         return SRED_OK;
     }
 };
@@ -174,10 +217,11 @@ int gat_setup(sred_ctx *ctx, const float *d_img, int w, int h, GatEval *e)
         set_error("unsupported GAT block grid");
         return SRED_ERR_ARG;
     }
-    SRED_TRY(ctx->misc.ensure(kMiscBytes));
     SRED_TRY(ctx->pm_part.ensure(sizeof(double2) * (size_t)kGatSets * e->nrows * e->nbx + 256));
     e->d_rows = ctx->pm_part.as<double2>();
-    e->d_err = reinterpret_cast<double *>(ctx->misc.as<char>() + 2048);  // 4 doubles in the mask-check area
+    if (!ctx->h_fit) SRED_HIP_CHECK(hipHostMalloc((void **)&ctx->h_fit, sizeof(double) * kGatSets, hipHostMallocDefault));
+    e->h_err = ctx->h_fit;
+    SRED_TRY(time_begin(ctx));  // creates the events; the fit sums its launches itself
     return SRED_OK;
 }
 
@@ -189,7 +233,11 @@ int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double ga
     GatEval e;
     SRED_TRY(gat_setup(ctx, d_img, w, h, &e));
     const double x[1][3] = {{gain, sigma, offset}};
-    return e.run(x, 1, error);
+    SRED_TRY(e.run(x, 1, error));
+    ctx->last_launches = 1;
+    ctx->last_pairs = e.pixel_evals;
+    ctx->summed_ms = (float)e.rows_ms;
+    return SRED_OK;
 }
 
 // Nelder-Mead as the CPU restatement (same expressions, no contraction), with
@@ -250,6 +298,8 @@ int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gai
     for (int k = 0; k < 3; k++) params[k] = v[b][k];
     if (error) *error = fv[b];
     ctx->last_launches = (int)std::min<int64_t>(e.launches, 1 << 30);
+    ctx->last_pairs = e.pixel_evals;
+    ctx->summed_ms = (float)e.rows_ms;
     return SRED_OK;
 }
 

@@ -65,5 +65,20 @@ def test_gat_minimize_device_path_and_errors(ctx):
                                               ctypes.byref(err)))
     p, f = O.gat_minimize(img, 1.0, 10.0, 100.0, 80)
     assert tuple(params) == tuple(p) and err.value == f
+    # measurement: the row kernel's time summed over the fit's launches (with
+    # the timing option on), and its pixel evaluations (4 sets per candidate
+    # launch, 3 per shrink); the fit itself is unchanged by the option
+    ms, launches, evals = ctx.last_contraction()
+    assert ms == 0.0 and 1 <= launches <= 81 + 80
+    assert 3 * 64 * 64 * launches <= evals <= 4 * 64 * 64 * launches
+    ctx.set_option(L.OPT_FIT_TIMING, 1)
+    try:
+        L.check(L.load().sred_gat_minimize_2d_dev(ctx.handle, d.data_ptr(), 64, 64, 1.0, 10.0, 100.0, 80, params,
+                                                  ctypes.byref(err)))
+    finally:
+        ctx.set_option(L.OPT_FIT_TIMING, 0)
+    assert tuple(params) == tuple(p) and err.value == f
+    ms2, launches2, evals2 = ctx.last_contraction()
+    assert ms2 > 0.0 and (launches2, evals2) == (launches, evals)
     with pytest.raises(sred_amd.SredError):
         utils.GATMinimizer2D(np.zeros((5, 40), np.float32), 40, 5, 1, 10, 100, 10, resources=ctx).run()

@@ -9,6 +9,9 @@ the Nelder-Mead simplex the build uses in place of ImageJ's Minimizer.
   (the float-squared mean square less the squared mean),
   and of an image whose blocks alternate two values a, b it is
   (((A - B)/2)^2 - 1)^2 with A, B the transformed values;
+* an FP64 numpy restatement of the objective (blocks of different
+  variance, 7 block columns or rows where w or h leaves a remainder, pixels
+  in the zero branch) to 1e-12 relative (the summation orders differ);
 * the simplex: a convex quadratic's analytic minimum, an immediate stop
   when all vertices tie (constant function), and the GAT fit lowering the
   objective of a Poisson-Gaussian image.
@@ -67,6 +70,42 @@ def test_objective_two_value_blocks():
     var = (float(np.float32(A) ** 2) + float(np.float32(B) ** 2)) / 2.0 - ((A + B) / 2.0) ** 2
     assert bw % 2 == 0 and var == pytest.approx(((A - B) / 2.0) ** 2, rel=1e-6)
     assert O.gat_objective(img, g, s, o) == pytest.approx((var - 1.0) ** 2, rel=1e-12)
+
+
+def _objective_numpy(img, g, s, o):
+    """GATMinimizer2D.userFunction (GATMinimizer2D.java:64-110) in numpy FP64:
+    blocks of (w//6) x (h//6) pixels, w // (w//6) of them per row."""
+    h, w = img.shape
+    bw, bh = w // 6, h // 6
+    nbx, nby = w // bw, h // bh
+    ref = 3.0 / 8.0 * g * g + s * s - g * o
+    d = img.astype(np.float64)
+    v = np.where(d <= -ref / g, 0.0, (2.0 / g) * np.sqrt(np.maximum(g * d + ref, 0.0))).astype(np.float32)
+    err = 0.0
+    for by in range(nby):
+        for bx in range(nbx):
+            blk = v[by * bh:(by + 1) * bh, bx * bw:(bx + 1) * bw]
+            mean = blk.astype(np.float64).sum() / blk.size
+            sq = (blk * blk).astype(np.float64).sum() / blk.size
+            err += ((sq - mean * mean) - 1.0) ** 2 / (nbx * nby)
+    return err
+
+
+@pytest.mark.parametrize("shape", [(60, 60), (37, 53), (13, 29), (100, 77), (7, 7), (90, 200)])
+def test_objective_numpy_restatement(shape):
+    # block-dependent brightness (every block a different variance), remainders
+    # that give 7 block columns or rows, and pixels in the zero branch
+    rng = np.random.default_rng(sum(shape))
+    h, w = shape
+    yy, xx = np.mgrid[0:h, 0:w]
+    lam = 5.0 + 60.0 * (xx / w) + 30.0 * (yy / h) ** 2
+    img = (1.7 * rng.poisson(lam) + rng.normal(4.0, 2.0, shape)).astype(np.float32)
+    img[rng.random(shape) < 0.05] = -200.0
+    for g, s, o in [(1.0, 10.0, 100.0), (1.7, 2.0, 4.0), (0.6, 5.0, -3.0)]:
+        ref = 3.0 / 8.0 * g * g + s * s - g * o
+        want = _objective_numpy(img, g, s, o)
+        assert np.any(img <= -ref / g)
+        assert O.gat_objective(img, g, s, o) == pytest.approx(want, rel=1e-12)
 
 
 def test_objective_small_image_is_nan():
