@@ -218,6 +218,36 @@ int sred_gat_minimize_2d(sred_ctx *ctx, const float *img, int w, int h, double g
 int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0,
                              double offset0, int max_iter, double *params, double *error);
 
+/* The "Quad/Octree" estimate of Utils.getInputImage2D (Utils.java:686-703):
+ * QuadTree_(w, h, min_size, alpha).buildTree, calculateRobustMeans(max_iter),
+ * calculateLTSVariances(lts_fraction), collectMeanVariancePairs and
+ * performLinearRegression (QuadTree_.java:135-407; the reference passes 4,
+ * 0.01f, 50, 0.75f).  The node statistics, robust means and LTS sums keep
+ * the reference's float arithmetic and sequential orders.  g0_edc[2] =
+ * (g0, eDC); *n_leaves (may be NULL) = the leaf count; up to cap leaves are
+ * written in pre-order to rects (x, y, w, h), means and variances (each may
+ * be NULL).  Errors: SRED_ERR_NONFINITE for a NaN/Inf pixel or a NaN node
+ * variance (Utils.java:879-895); SRED_ERR_ARG for a one-pixel image,
+ * min_size < 1, alpha or lts_fraction outside [0, 1], or fewer than 3 leaves
+ * (SimpleRegression.regress).  Synchronises. */
+int sred_quadtree_gat_2d(sred_ctx *ctx, const float *img, int w, int h, int min_size, float alpha, int max_iter,
+                         float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *rects, float *means,
+                         float *variances);
+int sred_quadtree_gat_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int min_size, float alpha,
+                             int max_iter, float lts_fraction, double *g0_edc, int *n_leaves, int cap,
+                             int32_t *rects, float *means, float *variances);
+/* QuadTree_.applyGATtree (QuadTree_.java:409-425) in place: ref = 3/8 g0^2 +
+ * eDC in double, v <- (g0 v + ref < 0) ? 0 : (2/g0) sqrt(g0 v + ref), rounded
+ * to float.  Synchronous; _dev enqueues on the context's stream. */
+int sred_gat_tree_2d(sred_ctx *ctx, float *img, int64_t n, double g0, double edc);
+int sred_gat_tree_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, double edc);
+/* Host-only helpers of the estimate (no GPU): the inverse CDF of the F(d1,
+ * d2) distribution (commons-math3 FDistribution.inverseCumulativeProbability,
+ * used at QuadTree_.java:241-245; bisection to adjacent doubles) and the
+ * least-squares line of commons-math3 SimpleRegression (QuadTree_.java:395-407). */
+int sred_f_quantile(double d1, double d2, double p, double *out);
+int sred_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept);
+
 /* ---- all-pairs vector maps (OpenCL kernels without a Java caller) ---- */
 /* Per-pixel Hu invariant hu = nu20 + nu02 of kernelGetHuMap.cl:146-172
  * (getInvariant on the bw x bh rectangle, patch = pixel - means[p]), in the

@@ -98,6 +98,24 @@ def lib():
         L.or_nelder_mead3.restype = c_double
         L.or_gat_minimize.argtypes = [_F, c_int, c_int, c_double, c_double, c_double, c_int, _D]
         L.or_gat_minimize.restype = c_double
+        L.or_beta_reg.argtypes = [c_double, c_double, c_double]
+        L.or_beta_reg.restype = c_double
+        L.or_f_quantile.argtypes = [c_double, c_double, c_double]
+        L.or_f_quantile.restype = c_double
+        L.or_simple_regression.argtypes = [_D, _D, c_i64, ctypes.POINTER(c_double), ctypes.POINTER(c_double)]
+        L.or_simple_regression.restype = c_int
+        L.or_qt_criterion.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_float),
+                                      ctypes.POINTER(c_float)]
+        L.or_qt_criterion.restype = None
+        L.or_qt_robust_mean.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int]
+        L.or_qt_robust_mean.restype = c_float
+        L.or_qt_lts_variance.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_float, c_float]
+        L.or_qt_lts_variance.restype = c_float
+        L.or_quadtree_gat.argtypes = [_F, c_int, c_int, c_int, c_float, c_int, c_float, c_int, vp, _F, _F,
+                                      ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.POINTER(c_double)]
+        L.or_quadtree_gat.restype = c_int
+        L.or_gat_tree.argtypes = [_F, c_i64, c_double, c_double]
+        L.or_gat_tree.restype = None
         _lib = L
     return _lib
 
@@ -390,7 +408,7 @@ def gat(img, gain, sigma, offset):
 
 
 def gat_objective(img, gain, sigma, offset):
-    """GATMinimizer2D.userFunction (block sums row by row)."""
+    """GATMinimizer2D.userFunction (fixed 64-lane summation order)."""
     img = _f32(img)
     h, w = img.shape
     return lib().or_gat_objective(img.reshape(-1), w, h, float(gain), float(sigma), float(offset))
@@ -411,3 +429,73 @@ def gat_minimize(img, gain0=1.0, sigma0=10.0, offset0=100.0, max_iter=5000):
     x = np.zeros(3)
     fx = lib().or_gat_minimize(img.reshape(-1), w, h, gain0, sigma0, offset0, int(max_iter), x)
     return x, fx
+
+
+def beta_reg(a, b, x):
+    """Regularised incomplete beta I_x(a, b)."""
+    return lib().or_beta_reg(float(a), float(b), float(x))
+
+
+def f_quantile(d1, d2, p):
+    """Inverse CDF of the F(d1, d2) distribution (FDistribution.inverseCumulativeProbability)."""
+    return lib().or_f_quantile(float(d1), float(d2), float(p))
+
+
+def simple_regression(x, y):
+    """commons-math3 SimpleRegression with intercept: (slope, intercept)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    s, i = ctypes.c_double(), ctypes.c_double()
+    lib().or_simple_regression(x, y, x.size, ctypes.byref(s), ctypes.byref(i))
+    return s.value, i.value
+
+
+def qt_criterion(img, x0, y0, nw, nh):
+    """QuadTree_.computeCriterion's two variances: (block variance, noise variance)."""
+    img = _f32(img)
+    h, w = img.shape
+    bv, nv = ctypes.c_float(), ctypes.c_float()
+    lib().or_qt_criterion(img.reshape(-1), w, h, x0, y0, nw, nh, ctypes.byref(bv), ctypes.byref(nv))
+    return bv.value, nv.value
+
+
+def qt_robust_mean(img, x0, y0, nw, nh, max_iter=50):
+    img = _f32(img)
+    return lib().or_qt_robust_mean(img.reshape(-1), img.shape[1], x0, y0, nw, nh, int(max_iter))
+
+
+def qt_lts_variance(img, x0, y0, nw, nh, mean, frac=0.75):
+    img = _f32(img)
+    return lib().or_qt_lts_variance(img.reshape(-1), img.shape[1], x0, y0, nw, nh, float(mean), float(frac))
+
+
+class QuadTreeResult:
+    """Leaves in pre-order (x, y, w, h), their robust means and LTS variances, g0 and eDC."""
+
+    def __init__(self, rects, means, variances, g0, edc):
+        self.rects, self.means, self.variances, self.g0, self.edc = rects, means, variances, g0, edc
+
+
+def quadtree_gat(img, min_size=4, alpha=0.01, max_iter=50, frac=0.75):
+    """Utils.getInputImage2D's Quad/Octree estimate; raises ValueError(status) on the reference's errors."""
+    img = _f32(img)
+    h, w = img.shape
+    cap = max(16, (w * h) // 2)
+    rects = np.zeros((cap, 4), np.int32)
+    means = np.zeros(cap, np.float32)
+    var = np.zeros(cap, np.float32)
+    n = ctypes.c_int()
+    g0, edc = ctypes.c_double(), ctypes.c_double()
+    rc = lib().or_quadtree_gat(img.reshape(-1), w, h, int(min_size), float(alpha), int(max_iter), float(frac), cap,
+                               rects.ctypes.data, means, var, ctypes.byref(n), ctypes.byref(g0), ctypes.byref(edc))
+    if rc != 0:
+        raise ValueError(rc)
+    k = n.value
+    return QuadTreeResult(rects[:k].copy(), means[:k].copy(), var[:k].copy(), g0.value, edc.value)
+
+
+def gat_tree(img, g0, edc):
+    """QuadTree_.applyGATtree (a copy)."""
+    out = _f32(img).copy()
+    lib().or_gat_tree(out.reshape(-1), out.size, float(g0), float(edc))
+    return out

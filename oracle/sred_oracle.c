@@ -1204,3 +1204,328 @@ double or_gat_minimize(const float *px, int w, int h, double gain0, double sigma
     double x0[3] = {gain0, sigma0, offset0}, d[3] = {1.0, 10.0, 100.0};
     return or_nelder_mead3(gat_obj3, &c, x0, d, max_iter, params);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Quadtree GAT estimate (QuadTree_.java:135-427, Utils.java:686-703)        */
+/* ------------------------------------------------------------------------ */
+
+#define OR_JEPS 1e-7f /* Utils.EPSILON (Utils.java:20) */
+
+/* java.lang.Math.max / min for floats (NaN wins; -0 < +0). */
+static float jmaxf(float a, float b)
+{
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && signbit(a)) return b;
+    return a >= b ? a : b;
+}
+static float jminf(float a, float b)
+{
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && signbit(b)) return b;
+    return a <= b ? a : b;
+}
+
+/* Regularised incomplete beta I_x(a, b): x^a (1-x)^b / (a B(a,b)) / g with
+ * g = 1 + d1/(1 + d2/(1 + ...)) the continued fraction of Abramowitz &
+ * Stegun 26.5.8, evaluated by Lentz's method (or_beta_cf returns 1/g); the
+ * symmetry I_x(a,b) = 1 - I_{1-x}(b,a) where the fraction converges slowly
+ * (x beyond the mean (a+1)/(a+b+2)). */
+static double or_beta_cf(double a, double b, double x)
+{
+    const double tiny = 1e-300;
+    double f = 1.0, C = 1.0, D = 0.0;  /* f -> g */
+    for (int m = 0; m < 2000000; m++) {
+        /* terms d_{2m+1} and d_{2m+2} (d_1 = -(a+b)x/(a+1)) */
+        for (int half = 0; half < 2; half++) {
+            double dm;
+            if (half == 0)
+                dm = -(a + m) * (a + b + m) * x / ((a + 2.0 * m) * (a + 2.0 * m + 1.0));
+            else
+                dm = (m + 1.0) * (b - m - 1.0) * x / ((a + 2.0 * m + 1.0) * (a + 2.0 * m + 2.0));
+            D = 1.0 + dm * D;
+            if (fabs(D) < tiny) D = tiny;
+            C = 1.0 + dm / C;
+            if (fabs(C) < tiny) C = tiny;
+            D = 1.0 / D;
+            double delta = C * D;
+            f *= delta;
+            if (half == 1 && fabs(delta - 1.0) < 1e-16) return 1.0 / f;
+        }
+    }
+    return 1.0 / f;
+}
+
+double or_beta_reg(double a, double b, double x)
+{
+    if (x <= 0.0) return 0.0;
+    if (x >= 1.0) return 1.0;
+    double lfront = lgamma(a + b) - lgamma(a) - lgamma(b) + a * log(x) + b * log1p(-x);
+    if (x < (a + 1.0) / (a + b + 2.0)) return exp(lfront) * or_beta_cf(a, b, x) / a;
+    return 1.0 - exp(lfront) * or_beta_cf(b, a, 1.0 - x) / b;
+}
+
+/* Inverse CDF of the F distribution with (d1, d2) degrees of freedom:
+ * commons-math3 3.6.1 FDistribution.inverseCumulativeProbability (the
+ * reference's dependency; its Brent solver stops at 1e-9 absolute): here
+ * bisection on F(x) = I_{d1 x/(d1 x + d2)}(d1/2, d2/2) to adjacent doubles. */
+double or_f_quantile(double d1, double d2, double p)
+{
+    if (!(p > 0.0)) return 0.0;
+    if (!(p < 1.0)) return INFINITY;
+    double lo = 0.0, hi = 1.0;
+    while (or_beta_reg(0.5 * d1, 0.5 * d2, d1 * hi / (d1 * hi + d2)) < p) {
+        lo = hi;
+        hi *= 2.0;
+    }
+    for (int it = 0; it < 2100; it++) {
+        double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (or_beta_reg(0.5 * d1, 0.5 * d2, d1 * mid / (d1 * mid + d2)) < p)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+/* commons-math3 3.6.1 SimpleRegression with intercept (the reference's
+ * dependency, QuadTree_.java:395-407): the updating sums of addData, then
+ * getSlope = sumXY/sumXX (NaN below 2 points or for |sumXX| < 10 Double.MIN_VALUE)
+ * and getIntercept = (sumY - slope sumX)/n.  Returns the number of points
+ * (regress() throws below 3). */
+int or_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept)
+{
+    double sx = 0.0, sy = 0.0, sxx = 0.0, syy = 0.0, sxy = 0.0, xbar = 0.0, ybar = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        if (i == 0) {
+            xbar = x[i];
+            ybar = y[i];
+        } else {
+            double fact1 = 1.0 + (double)i, fact2 = (double)i / (1.0 + (double)i);
+            double dx = x[i] - xbar, dy = y[i] - ybar;
+            sxx += dx * dx * fact2;
+            syy += dy * dy * fact2;
+            sxy += dx * dy * fact2;
+            xbar += dx / fact1;
+            ybar += dy / fact1;
+        }
+        sx += x[i];
+        sy += y[i];
+    }
+    (void)syy;
+    double s = NAN;
+    if (n >= 2 && !(fabs(sxx) < 10.0 * 4.9406564584124654e-324)) s = sxy / sxx;
+    *slope = s;
+    *intercept = (sy - s * sx) / (double)n;
+    return (int)(n < INT32_MAX ? n : INT32_MAX);
+}
+
+/* QuadTree_.computeCriterion (QuadTree_.java:170-183): the node's variance
+ * (Utils.getMeanAndVarBlock2D, Utils.java:854-898: float sums in raster
+ * order) and the variance of the pseudo-residuals (5 z - neighbours) /
+ * sqrt(30) (calculateNoiseVariance, :185-232; neighbours outside the image
+ * omitted, neighbours outside the node included). */
+void or_qt_criterion(const float *img, int w, int h, int x0, int y0, int nw, int nh, float *block_var,
+                     float *noise_var)
+{
+    int n = nw * nh;
+    float mean = 0.0f, sq = 0.0f;
+    for (int y = y0; y < y0 + nh; y++)
+        for (int x = x0; x < x0 + nw; x++) {
+            float v = img[(int64_t)y * w + x];
+            mean += v;
+            sq += v * v;
+        }
+    mean = mean / (float)n;
+    *block_var = sq / (float)n - mean * mean;
+
+    const float l = 5.0f;
+    const float scale = (float)(1.0 / sqrt((double)(l * l + l)));
+    float sum = 0.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        float mr = sum / (float)n, ss = 0.0f;
+        for (int y = y0; y < y0 + nh; y++)
+            for (int x = x0; x < x0 + nw; x++) {
+                const float *p = img + (int64_t)y * w + x;
+                float lap = l * p[0];
+                if (x > 0) lap -= p[-1];
+                if (x < w - 1) lap -= p[1];
+                if (y > 0) lap -= p[-w];
+                if (y < h - 1) lap -= p[w];
+                float r = lap * scale;
+                if (pass == 0) {
+                    sum += r;
+                } else {
+                    double d = (double)r - (double)mr;
+                    ss += (float)(d * d);  /* Math.pow(d, 2.0) */
+                }
+            }
+        if (pass == 1) *noise_var = ss / ((float)n - 1.0f);
+    }
+}
+
+/* QuadTree_.calculateRobustMean (:264-312): a Leclerc-weighted mean from the
+ * one-pass estimate (whose variance uses the running sum in place of the
+ * mean, as written), iterated while the mean moves by more than 1e-5. */
+float or_qt_robust_mean(const float *img, int w, int x0, int y0, int nw, int nh, int max_iter)
+{
+    float mean = 0.0f, var = 0.0f;
+    int n = 0;
+    for (int y = y0; y < y0 + nh; y++)
+        for (int x = x0; x < x0 + nw; x++) {
+            float v = img[(int64_t)y * w + x];
+            mean += v;
+            var += (v - mean) * (v - mean);
+            n++;
+        }
+    mean = mean / (float)n;
+    var = var / (float)(n - 1);
+    var = jmaxf(var, OR_JEPS);
+    float prev;
+    int it = 0;
+    do {
+        prev = mean;
+        float ws = 0.0f, wv = 0.0f;
+        for (int y = y0; y < y0 + nh; y++)
+            for (int x = x0; x < x0 + nw; x++) {
+                float v = img[(int64_t)y * w + x];
+                float wt = (float)exp(-((double)(v - mean) * (double)(v - mean)) / (2.0 * (double)var));
+                ws += wt;
+                wv += wt * v;
+            }
+        mean = wv / jmaxf(ws, OR_JEPS);
+        it++;
+    } while ((double)fabsf(mean - prev) > 1e-5 && it < max_iter);
+    return mean;
+}
+
+static int cmp_float_asc(const void *a, const void *b)
+{
+    float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* QuadTree_.calculateLTSVariance (:326-356): the squared residuals about the
+ * robust mean sorted ascending, the floor(frac n) smallest summed in float. */
+float or_qt_lts_variance(const float *img, int w, int x0, int y0, int nw, int nh, float mean, float frac)
+{
+    int n = nw * nh;
+    float *r = (float *)malloc(sizeof(float) * (size_t)n);
+    int k = 0;
+    for (int y = y0; y < y0 + nh; y++)
+        for (int x = x0; x < x0 + nw; x++) {
+            float z = img[(int64_t)y * w + x];
+            r[k++] = (z - mean) * (z - mean);
+        }
+    qsort(r, (size_t)n, sizeof(float), cmp_float_asc);
+    int hh = (int)floorf(frac * (float)n);
+    float s = 0.0f;
+    for (int i = 0; i < hh; i++) s += r[i];
+    free(r);
+    return s / ((float)hh + OR_JEPS);
+}
+
+typedef struct {
+    const float *img;
+    int w, h, min_size;
+    float alpha;
+    int n, cap, bad;
+    int32_t *rects;
+    int qn[64];
+    float qthr[64];
+} qt_build;
+
+static float qt_threshold(qt_build *b, int region)
+{
+    for (int i = 0; i < b->n && i < 64; i++)
+        if (b->qn[i] == region) return b->qthr[i];
+    double d = (double)(region - 1);
+    float t = (float)or_f_quantile(d, d, (double)b->alpha);
+    if (b->n < 64) {
+        b->qn[b->n] = region;
+        b->qthr[b->n] = t;
+        b->n++;
+    }
+    return t;
+}
+
+static int qt_leaves;
+
+static void qt_visit(qt_build *b, int x, int y, int nw, int nh)
+{
+    if (b->bad) return;
+    float bv, nv;
+    or_qt_criterion(b->img, b->w, b->h, x, y, nw, nh, &bv, &nv);
+    if (bv != bv) {  /* getMeanAndVarBlock2D: IllegalStateException */
+        b->bad = -5;
+        return;
+    }
+    /* performFisherTest (:236-249) */
+    float ratio = jminf(bv, nv) / jmaxf(bv, nv);
+    int divide = ratio < qt_threshold(b, nw * nh);
+    if (nw <= b->min_size || nh <= b->min_size || !divide) {
+        if (qt_leaves < b->cap) {
+            int32_t *r = b->rects + 4 * qt_leaves;
+            r[0] = x, r[1] = y, r[2] = nw, r[3] = nh;
+        }
+        qt_leaves++;
+        return;
+    }
+    int hw = nw / 2, hh = nh / 2;
+    qt_visit(b, x, y, hw, hh);
+    qt_visit(b, x + hw, y, hw, hh);
+    qt_visit(b, x, y + hh, hw, hh);
+    qt_visit(b, x + hw, y + hh, hw, hh);
+}
+
+/* Utils.getInputImage2D's "Quad/Octree" branch (Utils.java:686-703):
+ * QuadTree_(w, h, min_size, alpha).buildTree, calculateRobustMeans(max_iter),
+ * calculateLTSVariances(frac), collectMeanVariancePairs (leaves in pre-order:
+ * top-left, top-right, bottom-left, bottom-right) and performLinearRegression.
+ * rects (x, y, w, h), means and vars receive up to cap leaves; *nleaves the
+ * count.  Returns 0, -5 for a non-finite pixel or a NaN node variance
+ * (Utils.java:881-895), -1 for a one-pixel root (FDistribution(0, 0)) or fewer
+ * than 3 leaves (SimpleRegression.regress), -2 if cap is too small. */
+int or_quadtree_gat(const float *img, int w, int h, int min_size, float alpha, int max_iter, float frac, int cap,
+                    int32_t *rects, float *means, float *vars, int *nleaves, double *g0, double *edc)
+{
+    for (int64_t i = 0; i < (int64_t)w * h; i++)
+        if (!isfinite(img[i])) return -5;
+    if ((int64_t)w * h <= 1) return -1;
+    qt_build b;
+    memset(&b, 0, sizeof b);
+    b.img = img, b.w = w, b.h = h, b.min_size = min_size, b.alpha = alpha, b.cap = cap, b.rects = rects;
+    qt_leaves = 0;
+    qt_visit(&b, 0, 0, w, h);
+    *nleaves = qt_leaves;
+    if (b.bad) return b.bad;
+    if (qt_leaves > cap) return -2;
+    for (int i = 0; i < qt_leaves; i++) {
+        const int32_t *r = rects + 4 * i;
+        means[i] = or_qt_robust_mean(img, w, r[0], r[1], r[2], r[3], max_iter);
+        vars[i] = or_qt_lts_variance(img, w, r[0], r[1], r[2], r[3], means[i], frac);
+    }
+    if (qt_leaves < 3) return -1;
+    double *x = (double *)malloc(sizeof(double) * 2 * (size_t)qt_leaves), *y = x + qt_leaves;
+    for (int i = 0; i < qt_leaves; i++) {
+        x[i] = (double)means[i];
+        y[i] = (double)vars[i];
+    }
+    or_simple_regression(x, y, qt_leaves, g0, edc);
+    free(x);
+    return 0;
+}
+
+/* QuadTree_.applyGATtree (:409-425) in place: ref = 3/8 g0^2 + eDC. */
+void or_gat_tree(float *px, int64_t n, double g0, double edc)
+{
+    double ref = (3.0 / 8.0) * g0 * g0 + edc;
+    for (int64_t i = 0; i < n; i++) {
+        double v = (double)px[i];
+        if ((g0 * v + ref) < 0.0)
+            v = 0.0;
+        else
+            v = (2.0 / g0) * sqrt(g0 * v + ref);
+        px[i] = (float)v;
+    }
+}

@@ -165,8 +165,8 @@ void or_phase_map(const float *img, const float *means, const float *stds, int w
 
 /* Variance stabilisation (VarianceStabilisingTransform2D_.getGAT and
  * GATMinimizer2D).  or_gat: getGAT in place (:136-151).  or_gat_objective:
- * GATMinimizer2D.userFunction with block sums row by row (definition in
- * sred_oracle.c).  or_nelder_mead3 / or_gat_minimize: the build's own
+ * GATMinimizer2D.userFunction with a fixed 64-lane summation order
+ * (definition in sred_oracle.c).  or_nelder_mead3 / or_gat_minimize: the build's own
  * Nelder-Mead simplex in place of ImageJ's Minimizer (parity of the fit
  * with ImageJ unpinned; the library's simplex follows this one step for
  * step). */
@@ -176,6 +176,22 @@ typedef double (*or_fn3)(const double *x, void *ctx);
 double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, int max_iter, double *x);
 double or_gat_minimize(const float *px, int w, int h, double gain0, double sigma0, double offset0, int max_iter,
                        double *params);
+
+/* Quadtree GAT estimate (QuadTree_.java:135-427 as called by
+ * Utils.getInputImage2D, Utils.java:686-703).  Float arithmetic and sum
+ * orders as the reference; definitions and statuses in sred_oracle.c.
+ * or_beta_reg / or_f_quantile / or_simple_regression restate commons-math3
+ * 3.6.1 (the reference's dependency, not in the reference tree). */
+double or_beta_reg(double a, double b, double x);
+double or_f_quantile(double d1, double d2, double p);
+int or_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept);
+void or_qt_criterion(const float *img, int w, int h, int x0, int y0, int nw, int nh, float *block_var,
+                     float *noise_var);
+float or_qt_robust_mean(const float *img, int w, int x0, int y0, int nw, int nh, int max_iter);
+float or_qt_lts_variance(const float *img, int w, int x0, int y0, int nw, int nh, float mean, float frac);
+int or_quadtree_gat(const float *img, int w, int h, int min_size, float alpha, int max_iter, float frac, int cap,
+                    int32_t *rects, float *means, float *vars, int *nleaves, double *g0, double *edc);
+void or_gat_tree(float *px, int64_t n, double g0, double edc);
 
 #ifdef __cplusplus
 }

@@ -271,7 +271,8 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->tile_var, &ctx->counts, &ctx->valid_idx, &ctx->aux, &ctx->tflags, &ctx->u32, &ctx->uhi,
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
-                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v};
+                      &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v,
+                      &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -854,6 +855,76 @@ int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, do
         return SRED_ERR_ARG;
     }
     return gat_minimize_dev(ctx, d_img, w, h, gain0, sigma0, offset0, max_iter, params, error);
+}
+
+int sred_f_quantile(double d1, double d2, double p, double *out)
+{
+    if (!out || !(d1 > 0.0) || !(d2 > 0.0) || !(p >= 0.0 && p <= 1.0)) {
+        set_error("F quantile: needs d1, d2 > 0 and p in [0, 1]");
+        return SRED_ERR_ARG;
+    }
+    *out = f_quantile(d1, d2, p);
+    return SRED_OK;
+}
+
+int sred_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept)
+{
+    if (n < 0 || (n > 0 && (!x || !y)) || !slope || !intercept) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    simple_regression_host(x, y, n, slope, intercept);
+    return SRED_OK;
+}
+
+int sred_quadtree_gat_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int min_size, float alpha,
+                             int max_iter, float lts_fraction, double *g0_edc, int *n_leaves, int cap,
+                             int32_t *rects, float *means, float *variances)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !g0_edc) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return quadtree_gat_dev(ctx, d_img, w, h, min_size, alpha, max_iter, lts_fraction, g0_edc, n_leaves, cap, rects,
+                            means, variances);
+}
+
+int sred_quadtree_gat_2d(sred_ctx *ctx, const float *img, int w, int h, int min_size, float alpha, int max_iter,
+                         float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *rects, float *means,
+                         float *variances)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !g0_edc || w <= 0 || h <= 0) {
+        set_error("null buffer or empty image");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, (int64_t)w * h));
+    return quadtree_gat_dev(ctx, ctx->img.as<float>(), w, h, min_size, alpha, max_iter, lts_fraction, g0_edc,
+                            n_leaves, cap, rects, means, variances);
+}
+
+int sred_gat_tree_2d(sred_ctx *ctx, float *img, int64_t n, double g0, double edc)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    if (n == 0) return SRED_OK;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(gat_tree_dev(ctx, ctx->img.as<float>(), n, g0, edc));
+    return download(ctx, img, ctx->img, n);
+}
+
+int sred_gat_tree_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, double edc)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    return gat_tree_dev(ctx, d_img, n, g0, edc);
 }
 
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,

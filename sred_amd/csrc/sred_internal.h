@@ -126,6 +126,7 @@ struct sred_ctx {
     sred::DevBuf mmpart;       // per-workgroup min/max partials of a block map
     sred::DevBuf adj_v;        // transposed split-BF16 fragments of the adjoint
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
+    sred::DevBuf qt_res, qt_nodes, qt_leaves, qt_seg, qt_tmp;  // quadtree GAT estimate
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
@@ -285,6 +286,15 @@ int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double ga
                       double *error);
 int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0, double offset0,
                      int max_iter, double *params, double *error);
+// The GAT kernel with ref given directly: v <- (g v + ref < 0) ? 0 : (2/g) sqrt(g v + ref).
+int gat_ref_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double ref);
+// Quadtree GAT estimate (quadtree.hip): leaves, robust means, LTS variances
+// and (g0, eDC); applyGATtree; the host F quantile and regression it uses.
+int quadtree_gat_dev(sred_ctx *ctx, const float *d_img, int w, int h, int min_size, float alpha, int max_iter,
+                     float frac, double *g0_edc, int *n_leaves, int cap, int32_t *rects, float *means, float *vars);
+int gat_tree_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, double edc);
+double f_quantile(double d1, double d2, double p);
+void simple_regression_host(const double *x, const double *y, int64_t n, double *slope, double *intercept);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);

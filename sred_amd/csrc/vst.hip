@@ -148,13 +148,18 @@ static double gat_ref(double gain, double sigma, double offset)
     return (3.0 / 8.0) * gain * gain + sigma * sigma - gain * offset;
 }
 
-int gat_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset)
+int gat_ref_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double ref)
 {
     if (n <= 0) return SRED_OK;
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_gat, dim3(blocks), dim3(256), 0, ctx->stream, d_img, n, gain, gat_ref(gain, sigma, offset));
+    hipLaunchKernelGGL(k_gat, dim3(blocks), dim3(256), 0, ctx->stream, d_img, n, gain, ref);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
+}
+
+int gat_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double sigma, double offset)
+{
+    return gat_ref_dev(ctx, d_img, n, gain, gat_ref(gain, sigma, offset));
 }
 
 namespace {

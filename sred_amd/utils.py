@@ -119,25 +119,132 @@ class GATMinimizer2D:
         return err.value
 
 
+class QuadTree:
+    """QuadTree_ (QuadTree_.java:119-427): the Quad/Octree estimate of the GAT
+    parameters.  The device computes the tree, the robust means and the LTS
+    variances in one call (sred_quadtree_gat_2d) when the results are first
+    needed, with the parameters given to the constructor and to
+    calculate_robust_means / calculate_lts_variances."""
+
+    def __init__(self, image_width, image_height, min_size, alpha, resources=None):
+        self.image_width, self.image_height = int(image_width), int(image_height)
+        self.min_size, self.alpha = int(min_size), float(np.float32(alpha))
+        self.max_iterations, self.var_alpha = 50, float(np.float32(0.75))
+        self._pixels = None
+        self._res = resources
+        self._out = None
+
+    def build_tree(self, image_data):
+        self._pixels = L.f32(image_data).reshape(-1)
+        if self._pixels.size != self.image_width * self.image_height:
+            raise ValueError("image size does not match the tree")
+        self._out = None
+
+    def calculate_robust_means(self, image_data, max_iterations):
+        self.max_iterations = int(max_iterations)
+        self._out = None
+
+    def calculate_lts_variances(self, image_data, alpha):
+        self.var_alpha = float(np.float32(alpha))
+        self._out = None
+
+    def _run(self):
+        if self._out is not None:
+            return self._out
+        if self._pixels is None:
+            raise RuntimeError("build_tree first")
+        w, h = self.image_width, self.image_height
+        cap = max(16, (w * h) // 2)
+        rects = np.zeros((cap, 4), np.int32)
+        means = np.zeros(cap, np.float32)
+        var = np.zeros(cap, np.float32)
+        ge = (ctypes.c_double * 2)()
+        n = ctypes.c_int(0)
+        own = self._res is None
+        res = L.Context(0) if own else self._res
+        try:
+            rc = L.load().sred_quadtree_gat_2d(res.handle, L.fptr(self._pixels), w, h, self.min_size, self.alpha,
+                                               self.max_iterations, self.var_alpha, ge, ctypes.byref(n), cap,
+                                               rects.ctypes.data, L.fptr(means), L.fptr(var))
+        finally:
+            if own:
+                res.close()
+        k = n.value
+        if rc < 0 and not (rc == L.SRED_ERR_ARG and 0 < k < 3):
+            L.check(rc)  # raises
+        self._out = (rects[:k].copy(), means[:k].copy(), var[:k].copy())
+        return self._out
+
+    def leaves(self):
+        """Leaf rectangles (x, y, w, h) in pre-order (createOverlay's rectangles)."""
+        return self._run()[0]
+
+    def collect_mean_variance_pairs(self):
+        """[[robust mean, LTS variance], ...] over the leaves in pre-order (as doubles)."""
+        _, m, v = self._run()
+        return [[float(a), float(b)] for a, b in zip(m, v)]
+
+    def perform_linear_regression(self, pairs):
+        """(g0, eDC) = slope and intercept of SimpleRegression over the pairs;
+        ValueError below 3 pairs (SimpleRegression.regress throws)."""
+        p = np.asarray(pairs, np.float64).reshape(-1, 2)
+        if len(p) < 3:
+            raise ValueError("not enough data (at least 3 points) for a regression")
+        x, y = np.ascontiguousarray(p[:, 0]), np.ascontiguousarray(p[:, 1])
+        s, i = ctypes.c_double(), ctypes.c_double()
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.check(L.load().sred_simple_regression(x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(p), ctypes.byref(s),
+                                                ctypes.byref(i)))
+        return [s.value, i.value]
+
+    def apply_gat_tree(self, image_data, n_pixels, g0, edc):
+        """QuadTree_.applyGATtree: a new float32 array of the first n_pixels."""
+        out = L.f32(image_data).reshape(-1)[:n_pixels].copy()
+        own = self._res is None
+        res = L.Context(0) if own else self._res
+        try:
+            L.check(L.load().sred_gat_tree_2d(res.handle, L.fptr(out), out.size, float(g0), float(edc)))
+        finally:
+            if own:
+                res.close()
+        return out
+
+
+def f_quantile(d1, d2, p):
+    """Inverse CDF of the F(d1, d2) distribution (FDistribution.inverseCumulativeProbability)."""
+    out = ctypes.c_double()
+    L.check(L.load().sred_f_quantile(float(d1), float(d2), float(p), ctypes.byref(out)))
+    return out.value
+
+
 def get_input_image_2d(image, stabilise_noise_variance=False, normalize_output=True, gat_method="Simplex",
                        max_iter=5000, resources=None):
     """Utils.getInputImage2D (Utils.java:653-725, overload :746-775) for an array.
 
-    Converts to float32; with stabilise_noise_variance, fits the GAT
-    parameters (GATMinimizer2D with initial (1, 10, 100), Utils.java:695) and
-    applies getGAT with them rounded to float (:697-701); then min-max
-    normalises as (v-min)/(max-min+EPSILON).
+    Converts to float32; with stabilise_noise_variance and gat_method
+    "Simplex", fits the GAT parameters (GATMinimizer2D with initial
+    (1, 10, 100), Utils.java:695) and applies getGAT with them rounded to
+    float (:697-701); with "Quad/Octree", estimates (g0, eDC) with
+    QuadTree_(w, h, 4, 0.01f), 50 robust-mean iterations and LTS fraction
+    0.75f and applies applyGATtree (:686-703); another method string
+    stabilises nothing (the reference compares the strings by identity).
+    Then min-max normalises as (v-min)/(max-min+EPSILON).
     """
     arr = L.f32(np.array(image, dtype=np.float32, copy=True))
     if arr.ndim != 2:
         raise ValueError("expected a 2D image")
     h, w = arr.shape
-    if stabilise_noise_variance:
-        if gat_method != "Simplex":
-            raise NotImplementedError("the Quad/Octree GAT method is not built")
+    if stabilise_noise_variance and gat_method == "Simplex":
         m = GATMinimizer2D(arr, w, h, 1, 10, 100, max_iter, resources)
         m.run()
         arr = get_gat_2d(arr, np.float32(m.gain), np.float32(m.sigma), np.float32(m.offset), resources).reshape(h, w)
+    if stabilise_noise_variance and gat_method == "Quad/Octree":
+        qt = QuadTree(w, h, 4, 0.01, resources)
+        qt.build_tree(arr)
+        qt.calculate_robust_means(arr, 50)
+        qt.calculate_lts_variances(arr, 0.75)
+        g0, edc = qt.perform_linear_regression(qt.collect_mean_variance_pairs())
+        arr = qt.apply_gat_tree(arr, w * h, g0, edc).reshape(h, w)
     if normalize_output:
         L.check(L.load().sred_normalize_input_2d(L.fptr(arr), arr.size))
     return InputImage2D(arr, w, h, w * h)

@@ -59,8 +59,11 @@ def test_input_normalisation_matches_oracle_bitwise():
     np.testing.assert_array_equal(a, O.normalize_input(img))
     k = K["input_image_2d_normalize"]
     np.testing.assert_allclose(sred_amd.get_input_image_2d(k["image"]).image_array, k["expected"], atol=k["tol"])
-    with pytest.raises(NotImplementedError):  # the quadtree GAT method is not built
-        sred_amd.get_input_image_2d(img, stabilise_noise_variance=True, gat_method="Quad/Octree")
+    # a method string other than "Simplex" or "Quad/Octree" stabilises nothing
+    # (Utils.java:674, :686 compare the strings by identity)
+    np.testing.assert_array_equal(
+        sred_amd.get_input_image_2d(img, stabilise_noise_variance=True, gat_method="none").image_array,
+        O.normalize_input(img))
 
 
 def test_reference_block_kat_and_oracle():
@@ -189,3 +192,33 @@ def test_contraction_plan_covers_each_pair_once(nv, k, metric):
                 assert ch * cpc < npad
                 slots[ch, rt * rows:(rt + 1) * rows] += 1
         assert (slots == 1).all()  # every partial slot has exactly one writer
+
+
+def test_host_f_quantile_and_regression():
+    # the library's host helpers of the quadtree GAT estimate (no GPU):
+    # the F quantile against scipy and closed forms, the regression against
+    # numpy's least squares and an exact line
+    st = pytest.importorskip("scipy.stats")
+    from sred_amd import utils
+    p = float(np.float32(0.01))
+    for d in [1, 2, 3, 15, 63, 1023, 16383, 262143]:
+        q = utils.f_quantile(d, d, p)
+        assert q == pytest.approx(st.f.ppf(p, d, d), rel=1e-11)
+        assert np.float32(q) == np.float32(O.f_quantile(d, d, p))
+    assert utils.f_quantile(2, 2, 0.3) == pytest.approx(0.3 / 0.7, rel=1e-14)  # F(2,2): x/(1+x)
+    assert utils.f_quantile(40, 40, 0.5) == pytest.approx(1.0, abs=1e-14)
+    assert utils.f_quantile(7, 12, 0.2) == pytest.approx(st.f.ppf(0.2, 7, 12), rel=1e-11)
+    with pytest.raises(sred_amd.SredError):
+        utils.f_quantile(0, 3, 0.5)
+    qt = utils.QuadTree(8, 8, 4, 0.01)
+    rng = np.random.default_rng(4)
+    x = rng.random(200) * 50
+    y = 1.7 * x - 3.25 + rng.normal(0, 0.5, 200)
+    s, i = qt.perform_linear_regression(np.stack([x, y], 1))
+    ps, pi = np.polyfit(x, y, 1)
+    assert s == pytest.approx(ps, rel=1e-12) and i == pytest.approx(pi, rel=1e-11)
+    assert (s, i) == O.simple_regression(x, y)
+    s, i = qt.perform_linear_regression([[1.0, 5.0], [2.0, 7.0], [4.0, 11.0]])
+    assert (s, i) == (2.0, 3.0)
+    with pytest.raises(ValueError):
+        qt.perform_linear_regression([[1.0, 2.0], [2.0, 3.0]])
