@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: bf16x3 steps only, then a 256 MiB device copy (PMC calibration)")
-    ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase", "vst", "quadtree"],
+    ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase", "vst", "quadtree",
+                                           "timelapse"],
                     default="global",
                     help="global: the headline metric (default); block: block repetition 2048x2048 with a 15x15 "
                          "ring template (four metrics and the Hu map); adjoint: global Pearson map + adjoint, "
@@ -60,7 +61,7 @@ def parse():
                          "in one batched call; hu: global Hu map 1024x1024, 11x11; phase: phase correlation map "
                          "256x256, 5x5; vst: variance-stabilised input image 1024x1024 (GAT fit, 5000 simplex "
                          "iterations, and transform); quadtree: the same image stabilised by the Quad/Octree "
-                         "estimate")
+                         "estimate; timelapse: block repetition over a 100-frame 512x512 stack, 15x15 ring template")
     return ap.parse_args()
 
 
@@ -293,6 +294,65 @@ def pairmaps_run(args):
         "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
                      "traffic": None, "kernel": "k_hu_pairs" if hu else "k_phase_pairs<5, 5, true>",
                      "flops_per_pair": flops_pair},
+        "cpu_baseline": cpu}))
+
+
+def timelapse_run(args):
+    """The time-lapse branch of BlockRepetition2D_ (BlockRepetition2D_.java:95-137),
+    no stabilisation, Pearson, relevance constant 0, normalised: a stack of
+    100 frames of 512x512 Poisson rings and blobs (seed 1 + t), 15x15 ring
+    template (K = 149), stack and maps resident in HBM.  One step = the whole
+    stack: stack and frame normalisations, one block pipeline per frame, the
+    final stack normalisation.  Roofline of the stencil: the FP32 vector ALU
+    at 6K flops per pixel, as the single-image block workload."""
+    import torch
+
+    import oracle as O
+    import sred_amd
+    from sred_amd import _lib as L
+    from sred_amd import synthetic
+
+    D, S, T = 100, 512, 15
+    frames = [synthetic.rings_and_blobs(S, 180, 180, seed=1 + t) for t in range(D)]
+    st = np.stack(frames).astype(np.float32)
+    tmpl = synthetic.ring_template(T)
+    rb = sred_amd.get_reference_block_2d(tmpl)
+    d_st = torch.from_numpy(st).cuda()
+    d_out = torch.empty_like(d_st)
+    ctx = sred_amd.Context(0)
+    tp = L.f32(rb.pixels)
+
+    def step():
+        L.check(L.load().sred_block_timelapse_2d_dev(ctx.handle, O.PEARSON, d_st.data_ptr(), S, S, D, L.fptr(tp), T,
+                                                     T, rb.mean, rb.std, 0.0, 1, d_out.data_ptr()))
+
+    t, kms = _timed(step, args.steps, args.warmup, lambda: ctx.last_contraction()[0])
+    ctx.close()
+    px = D * S * S
+    flops = 6.0 * rb.size * px
+    ach = flops / t / 1e12
+    cpu = None
+    if not args.no_cpu:
+        n_fr = 30
+        t0 = time.perf_counter()
+        O.block_timelapse(O.PEARSON, st[:n_fr], rb.pixels, T, T, rb.mean, rb.std, 0.0, True)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": n_fr / t_cpu, "unit": "frames/s", "cores": host_cores(), "kind": "port",
+               "sample": f"oracle restatement (refsim block maps, OpenMP), first {n_fr} frames of the stack, "
+                         f"{t_cpu:.2f} s"}
+    print(f"# timelapse: {t * 1e3:.2f} ms per {D}-frame stack ({t * 1e3 / D:.3f} ms per frame), stencil "
+          f"{kms:.2f} ms", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "frames per second, time-lapse block repetition (512², 15×15 ring template, Pearson)",
+        "value": D / t, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {D} frames of Poisson rings and blobs at {S}x{S}, seeds 1..{D}",
+        "config": {"workload": f"BlockRepetition2D_ time-lapse, {D} frames {S}x{S}, 15x15 ring template (K="
+                               f"{rb.size}), relevance 0, normalised"},
+        "stencil_ms": kms,
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_VEC_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP32_VEC_TFLOPS, "traffic": None,
+                     "flops_per_pixel": 6 * rb.size, "note": "whole step, launch overheads included"},
         "cpu_baseline": cpu}))
 
 
@@ -555,6 +615,9 @@ def main():
         return
     if args.workload == "quadtree":
         quadtree_run(args)
+        return
+    if args.workload == "timelapse":
+        timelapse_run(args)
         return
     if args.workload == "vst":
         vst_run(args)

@@ -151,6 +151,20 @@ int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float
 int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl,
                           int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
                           int normalize, float *d_out);
+/* The time-lapse branch of BlockRepetition2D_.run (BlockRepetition2D_.java:95-137)
+ * on a stack of depth >= 3 frames (frame t at stack + t*w*h), without noise
+ * stabilisation: the stack min-max normalised as a whole (getInputImage3D),
+ * each frame normalised again (getInputImage2D(float[], ...)) and mapped as
+ * sred_block_map_2d_dev does, then, when normalize, the stacked maps
+ * normalised as a whole.  Synchronises once per frame when
+ * relevance_constant != 0.  The _dev form takes device buffers (the template
+ * stays a host array); the other copies the stack in and the maps out. */
+int sred_block_timelapse_2d(sred_ctx *ctx, int metric, const float *stack, int w, int h, int depth,
+                            const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std,
+                            float relevance_constant, int normalize, float *out);
+int sred_block_timelapse_2d_dev(sred_ctx *ctx, int metric, const float *d_stack, int w, int h, int depth,
+                                const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std,
+                                float relevance_constant, int normalize, float *d_out);
 
 /* ---- batched global repetition (no counterpart in the reference: SReD
  * runs one image per command and loops stacks in Java) ---- */

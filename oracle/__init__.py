@@ -499,3 +499,22 @@ def gat_tree(img, g0, edc):
     out = _f32(img).copy()
     lib().or_gat_tree(out.reshape(-1), out.size, float(g0), float(edc))
     return out
+
+
+def block_timelapse(metric, stack, tmpl_px, bw, bh, tmpl_mean, tmpl_std, c, normalize):
+    """The time-lapse branch of BlockRepetition2D_ (BlockRepetition2D_.java:95-137)
+    without stabilisation, composed of the oracle's steps: the stack
+    normalised as a whole (Utils.normalizeArray), each frame normalised
+    (getInputImage2D(float[])) and mapped (block_map), the maps normalised as
+    a whole when normalize."""
+    st = _f32(stack)
+    d, h, w = st.shape
+    if d < 3:
+        raise ValueError("Image must have at least 3 slices.")
+    flat = normalize_input(st.reshape(-1)).reshape(d, h, w)
+    out = np.zeros_like(flat)
+    for t in range(d):
+        out[t] = block_map(metric, normalize_input(flat[t]), tmpl_px, bw, bh, tmpl_mean, tmpl_std, c, normalize)
+    if normalize:
+        out = normalize_input(out.reshape(-1)).reshape(d, h, w)
+    return out

@@ -38,7 +38,7 @@ EXPORTS = [
     "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev", "sred_ctx_last_skipped",
     "sred_gat_2d", "sred_gat_2d_dev", "sred_gat_objective_2d", "sred_gat_minimize_2d", "sred_gat_minimize_2d_dev",
     "sred_quadtree_gat_2d", "sred_quadtree_gat_2d_dev", "sred_gat_tree_2d", "sred_gat_tree_2d_dev", "sred_f_quantile",
-    "sred_simple_regression",
+    "sred_simple_regression", "sred_block_timelapse_2d", "sred_block_timelapse_2d_dev",
 ]
 
 
@@ -124,6 +124,8 @@ def load():
         "sred_quadtree_gat_2d_dev": ([_VP, _VP, _I, _I, _I, _FL, _I, _FL, ctypes.POINTER(_D), ctypes.POINTER(_I),
                                       _I, _VP, _F, _F], _I),
         "sred_gat_tree_2d": ([_VP, _F, _I64, _D, _D], _I),
+        "sred_block_timelapse_2d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _F], _I),
+        "sred_block_timelapse_2d_dev": ([_VP, _I, _VP, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_gat_tree_2d_dev": ([_VP, _VP, _I64, _D, _D], _I),
         "sred_f_quantile": ([_D, _D, _D, ctypes.POINTER(_D)], _I),
         "sred_simple_regression": ([ctypes.POINTER(_D), ctypes.POINTER(_D), _I64, ctypes.POINTER(_D),

@@ -53,6 +53,34 @@ def block_repetition_2d(block, image, relevance_constant=0.0, metric="Pearson's 
             res.close()
 
 
+def block_repetition_timelapse_2d(block, stack, relevance_constant=0.0, metric="Pearson's R",
+                                  normalize_output=True, resources=None):
+    """The time-lapse branch of BlockRepetition2D_ (BlockRepetition2D_.java:95-137),
+    without noise stabilisation: one repetition map per frame of a (depth, H, W)
+    stack (depth >= 3), as a float32 (depth, H, W) array."""
+    ref = utils.get_reference_block_2d(block)
+    if ref is None:
+        raise ValueError("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.")
+    st = L.f32(stack)
+    if st.ndim != 3:
+        raise ValueError("expected a (depth, height, width) stack")
+    d, h, w = st.shape
+    m = L.load().sred_metric_from_name(metric.encode(), 1)
+    if m < 0:
+        return None
+    out = np.zeros_like(st)
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        L.check(L.load().sred_block_timelapse_2d(res.handle, m, L.fptr(st), w, h, d, L.fptr(L.f32(ref.pixels)),
+                                                 ref.width, ref.height, ref.mean, ref.std, float(relevance_constant),
+                                                 int(bool(normalize_output)), L.fptr(out)))
+    finally:
+        if own:
+            res.close()
+    return out
+
+
 def global_map_device(ctx, metric_id, precision_id, d_img, width, height, block_width, block_height,
                       relevance_constant, normalize_output, d_out, d_means=0, d_stds=0, d_mask=0, d_S=0, d_W=0):
     """sred_global_map_2d_dev on device pointers (ints, e.g. torch data_ptr())."""

@@ -228,6 +228,47 @@ static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const fl
     return SRED_OK;
 }
 
+// The time-lapse branch of BlockRepetition2D_.run (BlockRepetition2D_.java:95-137)
+// without stabilisation: the stack normalised as a whole (getInputImage3D,
+// Utils.java:1257-1260), each frame normalised again (getInputImage2D(float[]),
+// Utils.java:760-775) and mapped by the block pipeline, the stacked maps
+// normalised as a whole when normalize (getInputImage3D(float[], ...)).
+static int block_timelapse_dev(sred_ctx *ctx, int metric, const float *d_stack, int w, int h, int depth,
+                               const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std, float c,
+                               int normalize, float *d_out)
+{
+    if (depth < 3) {
+        set_error("Image must have at least 3 slices. Please try again.");
+        return SRED_ERR_ARG;
+    }
+    Geometry g;
+    SRED_TRY(make_geometry(w, h, bw, bh, &g));
+    const int64_t n = g.n(), total = n * depth;
+    SRED_TRY(ctx->stack_work.ensure(sizeof(float) * total));
+    float *work = ctx->stack_work.as<float>();
+    SRED_TRY(stack_frames_normalize_dev(ctx, d_stack, work, n, depth));
+    if ((metric == SRED_PEARSON || metric == SRED_SSIM) && c == 0.0f && normalize) {
+        // the default configuration: all frames in one stencil launch (grid z = frame), whose
+        // workgroups supply each frame's min/max partials for its normalisation
+        for (int t0 = 0; t0 < depth; t0 += 65535) {
+            const int nf = std::min(depth - t0, 65535);
+            const int64_t nwg = block_workgroups(g);
+            SRED_TRY(ctx->mmpart.ensure(sizeof(unsigned int) * 3 * (size_t)nwg * nf));
+            int nparts = 0;
+            SRED_TRY(block_raw_dev(ctx, g, metric, work + t0 * n, nullptr, nullptr, tmpl, tmpl_mean, tmpl_std,
+                                   d_out + t0 * n, ctx->mmpart.as<unsigned int>(), &nparts, nf));
+            SRED_TRY(normalize_frames_from_parts_dev(ctx, g, d_out + t0 * n, nf, ctx->mmpart.as<unsigned int>(),
+                                                     nparts, 0));
+        }
+    } else {
+        for (int t = 0; t < depth; t++)
+            SRED_TRY(block_pipeline(ctx, g, metric, work + t * n, tmpl, tmpl_mean, tmpl_std, c, normalize,
+                                    d_out + t * n));
+    }
+    if (normalize) SRED_TRY(seg_normalize_dev(ctx, d_out, d_out, total, 1));
+    return SRED_OK;
+}
+
 }  // namespace sred
 
 using namespace sred;
@@ -272,7 +313,8 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->ulo, &ctx->partial, &ctx->misc, &ctx->offs, &ctx->offs2, &ctx->tmpl, &ctx->dmap, &ctx->grad,
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
                       &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v,
-                      &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp};
+                      &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp,
+                      &ctx->stack_work, &ctx->seg_mm};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -855,6 +897,44 @@ int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, do
         return SRED_ERR_ARG;
     }
     return gat_minimize_dev(ctx, d_img, w, h, gain0, sigma0, offset0, max_iter, params, error);
+}
+
+int sred_block_timelapse_2d_dev(sred_ctx *ctx, int metric, const float *d_stack, int w, int h, int depth,
+                                const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std,
+                                float relevance_constant, int normalize, float *d_out)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_stack || !tmpl || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric < SRED_PEARSON || metric > SRED_ABSDIFF) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    return block_timelapse_dev(ctx, metric, d_stack, w, h, depth, tmpl, bw, bh, tmpl_mean, tmpl_std,
+                               relevance_constant, normalize, d_out);
+}
+
+int sred_block_timelapse_2d(sred_ctx *ctx, int metric, const float *stack, int w, int h, int depth,
+                            const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std,
+                            float relevance_constant, int normalize, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!stack || !tmpl || !out || w <= 0 || h <= 0 || depth <= 0) {
+        set_error("null buffer or empty stack");
+        return SRED_ERR_ARG;
+    }
+    if (metric < SRED_PEARSON || metric > SRED_ABSDIFF) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
+    }
+    const int64_t total = (int64_t)w * h * depth;
+    SRED_TRY(upload(ctx, ctx->img, stack, total));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * total));
+    SRED_TRY(block_timelapse_dev(ctx, metric, ctx->img.as<float>(), w, h, depth, tmpl, bw, bh, tmpl_mean, tmpl_std,
+                                 relevance_constant, normalize, ctx->out.as<float>()));
+    return download(ctx, out, ctx->out, total);
 }
 
 int sred_f_quantile(double d1, double d2, double p, double *out)

@@ -92,6 +92,11 @@ __global__ __launch_bounds__(256) void k_block(const float *__restrict__ img, in
     float *tile = sm;
     float4 *tp = reinterpret_cast<float4 *>(sm + ((th * RS + 3) & ~3));
     const int x0 = blockIdx.x * kBx, y0 = blockIdx.y * kBy;
+    if (blockIdx.z) {  // frame of a stack (batched frames: no statistics outputs)
+        const int64_t zoff = (int64_t)blockIdx.z * w * h;
+        img += zoff;
+        out += zoff;
+    }
     for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
         const int ly = i / tw, lx = i - ly * tw;
         const int gy = y0 - brh + ly, gx = x0 - brw + lx;
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(256) void k_block(const float *__restrict__ img, in
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            const int b = blockIdx.y * gridDim.x + blockIdx.x;
+            const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
             mmpart[3 * b] = slo[0];
             mmpart[3 * b + 1] = shi[0];
             mmpart[3 * b + 2] = 0u;
@@ -215,8 +220,12 @@ int64_t block_workgroups(const Geometry &g) { return (int64_t)((g.w + kBx - 1) /
 
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
                   float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out,
-                  unsigned int *d_mmpart, int *nparts)
+                  unsigned int *d_mmpart, int *nparts, int nframes)
 {
+    if (nframes < 1 || nframes > 65535 || (nframes > 1 && (d_means_out || d_stds_out))) {
+        set_error("invalid frame count for the block stencil");
+        return SRED_ERR_ARG;
+    }
     if (metric < 0 || metric > 3) {
         set_error("unknown metric");
         return SRED_ERR_UNKNOWN_METRIC;
@@ -254,7 +263,7 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
     const int tw = kBx + 2 * g.brw, th = kBy + 2 * g.brh;
     const int RS = (tw + 62) / 64 * 64 + 1;
     const size_t lds = sizeof(float) * (size_t)((th * RS + 3) & ~3) + sizeof(float4) * (size_t)(g.K + 3);
-    dim3 grid((g.w + kBx - 1) / kBx, (g.h + kBy - 1) / kBy);
+    dim3 grid((g.w + kBx - 1) / kBx, (g.h + kBy - 1) / kBy, nframes);
     if (nparts) *nparts = (int)(grid.x * grid.y);
     float *mo = d_stds_out ? d_means_out : nullptr, *so = d_means_out ? d_stds_out : nullptr;
     const float4 *d_tp = ctx->tmpl.as<float4>();
@@ -278,7 +287,7 @@ int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_i
                            g.K, rt, d_tp, tm, ts, tmpl_std, ed, d_out, mo, so, d_mmpart);
     }
     SRED_HIP_CHECK(hipGetLastError());
-    SRED_TRY(time_end(ctx, g.n()));
+    SRED_TRY(time_end(ctx, g.n() * nframes));
     return SRED_OK;
 }
 

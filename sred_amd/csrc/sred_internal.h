@@ -127,6 +127,7 @@ struct sred_ctx {
     sred::DevBuf adj_v;        // transposed split-BF16 fragments of the adjoint
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf qt_res, qt_nodes, qt_leaves, qt_seg, qt_tmp;  // quadtree GAT estimate
+    sred::DevBuf stack_work, seg_mm;                           // time-lapse stacks
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
@@ -252,6 +253,8 @@ int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const floa
 // Normalisation without mask of a map whose interior min/max partials (3
 // words each: min, max, 0 as ordered ints) are given; init_max_min_value as
 // normalize_map_dev.
+int normalize_frames_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_maps, int nframes,
+                                    const unsigned int *d_parts, int nparts, int init_max_min_value);
 int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const unsigned int *d_parts,
                                  int nparts, int init_max_min_value);
 // d_mmpart (optional): per-workgroup interior min/max partials of the output
@@ -259,7 +262,7 @@ int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map,
 int64_t block_workgroups(const Geometry &g);
 int block_raw_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, float *d_means_out,
                   float *d_stds_out, const float *h_tmpl, float tmpl_mean, float tmpl_std, float *d_out,
-                  unsigned int *d_mmpart = nullptr, int *nparts = nullptr);
+                  unsigned int *d_mmpart = nullptr, int *nparts = nullptr, int nframes = 1);
 // Batched global maps (see batch.hip).
 int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem *probs, float c, int normalize);
 // Hu template map (kernelGetPatchHu2D.cl) of a device image; h_tmpl is the
@@ -294,6 +297,11 @@ int quadtree_gat_dev(sred_ctx *ctx, const float *d_img, int w, int h, int min_si
                      float frac, double *g0_edc, int *n_leaves, int cap, int32_t *rects, float *means, float *vars);
 int gat_tree_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, double edc);
 double f_quantile(double d1, double d2, double p);
+// nseg min-max normalisations (v - min) / (max - min + EPSILON) of
+// consecutive segments of seg floats (stack.hip); d_in may equal d_out.
+int seg_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
+// The stack normalised as a whole, then each of its nseg frames of seg floats.
+int stack_frames_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
 void simple_regression_host(const double *x, const double *y, int64_t n, double *slope, double *intercept);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).

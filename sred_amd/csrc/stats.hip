@@ -365,6 +365,55 @@ int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map,
     return SRED_OK;
 }
 
+// The same for nframes maps of a stack (frame z at d_maps + z*w*h, partials
+// at d_parts + 3*nparts*z): one workgroup per frame, then one remap launch.
+__global__ __launch_bounds__(256) void k_frames_minmax_from_parts(const unsigned int *__restrict__ part, int nparts,
+                                                                  float init_max, unsigned int *__restrict__ mm)
+{
+    unsigned int *m = mm + 4 * blockIdx.x;
+    if (threadIdx.x == 0) {
+        m[0] = f2ord(FLT_MAX);
+        m[1] = f2ord(init_max);
+        m[2] = 0;
+    }
+    __syncthreads();
+    k_minmax_final_body(part + 3 * (size_t)nparts * blockIdx.x, nparts, m);
+}
+
+__global__ __launch_bounds__(256) void k_frames_remap(float *__restrict__ maps, int w, int h, int brw, int brh,
+                                                      const unsigned int *__restrict__ mm)
+{
+#pragma clang fp contract(off)
+    const unsigned int *m = mm + 4 * blockIdx.y;
+    const float mn = ord2f(m[0]), mx = ord2f(m[1]);
+    const float den = mx - mn + kEps;
+    float *map = maps + (int64_t)blockIdx.y * w * h;
+    for (int y = brh + blockIdx.x; y < h - brh; y += gridDim.x) {
+        const int64_t row = (int64_t)y * w;
+        for (int x = brw + threadIdx.x; x < w - brw; x += blockDim.x) map[row + x] = (map[row + x] - mn) / den;
+    }
+}
+
+int normalize_frames_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_maps, int nframes,
+                                    const unsigned int *d_parts, int nparts, int init_max_min_value)
+{
+    if (nframes < 1 || nframes > 65535) {
+        set_error("invalid frame count");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(ctx->seg_mm.ensure(sizeof(unsigned int) * 4 * (size_t)nframes + 256));
+    unsigned int *mm = ctx->seg_mm.as<unsigned int>();
+    const float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
+    hipLaunchKernelGGL(k_frames_minmax_from_parts, dim3(nframes), dim3(256), 0, ctx->stream, d_parts, nparts,
+                       init_max, mm);
+    const int rows = std::max(1, g.h - 2 * g.brh);
+    const int gx = std::max(1, std::min(rows, std::max(1, 8192 / nframes)));
+    hipLaunchKernelGGL(k_frames_remap, dim3(gx, nframes), dim3(256), 0, ctx->stream, d_maps, g.w, g.h, g.brw, g.brh,
+                       mm);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
 int mask_is_binary_dev(sred_ctx *ctx, const Geometry &g, const float *d_mask, int *is_binary)
 {
     SRED_TRY(ctx->misc.ensure(kMiscBytes));
