@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: bf16x3 steps only, then a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase", "vst", "quadtree",
-                                           "timelapse"],
+                                           "timelapse", "tools"],
                     default="global",
                     help="global: the headline metric (default); block: block repetition 2048x2048 with a 15x15 "
                          "ring template (four metrics and the Hu map); adjoint: global Pearson map + adjoint, "
@@ -61,7 +61,8 @@ def parse():
                          "in one batched call; hu: global Hu map 1024x1024, 11x11; phase: phase correlation map "
                          "256x256, 5x5; vst: variance-stabilised input image 1024x1024 (GAT fit, 5000 simplex "
                          "iterations, and transform); quadtree: the same image stabilised by the Quad/Octree "
-                         "estimate; timelapse: block repetition over a 100-frame 512x512 stack, 15x15 ring template")
+                         "estimate; timelapse: block repetition over a 100-frame 512x512 stack, 15x15 ring template; "
+                         "tools: the relevance-mask tools on a 1024x1024 image")
     return ap.parse_args()
 
 
@@ -295,6 +296,80 @@ def pairmaps_run(args):
                      "traffic": None, "kernel": "k_hu_pairs" if hu else "k_phase_pairs<5, 5, true>",
                      "flops_per_pair": flops_pair},
         "cpu_baseline": cpu}))
+
+
+def tools_run(args):
+    """The relevance-mask tools on a 1024x1024 Poisson rings-and-blobs image
+    (seed 1), 3x3 blocks: the shared input step (GAT fit from gain 0, 5000
+    iterations at most, GAT, double normalisation), then RelevanceMask2D_'s
+    mask (constant 1.0) and OptimiseRelevanceMask_'s scan (step 0.001) on the
+    prepared image, device buffers throughout.  One step = the input step
+    plus both tools; their times are reported separately.  The input step is
+    the simplex fit (see the vst workload); the tools are a stencil, a tile
+    reduction and, for the scan, one extremes reduction and a host loop."""
+    import ctypes
+
+    import torch
+
+    import oracle as O
+    import sred_amd
+    from sred_amd import _lib as L
+    from sred_amd import synthetic
+
+    S = 1024
+    img = synthetic.rings_and_blobs(S, seed=1)
+    d_src = torch.from_numpy(img).cuda()
+    d_img = torch.empty_like(d_src)
+    d_mask = torch.empty_like(d_src)
+    ctx = sred_amd.Context(0)
+    p = (ctypes.c_double * 3)()
+    nv, thr, ones, zeros = ctypes.c_float(), ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    lib = L.load()
+    parts = {"prep": [], "mask": [], "scan": []}
+
+    def step():
+        t0 = time.perf_counter()
+        d_img.copy_(d_src)
+        L.check(lib.sred_tool_prep_2d_dev(ctx.handle, d_img.data_ptr(), S, S, 5000, p))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.check(lib.sred_relevance_mask_tool_2d_dev(ctx.handle, d_img.data_ptr(), S, S, 3, 3, 1.0, d_mask.data_ptr(),
+                                                    ctypes.byref(nv), ctypes.byref(thr)))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        L.check(lib.sred_optimise_relevance_mask_2d_dev(ctx.handle, d_img.data_ptr(), S, S, 3, 3, 0.001, 100000000,
+                                                        ctypes.byref(ones), ctypes.byref(zeros)))
+        t3 = time.perf_counter()
+        parts["prep"].append(t1 - t0)
+        parts["mask"].append(t2 - t1)
+        parts["scan"].append(t3 - t2)
+
+    t, _ = _timed(step, args.steps, args.warmup, lambda: 0.0)
+    ctx.close()
+    k = args.steps
+    ms = {name: 1e3 * float(np.mean(v[-k:])) for name, v in parts.items()}
+    cpu = None
+    if not args.no_cpu:
+        prep = d_img.cpu().numpy()
+        t0 = time.perf_counter()
+        O.relevance_mask_tool(prep, 3, 3, 1.0)
+        O.optimise_relevance_mask(prep, 3, 3, 0.001)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": t_cpu * 1e3, "unit": "ms for both tools", "cores": 1, "kind": "port",
+               "sample": f"oracle restatements of RelevanceMask2D_ and OptimiseRelevanceMask_ (the scan rebuilding "
+                         f"every mask, as the reference does) on the prepared {S}x{S} image"}
+    print(f"# tools: {t * 1e3:.2f} ms per step (input step {ms['prep']:.2f}, mask {ms['mask']:.3f}, scan "
+          f"{ms['scan']:.3f} ms), constants ({ones.value}, {zeros.value})", file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "ms per relevance-mask tool run (input step + RelevanceMask2D_ + OptimiseRelevanceMask_), 1024²",
+        "value": t * 1e3, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: Poisson counts of 3000 rings + 3000 blobs at 1024x1024 (200 photons peak), seed 1",
+        "config": {"workload": "RelevanceMask2D_ (3x3, constant 1.0) and OptimiseRelevanceMask_ (3x3, step 0.001) "
+                               "after their shared input step, 1024x1024"},
+        "input_step_ms": ms["prep"], "mask_ms": ms["mask"], "scan_ms": ms["scan"],
+        "gat_fit": list(p), "noise_var": nv.value, "constant_range": [ones.value, zeros.value],
+        "roofline": None, "cpu_baseline": cpu}))
 
 
 def timelapse_run(args):
@@ -618,6 +693,9 @@ def main():
         return
     if args.workload == "timelapse":
         timelapse_run(args)
+        return
+    if args.workload == "tools":
+        tools_run(args)
         return
     if args.workload == "vst":
         vst_run(args)

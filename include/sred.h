@@ -260,6 +260,40 @@ int sred_gat_tree_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, doub
  * used at QuadTree_.java:241-245; bisection to adjacent doubles) and the
  * least-squares line of commons-math3 SimpleRegression (QuadTree_.java:395-407). */
 int sred_f_quantile(double d1, double d2, double p, double *out);
+
+/* ---- relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,
+ *      CalculateLocalMeans_) ---- */
+/* The tools' shared input step, in place (RelevanceMask2D_.java:133-164,
+ * OptimiseRelevanceMask_.java:124-154, CalculateLocalMeans_.java:163-190):
+ * the GAT fit from (gain, sigma, offset) = (0, 100, 0) (as
+ * sred_gat_minimize_2d; invalid parameters rank worst), getGAT with the
+ * fitted doubles, and (v - min)/(max - min + EPSILON) in double rounded to
+ * float.  params[3] (may be NULL) receives the fit.  Synchronises. */
+int sred_tool_prep_2d(sred_ctx *ctx, float *img, int w, int h, int max_iter, double *params);
+int sred_tool_prep_2d_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params);
+/* RelevanceMask2D_ after the input step (RelevanceMask2D_.java:165-250):
+ * local standard deviations (kernelGetLocalMeans.cl), the noise variance
+ * from double tile sums, threshold = noiseVar * filter_constant
+ * (filter_constant alone when noiseVar is 0), mask 0 where std^2 < threshold
+ * or std^2 == 0, else 1, 0 on the borders.  Odd square blocks only (the
+ * reference swaps the radii and overruns its patch array otherwise).
+ * *noise_var and *threshold may be NULL.  Synchronises. */
+int sred_relevance_mask_tool_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float filter_constant,
+                                float *mask, float *noise_var, float *threshold);
+int sred_relevance_mask_tool_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh,
+                                    float filter_constant, float *d_mask, float *noise_var, float *threshold);
+/* OptimiseRelevanceMask_ after the input step (OptimiseRelevanceMask_.java:
+ * 155-287): filter constants 0, step, 2 step, ... (a float sum); the
+ * plugin's masks compare the squares of kernelGetRelevanceMask2D.cl's local
+ * variances with noiseVar * c (c when noiseVar is 0).  *all_ones = the last
+ * constant whose mask is all ones inside the borders, *all_zeros = the first
+ * whose mask is all zeros.  Errors where the reference loops for ever: step
+ * <= 0, a NaN variance, a sum that stops growing, or more than max_steps
+ * constants.  Odd square blocks only.  Synchronises. */
+int sred_optimise_relevance_mask_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float step,
+                                    int64_t max_steps, float *all_ones, float *all_zeros);
+int sred_optimise_relevance_mask_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float step,
+                                        int64_t max_steps, float *all_ones, float *all_zeros);
 int sred_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept);
 
 /* ---- all-pairs vector maps (OpenCL kernels without a Java caller) ---- */

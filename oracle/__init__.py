@@ -116,6 +116,18 @@ def lib():
         L.or_quadtree_gat.restype = c_int
         L.or_gat_tree.argtypes = [_F, c_i64, c_double, c_double]
         L.or_gat_tree.restype = None
+        L.or_tool_prep.argtypes = [_F, c_int, c_int, c_int, _D]
+        L.or_tool_prep.restype = None
+        L.or_noise_var_double.argtypes = [_F, c_int, c_int]
+        L.or_noise_var_double.restype = c_float
+        L.or_relevance_mask_tool.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, _F, ctypes.POINTER(c_float),
+                                             ctypes.POINTER(c_float)]
+        L.or_relevance_mask_tool.restype = c_int
+        L.or_local_var_f32.argtypes = [_F, c_int, c_int, c_int, c_int, _F]
+        L.or_local_var_f32.restype = None
+        L.or_optimise_relevance_mask.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, c_int,
+                                                 ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_optimise_relevance_mask.restype = c_int
         _lib = L
     return _lib
 
@@ -518,3 +530,50 @@ def block_timelapse(metric, stack, tmpl_px, bw, bh, tmpl_mean, tmpl_std, c, norm
     if normalize:
         out = normalize_input(out.reshape(-1)).reshape(d, h, w)
     return out
+
+
+def tool_prep(img, max_iter=5000):
+    """The relevance-mask tools' input step: (prepared image, GAT fit)."""
+    out = _f32(img).copy()
+    h, w = out.shape
+    p = np.zeros(3)
+    lib().or_tool_prep(out.reshape(-1), w, h, int(max_iter), p)
+    return out, p
+
+
+def noise_var_double(img):
+    img = _f32(img)
+    h, w = img.shape
+    return lib().or_noise_var_double(img.reshape(-1), w, h)
+
+
+def relevance_mask_tool(img, bw, bh, c):
+    """RelevanceMask2D_ after its input step: (mask, noise variance, threshold)."""
+    img = _f32(img)
+    h, w = img.shape
+    mask = np.zeros_like(img)
+    nv, thr = ctypes.c_float(), ctypes.c_float()
+    if lib().or_relevance_mask_tool(img.reshape(-1), w, h, bw, bh, float(c), mask.reshape(-1), ctypes.byref(nv),
+                                    ctypes.byref(thr)) != 0:
+        raise ValueError("blocks must be odd and square")
+    return mask, nv.value, thr.value
+
+
+def local_var_f32(img, bw, bh):
+    img = _f32(img)
+    h, w = img.shape
+    out = np.zeros_like(img)
+    lib().or_local_var_f32(img.reshape(-1), w, h, bw, bh, out.reshape(-1))
+    return out
+
+
+def optimise_relevance_mask(img, bw, bh, step, max_steps=100000):
+    """OptimiseRelevanceMask_ after its input step: (all-ones constant, all-zeros constant)."""
+    img = _f32(img)
+    h, w = img.shape
+    a, b = ctypes.c_float(), ctypes.c_float()
+    rc = lib().or_optimise_relevance_mask(img.reshape(-1), w, h, bw, bh, float(step), int(max_steps), ctypes.byref(a),
+                                          ctypes.byref(b))
+    if rc != 0:
+        raise ValueError(rc)
+    return a.value, b.value

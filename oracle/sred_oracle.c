@@ -1144,13 +1144,22 @@ double or_gat_objective(const float *px, int w, int h, double gain, double sigma
  * returns its f.  f(params, ctx) is the objective. */
 typedef double (*or_fn3)(const double *x, void *ctx);
 
+/* A NaN objective marks invalid parameters: it ranks worst (+infinity), as
+ * ImageJ's Minimizer treats NaN results (GATMinimizer3D returns NaN for
+ * gain <= 0 on purpose). */
+static double nm_eval(or_fn3 f, const double *x, void *ctx)
+{
+    double r = f(x, ctx);
+    return r != r ? INFINITY : r;
+}
+
 double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, int max_iter, double *x)
 {
     double v[4][3], fv[4];
     for (int i = 0; i < 4; i++) {
         for (int k = 0; k < 3; k++) v[i][k] = x0[k];
         if (i > 0) v[i][i - 1] += d[i - 1];
-        fv[i] = f(v[i], ctx);
+        fv[i] = nm_eval(f, v[i], ctx);
     }
     for (int it = 0; it < max_iter; it++) {
         /* stable insertion sort by f */
@@ -1163,10 +1172,10 @@ double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, i
         double c[3], xr[3], xe[3], xc[3];
         for (int k = 0; k < 3; k++) c[k] = (v[0][k] + v[1][k] + v[2][k]) / 3.0;
         for (int k = 0; k < 3; k++) xr[k] = c[k] + (c[k] - v[3][k]);
-        double fr = f(xr, ctx);
+        double fr = nm_eval(f, xr, ctx);
         if (fr < fv[0]) {
             for (int k = 0; k < 3; k++) xe[k] = c[k] + 2.0 * (c[k] - v[3][k]);
-            double fe = f(xe, ctx);
+            double fe = nm_eval(f, xe, ctx);
             if (fe < fr) { for (int k = 0; k < 3; k++) v[3][k] = xe[k]; fv[3] = fe; }
             else { for (int k = 0; k < 3; k++) v[3][k] = xr[k]; fv[3] = fr; }
             continue;
@@ -1174,11 +1183,11 @@ double or_nelder_mead3(or_fn3 f, void *ctx, const double *x0, const double *d, i
         if (fr < fv[2]) { for (int k = 0; k < 3; k++) v[3][k] = xr[k]; fv[3] = fr; continue; }
         int outside = fr < fv[3];
         for (int k = 0; k < 3; k++) xc[k] = outside ? c[k] + 0.5 * (xr[k] - c[k]) : c[k] + 0.5 * (v[3][k] - c[k]);
-        double fc = f(xc, ctx);
+        double fc = nm_eval(f, xc, ctx);
         if (outside ? fc <= fr : fc < fv[3]) { for (int k = 0; k < 3; k++) v[3][k] = xc[k]; fv[3] = fc; continue; }
         for (int i = 1; i < 4; i++) {  /* shrink towards the best vertex */
             for (int k = 0; k < 3; k++) v[i][k] = v[0][k] + 0.5 * (v[i][k] - v[0][k]);
-            fv[i] = f(v[i], ctx);
+            fv[i] = nm_eval(f, v[i], ctx);
         }
     }
     int b = 0;
@@ -1528,4 +1537,163 @@ void or_gat_tree(float *px, int64_t n, double g0, double edc)
             v = (2.0 / g0) * sqrt(g0 * v + ref);
         px[i] = (float)v;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,           */
+/* CalculateLocalMeans_)                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* The tools' shared input step (RelevanceMask2D_.java:133-164,
+ * OptimiseRelevanceMask_.java:124-154, CalculateLocalMeans_.java:163-190):
+ * GATMinimizer2D from (0, 100, 0), getGAT with the fitted doubles, min-max
+ * normalisation in double, rounded to float.  In place; params receives the
+ * fit. */
+void or_tool_prep(float *img, int w, int h, int max_iter, double *params)
+{
+    double p[3];
+    or_gat_minimize(img, w, h, 0.0, 100.0, 0.0, max_iter, p);
+    int64_t n = (int64_t)w * h;
+    or_gat(img, n, p[0], p[1], p[2]);
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    for (int64_t i = 0; i < n; i++) {
+        double v = (double)img[i];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    for (int64_t i = 0; i < n; i++) img[i] = (float)(((double)img[i] - mn) / (mx - mn + (double)1e-7f));
+    if (params) memcpy(params, p, sizeof p);
+}
+
+/* RelevanceMask2D_'s noise variance (:203-233): 8x8 tiles up to CIF size,
+ * 16x16 above; per tile the double sums of v and of the float squares v*v
+ * (getMeanAndVarBlock, :401-419), the variance rounded to float; the lowest
+ * 3% averaged in float and scaled as Utils.getMeanNoiseVar2D does. */
+float or_noise_var_double(const float *img, int w, int h)
+{
+    int tb = ((int64_t)w * h <= 352 * 288) ? 8 : 16;
+    int nbx = w / tb, nby = h / tb, nb = nbx * nby;
+    if (nb <= 0) return NAN;
+    float *vars = (float *)malloc(sizeof(float) * (size_t)nb);
+    for (int by = 0; by < nby; by++)
+        for (int bx = 0; bx < nbx; bx++) {
+            double mean = 0.0, sq = 0.0;
+            for (int y = by * tb; y < (by + 1) * tb; y++)
+                for (int x = bx * tb; x < (bx + 1) * tb; x++) {
+                    float v = img[(int64_t)y * w + x];
+                    mean += v;
+                    sq += v * v;
+                }
+            mean = mean / (tb * tb);
+            double var = sq / (tb * tb) - mean * mean;
+            vars[by * nbx + bx] = (float)var;
+        }
+    qsort(vars, (size_t)nb, sizeof(float), cmp_float_asc);
+    int nv_n = (int)(0.03f * (float)nb + 1.0f);
+    float nv = 0.0f;
+    for (int i = 0; i < nv_n; i++) nv += vars[i];
+    nv = fabsf(nv / (float)nv_n);
+    nv = (1.0f + 0.001f * (nv - 40.0f)) * nv;
+    free(vars);
+    return nv;
+}
+
+/* RelevanceMask2D_.run after the input step (:165-250), for odd square
+ * blocks: local standard deviations as kernelGetLocalMeans.cl (the float
+ * arithmetic of kernelGetLocalStatistics2D.cl), the noise variance above,
+ * threshold = noiseVar * c (c itself when noiseVar is 0), and mask 0 where
+ * std*std < threshold or std*std == 0, 1 elsewhere inside the borders.
+ * Returns -1 for blocks that are not odd and square (the reference swaps the
+ * radii and overruns its patch array otherwise). */
+int or_relevance_mask_tool(const float *img, int w, int h, int bw, int bh, float c, float *mask, float *noise_var,
+                           float *threshold)
+{
+    if (bw != bh || bw % 2 == 0 || bw < 3) return -1;
+    int64_t n = (int64_t)w * h;
+    float *means = (float *)malloc(sizeof(float) * (size_t)n), *stds = (float *)malloc(sizeof(float) * (size_t)n);
+    or_local_stats_f32(img, w, h, bw, bh, means, stds);
+    float nv = or_noise_var_double(img, w, h);
+    float thr = (nv == 0.0f) ? c : nv * c;
+    int br = bw / 2;
+    for (int64_t i = 0; i < n; i++) mask[i] = 0.0f;
+    for (int y = br; y < h - br; y++)
+        for (int x = br; x < w - br; x++) {
+            float s = stds[(int64_t)y * w + x];
+            float v = s * s;
+            mask[(int64_t)y * w + x] = (v < thr || v == 0.0f) ? 0.0f : 1.0f;
+        }
+    *noise_var = nv;
+    *threshold = thr;
+    free(means);
+    free(stds);
+    return 0;
+}
+
+/* kernelGetRelevanceMask2D.cl: the local variance (no square root) of each
+ * inner pixel, the mean accumulated while gathering the ellipse. */
+void or_local_var_f32(const float *img, int w, int h, int bw, int bh, float *var)
+{
+    int brw = bw / 2, brh = bh / 2;
+    int K = or_footprint(bw, bh, NULL, NULL);
+    int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
+    or_footprint(bw, bh, dx, dy);
+    for (int gy = 0; gy < h; gy++)
+        for (int gx = 0; gx < w; gx++) {
+            int64_t o = (int64_t)gy * w + gx;
+            if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { var[o] = 0.0f; continue; }
+            float m = 0.0f;
+            for (int i = 0; i < K; i++) m += img[(int64_t)(gy + dy[i]) * w + gx + dx[i]];
+            m /= (float)K;
+            float v = 0.0f;
+            for (int i = 0; i < K; i++) {
+                float p = img[(int64_t)(gy + dy[i]) * w + gx + dx[i]];
+                v += (p - m) * (p - m);
+            }
+            var[o] = v / (float)(K - 1);
+        }
+    free(dx); free(dy);
+}
+
+/* OptimiseRelevanceMask_.run after the input step (:155-287), literally:
+ * the "standard deviations" are kernelGetRelevanceMask2D's variances, so the
+ * mask compares their squares; the filter constant grows from 0 by step (a
+ * float sum) and each value's mask is built (noise variance as
+ * Utils.getMeanNoiseVar2D, threshold noiseVar * c, or c when noiseVar is 0);
+ * *all_ones = the last constant whose mask is all ones inside the borders,
+ * *all_zeros = the first whose mask is all zeros.  Returns -1 for blocks that
+ * are not odd and square or step <= 0, -2 if no mask is all zeros within
+ * max_steps constants (the reference loops for ever). */
+int or_optimise_relevance_mask(const float *img, int w, int h, int bw, int bh, float step, int max_steps,
+                               float *all_ones, float *all_zeros)
+{
+    if (bw != bh || bw % 2 == 0 || bw < 3 || !(step > 0.0f)) return -1;
+    int64_t n = (int64_t)w * h;
+    float *var = (float *)malloc(sizeof(float) * (size_t)n);
+    or_local_var_f32(img, w, h, bw, bh, var);
+    float nv = or_noise_var(img, w, h);
+    int br = bw / 2;
+    float c = 0.0f, ones = 0.0f;
+    int rc = -2;
+    for (int it = 0; it < max_steps; it++) {
+        float thr = (nv == 0.0f) ? c : nv * c;
+        int every_one = 1, every_zero = 1;
+        for (int y = br; y < h - br; y++)
+            for (int x = br; x < w - br; x++) {
+                float s = var[(int64_t)y * w + x];
+                float v = s * s;
+                float m = (v < thr || v == 0.0f) ? 0.0f : 1.0f;
+                if (m != 1.0f) every_one = 0;
+                if (m != 0.0f) every_zero = 0;
+            }
+        if (every_one) ones = c;
+        if (every_zero) {
+            *all_zeros = c;
+            rc = 0;
+            break;
+        }
+        c += step;
+    }
+    *all_ones = ones;
+    free(var);
+    return rc;
 }

@@ -193,6 +193,16 @@ int or_quadtree_gat(const float *img, int w, int h, int min_size, float alpha, i
                     int32_t *rects, float *means, float *vars, int *nleaves, double *g0, double *edc);
 void or_gat_tree(float *px, int64_t n, double g0, double edc);
 
+/* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
+ * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */
+void or_tool_prep(float *img, int w, int h, int max_iter, double *params);
+float or_noise_var_double(const float *img, int w, int h);
+int or_relevance_mask_tool(const float *img, int w, int h, int bw, int bh, float c, float *mask, float *noise_var,
+                           float *threshold);
+void or_local_var_f32(const float *img, int w, int h, int bw, int bh, float *var);
+int or_optimise_relevance_mask(const float *img, int w, int h, int bw, int bh, float step, int max_steps,
+                               float *all_ones, float *all_zeros);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,7 +38,9 @@ EXPORTS = [
     "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev", "sred_ctx_last_skipped",
     "sred_gat_2d", "sred_gat_2d_dev", "sred_gat_objective_2d", "sred_gat_minimize_2d", "sred_gat_minimize_2d_dev",
     "sred_quadtree_gat_2d", "sred_quadtree_gat_2d_dev", "sred_gat_tree_2d", "sred_gat_tree_2d_dev", "sred_f_quantile",
-    "sred_simple_regression", "sred_block_timelapse_2d", "sred_block_timelapse_2d_dev",
+    "sred_simple_regression", "sred_block_timelapse_2d", "sred_block_timelapse_2d_dev", "sred_tool_prep_2d",
+    "sred_tool_prep_2d_dev", "sred_relevance_mask_tool_2d", "sred_relevance_mask_tool_2d_dev",
+    "sred_optimise_relevance_mask_2d", "sred_optimise_relevance_mask_2d_dev",
 ]
 
 
@@ -125,6 +127,12 @@ def load():
                                       _I, _VP, _F, _F], _I),
         "sred_gat_tree_2d": ([_VP, _F, _I64, _D, _D], _I),
         "sred_block_timelapse_2d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _F], _I),
+        "sred_tool_prep_2d": ([_VP, _F, _I, _I, _I, ctypes.POINTER(_D)], _I),
+        "sred_tool_prep_2d_dev": ([_VP, _VP, _I, _I, _I, ctypes.POINTER(_D)], _I),
+        "sred_relevance_mask_tool_2d": ([_VP, _F, _I, _I, _I, _I, _FL, _F, _F, _F], _I),
+        "sred_relevance_mask_tool_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _VP, _F, _F], _I),
+        "sred_optimise_relevance_mask_2d": ([_VP, _F, _I, _I, _I, _I, _FL, _I64, _F, _F], _I),
+        "sred_optimise_relevance_mask_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _I64, _F, _F], _I),
         "sred_block_timelapse_2d_dev": ([_VP, _I, _VP, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_gat_tree_2d_dev": ([_VP, _VP, _I64, _D, _D], _I),
         "sred_f_quantile": ([_D, _D, _D, ctypes.POINTER(_D)], _I),

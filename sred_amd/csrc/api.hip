@@ -937,6 +937,79 @@ int sred_block_timelapse_2d(sred_ctx *ctx, int metric, const float *stack, int w
     return download(ctx, out, ctx->out, total);
 }
 
+int sred_tool_prep_2d_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || w < 6 || h < 6 || max_iter < 0) {
+        set_error("null buffer, an image below 6 x 6 pixels or a negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    return tool_prep_dev(ctx, d_img, w, h, max_iter, params);
+}
+
+int sred_tool_prep_2d(sred_ctx *ctx, float *img, int w, int h, int max_iter, double *params)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || w < 6 || h < 6 || max_iter < 0) {
+        set_error("null buffer, an image below 6 x 6 pixels or a negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = (int64_t)w * h;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(tool_prep_dev(ctx, ctx->img.as<float>(), w, h, max_iter, params));
+    return download(ctx, img, ctx->img, n);
+}
+
+int sred_relevance_mask_tool_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh,
+                                    float filter_constant, float *d_mask, float *noise_var, float *threshold)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !d_mask) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return relevance_mask_tool_dev(ctx, d_img, w, h, bw, bh, filter_constant, d_mask, noise_var, threshold);
+}
+
+int sred_relevance_mask_tool_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float filter_constant,
+                                float *mask, float *noise_var, float *threshold)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !mask || w <= 0 || h <= 0) {
+        set_error("null buffer or empty image");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = (int64_t)w * h;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
+    SRED_TRY(relevance_mask_tool_dev(ctx, ctx->img.as<float>(), w, h, bw, bh, filter_constant, ctx->mask.as<float>(),
+                                     noise_var, threshold));
+    return download(ctx, mask, ctx->mask, n);
+}
+
+int sred_optimise_relevance_mask_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float step,
+                                        int64_t max_steps, float *all_ones, float *all_zeros)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !all_ones || !all_zeros) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return optimise_relevance_mask_dev(ctx, d_img, w, h, bw, bh, step, max_steps, all_ones, all_zeros);
+}
+
+int sred_optimise_relevance_mask_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float step,
+                                    int64_t max_steps, float *all_ones, float *all_zeros)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !all_ones || !all_zeros || w <= 0 || h <= 0) {
+        set_error("null buffer or empty image");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, img, (int64_t)w * h));
+    return optimise_relevance_mask_dev(ctx, ctx->img.as<float>(), w, h, bw, bh, step, max_steps, all_ones, all_zeros);
+}
+
 int sred_f_quantile(double d1, double d2, double p, double *out)
 {
     if (!out || !(d1 > 0.0) || !(d2 > 0.0) || !(p >= 0.0 && p <= 1.0)) {

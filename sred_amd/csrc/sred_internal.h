@@ -166,10 +166,11 @@ int zero_dev(sred_ctx *ctx, void *ptr, size_t bytes);
 
 // ---- launchers (stream = ctx->stream) ----
 // Local statistics, reference float order (kernelGetLocalStatistics2D.cl).
-int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds);
+int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds,
+                       bool var_out = false);
 // Java float noise variance (Utils.getMeanNoiseVar2D): tile variances on the
 // GPU, sort and mean of the lowest 3% on the host (synchronises).
-int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var);
+int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var, bool double_sums = false);
 // Relevance mask + Java float nPixels (device); n_pixels returned on host
 // (synchronises).
 int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
@@ -300,6 +301,14 @@ double f_quantile(double d1, double d2, double p);
 // nseg min-max normalisations (v - min) / (max - min + EPSILON) of
 // consecutive segments of seg floats (stack.hip); d_in may equal d_out.
 int seg_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
+// Exact (min, max) of nseg segments as order-preserving keys, mm[2*s], mm[2*s+1].
+int seg_minmax_dev(sred_ctx *ctx, const float *d_in, int64_t seg, int nseg, unsigned int *mm);
+// Relevance-mask tools (tools.hip).
+int tool_prep_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params);
+int relevance_mask_tool_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float c, float *d_mask,
+                            float *noise_var, float *threshold);
+int optimise_relevance_mask_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float step,
+                                int64_t max_steps, float *all_ones, float *all_zeros);
 // The stack normalised as a whole, then each of its nseg frames of seg floats.
 int stack_frames_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
 void simple_regression_host(const double *x, const double *y, int64_t n, double *slope, double *intercept);

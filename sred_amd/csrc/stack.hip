@@ -118,6 +118,15 @@ __global__ __launch_bounds__(256) void k_stack_frames_remap(const float *__restr
 
 }  // namespace
 
+int seg_minmax_dev(sred_ctx *ctx, const float *d_in, int64_t seg, int nseg, unsigned int *mm)
+{
+    hipLaunchKernelGGL(k_seg_mm_init, dim3((nseg + 255) / 256), dim3(256), 0, ctx->stream, mm, nseg);
+    const int per = (int)std::max<int64_t>(1, std::min<int64_t>((seg + 1023) / 1024, std::max(1, 2048 / nseg)));
+    hipLaunchKernelGGL(k_seg_minmax, dim3(per, nseg), dim3(256), 0, ctx->stream, d_in, seg, mm);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
 int stack_frames_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg)
 {
     if (seg <= 0 || nseg <= 0) return SRED_OK;

@@ -24,6 +24,8 @@ namespace sred {
 // ---------------------------------------------------------------------------
 constexpr int kStatTile = 16;
 
+// kVar: stds receives the variance itself (kernelGetRelevanceMask2D.cl), not its root.
+template <bool kVar>
 __global__ __launch_bounds__(256) void k_local_stats(const float *__restrict__ img, int w, int h, int brw, int brh,
                                                      const int *__restrict__ offs, int K, float *__restrict__ means,
                                                      float *__restrict__ stds)
@@ -59,17 +61,22 @@ __global__ __launch_bounds__(256) void k_local_stats(const float *__restrict__ i
     }
     v /= (float)(K - 1);
     means[o] = m;
-    stds[o] = sqrtf(v);
+    stds[o] = kVar ? v : sqrtf(v);
 }
 
-int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds)
+int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds,
+                       bool var_out)
 {
     const int tw = kStatTile + 2 * g.brw;
     SRED_TRY(upload_offsets(ctx, g, tw, &ctx->offs, ctx->offs_key));
     dim3 grid((g.w + kStatTile - 1) / kStatTile, (g.h + kStatTile - 1) / kStatTile);
     size_t lds = sizeof(float) * (size_t)(kStatTile + 2 * g.brw) * (kStatTile + 2 * g.brh);
-    hipLaunchKernelGGL(k_local_stats, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
-                       ctx->offs.as<int>(), g.K, d_means, d_stds);
+    if (var_out)
+        hipLaunchKernelGGL(k_local_stats<true>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
+                           ctx->offs.as<int>(), g.K, d_means, d_stds);
+    else
+        hipLaunchKernelGGL(k_local_stats<false>, grid, dim3(256), lds, ctx->stream, d_img, g.w, g.h, g.brw, g.brh,
+                           ctx->offs.as<int>(), g.K, d_means, d_stds);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
@@ -78,8 +85,12 @@ int launch_local_stats(sred_ctx *ctx, const Geometry &g, const float *d_img, flo
 // Noise variance: Utils.getMeanNoiseVar2D (Utils.java:790-837) with
 // getMeanAndVarBlock2D (Utils.java:854-900).  One thread per tile runs the
 // reference's one-pass float loop; the lowest-3% mean is taken on the host
-// exactly as the Java code does (sort, sequential float sum).
+// exactly as the Java code does (sort, sequential float sum).  With T =
+// double, the tile sums are the double ones of RelevanceMask2D_'s
+// getMeanAndVarBlock (RelevanceMask2D_.java:401-419), the variance rounded
+// to float.
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ void k_tile_var(const float *__restrict__ img, int w, int tbw, int tbh, int nbx, int nb,
                            float *__restrict__ out, int *__restrict__ bad)
 {
@@ -87,23 +98,23 @@ __global__ void k_tile_var(const float *__restrict__ img, int w, int tbw, int tb
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb) return;
     int bx = t % nbx, by = t / nbx;
-    float m = 0.0f, sq = 0.0f;
+    T m = 0, sq = 0;
     int nonfinite = 0;
     for (int y = by * tbh; y < (by + 1) * tbh; y++)
         for (int x = bx * tbw; x < (bx + 1) * tbw; x++) {
             float v = img[(int64_t)y * w + x];
             nonfinite |= !isfinite(v);
             m += v;
-            sq += v * v;
+            sq += v * v;  // the float square in both forms
         }
     int n = tbw * tbh;
-    m /= (float)n;
-    float var = sq / (float)n - m * m;
+    m = m / (T)n;
+    const float var = (float)(sq / (T)n - m * m);
     if (nonfinite || isnan(var)) atomicOr(bad, 1);
     out[t] = var;
 }
 
-int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var)
+int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var, bool double_sums)
 {
 #pragma clang fp contract(off)
     int tbw = 8, tbh = 8;
@@ -117,8 +128,12 @@ int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_
     float *d_var = ctx->tile_var.as<float>();
     int *d_bad = reinterpret_cast<int *>(d_var + nb);
     SRED_TRY(zero_dev(ctx, d_bad, sizeof(int)));
-    hipLaunchKernelGGL(k_tile_var, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, d_img, w, tbw, tbh, nbx, nb,
-                       d_var, d_bad);
+    if (double_sums)
+        hipLaunchKernelGGL(k_tile_var<double>, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, d_img, w, tbw, tbh,
+                           nbx, nb, d_var, d_bad);
+    else
+        hipLaunchKernelGGL(k_tile_var<float>, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, d_img, w, tbw, tbh,
+                           nbx, nb, d_var, d_bad);
     SRED_HIP_CHECK(hipGetLastError());
     ctx->host_tile_var.resize(nb + 1);
     SRED_HIP_CHECK(hipMemcpyAsync(ctx->host_tile_var.data(), d_var, sizeof(float) * (nb + 1), hipMemcpyDeviceToHost,

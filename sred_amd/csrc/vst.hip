@@ -27,7 +27,9 @@
 //     is third-party and not restated; this is a Nelder-Mead simplex with
 //     the Lagarias et al. (1998) coefficients, initial simplex x0 + d_i e_i
 //     with d = (1, 10, 100), stopping after max_iter iterations or when all
-//     vertices have the same objective (the reference's setMaxError(0)).
+//     vertices have the same objective (the reference's setMaxError(0)); a
+//     NaN objective (invalid parameters, e.g. gain 0) ranks worst, as
+//     ImageJ's Minimizer treats NaN.
 //     Each iteration evaluates the reflection, expansion and both
 //     contractions in one launch (four parameter sets) and then takes the
 //     branch the values select, so the path is the sequential algorithm's.
@@ -173,6 +175,7 @@ struct GatEval {
     double *h_err;  // pinned host memory written by k_gat_error
     int64_t launches = 0, pixel_evals = 0;
     double rows_ms = 0.0;  // k_gat_rows time summed over the launches (HIP events, when fit_timing is on)
+    bool rank_nan_worst = false;  // the simplex's evaluations map NaN to +infinity
 
     int run(const double (*x)[3], int nsets, double *f)
     {
@@ -191,6 +194,9 @@ struct GatEval {
         SRED_HIP_CHECK(hipGetLastError());
         SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (int s = 0; s < nsets; s++) f[s] = h_err[s];
+        if (rank_nan_worst)  // invalid parameters (NaN objective) rank worst, as ImageJ's Minimizer treats NaN
+            for (int s = 0; s < nsets; s++)
+                if (f[s] != f[s]) f[s] = INFINITY;
         if (ctx->fit_timing) {
             float ms = 0.0f;
             SRED_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -253,6 +259,7 @@ int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gai
 #pragma clang fp contract(off)
     GatEval e;
     SRED_TRY(gat_setup(ctx, d_img, w, h, &e));
+    e.rank_nan_worst = true;
     const double d[3] = {1.0, 10.0, 100.0};  // GATMinimizer2D.java:43-45
     double v[4][3], fv[4];
     for (int i = 0; i < 4; i++) {
