@@ -261,6 +261,33 @@ int sred_gat_tree_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, doub
  * least-squares line of commons-math3 SimpleRegression (QuadTree_.java:395-407). */
 int sred_f_quantile(double d1, double d2, double p, double *out);
 
+/* ---- volumes (GlobalRepetition3D_) ---- */
+/* Ellipsoid block size K of GlobalRepetition3D_.java:101-112 (odd sides,
+ * depth >= 3), host only. */
+int sred_block_size_3d(int bw, int bh, int bd);
+/* CLUtils.getLocalStatistics3D with kernelGetLocalStatistics3D.cl (float
+ * mean and unbiased std over the ellipsoid, borders 0), with the block size
+ * and radii as the function declares them (GlobalRepetition3D_ passes them
+ * in another order). */
+int sred_local_stats_3d(sred_ctx *ctx, const float *img, int w, int h, int d, int bw, int bh, int bd, float *means,
+                        float *stds);
+/* GlobalRepetition3D_.run with the Pearson metric (GlobalRepetition3D_.java:
+ * 113-160 -> CLUtils.getGlobalPearson3D :2225-2335) on a w x h x d volume
+ * (slice z at z*w*h), built as intended where the reference does not run:
+ * the statistics above, the 3D relevance mask of Utils.getRelevanceMask3D
+ * (noise variance = float mean of std^2 over the inner voxels; 0 where
+ * std^2 <= noiseVar * c), nPixels over it, the reference block gathered
+ * over the full ellipsoid (kernelGetGlobalPearson3D.cl stops one slice
+ * short), flat reference blocks scoring 0, then S/(Wsum*nPixels + eps),
+ * the mask when c > 0 and Utils.normalizeImage3D.  The contraction is the
+ * 2D one (precision as sred_global_repetition_2d).  Other metrics return
+ * SRED_ERR_ARG. */
+int sred_global_repetition_3d(sred_ctx *ctx, int metric, int precision, const float *img, int w, int h, int d, int bw,
+                              int bh, int bd, float relevance_constant, int normalize, float *out);
+int sred_global_repetition_3d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int d,
+                                  int bw, int bh, int bd, float relevance_constant, int normalize, float *d_out,
+                                  float *d_means, float *d_stds, float *d_mask, float *d_S, float *d_Wsum);
+
 /* ---- relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,
  *      CalculateLocalMeans_) ---- */
 /* The tools' shared input step, in place (RelevanceMask2D_.java:133-164,

@@ -128,6 +128,15 @@ def lib():
         L.or_optimise_relevance_mask.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, c_int,
                                                  ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
         L.or_optimise_relevance_mask.restype = c_int
+        L.or_footprint3d.argtypes = [c_int, c_int, c_int, vp, vp, vp]
+        L.or_footprint3d.restype = c_int
+        L.or_local_stats3d_f32.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, _F, _F]
+        L.or_local_stats3d_f32.restype = None
+        L.or_vol_noise_var.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int]
+        L.or_vol_noise_var.restype = c_float
+        L.or_global_pearson3d.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_int, c_int, _D,
+                                          vp, vp, vp, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_global_pearson3d.restype = c_int
         _lib = L
     return _lib
 
@@ -577,3 +586,48 @@ def optimise_relevance_mask(img, bw, bh, step, max_steps=100000):
     if rc != 0:
         raise ValueError(rc)
     return a.value, b.value
+
+
+def footprint3d(bw, bh, bd):
+    """(K, dx, dy, dz) of the ellipsoid block, z-y-x order."""
+    k = lib().or_footprint3d(bw, bh, bd, None, None, None)
+    d = np.zeros((3, k), np.int32)
+    lib().or_footprint3d(bw, bh, bd, d[0].ctypes.data, d[1].ctypes.data, d[2].ctypes.data)
+    return k, d[0], d[1], d[2]
+
+
+def local_stats3d_f32(vol, bw, bh, bd):
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    m = np.zeros_like(vol)
+    s = np.zeros_like(vol)
+    lib().or_local_stats3d_f32(vol.reshape(-1), w, h, d, bw, bh, bd, m.reshape(-1), s.reshape(-1))
+    return m, s
+
+
+def vol_noise_var(stds, bw, bh, bd):
+    stds = _f32(stds)
+    d, h, w = stds.shape
+    return lib().or_vol_noise_var(stds.reshape(-1), w, h, d, bw // 2, bh // 2, bd // 2)
+
+
+class Pearson3D:
+    """The 3D Pearson map and its intermediates (float form)."""
+
+    def __init__(self, out, S, W, mask, noise_var, n_pixels):
+        self.out, self.S, self.W, self.mask, self.noise_var, self.n_pixels = out, S, W, mask, noise_var, n_pixels
+
+
+def global_pearson3d(vol, bw, bh, bd, c=0.0, normalize=True, fp64=False):
+    """GlobalRepetition3D_ with Pearson's R as intended (refsim float, or FP64 with fp64=True)."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    out = np.zeros(vol.shape, np.float64)
+    S = np.zeros_like(vol)
+    W = np.zeros_like(vol)
+    mask = np.zeros_like(vol)
+    nv, npx = ctypes.c_float(), ctypes.c_float()
+    lib().or_global_pearson3d(vol.reshape(-1), w, h, d, bw, bh, bd, float(c), int(normalize), int(fp64),
+                              out.reshape(-1), S.ctypes.data, W.ctypes.data, mask.ctypes.data, ctypes.byref(nv),
+                              ctypes.byref(npx))
+    return Pearson3D(out, S, W, mask, nv.value, npx.value)

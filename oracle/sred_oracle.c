@@ -1697,3 +1697,202 @@ int or_optimise_relevance_mask(const float *img, int w, int h, int bw, int bh, f
     free(var);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Volumes: GlobalRepetition3D_ with the Pearson metric, as intended         */
+/* ------------------------------------------------------------------------ */
+
+/* Ellipsoid footprint (GlobalRepetition3D_.java:101-112, the kernels' z, y,
+ * x loops and float predicate). */
+int or_footprint3d(int bw, int bh, int bd, int *dx, int *dy, int *dz)
+{
+    int brw = bw / 2, brh = bh / 2, brd = bd / 2, k = 0;
+    for (int z = 0; z < bd; z++)
+        for (int y = 0; y < bh; y++)
+            for (int x = 0; x < bw; x++) {
+                float fx = (float)(x - brw), fy = (float)(y - brh), fz = (float)(z - brd);
+                if (((fx * fx) / (float)(brw * brw)) + ((fy * fy) / (float)(brh * brh)) +
+                        ((fz * fz) / (float)(brd * brd)) <= 1.0f) {
+                    if (dx) { dx[k] = x - brw; dy[k] = y - brh; dz[k] = z - brd; }
+                    k++;
+                }
+            }
+    return k;
+}
+
+static int64_t *vol_offsets(int w, int h, int bw, int bh, int bd, int *K)
+{
+    int k = or_footprint3d(bw, bh, bd, NULL, NULL, NULL);
+    int *dx = malloc(sizeof(int) * 3 * k), *dy = dx + k, *dz = dy + k;
+    or_footprint3d(bw, bh, bd, dx, dy, dz);
+    int64_t *off = malloc(sizeof(int64_t) * k);
+    for (int i = 0; i < k; i++) off[i] = ((int64_t)dz[i] * h + dy[i]) * w + dx[i];
+    free(dx);
+    *K = k;
+    return off;
+}
+
+static int vol_inner(int64_t o, int w, int h, int d, int brw, int brh, int brd)
+{
+    int x = (int)(o % w), y = (int)((o / w) % h), z = (int)(o / ((int64_t)w * h));
+    return x >= brw && x < w - brw && y >= brh && y < h - brh && z >= brd && z < d - brd;
+}
+
+/* kernelGetLocalStatistics3D.cl: float mean and unbiased std over the
+ * ellipsoid in footprint order; borders 0. */
+void or_local_stats3d_f32(const float *img, int w, int h, int d, int bw, int bh, int bd, float *means, float *stds)
+{
+    int K;
+    int64_t *off = vol_offsets(w, h, bw, bh, bd, &K);
+    int64_t n = (int64_t)w * h * d;
+#pragma omp parallel for schedule(static)
+    for (int64_t o = 0; o < n; o++) {
+        if (!vol_inner(o, w, h, d, bw / 2, bh / 2, bd / 2)) { means[o] = 0.0f; stds[o] = 0.0f; continue; }
+        float m = 0.0f;
+        for (int i = 0; i < K; i++) m += img[o + off[i]];
+        m /= (float)K;
+        float v = 0.0f;
+        for (int i = 0; i < K; i++) v += (img[o + off[i]] - m) * (img[o + off[i]] - m);
+        means[o] = m;
+        stds[o] = sqrtf(v / (float)(K - 1));
+    }
+    free(off);
+}
+
+/* Utils.getMeanNoiseVar3D(..., localStds) (Utils.java:1463-1499): float sum
+ * of std^2 over the inner voxels in raster order over a float count. */
+float or_vol_noise_var(const float *stds, int w, int h, int d, int brw, int brh, int brd)
+{
+    float s = 0.0f, cnt = 0.0f;
+    for (int z = brd; z < d - brd; z++)
+        for (int y = brh; y < h - brh; y++)
+            for (int x = brw; x < w - brw; x++) {
+                float v = stds[((int64_t)z * h + y) * w + x];
+                s += v * v;
+                cnt += 1.0f;
+            }
+    return s / cnt;
+}
+
+/* GlobalRepetition3D_ with Pearson's R as intended (the statistics with
+ * their declared argument order, Utils.getRelevanceMask3D, the reference
+ * block over the full ellipsoid): kernelGetGlobalPearson3D.cl per pair in
+ * float (fp64 = 0: its float operations and sequential sums) or in double
+ * (fp64 = 1: double block values, products, weights and sums; validity and
+ * flatness still from the float statistics), then CLUtils.java:2309-2330.
+ * out (double) receives the map; S, W, mask, *noise_var, *n_pixels (each
+ * may be NULL) the intermediates of the float form. */
+int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, int bd, float c, int normalize,
+                        int fp64, double *out, float *S_out, float *W_out, float *mask_out, float *noise_var,
+                        float *n_pixels)
+{
+    const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
+    int K;
+    int64_t *off = vol_offsets(w, h, bw, bh, bd, &K);
+    const int64_t n = (int64_t)w * h * d;
+    float *means = malloc(sizeof(float) * n), *stds = malloc(sizeof(float) * n), *mask = malloc(sizeof(float) * n);
+    double *S = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n);
+    or_local_stats3d_f32(img, w, h, d, bw, bh, bd, means, stds);
+    float nv = or_vol_noise_var(stds, w, h, d, brw, brh, brd);
+    float thr = nv * c;
+    float np = 0.0f;
+    for (int64_t o = 0; o < n; o++) {
+        mask[o] = 0.0f;
+        if (!vol_inner(o, w, h, d, brw, brh, brd)) continue;
+        mask[o] = (stds[o] * stds[o] <= thr) ? 0.0f : 1.0f;
+        if (mask[o] > 0.0f) np += 1.0f;
+    }
+    const double fp_d = or_java_float_literal(nv), thr_d = or_java_float_literal(thr), eps_d = or_java_float_literal(EPS_F);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t o = 0; o < n; o++) {
+        S[o] = 0.0; W[o] = 0.0;
+        if (!vol_inner(o, w, h, d, brw, brh, brd)) continue;
+        const float rs = stds[o], rv = rs * rs;
+        if ((double)rv < thr_d || rv == 0.0f) continue;  /* kernelGetGlobalPearson3D.cl:27-30 */
+        float rb[4096], tb[4096];
+        double rbd[4096], tbd[4096];
+        double rmd = 0.0;
+        if (fp64) {
+            for (int i = 0; i < K; i++) rmd += img[o + off[i]];
+            rmd /= K;
+            for (int i = 0; i < K; i++) rbd[i] = (double)img[o + off[i]] - rmd;
+        } else {
+            for (int i = 0; i < K; i++) rb[i] = img[o + off[i]] - means[o];
+        }
+        double rsd = 0.0;
+        if (fp64) {
+            for (int i = 0; i < K; i++) rsd += rbd[i] * rbd[i];
+            rsd = sqrt(rsd / (K - 1));
+        }
+        float acc = 0.0f, wacc = 0.0f;
+        double accd = 0.0, waccd = 0.0;
+        for (int64_t t = 0; t < n; t++) {
+            if (!vol_inner(t, w, h, d, brw, brh, brd)) continue;
+            const float ts = stds[t], tv = ts * ts;
+            if ((double)tv < thr_d) continue;
+            if (fp64) {
+                double tmd = 0.0, tsd = 0.0, cov = 0.0;
+                for (int i = 0; i < K; i++) tmd += img[t + off[i]];
+                tmd /= K;
+                for (int i = 0; i < K; i++) { tbd[i] = (double)img[t + off[i]] - tmd; tsd += tbd[i] * tbd[i]; }
+                tsd = sqrt(tsd / (K - 1));
+                for (int i = 0; i < K; i++) cov += rbd[i] * tbd[i];
+                cov /= (K - 1);
+                double wt = exp(-((rsd - tsd) * (rsd - tsd)) / (fp_d + eps_d));
+                waccd += wt;
+                if (ts == 0.0f) continue;  /* reference non-flat, test flat: 0 */
+                double r = cov / (rsd * tsd + eps_d);
+                accd += (r > 0.0 ? r : 0.0) * wt;
+            } else {
+                const float tm = means[t];
+                for (int i = 0; i < K; i++) tb[i] = img[t + off[i]] - tm;
+                float cov = 0.0f;
+                for (int i = 0; i < K; i++) cov += rb[i] * tb[i];
+                cov /= (float)(K - 1);
+                float dd = rs - ts;
+                float q = (float)((double)(dd * dd) / (fp_d + eps_d));
+                float wt = (float)expf((float)(-1.0f) * q);
+                wacc += wt;
+                if (ts == 0.0f) continue;
+                float r = (float)((double)cov / ((double)(rs * ts) + eps_d));
+                acc += fmaxf(0.0f, r) * wt;
+            }
+        }
+        S[o] = fp64 ? accd : (double)acc;
+        W[o] = fp64 ? waccd : (double)wacc;
+    }
+    /* CLUtils.java:2309-2330: finalize on the inner voxels, mask, normalise */
+    for (int64_t o = 0; o < n; o++) {
+        double r = 0.0;
+        if (vol_inner(o, w, h, d, brw, brh, brd)) {
+            if (fp64) r = S[o] / (W[o] * (double)np + eps_d);
+            else {
+                float den = (float)W[o] * np;
+                den = den + EPS_F;
+                r = (double)((float)S[o] / den);
+            }
+        }
+        if (c > 0.0f) r = fp64 ? r * mask[o] : (double)((float)r * mask[o]);
+        out[o] = r;
+    }
+    if (normalize) {
+        double mn = DBL_MAX, mx = -DBL_MAX;
+        for (int64_t o = 0; o < n; o++)
+            if (vol_inner(o, w, h, d, brw, brh, brd) && mask[o] > 0.0f) {
+                mn = out[o] < mn ? out[o] : mn;
+                mx = out[o] > mx ? out[o] : mx;
+            }
+        for (int64_t o = 0; o < n; o++)
+            if (vol_inner(o, w, h, d, brw, brh, brd) && mask[o] > 0.0f) {
+                if (fp64) out[o] = (out[o] - mn) / (mx - mn + eps_d);
+                else out[o] = (double)(((float)out[o] - (float)mn) / ((float)mx - (float)mn + EPS_F));
+            }
+    }
+    if (S_out) for (int64_t o = 0; o < n; o++) S_out[o] = (float)S[o];
+    if (W_out) for (int64_t o = 0; o < n; o++) W_out[o] = (float)W[o];
+    if (mask_out) memcpy(mask_out, mask, sizeof(float) * n);
+    if (noise_var) *noise_var = nv;
+    if (n_pixels) *n_pixels = np;
+    free(off); free(means); free(stds); free(mask); free(S); free(W);
+    return 0;
+}

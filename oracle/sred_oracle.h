@@ -193,6 +193,15 @@ int or_quadtree_gat(const float *img, int w, int h, int min_size, float alpha, i
                     int32_t *rects, float *means, float *vars, int *nleaves, double *g0, double *edc);
 void or_gat_tree(float *px, int64_t n, double g0, double edc);
 
+/* Volumes: GlobalRepetition3D_ with Pearson's R as intended (definitions
+ * and the decisions on the reference's own defects in sred_oracle.c). */
+int or_footprint3d(int bw, int bh, int bd, int *dx, int *dy, int *dz);
+void or_local_stats3d_f32(const float *img, int w, int h, int d, int bw, int bh, int bd, float *means, float *stds);
+float or_vol_noise_var(const float *stds, int w, int h, int d, int brw, int brh, int brd);
+int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, int bd, float c, int normalize,
+                        int fp64, double *out, float *S_out, float *W_out, float *mask_out, float *noise_var,
+                        float *n_pixels);
+
 /* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
  * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */
 void or_tool_prep(float *img, int w, int h, int max_iter, double *params);

@@ -40,7 +40,8 @@ EXPORTS = [
     "sred_quadtree_gat_2d", "sred_quadtree_gat_2d_dev", "sred_gat_tree_2d", "sred_gat_tree_2d_dev", "sred_f_quantile",
     "sred_simple_regression", "sred_block_timelapse_2d", "sred_block_timelapse_2d_dev", "sred_tool_prep_2d",
     "sred_tool_prep_2d_dev", "sred_relevance_mask_tool_2d", "sred_relevance_mask_tool_2d_dev",
-    "sred_optimise_relevance_mask_2d", "sred_optimise_relevance_mask_2d_dev",
+    "sred_optimise_relevance_mask_2d", "sred_optimise_relevance_mask_2d_dev", "sred_block_size_3d",
+    "sred_local_stats_3d", "sred_global_repetition_3d", "sred_global_repetition_3d_dev",
 ]
 
 
@@ -128,6 +129,11 @@ def load():
         "sred_gat_tree_2d": ([_VP, _F, _I64, _D, _D], _I),
         "sred_block_timelapse_2d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _F], _I),
         "sred_tool_prep_2d": ([_VP, _F, _I, _I, _I, ctypes.POINTER(_D)], _I),
+        "sred_block_size_3d": ([_I, _I, _I], _I),
+        "sred_local_stats_3d": ([_VP, _F, _I, _I, _I, _I, _I, _I, _F, _F], _I),
+        "sred_global_repetition_3d": ([_VP, _I, _I, _F, _I, _I, _I, _I, _I, _I, _FL, _I, _F], _I),
+        "sred_global_repetition_3d_dev": ([_VP, _I, _I, _VP, _I, _I, _I, _I, _I, _I, _FL, _I, _VP, _VP, _VP, _VP,
+                                           _VP, _VP], _I),
         "sred_tool_prep_2d_dev": ([_VP, _VP, _I, _I, _I, ctypes.POINTER(_D)], _I),
         "sred_relevance_mask_tool_2d": ([_VP, _F, _I, _I, _I, _I, _FL, _F, _F, _F], _I),
         "sred_relevance_mask_tool_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _VP, _F, _F], _I),

@@ -40,6 +40,18 @@ def get_local_statistics_2d(resources, input_image, block_width, block_height):
     return LocalStatistics(means, stds)
 
 
+def get_local_statistics_3d(resources, volume, block_width, block_height, block_depth):
+    """CLUtils.getLocalStatistics3D (CLUtils.java:1030-1114) with its declared
+    argument meaning: (means, stds) of a (depth, H, W) volume."""
+    vol = L.f32(volume)
+    d, h, w = vol.shape
+    means = np.zeros_like(vol)
+    stds = np.zeros_like(vol)
+    L.check(L.load().sred_local_stats_3d(resources.handle, L.fptr(vol), w, h, d, block_width, block_height,
+                                         block_depth, L.fptr(means), L.fptr(stds)))
+    return LocalStatistics(means, stds)
+
+
 def _metric_id(metric, block):
     rc = L.load().sred_metric_from_name(metric.encode(), 1 if block else 0)
     return None if rc == L.SRED_ERR_UNKNOWN_METRIC else L.check(rc)

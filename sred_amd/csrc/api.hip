@@ -74,6 +74,58 @@ int footprint(int bw, int bh, std::vector<int> *dx, std::vector<int> *dy)
     return k;
 }
 
+// Ellipsoid footprint of the 3D kernels (kernelGetLocalStatistics3D.cl,
+// kernelGetGlobalPearson3D.cl; GlobalRepetition3D_.java:101-112): z, then y,
+// then x, with the float predicate dx^2/brw^2 + dy^2/brh^2 + dz^2/brd^2 <= 1
+// summed left to right.
+int footprint3d(int bw, int bh, int bd, std::vector<int> *dx, std::vector<int> *dy, std::vector<int> *dz)
+{
+#pragma clang fp contract(off)
+    const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
+    int k = 0;
+    if (dx) { dx->clear(); dy->clear(); dz->clear(); }
+    for (int z = 0; z < bd; z++)
+        for (int y = 0; y < bh; y++)
+            for (int x = 0; x < bw; x++) {
+                const float fx = (float)(x - brw), fy = (float)(y - brh), fz = (float)(z - brd);
+                if (((fx * fx) / (float)(brw * brw)) + ((fy * fy) / (float)(brh * brh)) +
+                        ((fz * fz) / (float)(brd * brd)) <=
+                    1.0f) {
+                    if (dx) { dx->push_back(x - brw); dy->push_back(y - brh); dz->push_back(z - brd); }
+                    k++;
+                }
+            }
+    return k;
+}
+
+int make_geometry3d(int w, int h, int d, int bw, int bh, int bd, Geometry *g)
+{
+    if (w <= 0 || h <= 0 || d <= 0) {
+        set_error("volume dimensions must be positive");
+        return SRED_ERR_ARG;
+    }
+    if (bw < 3 || bh < 3 || bd < 3 || bw % 2 == 0 || bh % 2 == 0 || bd % 2 == 0) {
+        set_error("Block dimensions must be odd, and the block depth at least 3. Please try again.");
+        return SRED_ERR_ARG;
+    }
+    if (bw > SRED_MAX_BLOCK_SIDE || bh > SRED_MAX_BLOCK_SIDE || bd > SRED_MAX_BLOCK_SIDE) {
+        set_error("block side larger than SRED_MAX_BLOCK_SIDE");
+        return SRED_ERR_ARG;
+    }
+    if ((int64_t)w * h * d >= (int64_t)1 << 31 || h > 65535 || d > 65535) {
+        set_error("volume larger than 2^31 voxels, or more than 65535 rows or slices");
+        return SRED_ERR_ARG;
+    }
+    g->w = w; g->h = h; g->d = d; g->bw = bw; g->bh = bh; g->bd = bd;
+    g->brw = bw / 2; g->brh = bh / 2; g->brd = bd / 2;
+    g->K = footprint3d(bw, bh, bd, nullptr, nullptr, nullptr);
+    if (g->K < 2 || g->K > kMaxK) {
+        set_error("unsupported block size");
+        return SRED_ERR_ARG;
+    }
+    return SRED_OK;
+}
+
 double java_float_literal(float f)
 {
     if (f == 0.0f) return 0.0;
@@ -114,6 +166,18 @@ int make_geometry(int w, int h, int bw, int bh, Geometry *g)
 
 int upload_offsets(sred_ctx *ctx, const Geometry &g, int pitch, DevBuf *buf, int *key)
 {
+    if (g.bd > 1) {  // ellipsoid: dz*(w*h) + dy*pitch + dx, uploaded every time (the key marks 2D offsets)
+        std::vector<int> dx, dy, dz;
+        footprint3d(g.bw, g.bh, g.bd, &dx, &dy, &dz);
+        std::vector<int> off(g.K);
+        const int64_t plane = (int64_t)g.w * g.h;
+        for (int k = 0; k < g.K; k++) off[k] = (int)(dz[k] * plane + (int64_t)dy[k] * pitch + dx[k]);
+        SRED_TRY(buf->ensure(sizeof(int) * g.K));
+        SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        SRED_HIP_CHECK(hipMemcpy(buf->ptr, off.data(), sizeof(int) * g.K, hipMemcpyHostToDevice));
+        key[0] = key[1] = key[2] = -1;
+        return SRED_OK;
+    }
     if (key[0] == g.bw && key[1] == g.bh && key[2] == pitch && buf->ptr) return SRED_OK;
     std::vector<int> dx, dy;
     footprint(g.bw, g.bh, &dx, &dy);
@@ -935,6 +999,80 @@ int sred_block_timelapse_2d(sred_ctx *ctx, int metric, const float *stack, int w
     SRED_TRY(block_timelapse_dev(ctx, metric, ctx->img.as<float>(), w, h, depth, tmpl, bw, bh, tmpl_mean, tmpl_std,
                                  relevance_constant, normalize, ctx->out.as<float>()));
     return download(ctx, out, ctx->out, total);
+}
+
+int sred_global_repetition_3d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int d,
+                                  int bw, int bh, int bd, float relevance_constant, int normalize, float *d_out,
+                                  float *d_means, float *d_stds, float *d_mask, float *d_S, float *d_Wsum)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry3d(w, h, d, bw, bh, bd, &g));
+    if (!d_img || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (metric != SRED_PEARSON) {
+        set_error("volumes: only the Pearson metric is built (kernelGetGlobalSsim3D.cl does not compile)");
+        return metric >= 0 && metric <= 3 ? SRED_ERR_ARG : SRED_ERR_UNKNOWN_METRIC;
+    }
+    if (precision != SRED_FP32 && precision != SRED_BF16X3) {
+        set_error("unknown precision");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = g.n();
+    if (!d_means) { SRED_TRY(ctx->means.ensure(sizeof(float) * n)); d_means = ctx->means.as<float>(); }
+    if (!d_stds) { SRED_TRY(ctx->stds.ensure(sizeof(float) * n)); d_stds = ctx->stds.as<float>(); }
+    if (!d_mask) { SRED_TRY(ctx->mask.ensure(sizeof(float) * n)); d_mask = ctx->mask.as<float>(); }
+    if (!d_S) { SRED_TRY(ctx->S.ensure(sizeof(float) * n)); d_S = ctx->S.as<float>(); }
+    if (!d_Wsum) { SRED_TRY(ctx->W.ensure(sizeof(float) * n)); d_Wsum = ctx->W.as<float>(); }
+    return global_map3d_dev(ctx, g, metric, precision, d_img, relevance_constant, normalize, d_out, d_means, d_stds,
+                            d_mask, d_S, d_Wsum);
+}
+
+int sred_global_repetition_3d(sred_ctx *ctx, int metric, int precision, const float *img, int w, int h, int d, int bw,
+                              int bh, int bd, float relevance_constant, int normalize, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !out || w <= 0 || h <= 0 || d <= 0) {
+        set_error("null buffer or empty volume");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = (int64_t)w * h * d;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * n));
+    SRED_TRY(sred_global_repetition_3d_dev(ctx, metric, precision, ctx->img.as<float>(), w, h, d, bw, bh, bd,
+                                           relevance_constant, normalize, ctx->out.as<float>(), nullptr, nullptr,
+                                           nullptr, nullptr, nullptr));
+    return download(ctx, out, ctx->out, n);
+}
+
+int sred_local_stats_3d(sred_ctx *ctx, const float *img, int w, int h, int d, int bw, int bh, int bd, float *means,
+                        float *stds)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry3d(w, h, d, bw, bh, bd, &g));
+    if (!img || !means || !stds) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = g.n();
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(ctx->means.ensure(sizeof(float) * n));
+    SRED_TRY(ctx->stds.ensure(sizeof(float) * n));
+    SRED_TRY(local_stats3d_dev(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>()));
+    SRED_TRY(download(ctx, means, ctx->means, n));
+    return download(ctx, stds, ctx->stds, n);
+}
+
+int sred_block_size_3d(int bw, int bh, int bd)
+{
+    if (bw < 3 || bh < 3 || bd < 3 || bw % 2 == 0 || bh % 2 == 0 || bd % 2 == 0) {
+        set_error("Block dimensions must be odd, and the block depth at least 3. Please try again.");
+        return SRED_ERR_ARG;
+    }
+    return footprint3d(bw, bh, bd, nullptr, nullptr, nullptr);
 }
 
 int sred_tool_prep_2d_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params)

@@ -74,14 +74,15 @@ constexpr int kCompactBlock = 256;
 constexpr int kCompactTile = kCompactItems * kCompactBlock;
 
 __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int64_t p, int64_t n, int w, int h, int brw,
-                                            int brh, double thr_d, int *valid, int *flat)
+                                            int brh, int d, int brd, double thr_d, int *valid, int *flat)
 {
 #pragma clang fp contract(off)
     *valid = 0;
     *flat = 0;
     if (p >= n) return;
-    int x = (int)(p % w), y = (int)(p / w);
-    if (x < brw || x >= w - brw || y < brh || y >= h - brh) return;
+    const int64_t t = p / w;
+    const int x = (int)(p - t * w), y = (int)(t % h), z = (int)(t / h);
+    if (x < brw || x >= w - brw || y < brh || y >= h - brh || z < brd || z >= d - brd) return;
     float s = stds[p];
     float v = s * s;
     if ((double)v < thr_d) return;  // kernelGetGlobalPearson2D.cl:33,67 (double threshold literal)
@@ -91,16 +92,16 @@ __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int6
 
 // counts[b] = (valid, flat) pixels of tile b; smax[b] = largest sigma of its valid pixels
 __global__ __launch_bounds__(kCompactBlock) void k_compact_count(const float *__restrict__ stds, int w, int h, int brw,
-                                                                 int brh, double thr_d, int2 *__restrict__ counts,
-                                                                 float *__restrict__ smax)
+                                                                 int brh, int d, int brd, double thr_d,
+                                                                 int2 *__restrict__ counts, float *__restrict__ smax)
 {
-    const int64_t n = (int64_t)w * h;
+    const int64_t n = (int64_t)w * h * d;
     const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
     int cv = 0, cf = 0;
     float mx = 0.0f;
     for (int i = 0; i < kCompactItems; i++) {
         int v, f;
-        valid_flags(stds, base + i, n, w, h, brw, brh, thr_d, &v, &f);
+        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, &v, &f);
         cv += v;
         cf += f;
         if (v) mx = fmaxf(mx, stds[base + i]);
@@ -169,16 +170,16 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ 
 }
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *__restrict__ stds, int w, int h,
-                                                                   int brw, int brh, double thr_d,
+                                                                   int brw, int brh, int d, int brd, double thr_d,
                                                                    const int2 *__restrict__ offsets,
                                                                    int *__restrict__ idx)
 {
-    const int64_t n = (int64_t)w * h;
+    const int64_t n = (int64_t)w * h * d;
     const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
     int flags[kCompactItems], cv = 0;
     for (int i = 0; i < kCompactItems; i++) {
         int f;
-        valid_flags(stds, base + i, n, w, h, brw, brh, thr_d, &flags[i], &f);
+        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, &flags[i], &f);
         cv += flags[i];
     }
     __shared__ int sv[kCompactBlock];
@@ -969,14 +970,15 @@ __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partia
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_finalize(const float *__restrict__ S, const float *__restrict__ W,
                                                   const float *__restrict__ mask, int w, int h, int brw, int brh,
-                                                  float n_pixels, int invert, float *__restrict__ out)
+                                                  int d, int brd, float n_pixels, int invert, float *__restrict__ out)
 {
 #pragma clang fp contract(off)
-    int64_t n = (int64_t)w * h;
+    int64_t n = (int64_t)w * h * d;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int x = (int)(i % w), y = (int)(i / w);
+        const int64_t t = i / w;
+        const int x = (int)(i - t * w), y = (int)(t % h), z = (int)(t / h);
         float r = 0.0f;
-        if (x >= brw && x < w - brw && y >= brh && y < h - brh) {
+        if (x >= brw && x < w - brw && y >= brh && y < h - brh && z >= brd && z < d - brd) {
             float den = W[i] * n_pixels;
             den = den + kEps;
             r = S[i] / den;
@@ -1221,10 +1223,10 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     SRED_TRY(b.valid_idx->ensure(sizeof(int) * (size_t)n + 256));
     int *d_idx = b.valid_idx->as<int>();
     hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
-                       g.brh, thr_d, d_counts, d_smax);
+                       g.brh, g.d, g.brd, thr_d, d_counts, d_smax);
     hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, d_smax, nblk, d_offsets);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
-                       g.brh, thr_d, d_offsets, d_idx);
+                       g.brh, g.d, g.brd, thr_d, d_offsets, d_idx);
     SRED_HIP_CHECK(hipGetLastError());
     int2 tot[2];
     SRED_HIP_CHECK(hipMemcpyAsync(tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
@@ -1233,7 +1235,8 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     float sigma_max;
     std::memcpy(&sigma_max, &tot[1].x, sizeof sigma_max);
     r.Nv = Nv;
-    r.flat_count = (float)std::min(tot[0].y, 16777216);
+    // both-flat pairs give a flat row the flat-column count; rows the 3D kernels skip score 0
+    r.flat_count = p.skip_flat_rows ? 0.0f : (float)std::min(tot[0].y, 16777216);
     r.d_idx = d_idx;
 
     SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));
@@ -1259,7 +1262,7 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
         r.d_idx = d_idx;
     }
 
-    // 2. offsets relative to the image and per-block scalars
+    // 2. offsets relative to the image (or volume) and per-block scalars
     SRED_TRY(upload_offsets(ctx, g, g.w, b.offs2, b.offs2_key));
     const int *d_offs = b.offs2->as<int>();
     SRED_TRY(b.aux->ensure(sizeof(float4) * (size_t)Nv_rows));
@@ -1393,7 +1396,7 @@ int global_finalize_dev(sred_ctx *ctx, const Geometry &g, int metric, const floa
     int blocks = (int)std::min<int64_t>((g.n() + 255) / 256, 8192);
     int invert = (metric == SRED_NRMSE || metric == SRED_ABSDIFF);
     hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, ctx->stream, d_S, d_W, d_mask, g.w, g.h, g.brw, g.brh,
-                       n_pixels, invert, d_out);
+                       g.d, g.brd, n_pixels, invert, d_out);
     SRED_HIP_CHECK(hipGetLastError());
     if (c > 0.0f) SRED_TRY(apply_mask_dev(ctx, g, d_out, d_mask));
     if (normalize) SRED_TRY(normalize_map_dev(ctx, g, d_out, d_mask, 0));

@@ -68,12 +68,17 @@ struct DevBuf {
 };
 
 // Geometry of one 2D problem.
+// Geometry of one problem: a w x h image (d = 1, bd = 1, brd = 0) or a
+// w x h x d volume with bw x bh x bd blocks (slices of w*h voxels).
 struct Geometry {
     int w = 0, h = 0, bw = 0, bh = 0, brw = 0, brh = 0, K = 0;
-    int64_t n() const { return (int64_t)w * h; }
+    int d = 1, bd = 1, brd = 0;
+    int64_t n() const { return (int64_t)w * h * d; }
 };
 
 int make_geometry(int w, int h, int bw, int bh, Geometry *g);
+int make_geometry3d(int w, int h, int d, int bw, int bh, int bd, Geometry *g);
+int footprint3d(int bw, int bh, int bd, std::vector<int> *dx, std::vector<int> *dy, std::vector<int> *dz);
 // Row-major footprint offsets (dx, dy), reference predicate.
 int footprint(int bw, int bh, std::vector<int> *dx, std::vector<int> *dy);
 // The double an OpenCL compiler reads from Java's ""+f.
@@ -191,6 +196,7 @@ struct GlobalParams {
     int precision = SRED_FP32;
     float filter_param = 0.0f;  // noise variance
     float threshold = 0.0f;     // noise variance * relevance constant
+    bool skip_flat_rows = false;  // flat reference blocks score 0 (the 3D kernels skip them)
 };
 // Scratch of one global problem (pointers to buffers owned by a context).
 struct GlobalBufs {
@@ -303,6 +309,14 @@ double f_quantile(double d1, double d2, double p);
 int seg_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
 // Exact (min, max) of nseg segments as order-preserving keys, mm[2*s], mm[2*s+1].
 int seg_minmax_dev(sred_ctx *ctx, const float *d_in, int64_t seg, int nseg, unsigned int *mm);
+// Volumes (vol.hip): 3D local statistics, the 3D relevance mask, and the
+// global map of a volume Geometry.
+int local_stats3d_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds);
+int vol_relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float c, float *d_mask,
+                           float *noise_var, float *threshold, float *n_pixels);
+int global_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, int precision, const float *d_img, float c,
+                     int normalize, float *d_out, float *d_means, float *d_stds, float *d_mask, float *d_S,
+                     float *d_W);
 // Relevance-mask tools (tools.hip).
 int tool_prep_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params);
 int relevance_mask_tool_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float c, float *d_mask,

@@ -228,13 +228,17 @@ __global__ void k_minmax_init(unsigned int *mm, float init_max)
 // to part[3 * blockIdx.x ..], then k_minmax_final folds them into mm (which
 // holds the initial values).
 __global__ __launch_bounds__(256) void k_minmax(const float *__restrict__ map, const float *__restrict__ mask, int w,
-                                                int h, int brw, int brh, unsigned int *__restrict__ part)
+                                                int h, int brw, int brh, int d, int brd,
+                                                unsigned int *__restrict__ part)
 {
     unsigned int lo = 0xffffffffu, hi = 0u;
     int nonbinary = 0;
-    // whole interior rows per workgroup (no per-pixel index division)
-    for (int y = brh + blockIdx.x; y < h - brh; y += gridDim.x) {
-        const int64_t row = (int64_t)y * w;
+    // whole interior rows per workgroup (no per-pixel index division); a
+    // volume's rows run slice by slice
+    const int ih = h - 2 * brh, rows = ih * (d - 2 * brd);
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int z = brd + r / ih, y = brh + r % ih;
+        const int64_t row = ((int64_t)z * h + y) * w;
         for (int x = brw + threadIdx.x; x < w - brw; x += blockDim.x) {
             const int64_t i = row + x;
             if (mask) {
@@ -320,14 +324,16 @@ __global__ __launch_bounds__(256) void k_minmax_from_parts(const unsigned int *_
 
 // Utils.java:1014-1028: (x - min) / (max - min + EPSILON) on the selected pixels.
 __global__ __launch_bounds__(256) void k_remap(float *__restrict__ map, const float *__restrict__ mask, int w, int h,
-                                               int brw, int brh, const unsigned int *__restrict__ mm)
+                                               int brw, int brh, int d, int brd, const unsigned int *__restrict__ mm)
 {
 #pragma clang fp contract(off)
     if (mm[2]) return;  // non-binary mask: the reference throws before remapping
     const float mn = ord2f(mm[0]), mx = ord2f(mm[1]);
     const float den = mx - mn + kEps;
-    for (int y = brh + blockIdx.x; y < h - brh; y += gridDim.x) {
-        const int64_t row = (int64_t)y * w;
+    const int ih = h - 2 * brh, rows = ih * (d - 2 * brd);
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int z = brd + r / ih, y = brh + r % ih;
+        const int64_t row = ((int64_t)z * h + y) * w;
         for (int x = brw + threadIdx.x; x < w - brw; x += blockDim.x) {
             const int64_t i = row + x;
             if (mask && !(mask[i] > 0.0f)) continue;
@@ -340,14 +346,18 @@ static int grid_for(const Geometry &g) { return (int)std::min<int64_t>((g.n() + 
 constexpr int kMinmaxParts = 1024;  // workgroups of k_minmax (grid-stride); kMiscBytes holds their partials
 constexpr size_t kMiscPartOffset = 4096;
 // one workgroup per interior row, at most kMinmaxParts (partial slots)
-static int grid_minmax(const Geometry &g) { return std::max(1, std::min(g.h - 2 * g.brh, kMinmaxParts)); }
+static int grid_minmax(const Geometry &g)
+{
+    return std::max(1, std::min((g.h - 2 * g.brh) * (g.d - 2 * g.brd), kMinmaxParts));
+}
 
 // min/max (and the non-binary flag) of map over the selected pixels into mm[0..2]
 static int minmax_dev(sred_ctx *ctx, const Geometry &g, const float *d_map, const float *d_mask, unsigned int *mm)
 {
     unsigned int *part = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + kMiscPartOffset);
     const int grid = grid_minmax(g);
-    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh, part);
+    hipLaunchKernelGGL(k_minmax, dim3(grid), dim3(256), 0, ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh, g.d,
+                       g.brd, part);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, ctx->stream, part, grid, mm);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
@@ -360,9 +370,8 @@ int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const floa
     float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
     hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, ctx->stream, mm, init_max);
     SRED_TRY(minmax_dev(ctx, g, d_map, d_mask, mm));
-    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min(g.h - 2 * g.brh, 8192))), dim3(256), 0, ctx->stream, d_map,
-                       d_mask, g.w, g.h, g.brw, g.brh,
-                       mm);
+    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min((g.h - 2 * g.brh) * (g.d - 2 * g.brd), 8192))), dim3(256), 0,
+                       ctx->stream, d_map, d_mask, g.w, g.h, g.brw, g.brh, g.d, g.brd, mm);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
@@ -374,8 +383,8 @@ int normalize_map_from_parts_dev(sred_ctx *ctx, const Geometry &g, float *d_map,
     unsigned int *mm = reinterpret_cast<unsigned int *>(ctx->misc.as<char>() + 1024);
     const float init_max = init_max_min_value ? 1.4e-45f : -FLT_MAX;  // Float.MIN_VALUE (CLUtils.java:940)
     hipLaunchKernelGGL(k_minmax_from_parts, dim3(1), dim3(256), 0, ctx->stream, d_parts, nparts, init_max, mm);
-    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min(g.h - 2 * g.brh, 8192))), dim3(256), 0, ctx->stream, d_map,
-                       nullptr, g.w, g.h, g.brw, g.brh, mm);
+    hipLaunchKernelGGL(k_remap, dim3(std::max(1, std::min((g.h - 2 * g.brh) * (g.d - 2 * g.brd), 8192))), dim3(256), 0,
+                       ctx->stream, d_map, nullptr, g.w, g.h, g.brw, g.brh, g.d, g.brd, mm);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
