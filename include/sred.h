@@ -271,17 +271,21 @@ int sred_block_size_3d(int bw, int bh, int bd);
  * in another order). */
 int sred_local_stats_3d(sred_ctx *ctx, const float *img, int w, int h, int d, int bw, int bh, int bd, float *means,
                         float *stds);
-/* GlobalRepetition3D_.run with the Pearson metric (GlobalRepetition3D_.java:
- * 113-160 -> CLUtils.getGlobalPearson3D :2225-2335) on a w x h x d volume
- * (slice z at z*w*h), built as intended where the reference does not run:
- * the statistics above, the 3D relevance mask of Utils.getRelevanceMask3D
- * (noise variance = float mean of std^2 over the inner voxels; 0 where
- * std^2 <= noiseVar * c), nPixels over it, the reference block gathered
- * over the full ellipsoid (kernelGetGlobalPearson3D.cl stops one slice
- * short), flat reference blocks scoring 0, then S/(Wsum*nPixels + eps),
- * the mask when c > 0 and Utils.normalizeImage3D.  The contraction is the
- * 2D one (precision as sred_global_repetition_2d).  Other metrics return
- * SRED_ERR_ARG. */
+/* GlobalRepetition3D_.run (GlobalRepetition3D_.java:113-160 ->
+ * CLUtils.calculateGlobalRepetitionMap3D :2793-2815) with any of the four
+ * metrics on a w x h x d volume (slice z at z*w*h), built as intended where
+ * the reference does not run: the statistics above, the 3D relevance mask of
+ * Utils.getRelevanceMask3D (noise variance = float mean of std^2 over the
+ * inner voxels; 0 where std^2 <= noiseVar * c), nPixels over it, the
+ * reference block gathered over the full ellipsoid (the kernels stop one
+ * slice short), flat reference blocks scoring 0; per pair the 2D kernels'
+ * forms, except NRMSE adding sqrt(mse) * weight (kernelGetGlobalNrmse3D.cl:
+ * 111) and AbsDiffStds skipping flat test blocks (kernelGetGlobalAbsDiffStds3D
+ * .cl:53); SSIM without the undeclared-name line that stops
+ * kernelGetGlobalSsim3D.cl compiling; then S/(Wsum*nPixels + eps), NRMSE and
+ * AbsDiffStds inverted where the mask is set, the mask when c > 0 and
+ * Utils.normalizeImage3D.  The contraction is the 2D one (precision as
+ * sred_global_repetition_2d). */
 int sred_global_repetition_3d(sred_ctx *ctx, int metric, int precision, const float *img, int w, int h, int d, int bw,
                               int bh, int bd, float relevance_constant, int normalize, float *out);
 int sred_global_repetition_3d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int d,

@@ -137,6 +137,9 @@ def lib():
         L.or_global_pearson3d.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_int, c_int, _D,
                                           vp, vp, vp, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
         L.or_global_pearson3d.restype = c_int
+        L.or_global_repetition3d.argtypes = [c_int, _F, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_int,
+                                             c_int, _D, vp, vp, vp, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_global_repetition3d.restype = c_int
         _lib = L
     return _lib
 
@@ -611,15 +614,19 @@ def vol_noise_var(stds, bw, bh, bd):
     return lib().or_vol_noise_var(stds.reshape(-1), w, h, d, bw // 2, bh // 2, bd // 2)
 
 
-class Pearson3D:
-    """The 3D Pearson map and its intermediates (float form)."""
+class Repetition3D:
+    """A 3D global map and its intermediates (float form)."""
 
     def __init__(self, out, S, W, mask, noise_var, n_pixels):
         self.out, self.S, self.W, self.mask, self.noise_var, self.n_pixels = out, S, W, mask, noise_var, n_pixels
 
 
-def global_pearson3d(vol, bw, bh, bd, c=0.0, normalize=True, fp64=False):
-    """GlobalRepetition3D_ with Pearson's R as intended (refsim float, or FP64 with fp64=True)."""
+Pearson3D = Repetition3D
+
+
+def global_repetition3d(vol, metric, bw, bh, bd, c=0.0, normalize=True, fp64=False):
+    """GlobalRepetition3D_ with `metric` (PEARSON, SSIM, NRMSE, ABSDIFF) as
+    intended: the kernels' float form, or FP64 with fp64=True."""
     vol = _f32(vol)
     d, h, w = vol.shape
     out = np.zeros(vol.shape, np.float64)
@@ -627,7 +634,14 @@ def global_pearson3d(vol, bw, bh, bd, c=0.0, normalize=True, fp64=False):
     W = np.zeros_like(vol)
     mask = np.zeros_like(vol)
     nv, npx = ctypes.c_float(), ctypes.c_float()
-    lib().or_global_pearson3d(vol.reshape(-1), w, h, d, bw, bh, bd, float(c), int(normalize), int(fp64),
-                              out.reshape(-1), S.ctypes.data, W.ctypes.data, mask.ctypes.data, ctypes.byref(nv),
-                              ctypes.byref(npx))
-    return Pearson3D(out, S, W, mask, nv.value, npx.value)
+    rc = lib().or_global_repetition3d(int(metric), vol.reshape(-1), w, h, d, bw, bh, bd, float(c), int(normalize),
+                                      int(fp64), out.reshape(-1), S.ctypes.data, W.ctypes.data, mask.ctypes.data,
+                                      ctypes.byref(nv), ctypes.byref(npx))
+    if rc != 0:
+        raise ValueError(f"unknown metric {metric}")
+    return Repetition3D(out, S, W, mask, nv.value, npx.value)
+
+
+def global_pearson3d(vol, bw, bh, bd, c=0.0, normalize=True, fp64=False):
+    """GlobalRepetition3D_ with Pearson's R as intended (refsim float, or FP64 with fp64=True)."""
+    return global_repetition3d(vol, PEARSON, bw, bh, bd, c, normalize, fp64)

@@ -1699,7 +1699,7 @@ int or_optimise_relevance_mask(const float *img, int w, int h, int bw, int bh, f
 }
 
 /* ------------------------------------------------------------------------ */
-/* Volumes: GlobalRepetition3D_ with the Pearson metric, as intended         */
+/* Volumes: GlobalRepetition3D_ with its four metrics, as intended            */
 /* ------------------------------------------------------------------------ */
 
 /* Ellipsoid footprint (GlobalRepetition3D_.java:101-112, the kernels' z, y,
@@ -1774,18 +1774,30 @@ float or_vol_noise_var(const float *stds, int w, int h, int d, int brw, int brh,
     return s / cnt;
 }
 
-/* GlobalRepetition3D_ with Pearson's R as intended (the statistics with
- * their declared argument order, Utils.getRelevanceMask3D, the reference
- * block over the full ellipsoid): kernelGetGlobalPearson3D.cl per pair in
- * float (fp64 = 0: its float operations and sequential sums) or in double
- * (fp64 = 1: double block values, products, weights and sums; validity and
- * flatness still from the float statistics), then CLUtils.java:2309-2330.
+/* GlobalRepetition3D_.run -> CLUtils.calculateGlobalRepetitionMap3D (:2793-2815)
+ * with the four metrics as intended (the statistics with their declared
+ * argument order, Utils.getRelevanceMask3D, the reference block over the
+ * full ellipsoid), in float (fp64 = 0: the kernels' float operations and
+ * sequential sums) or in double (fp64 = 1: double block values, products,
+ * weights and sums; validity and flatness still from the float statistics).
  * out (double) receives the map; S, W, mask, *noise_var, *n_pixels (each
- * may be NULL) the intermediates of the float form. */
-int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, int bd, float c, int normalize,
-                        int fp64, double *out, float *S_out, float *W_out, float *mask_out, float *noise_var,
-                        float *n_pixels)
+ * may be NULL) the intermediates of the float form.  Per pair
+ * (kernelGetGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}3D.cl): a reference voxel
+ * below the threshold or with std 0 scores 0 (line 37 of the first three,
+ * AbsDiffStds3D.cl:36); test voxels below the threshold are skipped, and
+ * AbsDiffStds3D.cl:53 also skips flat test voxels; the weight is the 2D one;
+ * NRMSE adds sqrt(mse) * weight (Nrmse3D.cl:111; the 2D kernel adds
+ * sqrt(mse)); SSIM is the 2D form (Ssim3D.cl:107-124; its no-op line 76 on an
+ * undeclared pearson_map is dropped); AbsDiffStds adds fmax(0, |d|) * weight
+ * with the weight's exp in double (AbsDiffStds3D.cl:55-60).  Finalize
+ * (CLUtils.java:2309-2330, 2435-2467, 2569-2618, 2720-2769): S / (W nPixels
+ * + eps) on the inner voxels, NRMSE and AbsDiffStds inverted where the mask
+ * is set, the mask applied when c > 0, normalised over the mask. */
+int or_global_repetition3d(int metric, const float *img, int w, int h, int d, int bw, int bh, int bd, float c,
+                           int normalize, int fp64, double *out, float *S_out, float *W_out, float *mask_out,
+                           float *noise_var, float *n_pixels)
 {
+    if (metric < OR_PEARSON || metric > OR_ABSDIFF) return -1;
     const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
     int K;
     int64_t *off = vol_offsets(w, h, bw, bh, bd, &K);
@@ -1803,21 +1815,23 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
         if (mask[o] > 0.0f) np += 1.0f;
     }
     const double fp_d = or_java_float_literal(nv), thr_d = or_java_float_literal(thr), eps_d = or_java_float_literal(EPS_F);
+    const float c1 = 0.0001f, c2 = 0.0009f;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t o = 0; o < n; o++) {
         S[o] = 0.0; W[o] = 0.0;
         if (!vol_inner(o, w, h, d, brw, brh, brd)) continue;
         const float rs = stds[o], rv = rs * rs;
-        if ((double)rv < thr_d || rv == 0.0f) continue;  /* kernelGetGlobalPearson3D.cl:27-30 */
+        if ((double)rv < thr_d || rv == 0.0f) continue;
         float rb[4096], tb[4096];
         double rbd[4096], tbd[4096];
         double rmd = 0.0;
+        const float rm = means[o];
         if (fp64) {
             for (int i = 0; i < K; i++) rmd += img[o + off[i]];
             rmd /= K;
             for (int i = 0; i < K; i++) rbd[i] = (double)img[o + off[i]] - rmd;
         } else {
-            for (int i = 0; i < K; i++) rb[i] = img[o + off[i]] - means[o];
+            for (int i = 0; i < K; i++) rb[i] = img[o + off[i]] - rm;
         }
         double rsd = 0.0;
         if (fp64) {
@@ -1830,38 +1844,71 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
             if (!vol_inner(t, w, h, d, brw, brh, brd)) continue;
             const float ts = stds[t], tv = ts * ts;
             if ((double)tv < thr_d) continue;
+            if (metric == OR_ABSDIFF && tv == 0.0f) continue;
             if (fp64) {
-                double tmd = 0.0, tsd = 0.0, cov = 0.0;
+                double tmd = 0.0, tsd = 0.0;
                 for (int i = 0; i < K; i++) tmd += img[t + off[i]];
                 tmd /= K;
                 for (int i = 0; i < K; i++) { tbd[i] = (double)img[t + off[i]] - tmd; tsd += tbd[i] * tbd[i]; }
                 tsd = sqrt(tsd / (K - 1));
-                for (int i = 0; i < K; i++) cov += rbd[i] * tbd[i];
-                cov /= (K - 1);
                 double wt = exp(-((rsd - tsd) * (rsd - tsd)) / (fp_d + eps_d));
                 waccd += wt;
+                if (metric == OR_ABSDIFF) { accd += fabs(rsd - tsd) * wt; continue; }
+                if (metric == OR_NRMSE) {
+                    double mse = 0.0;
+                    for (int i = 0; i < K; i++) { double e = rbd[i] - tbd[i]; mse += e * e; }
+                    accd += sqrt(mse / K) * wt;
+                    continue;
+                }
                 if (ts == 0.0f) continue;  /* reference non-flat, test flat: 0 */
-                double r = cov / (rsd * tsd + eps_d);
-                accd += (r > 0.0 ? r : 0.0) * wt;
+                double cov = 0.0;
+                for (int i = 0; i < K; i++) cov += rbd[i] * tbd[i];
+                cov /= (K - 1);
+                if (metric == OR_PEARSON) {
+                    double r = cov / (rsd * tsd + eps_d);
+                    accd += (r > 0.0 ? r : 0.0) * wt;
+                } else {
+                    double ss = (2.0 * rmd * tmd + c1) * (2.0 * cov + c2) / ((rmd * rmd) * (tmd * tmd) + c1) /
+                                (rsd * rsd + tsd * tsd + c2) * wt;
+                    accd += ss > 0.0 ? ss : 0.0;
+                }
             } else {
+                float dd = rs - ts;
+                if (metric == OR_ABSDIFF) {
+                    float wt = (float)exp(-1.0 * ((double)(dd * dd) / (fp_d + eps_d)));
+                    wacc += wt;
+                    acc += fmaxf(0.0f, fabsf(dd)) * wt;
+                    continue;
+                }
                 const float tm = means[t];
                 for (int i = 0; i < K; i++) tb[i] = img[t + off[i]] - tm;
-                float cov = 0.0f;
-                for (int i = 0; i < K; i++) cov += rb[i] * tb[i];
-                cov /= (float)(K - 1);
-                float dd = rs - ts;
                 float q = (float)((double)(dd * dd) / (fp_d + eps_d));
                 float wt = (float)expf((float)(-1.0f) * q);
                 wacc += wt;
+                if (metric == OR_NRMSE) {
+                    float mse = 0.0f;
+                    for (int i = 0; i < K; i++) mse += (rb[i] - tb[i]) * (rb[i] - tb[i]);
+                    mse /= (float)K;
+                    acc += sqrtf(mse) * wt;
+                    continue;
+                }
                 if (ts == 0.0f) continue;
-                float r = (float)((double)cov / ((double)(rs * ts) + eps_d));
-                acc += fmaxf(0.0f, r) * wt;
+                float cov = 0.0f;
+                for (int i = 0; i < K; i++) cov += rb[i] * tb[i];
+                cov /= (float)(K - 1);
+                if (metric == OR_PEARSON) {
+                    float r = (float)((double)cov / ((double)(rs * ts) + eps_d));
+                    acc += fmaxf(0.0f, r) * wt;
+                } else {
+                    float num = (2.0f * rm * tm + c1) * (2.0f * cov + c2);
+                    float den = ((rm * rm) * (tm * tm) + c1) * ((rs * rs) + (ts * ts) + c2);
+                    acc += fmaxf(0.0f, num / den * wt);
+                }
             }
         }
         S[o] = fp64 ? accd : (double)acc;
         W[o] = fp64 ? waccd : (double)wacc;
     }
-    /* CLUtils.java:2309-2330: finalize on the inner voxels, mask, normalise */
     for (int64_t o = 0; o < n; o++) {
         double r = 0.0;
         if (vol_inner(o, w, h, d, brw, brh, brd)) {
@@ -1871,6 +1918,8 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
                 den = den + EPS_F;
                 r = (double)((float)S[o] / den);
             }
+            if ((metric == OR_NRMSE || metric == OR_ABSDIFF) && mask[o] > 0.0f)
+                r = (r == 0.0) ? 1.0 : (fp64 ? 1.0 / r : (double)(1.0f / (float)r));
         }
         if (c > 0.0f) r = fp64 ? r * mask[o] : (double)((float)r * mask[o]);
         out[o] = r;
@@ -1895,4 +1944,12 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
     if (n_pixels) *n_pixels = np;
     free(off); free(means); free(stds); free(mask); free(S); free(W);
     return 0;
+}
+
+int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, int bd, float c, int normalize,
+                        int fp64, double *out, float *S_out, float *W_out, float *mask_out, float *noise_var,
+                        float *n_pixels)
+{
+    return or_global_repetition3d(OR_PEARSON, img, w, h, d, bw, bh, bd, c, normalize, fp64, out, S_out, W_out,
+                                  mask_out, noise_var, n_pixels);
 }

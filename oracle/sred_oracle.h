@@ -201,6 +201,9 @@ float or_vol_noise_var(const float *stds, int w, int h, int d, int brw, int brh,
 int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, int bd, float c, int normalize,
                         int fp64, double *out, float *S_out, float *W_out, float *mask_out, float *noise_var,
                         float *n_pixels);
+int or_global_repetition3d(int metric, const float *img, int w, int h, int d, int bw, int bh, int bd, float c,
+                           int normalize, int fp64, double *out, float *S_out, float *W_out, float *mask_out,
+                           float *noise_var, float *n_pixels);
 
 /* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
  * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */

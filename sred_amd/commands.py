@@ -56,19 +56,23 @@ def block_repetition_2d(block, image, relevance_constant=0.0, metric="Pearson's 
 def global_repetition_3d(volume, block_width, block_height, block_depth, relevance_constant=0.0,
                          metric="Pearson's R", normalize_output=True, precision="bf16x3", resources=None):
     """GlobalRepetition3D_ (GlobalRepetition3D_.java:101-160) on a (depth, H, W)
-    volume with Pearson's R, built as intended where the reference does not
-    run (see sred_global_repetition_3d); returns a float32 (depth, H, W) map."""
+    volume with one of its metrics ("Pearson's R", "SSIM", "NRMSE", "Abs. diff.
+    of StdDevs"), built as intended where the reference does not run (see
+    sred_global_repetition_3d); returns a float32 (depth, H, W) map, or None
+    for another metric name (CLUtils.calculateGlobalRepetitionMap3D returns
+    null)."""
     vol = L.f32(volume)
     if vol.ndim != 3:
         raise ValueError("expected a (depth, height, width) volume")
-    if metric != "Pearson's R":
-        raise NotImplementedError("volumes: only Pearson's R is built")
+    m = L.load().sred_metric_from_name(metric.encode(), 0)
+    if m < 0:
+        return None
     d, h, w = vol.shape
     out = np.zeros_like(vol)
     own = resources is None
     res = clutils.get_opencl_resources() if own else resources
     try:
-        L.check(L.load().sred_global_repetition_3d(res.handle, L.PEARSON, clutils.PRECISIONS[precision], L.fptr(vol),
+        L.check(L.load().sred_global_repetition_3d(res.handle, m, clutils.PRECISIONS[precision], L.fptr(vol),
                                                    w, h, d, block_width, block_height, block_depth,
                                                    float(relevance_constant), int(bool(normalize_output)),
                                                    L.fptr(out)))

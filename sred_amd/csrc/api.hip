@@ -1012,9 +1012,9 @@ int sred_global_repetition_3d_dev(sred_ctx *ctx, int metric, int precision, cons
         set_error("null buffer");
         return SRED_ERR_ARG;
     }
-    if (metric != SRED_PEARSON) {
-        set_error("volumes: only the Pearson metric is built (kernelGetGlobalSsim3D.cl does not compile)");
-        return metric >= 0 && metric <= 3 ? SRED_ERR_ARG : SRED_ERR_UNKNOWN_METRIC;
+    if (metric < SRED_PEARSON || metric > SRED_ABSDIFF) {
+        set_error("unknown metric");
+        return SRED_ERR_UNKNOWN_METRIC;
     }
     if (precision != SRED_FP32 && precision != SRED_BF16X3) {
         set_error("unknown precision");

@@ -74,7 +74,8 @@ constexpr int kCompactBlock = 256;
 constexpr int kCompactTile = kCompactItems * kCompactBlock;
 
 __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int64_t p, int64_t n, int w, int h, int brw,
-                                            int brh, int d, int brd, double thr_d, int *valid, int *flat)
+                                            int brh, int d, int brd, double thr_d, int drop_flat, int *valid,
+                                            int *flat)
 {
 #pragma clang fp contract(off)
     *valid = 0;
@@ -86,6 +87,7 @@ __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int6
     float s = stds[p];
     float v = s * s;
     if ((double)v < thr_d) return;  // kernelGetGlobalPearson2D.cl:33,67 (double threshold literal)
+    if (drop_flat && s == 0.0f) return;
     *valid = 1;
     *flat = (s == 0.0f);
 }
@@ -93,7 +95,8 @@ __device__ __forceinline__ void valid_flags(const float *__restrict__ stds, int6
 // counts[b] = (valid, flat) pixels of tile b; smax[b] = largest sigma of its valid pixels
 __global__ __launch_bounds__(kCompactBlock) void k_compact_count(const float *__restrict__ stds, int w, int h, int brw,
                                                                  int brh, int d, int brd, double thr_d,
-                                                                 int2 *__restrict__ counts, float *__restrict__ smax)
+                                                                 int drop_flat, int2 *__restrict__ counts,
+                                                                 float *__restrict__ smax)
 {
     const int64_t n = (int64_t)w * h * d;
     const int64_t base = (int64_t)blockIdx.x * kCompactTile + threadIdx.x * kCompactItems;
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_count(const float *__
     float mx = 0.0f;
     for (int i = 0; i < kCompactItems; i++) {
         int v, f;
-        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, &v, &f);
+        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, drop_flat, &v, &f);
         cv += v;
         cf += f;
         if (v) mx = fmaxf(mx, stds[base + i]);
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const int2 *__restrict__ 
 
 __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *__restrict__ stds, int w, int h,
                                                                    int brw, int brh, int d, int brd, double thr_d,
-                                                                   const int2 *__restrict__ offsets,
+                                                                   int drop_flat, const int2 *__restrict__ offsets,
                                                                    int *__restrict__ idx)
 {
     const int64_t n = (int64_t)w * h * d;
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(kCompactBlock) void k_compact_scatter(const float *
     int flags[kCompactItems], cv = 0;
     for (int i = 0; i < kCompactItems; i++) {
         int f;
-        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, &flags[i], &f);
+        valid_flags(stds, base + i, n, w, h, brw, brh, d, brd, thr_d, drop_flat, &flags[i], &f);
         cv += flags[i];
     }
     __shared__ int sv[kCompactBlock];
@@ -336,7 +339,8 @@ __global__ __launch_bounds__(256) void k_tile_small(const float4 *__restrict__ a
 // can be copied into LDS as they are):
 //   Pearson: R = (t, eps log2(e)/s),  C = (t, 1/s)
 //   SSIM:    R = (t, 2mu, mu^2, s^2+c2), C = (t, mu, mu^2, s^2)   (inf for flat)
-//   NRMSE:   R = (t, |u|^2),          C = (t, |u|^2)              (-inf for padding)
+//   NRMSE:   R = (t, |u|^2),          C = (t, |u|^2)              (-inf for padding; also
+//            the weighted NRMSE, whose padding terms have weight 0)
 //   AbsDiff: R = (t),                 C = (t)
 template <int M> __device__ __forceinline__ float4 row_const(float4 a, const EpiConst &e)
 {
@@ -371,6 +375,12 @@ __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float
     } else if (M == SRED_NRMSE) {  // unweighted (kernelGetGlobalNrmse2D.cl)
         float mse = fmaf(-2.0f, dot, R.y + C.y);
         float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
+        S += t;
+        if (COLS) *cS += t;
+    } else if (M == kNrmseWeighted) {  // sqrt(mse) * weight (kernelGetGlobalNrmse3D.cl:111)
+        const float w = __builtin_amdgcn_exp2f(-(d * d));
+        float mse = fmaf(-2.0f, dot, R.y + C.y);
+        float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f)) * w;
         S += t;
         if (COLS) *cS += t;
     } else {  // ABSDIFF
@@ -949,7 +959,7 @@ __global__ __launch_bounds__(256) void k_contract_absdiff(const float4 *__restri
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partial, int nchunks, int Nv, int Nv_pad,
                                                 const int *__restrict__ idx, const float *__restrict__ stds,
-                                                int metric, float flat_count, double s_scale,
+                                                int flat_rows, float flat_count, double s_scale,
                                                 float *__restrict__ S_map)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -958,8 +968,9 @@ __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partia
     for (int c = 0; c < nchunks; c++) s += (double)partial[(size_t)c * Nv_pad + v];
     int p = idx[v];
     // both-flat pairs contribute 1*weight with weight exp(-0) = 1
-    // (kernelGetGlobalPearson2D.cl:101-102, kernelGetGlobalSsim2D.cl:108-109)
-    if ((metric == SRED_PEARSON || metric == SRED_SSIM) && stds[p] == 0.0f) s = (double)flat_count;
+    // (kernelGetGlobalPearson2D.cl:101-102, kernelGetGlobalSsim2D.cl:108-109);
+    // the 3D kernels skip flat rows (flat_count 0, every metric)
+    if (flat_rows && stds[p] == 0.0f) s = (double)flat_count;
     S_map[p] = (float)(s * s_scale);
 }
 
@@ -1044,6 +1055,8 @@ int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, cons
     if (metric == SRED_PEARSON) return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
     if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
     if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    if (metric == kNrmseWeighted)
+        return launch_sym<kNrmseWeighted>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
     set_error("the symmetric contraction takes Pearson, SSIM or NRMSE");
     return SRED_ERR_ARG;
 }
@@ -1223,10 +1236,10 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     SRED_TRY(b.valid_idx->ensure(sizeof(int) * (size_t)n + 256));
     int *d_idx = b.valid_idx->as<int>();
     hipLaunchKernelGGL(k_compact_count, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
-                       g.brh, g.d, g.brd, thr_d, d_counts, d_smax);
+                       g.brh, g.d, g.brd, thr_d, p.drop_flat ? 1 : 0, d_counts, d_smax);
     hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, ctx->stream, d_counts, d_smax, nblk, d_offsets);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
-                       g.brh, g.d, g.brd, thr_d, d_offsets, d_idx);
+                       g.brh, g.d, g.brd, thr_d, p.drop_flat ? 1 : 0, d_offsets, d_idx);
     SRED_HIP_CHECK(hipGetLastError());
     int2 tot[2];
     SRED_HIP_CHECK(hipMemcpyAsync(tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
@@ -1237,6 +1250,7 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     r.Nv = Nv;
     // both-flat pairs give a flat row the flat-column count; rows the 3D kernels skip score 0
     r.flat_count = p.skip_flat_rows ? 0.0f : (float)std::min(tot[0].y, 16777216);
+    r.zero_flat_rows = p.skip_flat_rows;
     r.d_idx = d_idx;
 
     SRED_TRY(zero_dev(ctx, d_S, sizeof(float) * n));
@@ -1304,8 +1318,9 @@ int global_reduce_dev(sred_ctx *ctx, const GlobalPrepared &r, int metric, const 
 {
     if (r.Nv == 0) return SRED_OK;
     const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)(float)r.tscale_d : 1.0;
+    const int flat_rows = r.zero_flat_rows || metric == SRED_PEARSON || metric == SRED_SSIM;
     hipLaunchKernelGGL(k_reduce, dim3((r.Nv + 255) / 256), dim3(256), 0, ctx->stream, d_part, nchunks, r.Nv, Nv_pad,
-                       r.d_idx, d_stds, metric, r.flat_count, s_scale, d_S);
+                       r.d_idx, d_stds, flat_rows, r.flat_count, s_scale, d_S);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
@@ -1316,6 +1331,8 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
                      const float *d_means, const float *d_stds, float *d_S, float *d_W)
 {
     const int metric = p.metric;
+    // template id of the contraction (data layout and reduction follow `metric`)
+    const int cm = (metric == SRED_NRMSE && p.nrmse_weighted) ? kNrmseWeighted : metric;
     GlobalPrepared r;
     ctx->last_ms = 0.0f;
     ctx->last_launches = 0;
@@ -1346,10 +1363,10 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
             reinterpret_cast<unsigned long long *>(ctx->symtab.as<char>() + (sizeof(SymProblem) + 127) / 128 * 128);
         SRED_TRY(zero_dev(ctx, d_next + 1, sizeof(unsigned long long)));
         ctx->last_skip = d_next + 1;
-        SRED_TRY(launch_sym_metric(metric, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, d_next, P, ec));
+        SRED_TRY(launch_sym_metric(cm, KS, pl.grid, ctx->stream, ctx->symtab.as<SymProblem>(), 1, d_next, P, ec));
     } else if (pl.mfma) {
         int rc = SRED_OK;
-        switch (metric) {
+        switch (cm) {
         case SRED_PEARSON:
             rc = launch_mfma<SRED_PEARSON>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc,
                                            Nv_pad, d_part, Nv_pad, ec);
@@ -1357,6 +1374,10 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         case SRED_SSIM:
             rc = launch_mfma<SRED_SSIM>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc, Nv_pad,
                                         d_part, Nv_pad, ec);
+            break;
+        case kNrmseWeighted:
+            rc = launch_mfma<kNrmseWeighted>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc,
+                                             Nv_pad, d_part, Nv_pad, ec);
             break;
         default:
             rc = launch_mfma<SRED_NRMSE>(KS, grid, ctx->stream, r.uhi, r.ulo, r.d_aux, r.d_tsm, nrowtiles, cpc,
@@ -1379,6 +1400,9 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         else if (metric == SRED_SSIM)
             launch_f32<SRED_SSIM>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc, d_part,
                                   ec);
+        else if (cm == kNrmseWeighted)
+            launch_f32<kNrmseWeighted>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc,
+                                       d_part, ec);
         else
             launch_f32<SRED_NRMSE>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc,
                                    d_part, ec);

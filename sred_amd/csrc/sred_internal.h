@@ -197,7 +197,12 @@ struct GlobalParams {
     float filter_param = 0.0f;  // noise variance
     float threshold = 0.0f;     // noise variance * relevance constant
     bool skip_flat_rows = false;  // flat reference blocks score 0 (the 3D kernels skip them)
+    bool nrmse_weighted = false;  // NRMSE adds sqrt(mse) * weight (kernelGetGlobalNrmse3D.cl:111)
+    bool drop_flat = false;       // flat blocks are neither rows nor columns (kernelGetGlobalAbsDiffStds3D.cl:36,53)
 };
+// Contraction template id of NRMSE with weighted terms (next to the public
+// metric ids 0-3).
+constexpr int kNrmseWeighted = 4;
 // Scratch of one global problem (pointers to buffers owned by a context).
 struct GlobalBufs {
     DevBuf *counts, *valid_idx, *aux, *tflags, *u32, *uhi, *ulo, *offs2;
@@ -209,6 +214,7 @@ GlobalBufs ctx_global_bufs(sred_ctx *ctx);
 struct GlobalPrepared {
     int Nv = 0, Nv_rows = 0;
     float flat_count = 0.0f;
+    bool zero_flat_rows = false;  // every metric's flat rows take flat_count (0)
     double tscale_d = 0.0;
     ContractionPlan pl{};
     const int *d_idx = nullptr;

@@ -1,10 +1,13 @@
-// vol.hip -- global repetition of volumes: GlobalRepetition3D_ with the
-// Pearson metric (GlobalRepetition3D_.java:101-160, CLUtils.getLocalStatistics3D
-// :1030-1114, getGlobalPearson3D :2225-2335, kernelGetLocalStatistics3D.cl,
-// kernelGetGlobalPearson3D.cl, Utils.getRelevanceMask3D :1501-1540).
+// vol.hip -- global repetition of volumes: GlobalRepetition3D_ with its four
+// metrics (GlobalRepetition3D_.java:101-160, CLUtils.getLocalStatistics3D
+// :1030-1114, calculateGlobalRepetitionMap3D :2793-2815 -> getGlobalPearson3D
+// :2225-2335, getGlobalSsim3D :2362-2471, getGlobalNrmse3D :2496-2622,
+// getGlobalAbsDiffStds3D :2647-2773, kernelGetLocalStatistics3D.cl,
+// kernelGetGlobal{Pearson,Ssim,Nrmse,AbsDiffStds}3D.cl, Utils.getRelevanceMask3D
+// :1501-1540).
 //
 // The reference's 3D path does not run as written; the intended computation
-// is built, with three decisions:
+// is built, with four decisions:
 //   * GlobalRepetition3D_ passes (radii..., blockSize) where
 //     getLocalStatistics3D expects (blockSize, radii...): the statistics use
 //     the block size and radii as meant;
@@ -13,14 +16,19 @@
 //     Utils.getRelevanceMask3D is used, whose noise variance is the float
 //     mean of std^2 over the inner voxels, threshold noiseVar * c, mask 0
 //     where std^2 <= threshold;
-//   * kernelGetGlobalPearson3D gathers the reference block over z < gz + bRZ
-//     but every test block over z <= z + bRZ: both use the full ellipsoid.
-// Per pair the kernel is the 2D one (weight, clamp, flat cases), except
+//   * the kernels gather the reference block over z < gz + bRZ but every test
+//     block over z <= z + bRZ: both use the full ellipsoid;
+//   * kernelGetGlobalSsim3D.cl:76 adds 0 to an undeclared pearson_map, so the
+//     kernel does not compile: the no-op is dropped.
+// Per pair the kernels are the 2D ones (weight, clamp, flat cases), except
 // that a flat reference block scores 0 (it is skipped with the others below
-// the threshold).  The pipeline is therefore the 2D one on a volume
-// Geometry: compaction of valid voxels, ellipsoid offsets (dz*w*h + dy*w +
-// dx) for the split-BF16 gather, the same symmetric MFMA contraction and
-// weight sums, and the 3D interior for the finalize and the normalisation.
+// the threshold), NRMSE adds sqrt(mse) * weight instead of sqrt(mse)
+// (kernelGetGlobalNrmse3D.cl:111) and AbsDiffStds also skips flat test
+// blocks (kernelGetGlobalAbsDiffStds3D.cl:53), so flat blocks leave its
+// compaction.  The pipeline is therefore the 2D one on a volume Geometry:
+// compaction of valid voxels, ellipsoid offsets (dz*w*h + dy*w + dx) for the
+// split-BF16 gather, the same symmetric MFMA contraction and weight sums, and
+// the 3D interior for the finalize, the inversion and the normalisation.
 #include <cmath>
 #include <vector>
 
@@ -192,6 +200,8 @@ int global_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, int precision
     p.filter_param = nv;
     p.threshold = thr;
     p.skip_flat_rows = true;
+    p.nrmse_weighted = (metric == SRED_NRMSE);
+    p.drop_flat = (metric == SRED_ABSDIFF);
     SRED_TRY(global_accum_dev(ctx, g, p, d_img, d_means, d_stds, d_S, d_W));
     return global_finalize_dev(ctx, g, metric, d_S, d_W, d_mask, np, c, normalize, d_out);
 }

@@ -1,8 +1,9 @@
-"""GPU volume path (GlobalRepetition3D_ with Pearson's R) against the
+"""GPU volume path (GlobalRepetition3D_ with its four metrics) against the
 oracle, through the C ABI: the 3D statistics bit for bit; the relevance mask
-and nPixels bit for bit; the normalised map within 1e-4 of the float
+and nPixels bit for bit; the normalised maps within 1e-4 of the float
 restatement and of the FP64 form for both contraction precisions, with and
-without a relevance constant; the 2D geometry untouched (covered by the 2D
+without a relevance constant, with flat regions (flat rows score 0, flat
+columns leave AbsDiffStds); the 2D geometry untouched (covered by the 2D
 tests); the errors."""
 import numpy as np
 import pytest
@@ -47,6 +48,22 @@ def test_global_pearson3d(ctx, precision, b, c):
     assert e32 <= 1e-4 and e64 <= 1e-4
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("metric", ["SSIM", "NRMSE", "Abs. diff. of StdDevs"])
+@pytest.mark.parametrize("b,c", [((3, 3, 3), 0.0), ((5, 5, 3), 0.8)])
+def test_global_metrics3d(ctx, precision, metric, b, c):
+    vol = _blobs(16, 26, 30, 5)
+    vol[4:12, 6:16, 8:20] = 0.5  # flat region: flat rows, flat columns
+    got = sred_amd.global_repetition_3d(vol, *b, c, metric, True, precision, resources=ctx)
+    m = {"SSIM": O.SSIM, "NRMSE": O.NRMSE, "Abs. diff. of StdDevs": O.ABSDIFF}[metric]
+    ref = O.global_repetition3d(vol, m, *b, c=c)
+    r64 = O.global_repetition3d(vol, m, *b, c=c, fp64=True)
+    e32, e64 = np.abs(got - ref.out).max(), np.abs(got - r64.out).max()
+    print(f"3d {metric} {b} c={c} {precision}: max|err| vs float {e32:.2e}, vs FP64 {e64:.2e}", flush=True)
+    assert e32 <= 1e-4 and e64 <= 1e-4
+    assert sred_amd.global_repetition_3d(vol, *b, c, "Hu moments", resources=ctx) is None
+
+
 def test_mask_and_device_entry(ctx):
     torch = pytest.importorskip("torch")
     vol = _blobs(12, 24, 28, 3)
@@ -74,7 +91,8 @@ def test_errors(ctx):
     with pytest.raises(sred_amd.SredError):  # block depth below 3
         L.check(L.load().sred_global_repetition_3d(ctx.handle, L.PEARSON, L.BF16X3, L.fptr(vol), 16, 16, 8, 3, 3, 1,
                                                    0.0, 1, L.fptr(np.zeros_like(vol))))
-    with pytest.raises(sred_amd.SredError):  # only Pearson's R for volumes
-        L.check(L.load().sred_global_repetition_3d(ctx.handle, L.SSIM, L.BF16X3, L.fptr(vol), 16, 16, 8, 3, 3, 3,
+    with pytest.raises(sred_amd.SredError) as e:  # unknown metric id
+        L.check(L.load().sred_global_repetition_3d(ctx.handle, 7, L.BF16X3, L.fptr(vol), 16, 16, 8, 3, 3, 3,
                                                    0.0, 1, L.fptr(np.zeros_like(vol))))
+    assert e.value.code == L.SRED_ERR_UNKNOWN_METRIC
     assert L.load().sred_block_size_3d(5, 5, 5) == 33

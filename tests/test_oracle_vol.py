@@ -1,14 +1,15 @@
-"""Pins the oracle's volume path (GlobalRepetition3D_ with Pearson's R, as
-intended where the reference does not run); no GPU needed.
+"""Pins the oracle's volume path (GlobalRepetition3D_ with its four metrics,
+as intended where the reference does not run); no GPU needed.
 
 * the ellipsoid footprint: Gauss's sphere lattice counts (radius 1, 2, 3, 4:
   7, 33, 123, 257 points) and an independent numpy evaluation of the float
   predicate for ellipsoids, in z-y-x order;
 * the 3D statistics and the noise variance (float mean of std^2 over the
   inner voxels) against FP64 numpy;
-* the map: float form against the FP64 form, and the FP64 form against an
-  independent vectorised numpy restatement (block matrix, correlation
-  matrix, weights, flat and threshold rules, finalize, mask, normalisation);
+* the maps of all four metrics: float form against the FP64 form, and the
+  FP64 form against an independent vectorised numpy restatement (block
+  matrix, correlation or distance matrices, weights, flat and threshold
+  rules, finalize, inversion, mask, normalisation);
 * symmetries that the full-ellipsoid reference gather makes exact: flipping
   the volume along z (or transposing x and y with a square block) flips
   (transposes) the FP64 map; the reference's one-slice-short gather would
@@ -75,7 +76,11 @@ def test_stats3d_and_noise_var():
     assert nv == pytest.approx((s64 ** 2).mean(), rel=1e-5)
 
 
-def _pearson3d_numpy(vol, bw, bh, bd, c):
+def _global3d_numpy(vol, metric, bw, bh, bd, c):
+    """GlobalRepetition3D_ per metric in FP64 numpy, from the kernels' rules:
+    rows need var >= threshold and std != 0, columns var >= threshold (and std
+    != 0 for AbsDiffStds); Pearson/SSIM score 0 against flat columns; NRMSE
+    adds sqrt(mse) * weight; finalize, inversion, mask and normalisation."""
     d, h, w = vol.shape
     brw, brh, brd = bw // 2, bh // 2, bd // 2
     _, s32 = O.local_stats3d_f32(vol, bw, bh, bd)
@@ -84,22 +89,45 @@ def _pearson3d_numpy(vol, bw, bh, bd, c):
     thr = np.float32(nv * np.float32(c))
     blocks, mu, sd = _stats64(vol, bw, bh, bd)
     B = blocks.reshape(-1, blocks.shape[-1]) - mu.reshape(-1, 1)
-    sd = sd.ravel()
+    mu, sd = mu.ravel(), sd.ravel()
     K = B.shape[1]
     v32 = (inner * inner).astype(np.float32)
     valid = v32.astype(np.float64) >= float(thr)
     rows = valid & (v32 != 0)
-    cov = (B @ B[valid].T) / (K - 1)
+    cols = rows if metric == O.ABSDIFF else valid
     eps = float(np.float32(1e-7))
     fp = float(nv)
-    wt = np.exp(-((sd[:, None] - sd[valid][None, :]) ** 2) / (fp + eps))
-    r = cov / (sd[:, None] * sd[valid][None, :] + eps)
-    r = np.where(inner[valid][None, :] == 0, 0.0, np.maximum(r, 0.0))
-    S = np.where(rows, (r * wt).sum(1), 0.0)
-    W = np.where(rows, wt.sum(1), 0.0)
+    c1, c2 = float(np.float32(0.0001)), float(np.float32(0.0009))
+    sc, mc, Bc = sd[cols], mu[cols], B[cols]
+    flat_c = (inner[cols] == 0)[None, :]
+    S = np.zeros(len(sd))
+    W = np.zeros(len(sd))
+    for r0 in range(0, len(sd), 64):  # row chunks keep the pair tensors small
+        r1 = min(r0 + 64, len(sd))
+        dd = sd[r0:r1, None] - sc[None, :]
+        wt = np.exp(-dd ** 2 / (fp + eps))
+        if metric == O.ABSDIFF:
+            T = np.abs(dd) * wt
+        elif metric == O.NRMSE:
+            mse = ((B[r0:r1, None, :] - Bc[None, :, :]) ** 2).sum(-1) / K
+            T = np.sqrt(mse) * wt
+        else:
+            cov = B[r0:r1] @ Bc.T / (K - 1)
+            if metric == O.PEARSON:
+                T = np.maximum(cov / (sd[r0:r1, None] * sc[None, :] + eps), 0.0) * wt
+            else:
+                m = mu[r0:r1, None]
+                T = (2 * m * mc[None, :] + c1) * (2 * cov + c2) / ((m * m) * (mc[None, :] ** 2) + c1) / \
+                    (sd[r0:r1, None] ** 2 + sc[None, :] ** 2 + c2) * wt
+                T = np.maximum(T, 0.0)
+            T = np.where(flat_c, 0.0, T)
+        S[r0:r1] = np.where(rows[r0:r1], T.sum(1), 0.0)
+        W[r0:r1] = np.where(rows[r0:r1], wt.sum(1), 0.0)
     mask = (v32 > thr).astype(np.float64)
     npx = float(mask.sum())
     out = S / (W * npx + eps)
+    if metric in (O.NRMSE, O.ABSDIFF):
+        out = np.where(mask > 0, np.where(out == 0, 1.0, 1.0 / np.where(out == 0, 1.0, out)), out)
     if c > 0:
         out = out * mask
     sel = mask > 0
@@ -114,9 +142,22 @@ def _pearson3d_numpy(vol, bw, bh, bd, c):
 def test_pearson3d_against_numpy(b, c):
     vol = _blobs(12, 16, 18, 2)
     r64 = O.global_pearson3d(vol, *b, c=c, fp64=True)
-    np.testing.assert_allclose(r64.out, _pearson3d_numpy(vol, *b, c), atol=1e-9)
+    np.testing.assert_allclose(r64.out, _global3d_numpy(vol, O.PEARSON, *b, c), atol=1e-9)
     r32 = O.global_pearson3d(vol, *b, c=c)
     assert np.abs(r32.out - r64.out).max() < 1e-4
+
+
+@pytest.mark.parametrize("metric", [O.SSIM, O.NRMSE, O.ABSDIFF])
+@pytest.mark.parametrize("b,c", [((3, 3, 3), 0.0), ((5, 3, 3), 0.5), ((3, 5, 5), 1.0)])
+def test_metrics3d_against_numpy(metric, b, c):
+    vol = _blobs(10, 14, 16, 5)
+    vol[2:8, 3:9, 4:10] = 0.5  # flat region: flat rows and (for AbsDiffStds) skipped flat columns
+    r64 = O.global_repetition3d(vol, metric, *b, c=c, fp64=True)
+    np.testing.assert_allclose(r64.out, _global3d_numpy(vol, metric, *b, c), atol=1e-8)
+    r32 = O.global_repetition3d(vol, metric, *b, c=c)
+    assert np.abs(r32.out - r64.out).max() < 2e-4
+    with pytest.raises(ValueError):
+        O.global_repetition3d(vol, 4, *b)
 
 
 def test_pearson3d_symmetries_and_flat_rows():
