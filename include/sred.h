@@ -292,6 +292,36 @@ int sred_global_repetition_3d_dev(sred_ctx *ctx, int metric, int precision, cons
                                   int bw, int bh, int bd, float relevance_constant, int normalize, float *d_out,
                                   float *d_means, float *d_stds, float *d_mask, float *d_S, float *d_Wsum);
 
+/* Utils.getReferenceBlock3D (Utils.java:1047-1155) on a bw x bh x bd block
+ * (slice z at z*bw*bh), host only: the ellipsoid's voxels in z, y, x order
+ * into px (K floats, K = sred_block_size_3d), normalised
+ * (Utils.normalizeArray), mean-subtracted, normalised again; *mean and *std
+ * of the result (float sums in order, unbiased std).  Returns K; odd sides
+ * of at least 3 (the depth check of :1072-1076). */
+int sred_reference_block_3d(const float *block, int bw, int bh, int bd, float *px, float *mean, float *std_);
+/* BlockRepetition3D_.run (BlockRepetition3D_.java:86-105 ->
+ * CLUtils.calculateBlockRepetitionMap3D :1639-1653) on a normalised
+ * w x h x d volume with the reference block of sred_reference_block_3d:
+ * the 3D statistics, the kernel of `metric` per inner voxel
+ * (kernelGetBlock{Pearson,Ssim,Nrmse,AbsDiffStds}3D.cl; Pearson remaps the
+ * test block, subtracts the local mean and remaps again before its dot
+ * product), 1/rmse (1 where rmse == 0) for NRMSE and 1 - d for AbsDiffStds,
+ * the 3D mask when c > 0 (Utils.getRelevanceMask3D, Utils.applyMask3D) and
+ * Utils.normalizeImage3D over it (all inner voxels when c <= 0).  Built as
+ * intended where the reference does not run: the block radii, not the
+ * block depth (getBlockPearson3D, CLUtils.java:1149) or width
+ * (getBlockSsim3D, :1310) that the reference passes.  Errors: block sides
+ * larger than the volume's (BlockRepetition3D_.java:97-100). */
+int sred_block_repetition_3d(sred_ctx *ctx, int metric, const float *img, int w, int h, int d, const float *tmpl,
+                             int bw, int bh, int bd, float tmpl_mean, float tmpl_std, float relevance_constant,
+                             int normalize, float *out);
+/* Device-pointer form; d_means, d_stds and d_mask (used when c > 0) may be
+ * NULL (context scratch) or receive the statistics and the mask. */
+int sred_block_repetition_3d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, int d,
+                                 const float *tmpl, int bw, int bh, int bd, float tmpl_mean, float tmpl_std,
+                                 float relevance_constant, int normalize, float *d_out, float *d_means,
+                                 float *d_stds, float *d_mask);
+
 /* ---- relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,
  *      CalculateLocalMeans_) ---- */
 /* The tools' shared input step, in place (RelevanceMask2D_.java:133-164,

@@ -140,6 +140,15 @@ def lib():
         L.or_global_repetition3d.argtypes = [c_int, _F, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_int,
                                              c_int, _D, vp, vp, vp, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
         L.or_global_repetition3d.restype = c_int
+        L.or_reference_block3d.argtypes = [_F, c_int, c_int, c_int, _F, ctypes.POINTER(c_float),
+                                           ctypes.POINTER(c_float)]
+        L.or_reference_block3d.restype = c_int
+        L.or_block_raw3d.argtypes = [c_int, _F, _F, _F, c_int, c_int, c_int, _F, c_int, c_int, c_int, c_float,
+                                     c_float, _F]
+        L.or_block_raw3d.restype = c_int
+        L.or_block_map3d.argtypes = [c_int, _F, c_int, c_int, c_int, _F, c_int, c_int, c_int, c_float, c_float,
+                                     c_float, c_int, _F]
+        L.or_block_map3d.restype = c_int
         _lib = L
     return _lib
 
@@ -645,3 +654,42 @@ def global_repetition3d(vol, metric, bw, bh, bd, c=0.0, normalize=True, fp64=Fal
 def global_pearson3d(vol, bw, bh, bd, c=0.0, normalize=True, fp64=False):
     """GlobalRepetition3D_ with Pearson's R as intended (refsim float, or FP64 with fp64=True)."""
     return global_repetition3d(vol, PEARSON, bw, bh, bd, c, normalize, fp64)
+
+
+def reference_block3d(block):
+    """Utils.getReferenceBlock3D on a (depth, H, W) block: (pixels, mean, std)
+    with the pixels in the ellipsoid's z, y, x order; None for even sides or
+    a depth below 3."""
+    blk = _f32(block)
+    bd, bh, bw = blk.shape
+    px = np.zeros(bw * bh * bd, np.float32)
+    m, sd = ctypes.c_float(), ctypes.c_float()
+    k = lib().or_reference_block3d(blk.reshape(-1), bw, bh, bd, px, ctypes.byref(m), ctypes.byref(sd))
+    if k < 0:
+        return None
+    return px[:k].copy(), m.value, sd.value
+
+
+def block_raw3d(metric, vol, means, stds, tmpl, bw, bh, bd, tmpl_mean, tmpl_std):
+    """Raw output of the 3D block kernel of `metric` on a (depth, H, W) volume."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    out = np.zeros_like(vol)
+    rc = lib().or_block_raw3d(int(metric), vol.reshape(-1), _f32(means).reshape(-1), _f32(stds).reshape(-1), w, h,
+                              d, _f32(tmpl), bw, bh, bd, float(tmpl_mean), float(tmpl_std), out.reshape(-1))
+    if rc != 0:
+        raise ValueError("unknown metric or block above 8192 voxels")
+    return out
+
+
+def block_map3d(metric, vol, tmpl, bw, bh, bd, tmpl_mean, tmpl_std, c=0.0, normalize=True):
+    """CLUtils.calculateBlockRepetitionMap3D as intended on a normalised
+    (depth, H, W) volume (float form)."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    out = np.zeros_like(vol)
+    rc = lib().or_block_map3d(int(metric), vol.reshape(-1), w, h, d, _f32(tmpl), bw, bh, bd, float(tmpl_mean),
+                              float(tmpl_std), float(c), int(normalize), out.reshape(-1))
+    if rc != 0:
+        raise ValueError("unknown metric or block above 8192 voxels")
+    return out

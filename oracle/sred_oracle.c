@@ -1953,3 +1953,165 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
     return or_global_repetition3d(OR_PEARSON, img, w, h, d, bw, bh, bd, c, normalize, fp64, out, S_out, W_out,
                                   mask_out, noise_var, n_pixels);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Volumes: BlockRepetition3D_, as intended                                 */
+/* ------------------------------------------------------------------------ */
+
+static void java_normalize_array(float *a, int n)
+{   /* Utils.normalizeArray (Utils.java:1642-1660), float */
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int i = 0; i < n; i++) { mn = fminf(a[i], mn); mx = fmaxf(a[i], mx); }
+    for (int i = 0; i < n; i++) a[i] = (a[i] - mn) / (mx - mn + EPS_F);
+}
+
+/* Utils.getReferenceBlock3D (Utils.java:1047-1155): the bw x bh x bd block
+ * (slice z at z*bw*bh) restricted to the ellipsoid in z, y, x order,
+ * normalised, mean-subtracted, normalised again; mean and unbiased std of
+ * the result (float sums in order).  Returns K, or -1 for even sides or a
+ * depth below 3 (:1066-1076; sides below 3 give an empty ellipsoid). */
+int or_reference_block3d(const float *blk, int bw, int bh, int bd, float *px, float *mean, float *std_)
+{
+    if (bw % 2 == 0 || bh % 2 == 0 || bd % 2 == 0 || bd < 3 || bw < 3 || bh < 3) return -1;
+    int K;
+    int64_t *off = vol_offsets(bw, bh, bw, bh, bd, &K);
+    const int64_t c = ((int64_t)(bd / 2) * bh + bh / 2) * bw + bw / 2;
+    for (int i = 0; i < K; i++) px[i] = blk[c + off[i]];
+    free(off);
+    java_normalize_array(px, K);
+    float m = 0.0f;
+    for (int i = 0; i < K; i++) m += px[i];
+    m /= (float)K;
+    for (int i = 0; i < K; i++) px[i] = px[i] - m;
+    java_normalize_array(px, K);
+    m = 0.0f;
+    for (int i = 0; i < K; i++) m += px[i];
+    m /= (float)K;
+    float s = 0.0f;
+    for (int i = 0; i < K; i++) s += (px[i] - m) * (px[i] - m);
+    s = (float)sqrt((double)(s / (float)(K - 1)));
+    *mean = m;
+    *std_ = s;
+    return K;
+}
+
+/* Raw per-voxel output of kernelGetBlock{Pearson,Ssim,Nrmse,AbsDiffStds}3D.cl
+ * with the block radii the kernels mean (getBlockPearson3D passes the block
+ * depth as bRZ, CLUtils.java:1149; getBlockSsim3D the block width as bRW,
+ * :1310).  The macro literals ref_mean, ref_std and EPSILON are double, so
+ * the remaps divide in double, Pearson's quotient is double and SSIM's
+ * numerator is rounded to float before its double division
+ * (kernelGetBlockSsim3D.cl:101).  Borders 0. */
+int or_block_raw3d(int metric, const float *img, const float *means, const float *stds, int w, int h, int d,
+                   const float *tmpl, int bw, int bh, int bd, float tmpl_mean, float tmpl_std, float *out)
+{
+    if (metric < OR_PEARSON || metric > OR_ABSDIFF) return -1;
+    const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
+    int K;
+    int64_t *off = vol_offsets(w, h, bw, bh, bd, &K);
+    if (K > 8192) { free(off); return -1; }
+    const double eps_d = or_java_float_literal(EPS_F);
+    const double std_d = or_java_float_literal(tmpl_std);
+    const double mean_d = or_java_float_literal(tmpl_mean);
+    const int64_t n = (int64_t)w * h * d;
+#pragma omp parallel for schedule(static)
+    for (int64_t zy = 0; zy < (int64_t)d * h; zy++) {
+        float cb[8192];
+        for (int x = 0; x < w; x++) {
+            const int64_t o = zy * w + x;
+            if (!vol_inner(o, w, h, d, brw, brh, brd)) { out[o] = 0.0f; continue; }
+            const float cs = stds[o];
+            if (metric == OR_ABSDIFF) { out[o] = fabsf((float)std_d - cs); continue; }  /* AbsDiffStds3D.cl:28 */
+            for (int i = 0; i < K; i++) cb[i] = img[o + off[i]];
+            const float cm = means[o];
+            if (metric == OR_PEARSON) {  /* kernelGetBlockPearson3D.cl:58-112 */
+                float mn = FLT_MAX, mx = -FLT_MAX;
+                for (int i = 0; i < K; i++) { mn = fminf(mn, cb[i]); mx = fmaxf(mx, cb[i]); }
+                for (int i = 0; i < K; i++) cb[i] = (float)((double)(cb[i] - mn) / ((double)(mx - mn) + eps_d));
+                for (int i = 0; i < K; i++) cb[i] = cb[i] - cm;
+                mn = FLT_MAX; mx = -FLT_MAX;
+                for (int i = 0; i < K; i++) { mn = fminf(mn, cb[i]); mx = fmaxf(mx, cb[i]); }
+                for (int i = 0; i < K; i++) cb[i] = (float)((double)(cb[i] - mn) / ((double)(mx - mn) + eps_d));
+                float cov = 0.0f;
+                for (int i = 0; i < K; i++) cov += tmpl[i] * cb[i];
+                cov /= (float)(K - 1);
+                if (std_d == 0.0 && cs == 0.0f) out[o] = 1.0f;
+                else if (std_d == 0.0 || cs == 0.0f) out[o] = 0.0f;
+                else out[o] = fmaxf(0.0f, (float)((double)cov / (std_d * (double)cs + eps_d)));
+                continue;
+            }
+            for (int i = 0; i < K; i++) cb[i] = cb[i] - cm;
+            if (metric == OR_NRMSE) {  /* kernelGetBlockNrmse3D.cl:61-75 */
+                float mse = 0.0f;
+                for (int i = 0; i < K; i++) mse += (tmpl[i] - cb[i]) * (tmpl[i] - cb[i]);
+                mse /= (float)K;
+                out[o] = sqrtf(mse);
+                continue;
+            }
+            /* kernelGetBlockSsim3D.cl:66-103 */
+            float cov = 0.0f;
+            for (int i = 0; i < K; i++) cov += tmpl[i] * cb[i];
+            cov /= (float)(K - 1);
+            const float c1 = 0.0001f, c2 = 0.0009f;
+            if (std_d == 0.0 && cs == 0.0f) { out[o] = 1.0f; continue; }
+            if (std_d == 0.0 || cs == 0.0f) { out[o] = 0.0f; continue; }
+            const float num = (float)((2.0 * mean_d * (double)cm + (double)c1) * (double)(2.0f * cov + c2));
+            const double den = ((mean_d * mean_d) + (double)(cm * cm) + (double)c1) *
+                               ((std_d * std_d) + (double)(cs * cs) + (double)c2);
+            out[o] = fmaxf(0.0f, (float)((double)num / den));
+        }
+    }
+    (void)n;
+    free(off);
+    return 0;
+}
+
+static void vol_normalize_map(float *map, int w, int h, int d, int brw, int brh, int brd, const float *mask)
+{   /* Utils.normalizeImage3D (Utils.java:1572-1637) */
+    const int64_t n = (int64_t)w * h * d;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int64_t o = 0; o < n; o++)
+        if (vol_inner(o, w, h, d, brw, brh, brd) && (!mask || mask[o] > 0.0f)) {
+            mn = fminf(map[o], mn);
+            mx = fmaxf(map[o], mx);
+        }
+    for (int64_t o = 0; o < n; o++)
+        if (vol_inner(o, w, h, d, brw, brh, brd) && (!mask || mask[o] > 0.0f))
+            map[o] = (map[o] - mn) / (mx - mn + EPS_F);
+}
+
+/* CLUtils.calculateBlockRepetitionMap3D (CLUtils.java:1639-1653) on a
+ * normalised volume: the statistics, the raw kernel output, then the host
+ * steps of getBlockPearson3D / getBlockSsim3D (:1208-1233, :1335-1360),
+ * getBlockNrmse3D (:1447-1496: 1/rmse, 1 where rmse == 0) and
+ * getBlockAbsDiffStds3D (:1585-1620: 1 - d), each followed by the 3D mask
+ * when c > 0 (Utils.getRelevanceMask3D, Utils.applyMask3D) and
+ * Utils.normalizeImage3D over the mask (all inner voxels when c <= 0). */
+int or_block_map3d(int metric, const float *img, int w, int h, int d, const float *tmpl, int bw, int bh, int bd,
+                   float tmpl_mean, float tmpl_std, float c, int normalize, float *out)
+{
+    const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
+    const int64_t n = (int64_t)w * h * d;
+    float *means = malloc(sizeof(float) * n), *stds = malloc(sizeof(float) * n), *mask = NULL;
+    or_local_stats3d_f32(img, w, h, d, bw, bh, bd, means, stds);
+    int rc = or_block_raw3d(metric, img, means, stds, w, h, d, tmpl, bw, bh, bd, tmpl_mean, tmpl_std, out);
+    if (rc) goto done;
+    for (int64_t o = 0; o < n; o++) {
+        if (!vol_inner(o, w, h, d, brw, brh, brd)) continue;
+        if (metric == OR_NRMSE) out[o] = (out[o] == 0.0f) ? 1.0f : 1.0f / out[o];
+        else if (metric == OR_ABSDIFF) out[o] = 1.0f - out[o];
+    }
+    if (c > 0.0f) {
+        mask = malloc(sizeof(float) * n);
+        const float thr = or_vol_noise_var(stds, w, h, d, brw, brh, brd) * c;
+        for (int64_t o = 0; o < n; o++) {
+            mask[o] = 0.0f;
+            if (vol_inner(o, w, h, d, brw, brh, brd)) mask[o] = (stds[o] * stds[o] <= thr) ? 0.0f : 1.0f;
+            out[o] *= mask[o];
+        }
+    }
+    if (normalize) vol_normalize_map(out, w, h, d, brw, brh, brd, mask);
+done:
+    free(means); free(stds); free(mask);
+    return rc;
+}

@@ -204,6 +204,13 @@ int or_global_pearson3d(const float *img, int w, int h, int d, int bw, int bh, i
 int or_global_repetition3d(int metric, const float *img, int w, int h, int d, int bw, int bh, int bd, float c,
                            int normalize, int fp64, double *out, float *S_out, float *W_out, float *mask_out,
                            float *noise_var, float *n_pixels);
+/* BlockRepetition3D_ (Utils.getReferenceBlock3D, the four block kernels and
+ * their host steps), as intended; definitions in sred_oracle.c. */
+int or_reference_block3d(const float *blk, int bw, int bh, int bd, float *px, float *mean, float *std_);
+int or_block_raw3d(int metric, const float *img, const float *means, const float *stds, int w, int h, int d,
+                   const float *tmpl, int bw, int bh, int bd, float tmpl_mean, float tmpl_std, float *out);
+int or_block_map3d(int metric, const float *img, int w, int h, int d, const float *tmpl, int bw, int bh, int bd,
+                   float tmpl_mean, float tmpl_std, float c, int normalize, float *out);
 
 /* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
  * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */

@@ -42,6 +42,7 @@ EXPORTS = [
     "sred_tool_prep_2d_dev", "sred_relevance_mask_tool_2d", "sred_relevance_mask_tool_2d_dev",
     "sred_optimise_relevance_mask_2d", "sred_optimise_relevance_mask_2d_dev", "sred_block_size_3d",
     "sred_local_stats_3d", "sred_global_repetition_3d", "sred_global_repetition_3d_dev",
+    "sred_reference_block_3d", "sred_block_repetition_3d", "sred_block_repetition_3d_dev",
 ]
 
 
@@ -130,6 +131,10 @@ def load():
         "sred_block_timelapse_2d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _F], _I),
         "sred_tool_prep_2d": ([_VP, _F, _I, _I, _I, ctypes.POINTER(_D)], _I),
         "sred_block_size_3d": ([_I, _I, _I], _I),
+        "sred_reference_block_3d": ([_F, _I, _I, _I, _F, _F, _F], _I),
+        "sred_block_repetition_3d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _F], _I),
+        "sred_block_repetition_3d_dev": ([_VP, _I, _VP, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _VP, _VP,
+                                          _VP, _VP], _I),
         "sred_local_stats_3d": ([_VP, _F, _I, _I, _I, _I, _I, _I, _F, _F], _I),
         "sred_global_repetition_3d": ([_VP, _I, _I, _F, _I, _I, _I, _I, _I, _I, _FL, _I, _F], _I),
         "sred_global_repetition_3d_dev": ([_VP, _I, _I, _VP, _I, _I, _I, _I, _I, _I, _FL, _I, _VP, _VP, _VP, _VP,

@@ -111,6 +111,26 @@ def calculate_block_repetition_map_2d(metric, input_image, reference_block, rele
     return out
 
 
+def calculate_block_repetition_map_3d(metric, input_image, reference_block, relevance_constant, normalize_output,
+                                      resources):
+    """CLUtils.calculateBlockRepetitionMap3D (CLUtils.java:1639-1653) with the
+    block radii as intended (see sred_block_repetition_3d).
+
+    Returns a float32 array of width*height*depth, or None for an unknown metric.
+    """
+    mid = _metric_id(metric, block=True)
+    if mid is None:
+        return None
+    out = np.zeros(input_image.size, np.float32)
+    img = L.f32(input_image.image_array).reshape(-1)
+    rb = reference_block
+    L.check(L.load().sred_block_repetition_3d(
+        resources.handle, mid, L.fptr(img), input_image.width, input_image.height, input_image.depth,
+        L.fptr(L.f32(rb.pixels)), rb.width, rb.height, rb.depth, rb.mean, rb.std, relevance_constant,
+        int(bool(normalize_output)), L.fptr(out)))
+    return out
+
+
 def calculate_block_hu_map_2d(input_image, template, resources):
     """Hu template map (kernelGetPatchHu2D.cl:11-282; the reference has no Java
     caller for it): sqrt(hu1(template) * hu1(window)) per interior pixel.

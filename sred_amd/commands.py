@@ -53,20 +53,28 @@ def block_repetition_2d(block, image, relevance_constant=0.0, metric="Pearson's 
             res.close()
 
 
-def global_repetition_3d(volume, block_width, block_height, block_depth, relevance_constant=0.0,
-                         metric="Pearson's R", normalize_output=True, precision="bf16x3", resources=None):
-    """GlobalRepetition3D_ (GlobalRepetition3D_.java:101-160) on a (depth, H, W)
-    volume with one of its metrics ("Pearson's R", "SSIM", "NRMSE", "Abs. diff.
-    of StdDevs"), built as intended where the reference does not run (see
+def global_repetition_3d(volume, block_width=5, block_height=5, block_depth=5, relevance_constant=0.0,
+                         metric="Pearson's R", normalize_output=True, precision="bf16x3", resources=None,
+                         stabilise_noise_variance=False, gat_method="Simplex"):
+    """GlobalRepetition3D_.run (GlobalRepetition3D_.java:35-160) on a (depth,
+    H, W) volume with one of its metrics ("Pearson's R", "SSIM", "NRMSE",
+    "Abs. diff. of StdDevs"): the input step of Utils.getInputImage3D
+    (utils.get_input_image_3d: normalised as a whole), then the map built as
+    intended where the reference does not run (see
     sred_global_repetition_3d); returns a float32 (depth, H, W) map, or None
     for another metric name (CLUtils.calculateGlobalRepetitionMap3D returns
     null)."""
-    vol = L.f32(volume)
-    if vol.ndim != 3:
-        raise ValueError("expected a (depth, height, width) volume")
+    if block_width % 2 == 0 or block_height % 2 == 0 or block_depth % 2 == 0:  # :80-84
+        raise ValueError("Block dimensions must be odd. Please try again.")
+    if block_depth < 3:  # :86-90
+        raise ValueError("Block depth must be at least 3. Please try again.")
+    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True)
+    if inp is None:
+        raise ValueError("Image must have at least 3 slices. Please try again.")
     m = L.load().sred_metric_from_name(metric.encode(), 0)
     if m < 0:
         return None
+    vol = inp.image_array
     d, h, w = vol.shape
     out = np.zeros_like(vol)
     own = resources is None
@@ -135,3 +143,26 @@ def global_batch_device(ctx, metric_id, problems, relevance_constant=0.0, normal
         arr[i] = L.GlobalProblem(d_img, w, h, bw, bh, d_out)
     L.check(L.load().sred_global_batch_2d(ctx.handle, metric_id, len(problems), arr, relevance_constant,
                                           int(bool(normalize_output))))
+
+
+def block_repetition_3d(block, volume, relevance_constant=0.0, metric="Pearson's R", normalize_output=True,
+                        resources=None, stabilise_noise_variance=False, gat_method="Simplex"):
+    """BlockRepetition3D_.run (BlockRepetition3D_.java:32-115) on a (depth, H,
+    W) volume and a (depth, H, W) reference block: Utils.getReferenceBlock3D,
+    the input step of Utils.getInputImage3D, then
+    CLUtils.calculateBlockRepetitionMap3D with the block radii as intended.
+    Returns a float32 (depth, H, W) map, or None for an unknown metric."""
+    ref = utils.get_reference_block_3d(block)
+    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True)
+    if inp is None:
+        raise ValueError("Image must have at least 3 slices. Please try again.")
+    if ref.width > inp.width or ref.height > inp.height or ref.depth > inp.depth:  # :97-100
+        raise ValueError("Block dimensions must be smaller than image dimensions.")
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        out = clutils.calculate_block_repetition_map_3d(metric, inp, ref, relevance_constant, normalize_output, res)
+        return None if out is None else out.reshape(inp.depth, inp.height, inp.width)
+    finally:
+        if own:
+            res.close()

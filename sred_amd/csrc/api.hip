@@ -378,7 +378,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
                       &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v,
                       &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp,
-                      &ctx->stack_work, &ctx->seg_mm};
+                      &ctx->stack_work, &ctx->seg_mm, &ctx->tmpl3d};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -1064,6 +1064,94 @@ int sred_local_stats_3d(sred_ctx *ctx, const float *img, int w, int h, int d, in
     SRED_TRY(local_stats3d_dev(ctx, g, ctx->img.as<float>(), ctx->means.as<float>(), ctx->stds.as<float>()));
     SRED_TRY(download(ctx, means, ctx->means, n));
     return download(ctx, stds, ctx->stds, n);
+}
+
+int sred_reference_block_3d(const float *block, int bw, int bh, int bd, float *px, float *mean, float *std_)
+{
+#pragma clang fp contract(off)
+    if (!block || !px || !mean || !std_) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    if (bw % 2 == 0 || bh % 2 == 0 || bd % 2 == 0 || bw < 3 || bh < 3 || bd < 3 || bw > SRED_MAX_BLOCK_SIDE ||
+        bh > SRED_MAX_BLOCK_SIDE || bd > SRED_MAX_BLOCK_SIDE) {
+        set_error("Block dimensions must be odd (e.g., 3x3x3 or 5x5x5), with at least 3 slices. Please try again.");
+        return SRED_ERR_ARG;
+    }
+    std::vector<int> dx, dy, dz;
+    const int K = footprint3d(bw, bh, bd, &dx, &dy, &dz);
+    if (K < 2 || K > kMaxK) {
+        set_error("unsupported block size");
+        return SRED_ERR_ARG;
+    }
+    const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
+    for (int k = 0; k < K; k++) px[k] = block[((dz[k] + brd) * bh + dy[k] + brh) * bw + dx[k] + brw];
+    // Utils.normalizeArray, mean subtraction, Utils.normalizeArray (Utils.java:1118-1135)
+    for (int pass = 0; pass < 2; pass++) {
+        float bmin = FLT_MAX, bmax = -FLT_MAX;
+        for (int k = 0; k < K; k++) { bmin = std::min(px[k], bmin); bmax = std::max(px[k], bmax); }
+        for (int k = 0; k < K; k++) px[k] = (px[k] - bmin) / (bmax - bmin + kEps);
+        if (pass == 0) {
+            float m = 0.0f;
+            for (int k = 0; k < K; k++) m += px[k];
+            m /= (float)K;
+            for (int k = 0; k < K; k++) px[k] = px[k] - m;
+        }
+    }
+    float m = 0.0f;
+    for (int k = 0; k < K; k++) m += px[k];
+    m /= (float)K;
+    float s = 0.0f;
+    for (int k = 0; k < K; k++) s += (px[k] - m) * (px[k] - m);
+    s = (float)std::sqrt((double)(s / (float)(K - 1)));
+    *mean = m;
+    *std_ = s;
+    return K;
+}
+
+int sred_block_repetition_3d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, int d,
+                                 const float *tmpl, int bw, int bh, int bd, float tmpl_mean, float tmpl_std,
+                                 float relevance_constant, int normalize, float *d_out, float *d_means,
+                                 float *d_stds, float *d_mask)
+{
+    SRED_TRY(check_ctx(ctx));
+    Geometry g;
+    SRED_TRY(make_geometry3d(w, h, d, bw, bh, bd, &g));
+    if (!d_img || !tmpl || !d_out) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    if (bw > w || bh > h || bd > d) {  // BlockRepetition3D_.java:97-100
+        set_error("Block dimensions can't be larger than image dimensions.");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = g.n();
+    if (!d_means) { SRED_TRY(ctx->means.ensure(sizeof(float) * n)); d_means = ctx->means.as<float>(); }
+    if (!d_stds) { SRED_TRY(ctx->stds.ensure(sizeof(float) * n)); d_stds = ctx->stds.as<float>(); }
+    if (!d_mask && relevance_constant > 0.0f) {
+        SRED_TRY(ctx->mask.ensure(sizeof(float) * n));
+        d_mask = ctx->mask.as<float>();
+    }
+    return block_map3d_dev(ctx, g, metric, d_img, tmpl, tmpl_mean, tmpl_std, relevance_constant, normalize, d_out,
+                           d_means, d_stds, d_mask);
+}
+
+int sred_block_repetition_3d(sred_ctx *ctx, int metric, const float *img, int w, int h, int d, const float *tmpl,
+                             int bw, int bh, int bd, float tmpl_mean, float tmpl_std, float relevance_constant,
+                             int normalize, float *out)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !out || w <= 0 || h <= 0 || d <= 0) {
+        set_error("null buffer or empty volume");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = (int64_t)w * h * d;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(ctx->out.ensure(sizeof(float) * n));
+    SRED_TRY(sred_block_repetition_3d_dev(ctx, metric, ctx->img.as<float>(), w, h, d, tmpl, bw, bh, bd, tmpl_mean,
+                                          tmpl_std, relevance_constant, normalize, ctx->out.as<float>(), nullptr,
+                                          nullptr, nullptr));
+    return download(ctx, out, ctx->out, n);
 }
 
 int sred_block_size_3d(int bw, int bh, int bd)

@@ -133,6 +133,7 @@ struct sred_ctx {
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf qt_res, qt_nodes, qt_leaves, qt_seg, qt_tmp;  // quadtree GAT estimate
     sred::DevBuf stack_work, seg_mm;                           // time-lapse stacks
+    sred::DevBuf tmpl3d;                                       // reference block of a volume
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
@@ -323,6 +324,12 @@ int vol_relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds
 int global_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, int precision, const float *d_img, float c,
                      int normalize, float *d_out, float *d_means, float *d_stds, float *d_mask, float *d_S,
                      float *d_W);
+// BlockRepetition3D_'s map of one reference block (h_tmpl: K host floats of
+// Utils.getReferenceBlock3D) over a volume; d_means/d_stds/d_mask receive
+// the statistics and (when c > 0) the 3D mask.
+int block_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *h_tmpl,
+                    float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out, float *d_means,
+                    float *d_stds, float *d_mask);
 // Relevance-mask tools (tools.hip).
 int tool_prep_dev(sred_ctx *ctx, float *d_img, int w, int h, int max_iter, double *params);
 int relevance_mask_tool_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float c, float *d_mask,

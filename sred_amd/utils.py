@@ -41,6 +41,33 @@ class ReferenceBlock2D:
 
 
 @dataclass
+class InputImage3D:
+    """Utils.InputImage3D: the (normalised) float32 volume, slice z at z*w*h."""
+
+    image_array: np.ndarray
+    width: int
+    height: int
+    depth: int
+    size: int
+
+
+@dataclass
+class ReferenceBlock3D:
+    """Utils.ReferenceBlock3D: the ellipsoid's voxels of the block and their statistics."""
+
+    pixels: np.ndarray
+    width: int
+    height: int
+    depth: int
+    radius_width: int
+    radius_height: int
+    radius_depth: int
+    size: int
+    mean: float
+    std: float
+
+
+@dataclass
 class RelevanceMask:
     """Utils.RelevanceMask (Utils.java:35-96)."""
 
@@ -264,6 +291,51 @@ def get_reference_block_2d(block):
     m, s = ctypes.c_float(), ctypes.c_float()
     k = L.check(L.load().sred_reference_block_2d(L.fptr(b), bw, bh, L.fptr(px), ctypes.pointer(m), ctypes.pointer(s)))
     return ReferenceBlock2D(px, bw, bh, bw // 2, bh // 2, k, m.value, s.value)
+
+
+def get_input_image_3d(volume, stabilise_noise_variance=False, gat_method="Simplex", normalize_output=True):
+    """Utils.getInputImage3D (Utils.java:1167-1263) for a (depth, H, W) array.
+
+    Returns None below 3 slices (:1192-1196).  The "Simplex" stabilisation
+    leaves the volume unchanged: the reference writes the transformed
+    slices into a stack array that it then discards (:1227-1229), so only
+    its (unused) fitted parameters differ; "Quad/Octree" (OcTree_) is not
+    built.  Then min-max normalises the whole volume as
+    (v-min)/(max-min+EPSILON) (Utils.normalizeArray, :1257-1260).
+    """
+    arr = L.f32(np.array(volume, dtype=np.float32, copy=True))
+    if arr.ndim != 3:
+        raise ValueError("expected a (depth, height, width) volume")
+    d, h, w = arr.shape
+    if d < 3:
+        return None
+    if stabilise_noise_variance and gat_method == "Quad/Octree":
+        raise NotImplementedError("the octree GAT estimate of volumes (OcTree_) is not built")
+    if normalize_output:
+        L.check(L.load().sred_normalize_input_2d(L.fptr(arr), arr.size))
+    return InputImage3D(arr, w, h, d, w * h * d)
+
+
+def get_reference_block_3d(block):
+    """Utils.getReferenceBlock3D (Utils.java:1047-1155) for a (depth, H, W) block.
+
+    Raises ValueError for even sides or fewer than 3 slices, as the
+    reference throws IllegalArgumentException (:1066-1076).
+    """
+    b = L.f32(block)
+    if b.ndim != 3:
+        raise ValueError("expected a (depth, height, width) block")
+    bd, bh, bw = b.shape
+    if bw % 2 == 0 or bh % 2 == 0 or bd % 2 == 0:
+        raise ValueError("Block dimensions must be odd.")
+    if bd < 3:
+        raise ValueError("Reference block must have at least 3 slices.")
+    k = L.check(L.load().sred_block_size_3d(bw, bh, bd))
+    px = np.zeros(k, np.float32)
+    m, s = ctypes.c_float(), ctypes.c_float()
+    k = L.check(L.load().sred_reference_block_3d(L.fptr(b), bw, bh, bd, L.fptr(px), ctypes.pointer(m),
+                                                 ctypes.pointer(s)))
+    return ReferenceBlock3D(px, bw, bh, bd, bw // 2, bh // 2, bd // 2, k, m.value, s.value)
 
 
 def get_mean_noise_var_2d(ref_pixels, w, h):

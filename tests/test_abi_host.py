@@ -79,6 +79,26 @@ def test_reference_block_kat_and_oracle():
     assert (rb.mean, rb.std) == (m, s)
 
 
+def test_reference_block_3d_and_input_3d_match_oracle():
+    rng = np.random.default_rng(2)
+    for shape in [(3, 3, 3), (5, 7, 5), (7, 5, 9), (3, 11, 3)]:
+        blk = rng.random(shape).astype(np.float32)
+        rb = sred_amd.get_reference_block_3d(blk)
+        px, m, s = O.reference_block3d(blk)
+        np.testing.assert_array_equal(rb.pixels, px)
+        assert (rb.mean, rb.std, rb.size) == (m, s, len(px))
+        assert (rb.width, rb.height, rb.depth) == (shape[2], shape[1], shape[0])
+    for bad in [(3, 4, 3), (1, 3, 3)]:
+        with pytest.raises(ValueError):
+            sred_amd.get_reference_block_3d(np.ones(bad, np.float32))
+    vol = (rng.random((4, 9, 7)) * 50 - 3).astype(np.float32)
+    inp = sred_amd.get_input_image_3d(vol)
+    np.testing.assert_array_equal(inp.image_array, O.normalize_input(vol))
+    assert (inp.width, inp.height, inp.depth, inp.size) == (7, 9, 4, 252)
+    np.testing.assert_array_equal(sred_amd.get_input_image_3d(vol, True, "Simplex").image_array, inp.image_array)
+    assert sred_amd.get_input_image_3d(vol[:2]) is None
+
+
 def test_noise_variance_and_mask_random42_kat():
     k = K["relevance_mask_random42"]
     img, stds = random42_inputs(k)

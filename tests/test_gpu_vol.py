@@ -41,8 +41,9 @@ def test_stats3d_bitexact(ctx, b):
 def test_global_pearson3d(ctx, precision, b, c):
     vol = _blobs(16, 26, 30, 2)
     got = sred_amd.global_repetition_3d(vol, *b, c, "Pearson's R", True, precision, resources=ctx)
-    ref = O.global_pearson3d(vol, *b, c=c)
-    r64 = O.global_pearson3d(vol, *b, c=c, fp64=True)
+    inp = O.normalize_input(vol)  # the command's input step (Utils.getInputImage3D)
+    ref = O.global_pearson3d(inp, *b, c=c)
+    r64 = O.global_pearson3d(inp, *b, c=c, fp64=True)
     e32, e64 = np.abs(got - ref.out).max(), np.abs(got - r64.out).max()
     print(f"3d {b} c={c} {precision}: max|err| vs float {e32:.2e}, vs FP64 {e64:.2e}", flush=True)
     assert e32 <= 1e-4 and e64 <= 1e-4
@@ -56,8 +57,9 @@ def test_global_metrics3d(ctx, precision, metric, b, c):
     vol[4:12, 6:16, 8:20] = 0.5  # flat region: flat rows, flat columns
     got = sred_amd.global_repetition_3d(vol, *b, c, metric, True, precision, resources=ctx)
     m = {"SSIM": O.SSIM, "NRMSE": O.NRMSE, "Abs. diff. of StdDevs": O.ABSDIFF}[metric]
-    ref = O.global_repetition3d(vol, m, *b, c=c)
-    r64 = O.global_repetition3d(vol, m, *b, c=c, fp64=True)
+    inp = O.normalize_input(vol)
+    ref = O.global_repetition3d(inp, m, *b, c=c)
+    r64 = O.global_repetition3d(inp, m, *b, c=c, fp64=True)
     e32, e64 = np.abs(got - ref.out).max(), np.abs(got - r64.out).max()
     print(f"3d {metric} {b} c={c} {precision}: max|err| vs float {e32:.2e}, vs FP64 {e64:.2e}", flush=True)
     assert e32 <= 1e-4 and e64 <= 1e-4
