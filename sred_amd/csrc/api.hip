@@ -378,7 +378,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
                       &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v,
                       &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp,
-                      &ctx->stack_work, &ctx->seg_mm, &ctx->tmpl3d};
+                      &ctx->stack_work, &ctx->seg_mm, &ctx->tmpl3d, &ctx->offs3d};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();

@@ -19,6 +19,7 @@
 // to float before its double division (kernelGetBlockSsim3D.cl:101).  The
 // host steps that follow the kernels (1/rmse, 1 where rmse == 0, for NRMSE:
 // CLUtils.java:1459-1472; 1 - d for AbsDiffStds: :1597-1605) are fused.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -115,13 +116,19 @@ int block_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d
         set_error("unknown metric");
         return SRED_ERR_UNKNOWN_METRIC;
     }
-    // statistics (uploads the ellipsoid's volume offsets into ctx->offs)
+    // statistics (with the ellipsoid's volume offsets in ctx->offs3d)
     SRED_TRY(local_stats3d_dev(ctx, g, d_img, d_means, d_stds));
-    SRED_TRY(ctx->tmpl3d.ensure(sizeof(float) * (size_t)g.K));
-    SRED_HIP_CHECK(hipMemcpy(ctx->tmpl3d.ptr, h_tmpl, sizeof(float) * (size_t)g.K, hipMemcpyHostToDevice));
+    if (ctx->host_tmpl3d.size() != (size_t)g.K || !std::equal(h_tmpl, h_tmpl + g.K, ctx->host_tmpl3d.begin()) ||
+        !ctx->tmpl3d.ptr) {
+        ctx->host_tmpl3d.clear();
+        SRED_TRY(ctx->tmpl3d.ensure(sizeof(float) * (size_t)g.K));
+        SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        SRED_HIP_CHECK(hipMemcpy(ctx->tmpl3d.ptr, h_tmpl, sizeof(float) * (size_t)g.K, hipMemcpyHostToDevice));
+        ctx->host_tmpl3d.assign(h_tmpl, h_tmpl + g.K);
+    }
     const double tm = java_float_literal(tmpl_mean), ts = java_float_literal(tmpl_std), ed = java_float_literal(kEps);
     const dim3 grid((g.w + 63) / 64, (g.h + 3) / 4, g.d);
-    const int *offs = ctx->offs.as<int>();
+    const int *offs = ctx->offs3d.as<int>();
     const float *tp = ctx->tmpl3d.as<float>();
     SRED_TRY(time_begin(ctx));
     switch (metric) {

@@ -133,7 +133,9 @@ struct sred_ctx {
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf qt_res, qt_nodes, qt_leaves, qt_seg, qt_tmp;  // quadtree GAT estimate
     sred::DevBuf stack_work, seg_mm;                           // time-lapse stacks
-    sred::DevBuf tmpl3d;                                       // reference block of a volume
+    sred::DevBuf tmpl3d, offs3d;  // reference block of a volume; ellipsoid offsets dz*w*h + dy*w + dx
+    int offs3d_key[5] = {-1, -1, -1, -1, -1};  // (bw, bh, bd, w, h) of the offsets in offs3d
+    std::vector<float> host_tmpl3d;            // reference block last copied into tmpl3d
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
     // batched global calls
@@ -327,6 +329,8 @@ int global_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, int precision
 // BlockRepetition3D_'s map of one reference block (h_tmpl: K host floats of
 // Utils.getReferenceBlock3D) over a volume; d_means/d_stds/d_mask receive
 // the statistics and (when c > 0) the 3D mask.
+// Uploads the ellipsoid offsets of g into ctx->offs3d unless they are there.
+int upload_offsets3d(sred_ctx *ctx, const Geometry &g);
 int block_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *h_tmpl,
                     float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out, float *d_means,
                     float *d_stds, float *d_mask);

@@ -29,6 +29,7 @@
 // compaction of valid voxels, ellipsoid offsets (dz*w*h + dy*w + dx) for the
 // split-BF16 gather, the same symmetric MFMA contraction and weight sums, and
 // the 3D interior for the finalize, the inversion and the normalisation.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -42,13 +43,15 @@ constexpr int kVolSeg = 1024;  // elements per stage of the in-order chain (16 p
 
 // kernelGetLocalStatistics3D.cl: mean and unbiased std of the ellipsoid
 // around every inner voxel, float sums in footprint order; borders 0.
+// 64 x 4 voxels per workgroup (x across the wave: each offset is one
+// coalesced row read, and no lanes idle on narrow volumes).
 __global__ __launch_bounds__(256) void k_local_stats3d(const float *__restrict__ img, int w, int h, int d, int brw,
                                                        int brh, int brd, const int *__restrict__ offs, int K,
                                                        float *__restrict__ means, float *__restrict__ stds)
 {
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
-    if (x >= w) return;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6), z = blockIdx.z;
+    if (x >= w || y >= h) return;
     const int64_t o = ((int64_t)z * h + y) * w + x;
     if (x < brw || x >= w - brw || y < brh || y >= h - brh || z < brd || z >= d - brd) {
         means[o] = 0.0f;
@@ -139,19 +142,27 @@ __global__ __launch_bounds__(256) void k_vol_mask(const float *__restrict__ stds
 
 }  // namespace
 
-int local_stats3d_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds)
+int upload_offsets3d(sred_ctx *ctx, const Geometry &g)
 {
+    const int key[5] = {g.bw, g.bh, g.bd, g.w, g.h};
+    if (ctx->offs3d.ptr && std::equal(key, key + 5, ctx->offs3d_key)) return SRED_OK;
     std::vector<int> dx, dy, dz;
     footprint3d(g.bw, g.bh, g.bd, &dx, &dy, &dz);
     std::vector<int> off(g.K);
     const int64_t plane = (int64_t)g.w * g.h;
     for (int k = 0; k < g.K; k++) off[k] = (int)(dz[k] * plane + (int64_t)dy[k] * g.w + dx[k]);
-    SRED_TRY(ctx->offs.ensure(sizeof(int) * g.K));
+    SRED_TRY(ctx->offs3d.ensure(sizeof(int) * g.K));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    SRED_HIP_CHECK(hipMemcpy(ctx->offs.ptr, off.data(), sizeof(int) * g.K, hipMemcpyHostToDevice));
-    ctx->offs_key[0] = ctx->offs_key[1] = ctx->offs_key[2] = -1;  // the 2D statistics' offsets are gone
-    hipLaunchKernelGGL(k_local_stats3d, dim3((g.w + 255) / 256, g.h, g.d), dim3(256), 0, ctx->stream, d_img, g.w, g.h,
-                       g.d, g.brw, g.brh, g.brd, ctx->offs.as<int>(), g.K, d_means, d_stds);
+    SRED_HIP_CHECK(hipMemcpy(ctx->offs3d.ptr, off.data(), sizeof(int) * g.K, hipMemcpyHostToDevice));
+    std::copy(key, key + 5, ctx->offs3d_key);
+    return SRED_OK;
+}
+
+int local_stats3d_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, float *d_means, float *d_stds)
+{
+    SRED_TRY(upload_offsets3d(ctx, g));
+    hipLaunchKernelGGL(k_local_stats3d, dim3((g.w + 63) / 64, (g.h + 3) / 4, g.d), dim3(256), 0, ctx->stream, d_img,
+                       g.w, g.h, g.d, g.brw, g.brh, g.brd, ctx->offs3d.as<int>(), g.K, d_means, d_stds);
     SRED_HIP_CHECK(hipGetLastError());
     return SRED_OK;
 }
