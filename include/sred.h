@@ -322,6 +322,32 @@ int sred_block_repetition_3d_dev(sred_ctx *ctx, int metric, const float *d_img, 
                                  float relevance_constant, int normalize, float *d_out, float *d_means,
                                  float *d_stds, float *d_mask);
 
+/* The octree GAT estimate of a w x h x d volume: the "Quad/Octree" branch of
+ * Utils.getInputImage3D (Utils.java:1234-1254) and the OcTree_ plugin
+ * (OcTree_.java:82-96): OcTree_(w, h, d, min_size, alpha).buildTree (a leaf
+ * when a side is <= min_size or the F test with (width*height - 1) degrees
+ * of freedom says keep; node variance from double sums, pseudo-residuals
+ * 7 v - the six neighbours over sqrt(56)), robust means (max_iter), LTS
+ * variances (lts_fraction), the leaves in pre-order (children x fastest,
+ * then y, then z) and the regression, whose slope g0 and intercept eDC are
+ * cast to float (g0_edc[0..1] receive the float values).  boxes (x, y, z,
+ * w, h, d), means and variances receive up to cap leaves (each may be
+ * NULL); *n_leaves the count.  Errors: a visited node with one pixel per
+ * slice (FDistribution(0, 0)), fewer than 3 leaves. */
+int sred_octree_gat_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, int min_size, float alpha, int max_iter,
+                       float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *boxes, float *means,
+                       float *variances);
+int sred_octree_gat_3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, int min_size, float alpha,
+                           int max_iter, float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *boxes,
+                           float *means, float *variances);
+/* OcTree_.applyGATtree (OcTree_.java:471-485) in place on n voxels:
+ * ref = 3/8 g0^2 + eDC in double, 0 where v < -ref/g0, else
+ * (2/g0) sqrt(g0 v + ref).  Utils.getInputImage3D passes width*height as n
+ * (Utils.java:1253), which leaves a one-slice array behind; pass all
+ * voxels, as the OcTree_ plugin does. */
+int sred_gat_tree_3d(sred_ctx *ctx, float *vol, int64_t n, float g0, float edc);
+int sred_gat_tree_3d_dev(sred_ctx *ctx, float *d_vol, int64_t n, float g0, float edc);
+
 /* ---- relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,
  *      CalculateLocalMeans_) ---- */
 /* The tools' shared input step, in place (RelevanceMask2D_.java:133-164,

@@ -116,6 +116,19 @@ def lib():
         L.or_quadtree_gat.restype = c_int
         L.or_gat_tree.argtypes = [_F, c_i64, c_double, c_double]
         L.or_gat_tree.restype = None
+        L.or_oc_criterion.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                      ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
+        L.or_oc_criterion.restype = None
+        L.or_octree_gat.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, c_int, c_float, c_int, vp, _F, _F,
+                                    ctypes.POINTER(c_int), ctypes.POINTER(c_double), ctypes.POINTER(c_double)]
+        L.or_octree_gat.restype = c_int
+        L.or_gat_tree3d.argtypes = [_F, c_i64, c_float, c_float]
+        L.or_gat_tree3d.restype = None
+        L.or_oc_robust_mean.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int]
+        L.or_oc_robust_mean.restype = c_float
+        L.or_oc_lts_variance.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_float,
+                                         c_float]
+        L.or_oc_lts_variance.restype = c_float
         L.or_tool_prep.argtypes = [_F, c_int, c_int, c_int, _D]
         L.or_tool_prep.restype = None
         L.or_noise_var_double.argtypes = [_F, c_int, c_int]
@@ -692,4 +705,44 @@ def block_map3d(metric, vol, tmpl, bw, bh, bd, tmpl_mean, tmpl_std, c=0.0, norma
                               float(tmpl_std), float(c), int(normalize), out.reshape(-1))
     if rc != 0:
         raise ValueError("unknown metric or block above 8192 voxels")
+    return out
+
+
+def oc_criterion(vol, x0, y0, z0, nw, nh, nd):
+    """OcTree_.computeCriterion's two variances: (block variance, noise variance)."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    bv, nv = ctypes.c_float(), ctypes.c_float()
+    lib().or_oc_criterion(vol.reshape(-1), w, h, d, x0, y0, z0, nw, nh, nd, ctypes.byref(bv), ctypes.byref(nv))
+    return bv.value, nv.value
+
+
+class OcTreeResult:
+    """Leaves in pre-order (x, y, z, w, h, d), their robust means and LTS variances, g0 and eDC (float values)."""
+
+    def __init__(self, boxes, means, variances, g0, edc):
+        self.boxes, self.means, self.variances, self.g0, self.edc = boxes, means, variances, g0, edc
+
+
+def octree_gat(vol, min_size=4, alpha=0.01, max_iter=50, frac=0.75):
+    """The OcTree_ estimate of a (depth, H, W) volume; raises ValueError(status) on the reference's errors."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    cap = max(16, vol.size // 2)
+    boxes = np.zeros((cap, 6), np.int32)
+    means = np.zeros(cap, np.float32)
+    var = np.zeros(cap, np.float32)
+    n, g0, edc = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    rc = lib().or_octree_gat(vol.reshape(-1), w, h, d, int(min_size), float(alpha), int(max_iter), float(frac), cap,
+                             boxes.ctypes.data, means, var, ctypes.byref(n), ctypes.byref(g0), ctypes.byref(edc))
+    if rc != 0:
+        raise ValueError(rc)
+    k = n.value
+    return OcTreeResult(boxes[:k].copy(), means[:k].copy(), var[:k].copy(), g0.value, edc.value)
+
+
+def gat_tree3d(vol, g0, edc):
+    """OcTree_.applyGATtree with float g0, eDC (a copy)."""
+    out = _f32(vol).copy()
+    lib().or_gat_tree3d(out.reshape(-1), out.size, float(g0), float(edc))
     return out

@@ -2115,3 +2115,201 @@ done:
     free(means); free(stds); free(mask);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Volumes: OcTree_ GAT estimate                                             */
+/* ------------------------------------------------------------------------ */
+
+/* OcTree_.computeCriterion (OcTree_.java:216-228): the node's variance from
+ * Utils.getMeanAndVarBlock3D (Utils.java:1356-1380: double sums of v and v*v
+ * in z, y, x order, var = sq/n - mean^2, cast to float) and
+ * calculateNoiseVariance3D (:230-280): pseudo-residuals (7 v - the six
+ * neighbours inside the volume) / sqrt(56), their float mean and the float
+ * sum of (float)((r - mean)^2) over n - 1. */
+void or_oc_criterion(const float *img, int w, int h, int d, int x0, int y0, int z0, int nw, int nh, int nd,
+                     float *block_var, float *noise_var)
+{
+    const int64_t n = (int64_t)nw * nh * nd, plane = (int64_t)w * h;
+    double mean = 0.0, sq = 0.0;
+    for (int z = z0; z < z0 + nd; z++)
+        for (int y = y0; y < y0 + nh; y++)
+            for (int x = x0; x < x0 + nw; x++) {
+                double v = (double)img[z * plane + (int64_t)y * w + x];
+                mean += v;
+                sq += v * v;
+            }
+    mean = mean / (double)n;
+    *block_var = (float)(sq / (double)n - mean * mean);
+
+    const float l = 7.0f;
+    const float scale = (float)(1.0 / sqrt((double)(l * l + l)));
+    float sum = 0.0f, mr = 0.0f, ss = 0.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) mr = sum / (float)n;
+        for (int z = z0; z < z0 + nd; z++)
+            for (int y = y0; y < y0 + nh; y++)
+                for (int x = x0; x < x0 + nw; x++) {
+                    const float *p = img + z * plane + (int64_t)y * w + x;
+                    float lap = l * p[0];
+                    if (x > 0) lap -= p[-1];
+                    if (x < w - 1) lap -= p[1];
+                    if (y > 0) lap -= p[-w];
+                    if (y < h - 1) lap -= p[w];
+                    if (z > 0) lap -= p[-plane];
+                    if (z < d - 1) lap -= p[plane];
+                    float r = lap * scale;
+                    if (pass == 0) {
+                        sum += r;
+                    } else {
+                        double dd = (double)r - (double)mr;
+                        ss += (float)(dd * dd);  /* Math.pow(dd, 2.0) */
+                    }
+                }
+    }
+    *noise_var = ss / ((float)n - 1.0f);
+}
+
+/* OcTree_.calculateRobustMean (:316-366): as the quadtree's, over the box. */
+float or_oc_robust_mean(const float *img, int w, int h, int x0, int y0, int z0, int nw, int nh, int nd,
+                        int max_iter)
+{
+    const int64_t plane = (int64_t)w * h;
+    float mean = 0.0f, var = 0.0f;
+    int n = 0;
+    for (int z = z0; z < z0 + nd; z++)
+        for (int y = y0; y < y0 + nh; y++)
+            for (int x = x0; x < x0 + nw; x++) {
+                float v = img[z * plane + (int64_t)y * w + x];
+                mean += v;
+                var += (v - mean) * (v - mean);
+                n++;
+            }
+    mean = mean / (float)n;
+    var = var / (float)(n - 1);
+    var = jmaxf(var, OR_JEPS);
+    float prev;
+    int it = 0;
+    do {
+        prev = mean;
+        float ws = 0.0f, wv = 0.0f;
+        for (int z = z0; z < z0 + nd; z++)
+            for (int y = y0; y < y0 + nh; y++)
+                for (int x = x0; x < x0 + nw; x++) {
+                    float v = img[z * plane + (int64_t)y * w + x];
+                    float wt = (float)exp(-((double)(v - mean) * (double)(v - mean)) / (2.0 * (double)var));
+                    ws += wt;
+                    wv += wt * v;
+                }
+        mean = wv / jmaxf(ws, OR_JEPS);
+        it++;
+    } while ((double)fabsf(mean - prev) > 1e-5 && it < max_iter);
+    return mean;
+}
+
+/* OcTree_.calculateLTSVariance (:382-420). */
+float or_oc_lts_variance(const float *img, int w, int h, int x0, int y0, int z0, int nw, int nh, int nd, float mean,
+                         float frac)
+{
+    const int64_t plane = (int64_t)w * h;
+    int n = nw * nh * nd, k = 0;
+    float *r = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int z = z0; z < z0 + nd; z++)
+        for (int y = y0; y < y0 + nh; y++)
+            for (int x = x0; x < x0 + nw; x++) {
+                float v = img[z * plane + (int64_t)y * w + x];
+                r[k++] = (v - mean) * (v - mean);
+            }
+    qsort(r, (size_t)n, sizeof(float), cmp_float_asc);
+    int hh = (int)floorf(frac * (float)n);
+    float s = 0.0f;
+    for (int i = 0; i < hh; i++) s += r[i];
+    free(r);
+    return s / ((float)hh + OR_JEPS);
+}
+
+typedef struct {
+    qt_build q;     /* thresholds cache, alpha, cap */
+    int d, bad, leaves;
+    int32_t *boxes;
+} oc_build;
+
+static void oc_visit(oc_build *b, int x, int y, int z, int nw, int nh, int nd)
+{
+    if (b->bad) return;
+    if (nw * nh <= 1) {  /* FDistribution(0, 0): NotStrictlyPositiveException */
+        b->bad = -1;
+        return;
+    }
+    float bv, nv;
+    or_oc_criterion(b->q.img, b->q.w, b->q.h, b->d, x, y, z, nw, nh, nd, &bv, &nv);
+    /* performFisherTest (:282-296) with regionSize = width * height (:225) */
+    float ratio = jminf(bv, nv) / jmaxf(bv, nv);
+    int divide = ratio < qt_threshold(&b->q, nw * nh);
+    if (nw <= b->q.min_size || nh <= b->q.min_size || nd <= b->q.min_size || !divide) {
+        if (b->leaves < b->q.cap) {
+            int32_t *r = b->boxes + 6 * b->leaves;
+            r[0] = x, r[1] = y, r[2] = z, r[3] = nw, r[4] = nh, r[5] = nd;
+        }
+        b->leaves++;
+        return;
+    }
+    int hw = nw / 2, hh = nh / 2, hd = nd / 2;  /* splitNode (:197-213): children 0..7, x fastest */
+    for (int c = 0; c < 8; c++)
+        oc_visit(b, x + (c & 1) * hw, y + ((c >> 1) & 1) * hh, z + (c >> 2) * hd, hw, hh, hd);
+}
+
+/* The "Quad/Octree" branch of Utils.getInputImage3D (Utils.java:1234-1254),
+ * the same steps as the OcTree_ plugin (OcTree_.java:82-96): OcTree_(w, h,
+ * d, min_size, alpha).buildTree (leaf when a side is <= min_size or the
+ * F test says keep), calculateRobustMeans(max_iter),
+ * calculateLTSVariances(frac), collectMeanVariancePairs (leaves in
+ * pre-order) and performLinearRegression, whose slope and intercept are cast
+ * to float (:458-469).  boxes (x, y, z, w, h, d), means and vars receive up
+ * to cap leaves; *nleaves the count.  Returns 0, -1 for a visited node of
+ * one pixel per slice (FDistribution(0, 0)) or fewer than 3 leaves, -2 if
+ * cap is too small. */
+int or_octree_gat(const float *img, int w, int h, int d, int min_size, float alpha, int max_iter, float frac,
+                  int cap, int32_t *boxes, float *means, float *vars, int *nleaves, double *g0, double *edc)
+{
+    oc_build b;
+    memset(&b, 0, sizeof b);
+    b.q.img = img, b.q.w = w, b.q.h = h, b.q.min_size = min_size, b.q.alpha = alpha, b.q.cap = cap;
+    b.d = d, b.boxes = boxes;
+    oc_visit(&b, 0, 0, 0, w, h, d);
+    *nleaves = b.leaves;
+    if (b.bad) return b.bad;
+    if (b.leaves > cap) return -2;
+    for (int i = 0; i < b.leaves; i++) {
+        const int32_t *r = boxes + 6 * i;
+        means[i] = or_oc_robust_mean(img, w, h, r[0], r[1], r[2], r[3], r[4], r[5], max_iter);
+        vars[i] = or_oc_lts_variance(img, w, h, r[0], r[1], r[2], r[3], r[4], r[5], means[i], frac);
+    }
+    if (b.leaves < 3) return -1;
+    double *x = (double *)malloc(sizeof(double) * 2 * (size_t)b.leaves), *y = x + b.leaves;
+    for (int i = 0; i < b.leaves; i++) {
+        x[i] = (double)means[i];
+        y[i] = (double)vars[i];
+    }
+    double s, c;
+    or_simple_regression(x, y, b.leaves, &s, &c);
+    *g0 = (double)(float)s;
+    *edc = (double)(float)c;
+    free(x);
+    return 0;
+}
+
+/* OcTree_.applyGATtree (:471-485) in place with float g0, eDC:
+ * ref = 3/8 g0^2 + eDC; 0 where v < -ref / g0. */
+void or_gat_tree3d(float *px, int64_t n, float g0, float edc)
+{
+    const double g = (double)g0;
+    const double ref = 3.0 / 8.0 * g * g + (double)edc;
+    for (int64_t i = 0; i < n; i++) {
+        double v = (double)px[i];
+        if (v < -ref / g)
+            v = 0.0;
+        else
+            v = (2.0 / g) * sqrt(g * v + ref);
+        px[i] = (float)v;
+    }
+}

@@ -211,6 +211,17 @@ int or_block_raw3d(int metric, const float *img, const float *means, const float
                    const float *tmpl, int bw, int bh, int bd, float tmpl_mean, float tmpl_std, float *out);
 int or_block_map3d(int metric, const float *img, int w, int h, int d, const float *tmpl, int bw, int bh, int bd,
                    float tmpl_mean, float tmpl_std, float c, int normalize, float *out);
+/* OcTree_ GAT estimate of a volume and its transform; definitions in
+ * sred_oracle.c. */
+void or_oc_criterion(const float *img, int w, int h, int d, int x0, int y0, int z0, int nw, int nh, int nd,
+                     float *block_var, float *noise_var);
+float or_oc_robust_mean(const float *img, int w, int h, int x0, int y0, int z0, int nw, int nh, int nd,
+                        int max_iter);
+float or_oc_lts_variance(const float *img, int w, int h, int x0, int y0, int z0, int nw, int nh, int nd, float mean,
+                         float frac);
+int or_octree_gat(const float *img, int w, int h, int d, int min_size, float alpha, int max_iter, float frac,
+                  int cap, int32_t *boxes, float *means, float *vars, int *nleaves, double *g0, double *edc);
+void or_gat_tree3d(float *px, int64_t n, float g0, float edc);
 
 /* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
  * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */

@@ -43,6 +43,7 @@ EXPORTS = [
     "sred_optimise_relevance_mask_2d", "sred_optimise_relevance_mask_2d_dev", "sred_block_size_3d",
     "sred_local_stats_3d", "sred_global_repetition_3d", "sred_global_repetition_3d_dev",
     "sred_reference_block_3d", "sred_block_repetition_3d", "sred_block_repetition_3d_dev",
+    "sred_octree_gat_3d", "sred_octree_gat_3d_dev", "sred_gat_tree_3d", "sred_gat_tree_3d_dev",
 ]
 
 
@@ -132,6 +133,10 @@ def load():
         "sred_tool_prep_2d": ([_VP, _F, _I, _I, _I, ctypes.POINTER(_D)], _I),
         "sred_block_size_3d": ([_I, _I, _I], _I),
         "sred_reference_block_3d": ([_F, _I, _I, _I, _F, _F, _F], _I),
+        "sred_octree_gat_3d": ([_VP, _F, _I, _I, _I, _I, _FL, _I, _FL, _VP, _VP, _I, _VP, _VP, _VP], _I),
+        "sred_octree_gat_3d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _I, _FL, _VP, _VP, _I, _VP, _VP, _VP], _I),
+        "sred_gat_tree_3d": ([_VP, _F, _I64, _FL, _FL], _I),
+        "sred_gat_tree_3d_dev": ([_VP, _VP, _I64, _FL, _FL], _I),
         "sred_block_repetition_3d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _F], _I),
         "sred_block_repetition_3d_dev": ([_VP, _I, _VP, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _VP, _VP,
                                           _VP, _VP], _I),

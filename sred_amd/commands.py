@@ -68,7 +68,7 @@ def global_repetition_3d(volume, block_width=5, block_height=5, block_depth=5, r
         raise ValueError("Block dimensions must be odd. Please try again.")
     if block_depth < 3:  # :86-90
         raise ValueError("Block depth must be at least 3. Please try again.")
-    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True)
+    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True, resources)
     if inp is None:
         raise ValueError("Image must have at least 3 slices. Please try again.")
     m = L.load().sred_metric_from_name(metric.encode(), 0)
@@ -153,7 +153,7 @@ def block_repetition_3d(block, volume, relevance_constant=0.0, metric="Pearson's
     CLUtils.calculateBlockRepetitionMap3D with the block radii as intended.
     Returns a float32 (depth, H, W) map, or None for an unknown metric."""
     ref = utils.get_reference_block_3d(block)
-    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True)
+    inp = utils.get_input_image_3d(volume, stabilise_noise_variance, gat_method, True, resources)
     if inp is None:
         raise ValueError("Image must have at least 3 slices. Please try again.")
     if ref.width > inp.width or ref.height > inp.height or ref.depth > inp.depth:  # :97-100

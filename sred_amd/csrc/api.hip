@@ -1306,6 +1306,56 @@ int sred_gat_tree_2d_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, doub
     return gat_tree_dev(ctx, d_img, n, g0, edc);
 }
 
+int sred_octree_gat_3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, int min_size, float alpha,
+                           int max_iter, float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *boxes,
+                           float *means, float *variances)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_vol || !g0_edc) {
+        set_error("null buffer");
+        return SRED_ERR_ARG;
+    }
+    return octree_gat_dev(ctx, d_vol, w, h, d, min_size, alpha, max_iter, lts_fraction, g0_edc, n_leaves, cap, boxes,
+                          means, variances);
+}
+
+int sred_octree_gat_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, int min_size, float alpha, int max_iter,
+                       float lts_fraction, double *g0_edc, int *n_leaves, int cap, int32_t *boxes, float *means,
+                       float *variances)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!vol || !g0_edc || w <= 0 || h <= 0 || d <= 0 || (int64_t)w * h * d >= (1LL << 31)) {
+        set_error("null buffer, empty volume or more than 2^31 voxels");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, vol, (int64_t)w * h * d));
+    return octree_gat_dev(ctx, ctx->img.as<float>(), w, h, d, min_size, alpha, max_iter, lts_fraction, g0_edc,
+                          n_leaves, cap, boxes, means, variances);
+}
+
+int sred_gat_tree_3d(sred_ctx *ctx, float *vol, int64_t n, float g0, float edc)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!vol || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    if (n == 0) return SRED_OK;
+    SRED_TRY(upload(ctx, ctx->img, vol, n));
+    SRED_TRY(gat_tree3d_dev(ctx, ctx->img.as<float>(), n, g0, edc));
+    return download(ctx, vol, ctx->img, n);
+}
+
+int sred_gat_tree_3d_dev(sred_ctx *ctx, float *d_vol, int64_t n, float g0, float edc)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_vol || n < 0) {
+        set_error("null buffer or negative size");
+        return SRED_ERR_ARG;
+    }
+    return gat_tree3d_dev(ctx, d_vol, n, g0, edc);
+}
+
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
                          float relevance_constant, int normalize)
 {

@@ -312,6 +312,10 @@ int gat_ref_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double ref)
 int quadtree_gat_dev(sred_ctx *ctx, const float *d_img, int w, int h, int min_size, float alpha, int max_iter,
                      float frac, double *g0_edc, int *n_leaves, int cap, int32_t *rects, float *means, float *vars);
 int gat_tree_dev(sred_ctx *ctx, float *d_img, int64_t n, double g0, double edc);
+// OcTree_'s estimate of a w x h x d volume (boxes: 6 ints per leaf) and its transform.
+int octree_gat_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, int min_size, float alpha, int max_iter,
+                   float frac, double *g0_edc, int *n_leaves, int cap, int32_t *boxes, float *means, float *vars);
+int gat_tree3d_dev(sred_ctx *ctx, float *d_vol, int64_t n, float g0, float edc);
 double f_quantile(double d1, double d2, double p);
 // nseg min-max normalisations (v - min) / (max - min + EPSILON) of
 // consecutive segments of seg floats (stack.hip); d_in may equal d_out.

@@ -112,6 +112,10 @@ int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_s
 {
     if (Nv <= 0) return SRED_OK;
     const double tmax = (double)sigma_max * tscale;
+    if (tmax != tmax) {
+        set_error("non-finite block statistics (NaN or infinite values in the image)");
+        return SRED_ERR_NONFINITE;
+    }
     if (!(tmax < (double)(kMaxBins - 2) * kBinW)) {
         set_error("block standard deviations span too wide a range of weights (normalise the image)");
         return SRED_ERR_ARG;

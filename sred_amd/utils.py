@@ -293,15 +293,103 @@ def get_reference_block_2d(block):
     return ReferenceBlock2D(px, bw, bh, bw // 2, bh // 2, k, m.value, s.value)
 
 
-def get_input_image_3d(volume, stabilise_noise_variance=False, gat_method="Simplex", normalize_output=True):
+class OcTree:
+    """OcTree_ (OcTree_.java:136-486): the octree estimate of the GAT
+    parameters of a volume.  The device computes the tree, the robust means
+    and the LTS variances in one call (sred_octree_gat_3d) when the results
+    are first needed; perform_linear_regression returns float values, as
+    the reference's does."""
+
+    def __init__(self, image_width, image_height, image_depth, min_size, alpha, resources=None):
+        self.image_width, self.image_height, self.image_depth = int(image_width), int(image_height), int(image_depth)
+        self.min_size, self.alpha = int(min_size), float(np.float32(alpha))
+        self.max_iterations, self.var_alpha = 50, float(np.float32(0.75))
+        self._voxels = None
+        self._res = resources
+        self._out = None
+
+    def build_tree(self, image_data):
+        self._voxels = L.f32(image_data).reshape(-1)
+        if self._voxels.size != self.image_width * self.image_height * self.image_depth:
+            raise ValueError("volume size does not match the tree")
+        self._out = None
+
+    def calculate_robust_means(self, image_data, max_iterations):
+        self.max_iterations = int(max_iterations)
+        self._out = None
+
+    def calculate_lts_variances(self, image_data, alpha):
+        self.var_alpha = float(np.float32(alpha))
+        self._out = None
+
+    def _run(self):
+        if self._out is not None:
+            return self._out
+        if self._voxels is None:
+            raise RuntimeError("build_tree first")
+        w, h, d = self.image_width, self.image_height, self.image_depth
+        cap = max(16, (w * h * d) // 2)
+        boxes = np.zeros((cap, 6), np.int32)
+        means = np.zeros(cap, np.float32)
+        var = np.zeros(cap, np.float32)
+        ge = (ctypes.c_double * 2)()
+        n = ctypes.c_int(0)
+        own = self._res is None
+        res = L.Context(0) if own else self._res
+        try:
+            rc = L.load().sred_octree_gat_3d(res.handle, L.fptr(self._voxels), w, h, d, self.min_size, self.alpha,
+                                             self.max_iterations, self.var_alpha, ge, ctypes.byref(n), cap,
+                                             boxes.ctypes.data, means.ctypes.data, var.ctypes.data)
+        finally:
+            if own:
+                res.close()
+        k = n.value
+        if rc < 0 and not (rc == L.SRED_ERR_ARG and 0 < k < 3):
+            L.check(rc)  # raises
+        self._out = (boxes[:k].copy(), means[:k].copy(), var[:k].copy())
+        return self._out
+
+    def leaves(self):
+        """Leaf boxes (x, y, z, w, h, d) in pre-order."""
+        return self._run()[0]
+
+    def collect_mean_variance_pairs(self):
+        """[[robust mean, LTS variance], ...] over the leaves in pre-order (as doubles)."""
+        _, m, v = self._run()
+        return [[float(a), float(b)] for a, b in zip(m, v)]
+
+    def perform_linear_regression(self, pairs):
+        """[g0, eDC] as float values (OcTree_.java:458-469); ValueError below 3 pairs."""
+        g0, edc = QuadTree.perform_linear_regression(self, pairs)
+        return [float(np.float32(g0)), float(np.float32(edc))]
+
+    def apply_gat_tree(self, image_data, n_pixels, g0, edc):
+        """OcTree_.applyGATtree: a new float32 array of the first n_pixels voxels."""
+        out = L.f32(image_data).reshape(-1)[:n_pixels].copy()
+        own = self._res is None
+        res = L.Context(0) if own else self._res
+        try:
+            L.check(L.load().sred_gat_tree_3d(res.handle, L.fptr(out), out.size, float(g0), float(edc)))
+        finally:
+            if own:
+                res.close()
+        return out
+
+
+def get_input_image_3d(volume, stabilise_noise_variance=False, gat_method="Simplex", normalize_output=True,
+                       resources=None):
     """Utils.getInputImage3D (Utils.java:1167-1263) for a (depth, H, W) array.
 
     Returns None below 3 slices (:1192-1196).  The "Simplex" stabilisation
     leaves the volume unchanged: the reference writes the transformed
     slices into a stack array that it then discards (:1227-1229), so only
-    its (unused) fitted parameters differ; "Quad/Octree" (OcTree_) is not
-    built.  Then min-max normalises the whole volume as
-    (v-min)/(max-min+EPSILON) (Utils.normalizeArray, :1257-1260).
+    its (unused) fitted parameters differ.  "Quad/Octree" estimates (g0, eDC)
+    with OcTree_(w, h, d, 4, 0.01f), 50 robust-mean iterations and LTS
+    fraction 0.75f and applies applyGATtree to every voxel (the reference
+    passes width*height, :1253, and the one-slice array it gets back
+    overruns the volume buffers downstream).  Then min-max normalises the
+    whole volume as (v-min)/(max-min+EPSILON) (Utils.normalizeArray,
+    :1257-1260).
     """
     arr = L.f32(np.array(volume, dtype=np.float32, copy=True))
     if arr.ndim != 3:
@@ -310,7 +398,12 @@ def get_input_image_3d(volume, stabilise_noise_variance=False, gat_method="Simpl
     if d < 3:
         return None
     if stabilise_noise_variance and gat_method == "Quad/Octree":
-        raise NotImplementedError("the octree GAT estimate of volumes (OcTree_) is not built")
+        oc = OcTree(w, h, d, 4, 0.01, resources)
+        oc.build_tree(arr)
+        oc.calculate_robust_means(arr, 50)
+        oc.calculate_lts_variances(arr, 0.75)
+        g0, edc = oc.perform_linear_regression(oc.collect_mean_variance_pairs())
+        arr = oc.apply_gat_tree(arr, w * h * d, g0, edc).reshape(d, h, w)
     if normalize_output:
         L.check(L.load().sred_normalize_input_2d(L.fptr(arr), arr.size))
     return InputImage3D(arr, w, h, d, w * h * d)
