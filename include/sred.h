@@ -348,6 +348,27 @@ int sred_octree_gat_3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int 
 int sred_gat_tree_3d(sred_ctx *ctx, float *vol, int64_t n, float g0, float edc);
 int sred_gat_tree_3d_dev(sred_ctx *ctx, float *d_vol, int64_t n, float g0, float edc);
 
+/* GATMinimizer3D.userFunction (GATMinimizer3D.java:79-157) at (gain, sigma,
+ * offset) on a w x h x d volume: NaN for gain <= 0, sigma < 0 or
+ * offset < 0; else the transformed volume (0 where v <= -ref/gain) in
+ * blocks of max(w/6,1) x max(h/6,1) x max(d/6,1) voxels, halved along axes
+ * with fewer than 2 blocks (:108-127), per block the variance from double
+ * sums of v and v*v, and the mean of (var - 1)^2.  The sums run in a fixed
+ * order (as sred_gat_objective_2d's).  Errors: a volume for which the
+ * reference's block search never ends (a 1-voxel axis while another side
+ * stays above 1), more than 4096 blocks. */
+int sred_gat_objective_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, double gain, double sigma,
+                          double offset, double *error);
+/* GATMinimizer3D.run (:40-70): the simplex of sred_gat_minimize_2d (steps
+ * 1, 10, 100; NaN ranks worst) over the 3D objective.  The
+ * VarianceStabilisingTransform3D_ plugin (VarianceStabilisingTransform3D_
+ * .java:70-80) starts it at (gain, sigma, offset) = (1, 100, 10) and applies
+ * sred_gat_2d with the fitted doubles to all w*h*d voxels. */
+int sred_gat_minimize_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, double gain0, double sigma0,
+                         double offset0, int max_iter, double *params, double *error);
+int sred_gat_minimize_3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain0, double sigma0,
+                             double offset0, int max_iter, double *params, double *error);
+
 /* ---- relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_,
  *      CalculateLocalMeans_) ---- */
 /* The tools' shared input step, in place (RelevanceMask2D_.java:133-164,

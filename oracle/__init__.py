@@ -124,6 +124,12 @@ def lib():
         L.or_octree_gat.restype = c_int
         L.or_gat_tree3d.argtypes = [_F, c_i64, c_float, c_float]
         L.or_gat_tree3d.restype = None
+        L.or_gat_blocks3d.argtypes = [c_int, c_int, c_int] + [ctypes.POINTER(c_int)] * 6
+        L.or_gat_blocks3d.restype = c_int
+        L.or_gat_objective3d.argtypes = [_F, c_int, c_int, c_int, c_double, c_double, c_double]
+        L.or_gat_objective3d.restype = c_double
+        L.or_gat_minimize3d.argtypes = [_F, c_int, c_int, c_int, c_double, c_double, c_double, c_int, _D]
+        L.or_gat_minimize3d.restype = c_double
         L.or_oc_robust_mean.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int]
         L.or_oc_robust_mean.restype = c_float
         L.or_oc_lts_variance.argtypes = [_F, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_float,
@@ -746,3 +752,29 @@ def gat_tree3d(vol, g0, edc):
     out = _f32(vol).copy()
     lib().or_gat_tree3d(out.reshape(-1), out.size, float(g0), float(edc))
     return out
+
+
+def gat_blocks3d(w, h, d):
+    """GATMinimizer3D's block grid: (bw, bh, bd, nbx, nby, nbz)."""
+    v = [ctypes.c_int() for _ in range(6)]
+    if lib().or_gat_blocks3d(w, h, d, *[ctypes.byref(x) for x in v]):
+        raise ValueError("the reference's block search does not terminate for this volume")
+    return tuple(x.value for x in v)
+
+
+def gat_objective3d(vol, gain, sigma, offset):
+    """GATMinimizer3D.userFunction (device sum order)."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    return lib().or_gat_objective3d(vol.reshape(-1), w, h, d, float(gain), float(sigma), float(offset))
+
+
+def gat_minimize3d(vol, gain0=1.0, sigma0=100.0, offset0=10.0, max_iter=5000):
+    """The simplex over the 3D objective: ((gain, sigma, offset), error)."""
+    vol = _f32(vol)
+    d, h, w = vol.shape
+    gat_blocks3d(w, h, d)  # raises where the reference never ends
+    p = np.zeros(3)
+    err = lib().or_gat_minimize3d(vol.reshape(-1), w, h, d, float(gain0), float(sigma0), float(offset0),
+                                  int(max_iter), p)
+    return tuple(p), err

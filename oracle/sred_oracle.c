@@ -2313,3 +2313,92 @@ void or_gat_tree3d(float *px, int64_t n, float g0, float edc)
         px[i] = (float)v;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Volumes: GATMinimizer3D                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* GATMinimizer3D's block grid (GATMinimizer3D.java:108-127): sides
+ * max(w/6, 1), max(h/6, 1), max(d/6, 1), halved (not below 1) along each
+ * axis with fewer than 2 blocks until every axis has 2 or all sides are 1.
+ * Returns 0, or -1 where the reference's loop never ends: an axis with one
+ * block at side 1 (a 1-voxel axis) while another side is above 1. */
+int or_gat_blocks3d(int w, int h, int d, int *bw, int *bh, int *bd, int *nbx, int *nby, int *nbz)
+{
+    int a = w / 6 > 1 ? w / 6 : 1, b = h / 6 > 1 ? h / 6 : 1, c = d / 6 > 1 ? d / 6 : 1;
+    int nx = w / a, ny = h / b, nz = d / c;
+    while (nx < 2 || ny < 2 || nz < 2) {
+        const int a0 = a, b0 = b, c0 = c;
+        if (nx < 2 && a > 1) a = a / 2 > 1 ? a / 2 : 1;
+        if (ny < 2 && b > 1) b = b / 2 > 1 ? b / 2 : 1;
+        if (nz < 2 && c > 1) c = c / 2 > 1 ? c / 2 : 1;
+        nx = w / a, ny = h / b, nz = d / c;
+        if (a == 1 && b == 1 && c == 1) break;
+        if (a == a0 && b == b0 && c == c0) return -1;
+    }
+    *bw = a, *bh = b, *bd = c, *nbx = nx, *nby = ny, *nbz = nz;
+    return 0;
+}
+
+/* GATMinimizer3D.userFunction (:79-157): NaN for gain <= 0, sigma < 0 or
+ * offset < 0; else the volume transformed (GATMinimizer2D.applyGAT: 0 where
+ * v <= -ref/gain), per block Utils.getMeanAndVarBlock3D (double v, double
+ * v*v; var = sq/n - mean^2) and the mean of (var - 1)^2 over the blocks.
+ * The sums run in the device's fixed order, as or_gat_objective's: a row's
+ * element i to lane i mod 64, a block's row r (z, then y) to lane r mod 64,
+ * block b (z, y, x order) to lane b mod 64, each combined by lane_tree64. */
+double or_gat_objective3d(const float *px, int w, int h, int d, double gain, double sigma, double offset)
+{
+    if (gain <= 0.0 || sigma < 0.0 || offset < 0.0) return NAN;
+    int bw, bh, bd, nbx, nby, nbz;
+    if (or_gat_blocks3d(w, h, d, &bw, &bh, &bd, &nbx, &nby, &nbz)) return NAN;
+    const int nb = nbx * nby * nbz;
+    const int64_t plane = (int64_t)w * h;
+    double ref = (3.0 / 8.0) * gain * gain + sigma * sigma - gain * offset;
+    double be[64], bm[64], bs[64], lm[64], ls[64];
+    for (int l = 0; l < 64; l++) be[l] = 0.0;
+    for (int bz = 0; bz < nbz; bz++)
+        for (int by = 0; by < nby; by++)
+            for (int bx = 0; bx < nbx; bx++) {
+                for (int l = 0; l < 64; l++) bm[l] = bs[l] = 0.0;
+                for (int r = 0; r < bh * bd; r++) {
+                    const int zz = r / bh, yy = r - zz * bh;
+                    const float *row = px + (bz * bd + zz) * plane + (int64_t)(by * bh + yy) * w + (int64_t)bx * bw;
+                    for (int l = 0; l < 64; l++) lm[l] = ls[l] = 0.0;
+                    for (int i = 0; i < bw; i++) {
+                        double v = (double)gat_fit_px(row[i], gain, ref);
+                        lm[i % 64] += v;
+                        ls[i % 64] += v * v;
+                    }
+                    bm[r % 64] += lane_tree64(lm);
+                    bs[r % 64] += lane_tree64(ls);
+                }
+                double mean = lane_tree64(bm), sq = lane_tree64(bs);
+                int64_t n = (int64_t)bw * bh * bd;
+                mean = mean / (double)n;
+                double var = sq / (double)n - mean * mean;
+                double delta = var - 1.0;
+                int b = (bz * nby + by) * nbx + bx;
+                be[b % 64] += (delta * delta) / (double)nb;
+            }
+    return lane_tree64(be);
+}
+
+typedef struct { const float *px; int w, h, d; } gat3_ctx;
+static double gat3_obj(const double *x, void *c)
+{
+    const gat3_ctx *g = (const gat3_ctx *)c;
+    return or_gat_objective3d(g->px, g->w, g->h, g->d, x[0], x[1], x[2]);
+}
+
+/* GATMinimizer3D.run (:40-70): the same simplex as the 2D fit (steps 1, 10,
+ * 100) over the 3D objective; NaN ranks worst. */
+double or_gat_minimize3d(const float *px, int w, int h, int d, double gain0, double sigma0, double offset0,
+                         int max_iter, double *params)
+{
+    int g[6];
+    if (or_gat_blocks3d(w, h, d, g, g + 1, g + 2, g + 3, g + 4, g + 5)) return NAN;  /* the reference never ends */
+    gat3_ctx c = {px, w, h, d};
+    double x0[3] = {gain0, sigma0, offset0}, dd[3] = {1.0, 10.0, 100.0};
+    return or_nelder_mead3(gat3_obj, &c, x0, dd, max_iter, params);
+}

@@ -222,6 +222,11 @@ float or_oc_lts_variance(const float *img, int w, int h, int x0, int y0, int z0,
 int or_octree_gat(const float *img, int w, int h, int d, int min_size, float alpha, int max_iter, float frac,
                   int cap, int32_t *boxes, float *means, float *vars, int *nleaves, double *g0, double *edc);
 void or_gat_tree3d(float *px, int64_t n, float g0, float edc);
+/* GATMinimizer3D: block grid, objective (device sum order) and the simplex. */
+int or_gat_blocks3d(int w, int h, int d, int *bw, int *bh, int *bd, int *nbx, int *nby, int *nbz);
+double or_gat_objective3d(const float *px, int w, int h, int d, double gain, double sigma, double offset);
+double or_gat_minimize3d(const float *px, int w, int h, int d, double gain0, double sigma0, double offset0,
+                         int max_iter, double *params);
 
 /* Relevance-mask tools (RelevanceMask2D_, OptimiseRelevanceMask_, the
  * intent of CalculateLocalMeans_); definitions in sred_oracle.c. */

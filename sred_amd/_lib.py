@@ -44,6 +44,7 @@ EXPORTS = [
     "sred_local_stats_3d", "sred_global_repetition_3d", "sred_global_repetition_3d_dev",
     "sred_reference_block_3d", "sred_block_repetition_3d", "sred_block_repetition_3d_dev",
     "sred_octree_gat_3d", "sred_octree_gat_3d_dev", "sred_gat_tree_3d", "sred_gat_tree_3d_dev",
+    "sred_gat_objective_3d", "sred_gat_minimize_3d", "sred_gat_minimize_3d_dev",
 ]
 
 
@@ -136,6 +137,9 @@ def load():
         "sred_octree_gat_3d": ([_VP, _F, _I, _I, _I, _I, _FL, _I, _FL, _VP, _VP, _I, _VP, _VP, _VP], _I),
         "sred_octree_gat_3d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _I, _FL, _VP, _VP, _I, _VP, _VP, _VP], _I),
         "sred_gat_tree_3d": ([_VP, _F, _I64, _FL, _FL], _I),
+        "sred_gat_objective_3d": ([_VP, _F, _I, _I, _I, _D, _D, _D, _VP], _I),
+        "sred_gat_minimize_3d": ([_VP, _F, _I, _I, _I, _D, _D, _D, _I, _VP, _VP], _I),
+        "sred_gat_minimize_3d_dev": ([_VP, _VP, _I, _I, _I, _D, _D, _D, _I, _VP, _VP], _I),
         "sred_gat_tree_3d_dev": ([_VP, _VP, _I64, _FL, _FL], _I),
         "sred_block_repetition_3d": ([_VP, _I, _F, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _F], _I),
         "sred_block_repetition_3d_dev": ([_VP, _I, _VP, _I, _I, _I, _F, _I, _I, _I, _FL, _FL, _FL, _I, _VP, _VP,

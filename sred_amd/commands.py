@@ -166,3 +166,25 @@ def block_repetition_3d(block, volume, relevance_constant=0.0, metric="Pearson's
     finally:
         if own:
             res.close()
+
+
+def variance_stabilising_transform_3d(volume, gain=1.0, offset=10.0, sigma=100.0, max_iter=5000, resources=None):
+    """The VarianceStabilisingTransform3D_ plugin (VarianceStabilisingTransform3D_
+    .java:28-110) on a (depth, H, W) volume: GATMinimizer3D from the dialog's
+    initial (gain, offset, noise std) and the GAT of getGAT with the fitted
+    (double) parameters on every voxel.  Returns (float32 volume, (gain,
+    sigma, offset))."""
+    vol = L.f32(volume)
+    if vol.ndim != 3 or vol.shape[0] < 2:  # :47-50
+        raise ValueError("Image is not 3D. Please open a 3D image and try again.")
+    d, h, w = vol.shape
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        m = utils.GATMinimizer3D(vol, w, h, d, gain, sigma, offset, max_iter, res)
+        m.run()
+        out = utils.get_gat_2d(vol, m.gain, m.sigma, m.offset, res).reshape(d, h, w)
+        return out, (m.gain, m.sigma, m.offset)
+    finally:
+        if own:
+            res.close()

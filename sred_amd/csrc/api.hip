@@ -963,6 +963,41 @@ int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, do
     return gat_minimize_dev(ctx, d_img, w, h, gain0, sigma0, offset0, max_iter, params, error);
 }
 
+int sred_gat_objective_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, double gain, double sigma,
+                          double offset, double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!vol || !error || w <= 0 || h <= 0 || d <= 0 || (int64_t)w * h * d >= (1LL << 31)) {
+        set_error("null buffer, empty volume or more than 2^31 voxels");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, vol, (int64_t)w * h * d));
+    return gat_objective3d_dev(ctx, ctx->img.as<float>(), w, h, d, gain, sigma, offset, error);
+}
+
+int sred_gat_minimize_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, double gain0, double sigma0,
+                         double offset0, int max_iter, double *params, double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!vol || !params || w <= 0 || h <= 0 || d <= 0 || (int64_t)w * h * d >= (1LL << 31) || max_iter < 0) {
+        set_error("null buffer, empty volume, more than 2^31 voxels or negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(upload(ctx, ctx->img, vol, (int64_t)w * h * d));
+    return gat_minimize3d_dev(ctx, ctx->img.as<float>(), w, h, d, gain0, sigma0, offset0, max_iter, params, error);
+}
+
+int sred_gat_minimize_3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain0, double sigma0,
+                             double offset0, int max_iter, double *params, double *error)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_vol || !params || w <= 0 || h <= 0 || d <= 0 || (int64_t)w * h * d >= (1LL << 31) || max_iter < 0) {
+        set_error("null buffer, empty volume, more than 2^31 voxels or negative iteration count");
+        return SRED_ERR_ARG;
+    }
+    return gat_minimize3d_dev(ctx, d_vol, w, h, d, gain0, sigma0, offset0, max_iter, params, error);
+}
+
 int sred_block_timelapse_2d_dev(sred_ctx *ctx, int metric, const float *d_stack, int w, int h, int depth,
                                 const float *tmpl, int bw, int bh, float tmpl_mean, float tmpl_std,
                                 float relevance_constant, int normalize, float *d_out)

@@ -305,6 +305,11 @@ int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double ga
                       double *error);
 int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0, double offset0,
                      int max_iter, double *params, double *error);
+// GATMinimizer3D's objective and fit on a w x h x d volume.
+int gat_objective3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain, double sigma,
+                        double offset, double *error);
+int gat_minimize3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain0, double sigma0,
+                       double offset0, int max_iter, double *params, double *error);
 // The GAT kernel with ref given directly: v <- (g v + ref < 0) ? 0 : (2/g) sqrt(g v + ref).
 int gat_ref_dev(sred_ctx *ctx, float *d_img, int64_t n, double gain, double ref);
 // Quadtree GAT estimate (quadtree.hip): leaves, robust means, LTS variances

@@ -33,6 +33,13 @@
 //     Each iteration evaluates the reflection, expansion and both
 //     contractions in one launch (four parameter sets) and then takes the
 //     branch the values select, so the path is the sequential algorithm's.
+//   * Volumes (GATMinimizer3D.java:79-157, run by the
+//     VarianceStabilisingTransform3D_ plugin): the same kernels on rows
+//     addressed by (slice, row), blocks of bw x bh x bd voxels whose rows are
+//     taken slice by slice, squares in double (Utils.getMeanAndVarBlock3D),
+//     NaN for gain <= 0, sigma < 0 or offset < 0, and the 3D block grid
+//     (sides max(n/6, 1), halved along axes with fewer than 2 blocks).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -42,6 +49,7 @@ namespace sred {
 
 constexpr int kGatSets = 4;        // parameter sets per objective launch
 constexpr int kGatErrThreads = 1024;  // k_gat_error workgroup (16 waves)
+constexpr int kGatMaxBlocks = 4096;   // blocks of the objective (LDS table of block terms)
 
 struct GatSets {
     double gain[kGatSets], ref[kGatSets];
@@ -71,9 +79,13 @@ __device__ __forceinline__ double lane_tree64(double a)
 }
 
 // rows[set][j][bx] = (sum v, sum v*v) over row j of block column bx; one
-// wave per (j, bx) evaluates every parameter set of the launch
+// wave per (j, bx) evaluates every parameter set of the launch.  Row j is
+// row j mod rps of slice j / rps (plane voxels apart); DSQ squares in double
+// (the volume objective), else in float (the image objective).
+template <bool DSQ>
 __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, int w, int bw, int nbx, int nrows,
-                                                  int nsets, GatSets ps, double2 *__restrict__ rows)
+                                                  int rps, int64_t plane, int nsets, GatSets ps,
+                                                  double2 *__restrict__ rows)
 {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
@@ -81,6 +93,7 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
     const int per_set = nrows * nbx;
     if (t >= per_set) return;
     const int j = t / nbx, bx = t - j * nbx;
+    const int z = j / rps, y = j - z * rps;
     double lim[kGatSets], two_g[kGatSets], rm[kGatSets], rs[kGatSets];
 #pragma unroll
     for (int s = 0; s < kGatSets; s++) {
@@ -88,7 +101,7 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
         two_g[s] = 2.0 / ps.gain[s];
         rm[s] = rs[s] = 0.0;
     }
-    const float *p = px + (int64_t)j * w + (int64_t)bx * bw;
+    const float *p = px + z * plane + (int64_t)y * w + (int64_t)bx * bw;
     for (int i = lane; i < bw; i += 64) {
         const double d = (double)p[i];
 #pragma unroll
@@ -96,7 +109,8 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
             if (s < nsets) {
                 const float v = d <= lim[s] ? 0.0f : (float)(two_g[s] * sqrt(ps.gain[s] * d + ps.ref[s]));
                 rm[s] += v;
-                rs[s] += v * v;
+                if (DSQ) rs[s] += (double)v * (double)v;
+                else rs[s] += v * v;
             }
         }
     }
@@ -110,25 +124,29 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
 }
 
 // err[set] from the row sums of the set (one workgroup per set): each wave
-// takes blocks wave, wave + 16, ...; the block terms are combined by wave 0.
+// takes blocks wave, wave + 16, ... (z, y, x order; a block's bh * bd rows
+// slice by slice, row k to lane k mod 64); the block terms (block b to lane
+// b mod 64) are combined by wave 0.
 __global__ __launch_bounds__(kGatErrThreads) void k_gat_error(const double2 *__restrict__ rows, int bw, int bh,
-                                                              int nbx, int nby, int nrows, double *__restrict__ err)
+                                                              int bd, int nbx, int nby, int nbz, int nrows, int rps,
+                                                              double *__restrict__ err)
 {
 #pragma clang fp contract(off)
-    __shared__ double be[128];
-    const int set = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nb = nbx * nby;
+    __shared__ double be[kGatMaxBlocks];
+    const int set = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nb = nbx * nby * nbz;
     const double2 *r = rows + (size_t)set * nrows * nbx;
     for (int b = wave; b < nb; b += kGatErrThreads / 64) {
-        const int by = b / nbx, bx = b - by * nbx;
+        const int bz = b / (nbx * nby), rem = b - bz * (nbx * nby), by = rem / nbx, bx = rem - by * nbx;
         double mean = 0.0, sq = 0.0;
-        for (int k = lane; k < bh; k += 64) {
-            const double2 q = r[(size_t)(by * bh + k) * nbx + bx];
+        for (int k = lane; k < bh * bd; k += 64) {
+            const int kz = k / bh, ky = k - kz * bh;
+            const double2 q = r[((size_t)(bz * bd + kz) * rps + by * bh + ky) * nbx + bx];
             mean += q.x;
             sq += q.y;
         }
         mean = lane_tree64(mean);
         sq = lane_tree64(sq);
-        const int n = bw * bh;
+        const double n = (double)((int64_t)bw * bh * bd);
         mean = mean / n;
         const double var = sq / n - mean * mean;
         const double delta = var - 1.0;
@@ -137,8 +155,7 @@ __global__ __launch_bounds__(kGatErrThreads) void k_gat_error(const double2 *__r
     __syncthreads();
     if (wave == 0) {
         double e = 0.0;
-        if (lane < nb) e += be[lane];
-        if (lane + 64 < nb) e += be[lane + 64];
+        for (int q = lane; q < nb; q += 64) e += be[q];
         e = lane_tree64(e);
         if (lane == 0) err[set] = e;
     }
@@ -171,6 +188,9 @@ struct GatEval {
     sred_ctx *ctx;
     const float *d_img;
     int w, h, bw, bh, nbx, nby, nrows;
+    int bd = 1, nbz = 1, rps = 0;  // volumes: block depth, block slices, rows per slice
+    int64_t plane = 0;             // voxels per slice (0 for an image)
+    bool volume = false;           // double squares, NaN for invalid parameters
     double2 *d_rows;
     double *h_err;  // pinned host memory written by k_gat_error
     int64_t launches = 0, pixel_evals = 0;
@@ -186,14 +206,21 @@ struct GatEval {
         }
         const int waves = nrows * nbx;
         if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL(k_gat_rows, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx, nrows,
-                           nsets, ps, d_rows);
+        if (volume)
+            hipLaunchKernelGGL(k_gat_rows<true>, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
+                               nrows, rps, plane, nsets, ps, d_rows);
+        else
+            hipLaunchKernelGGL(k_gat_rows<false>, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
+                               nrows, rps, plane, nsets, ps, d_rows);
         if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
-        hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw, bh, nbx, nby,
-                           nrows, h_err);
+        hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw, bh, bd, nbx,
+                           nby, nbz, nrows, rps, h_err);
         SRED_HIP_CHECK(hipGetLastError());
         SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (int s = 0; s < nsets; s++) f[s] = h_err[s];
+        if (volume)  // GATMinimizer3D.java:84-86
+            for (int s = 0; s < nsets; s++)
+                if (x[s][0] <= 0.0 || x[s][1] < 0.0 || x[s][2] < 0.0) f[s] = NAN;
         if (rank_nan_worst)  // invalid parameters (NaN objective) rank worst, as ImageJ's Minimizer treats NaN
             for (int s = 0; s < nsets; s++)
                 if (f[s] != f[s]) f[s] = INFINITY;
@@ -224,7 +251,8 @@ int gat_setup(sred_ctx *ctx, const float *d_img, int w, int h, GatEval *e)
     e->nbx = w / bw;
     e->nby = h / bh;
     e->nrows = e->nby * bh;
-    if (e->nbx * e->nby > 128) {
+    e->rps = e->nrows;
+    if (e->nbx * e->nby > kGatMaxBlocks) {
         set_error("unsupported GAT block grid");
         return SRED_ERR_ARG;
     }
@@ -236,7 +264,77 @@ int gat_setup(sred_ctx *ctx, const float *d_img, int w, int h, GatEval *e)
     return SRED_OK;
 }
 
+// The block grid of GATMinimizer3D.userFunction (GATMinimizer3D.java:108-127).
+int gat_setup3d(sred_ctx *ctx, const float *d_vol, int w, int h, int d, GatEval *e)
+{
+    int a = std::max(w / 6, 1), b = std::max(h / 6, 1), c = std::max(d / 6, 1);
+    int nx = w / a, ny = h / b, nz = d / c;
+    while (nx < 2 || ny < 2 || nz < 2) {
+        const int a0 = a, b0 = b, c0 = c;
+        if (nx < 2 && a > 1) a = std::max(a / 2, 1);
+        if (ny < 2 && b > 1) b = std::max(b / 2, 1);
+        if (nz < 2 && c > 1) c = std::max(c / 2, 1);
+        nx = w / a, ny = h / b, nz = d / c;
+        if (a == 1 && b == 1 && c == 1) break;
+        if (a == a0 && b == b0 && c == c0) {
+            set_error("GATMinimizer3D's block search does not terminate for this volume (a 1-voxel axis while "
+                      "another block side is above 1)");
+            return SRED_ERR_ARG;
+        }
+    }
+    if ((int64_t)nx * ny * nz > kGatMaxBlocks) {
+        set_error("unsupported GAT block grid (more than 4096 blocks)");
+        return SRED_ERR_ARG;
+    }
+    e->ctx = ctx;
+    e->d_img = d_vol;
+    e->w = w;
+    e->h = h;
+    e->bw = a;
+    e->bh = b;
+    e->bd = c;
+    e->nbx = nx;
+    e->nby = ny;
+    e->nbz = nz;
+    e->rps = ny * b;
+    e->nrows = nz * c * e->rps;
+    e->plane = (int64_t)w * h;
+    e->volume = true;
+    SRED_TRY(ctx->pm_part.ensure(sizeof(double2) * (size_t)kGatSets * e->nrows * e->nbx + 256));
+    e->d_rows = ctx->pm_part.as<double2>();
+    if (!ctx->h_fit) SRED_HIP_CHECK(hipHostMalloc((void **)&ctx->h_fit, sizeof(double) * kGatSets, hipHostMallocDefault));
+    e->h_err = ctx->h_fit;
+    SRED_TRY(time_begin(ctx));
+    return SRED_OK;
+}
+
+// Nelder-Mead as the CPU restatement (same expressions, no contraction), with
+// the candidate points of an iteration evaluated together.
+int gat_minimize_run(sred_ctx *ctx, GatEval &e, double gain0, double sigma0, double offset0, int max_iter,
+                     double *params, double *error);
+
 }  // namespace
+
+int gat_objective3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain, double sigma,
+                        double offset, double *error)
+{
+    GatEval e;
+    SRED_TRY(gat_setup3d(ctx, d_vol, w, h, d, &e));
+    const double x[1][3] = {{gain, sigma, offset}};
+    SRED_TRY(e.run(x, 1, error));
+    ctx->last_launches = 1;
+    ctx->last_pairs = e.pixel_evals;
+    ctx->summed_ms = (float)e.rows_ms;
+    return SRED_OK;
+}
+
+int gat_minimize3d_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, double gain0, double sigma0,
+                       double offset0, int max_iter, double *params, double *error)
+{
+    GatEval e;
+    SRED_TRY(gat_setup3d(ctx, d_vol, w, h, d, &e));
+    return gat_minimize_run(ctx, e, gain0, sigma0, offset0, max_iter, params, error);
+}
 
 int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain, double sigma, double offset,
                       double *error)
@@ -251,14 +349,20 @@ int gat_objective_dev(sred_ctx *ctx, const float *d_img, int w, int h, double ga
     return SRED_OK;
 }
 
-// Nelder-Mead as the CPU restatement (same expressions, no contraction), with
-// the candidate points of an iteration evaluated together.
 int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gain0, double sigma0, double offset0,
                      int max_iter, double *params, double *error)
 {
-#pragma clang fp contract(off)
     GatEval e;
     SRED_TRY(gat_setup(ctx, d_img, w, h, &e));
+    return gat_minimize_run(ctx, e, gain0, sigma0, offset0, max_iter, params, error);
+}
+
+namespace {
+
+int gat_minimize_run(sred_ctx *ctx, GatEval &e, double gain0, double sigma0, double offset0, int max_iter,
+                     double *params, double *error)
+{
+#pragma clang fp contract(off)
     e.rank_nan_worst = true;
     const double d[3] = {1.0, 10.0, 100.0};  // GATMinimizer2D.java:43-45
     double v[4][3], fv[4];
@@ -314,5 +418,7 @@ int gat_minimize_dev(sred_ctx *ctx, const float *d_img, int w, int h, double gai
     ctx->summed_ms = (float)e.rows_ms;
     return SRED_OK;
 }
+
+}  // namespace
 
 }  // namespace sred

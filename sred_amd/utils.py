@@ -146,6 +146,41 @@ class GATMinimizer2D:
         return err.value
 
 
+class GATMinimizer3D(GATMinimizer2D):
+    """GATMinimizer3D (GATMinimizer3D.java:19-158): the same fit over the
+    volume's blocks (sides max(n/6, 1), halved along axes with fewer than 2
+    blocks), squares in double, NaN for gain <= 0, sigma < 0 or offset < 0.
+    run() uses the library's Nelder-Mead simplex in place of ImageJ's
+    Minimizer."""
+
+    def __init__(self, pixels, width, height, depth, gain, sigma, offset, max_iter, resources=None):
+        super().__init__(pixels, width, height, gain, sigma, offset, max_iter, resources)
+        self.depth = int(depth)
+
+    def run(self):
+        params = (ctypes.c_double * 3)()
+        err = ctypes.c_double()
+
+        def go(res):
+            L.check(L.load().sred_gat_minimize_3d(res.handle, L.fptr(self.pixels), self.width, self.height,
+                                                  self.depth, self.gain, self.sigma, self.offset, self.max_iter,
+                                                  params, ctypes.byref(err)))
+        self._with_ctx(go)
+        self.gain, self.sigma, self.offset = params[0], params[1], params[2]
+        self.error = err.value
+
+    def user_function(self, params, _v=0.0):
+        """The objective at params = (gain, sigma, offset) (GATMinimizer3D.java:79-157)."""
+        err = ctypes.c_double()
+
+        def go(res):
+            L.check(L.load().sred_gat_objective_3d(res.handle, L.fptr(self.pixels), self.width, self.height,
+                                                   self.depth, float(params[0]), float(params[1]),
+                                                   float(params[2]), ctypes.byref(err)))
+        self._with_ctx(go)
+        return err.value
+
+
 class QuadTree:
     """QuadTree_ (QuadTree_.java:119-427): the Quad/Octree estimate of the GAT
     parameters.  The device computes the tree, the robust means and the LTS
