@@ -402,6 +402,18 @@ int sred_optimise_relevance_mask_2d(sred_ctx *ctx, const float *img, int w, int 
                                     int64_t max_steps, float *all_ones, float *all_zeros);
 int sred_optimise_relevance_mask_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float step,
                                         int64_t max_steps, float *all_ones, float *all_zeros);
+/* RelevanceMaskStack2D_ after the input step (RelevanceMaskStack2D_.java:
+ * 140-285): the masks of OptimiseRelevanceMask_.getRelevanceMap (the
+ * plugin's local variances squared against noiseVar * c, c when noiseVar is
+ * 0) for the constants lower, lower + step, ... (a float sum), (int)
+ * ceil((upper - lower) / step) + 1 of them; masks receives them one image
+ * after the other, constants (may be NULL) their values, *n_masks the
+ * count (also when it exceeds cap, which is an error).  Odd square blocks
+ * only.  Synchronises. */
+int sred_relevance_mask_stack_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float lower,
+                                float upper, float step, int cap, float *masks, float *constants, int *n_masks);
+int sred_relevance_mask_stack_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float lower,
+                                    float upper, float step, int cap, float *d_masks, float *constants, int *n_masks);
 int sred_simple_regression(const double *x, const double *y, int64_t n, double *slope, double *intercept);
 
 /* ---- all-pairs vector maps (OpenCL kernels without a Java caller) ---- */

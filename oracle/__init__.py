@@ -124,6 +124,9 @@ def lib():
         L.or_octree_gat.restype = c_int
         L.or_gat_tree3d.argtypes = [_F, c_i64, c_float, c_float]
         L.or_gat_tree3d.restype = None
+        L.or_relevance_mask_stack.argtypes = [_F, c_int, c_int, c_int, c_int, c_float, c_float, c_float, c_int, _F,
+                                              _F, ctypes.POINTER(c_int)]
+        L.or_relevance_mask_stack.restype = c_int
         L.or_gat_blocks3d.argtypes = [c_int, c_int, c_int] + [ctypes.POINTER(c_int)] * 6
         L.or_gat_blocks3d.restype = c_int
         L.or_gat_objective3d.argtypes = [_F, c_int, c_int, c_int, c_double, c_double, c_double]
@@ -778,3 +781,22 @@ def gat_minimize3d(vol, gain0=1.0, sigma0=100.0, offset0=10.0, max_iter=5000):
     err = lib().or_gat_minimize3d(vol.reshape(-1), w, h, d, float(gain0), float(sigma0), float(offset0),
                                   int(max_iter), p)
     return tuple(p), err
+
+
+def relevance_mask_stack(img, bw, bh, lower, upper, step, cap=4096):
+    """RelevanceMaskStack2D_ after the input step: (masks (n, h, w), constants (n,))."""
+    img = _f32(img)
+    h, w = img.shape
+    n = ctypes.c_int()
+    rc = lib().or_relevance_mask_stack(img.reshape(-1), w, h, bw, bh, float(lower), float(upper), float(step), 0,
+                                       np.zeros(1, np.float32), np.zeros(1, np.float32), ctypes.byref(n))
+    if rc == -1:
+        raise ValueError("invalid block or constants")
+    k = n.value
+    masks = np.zeros((k, h, w), np.float32)
+    consts = np.zeros(k, np.float32)
+    rc = lib().or_relevance_mask_stack(img.reshape(-1), w, h, bw, bh, float(lower), float(upper), float(step), k,
+                                       masks.reshape(-1), consts, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError(rc)
+    return masks, consts

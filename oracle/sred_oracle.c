@@ -2402,3 +2402,46 @@ double or_gat_minimize3d(const float *px, int w, int h, int d, double gain0, dou
     double x0[3] = {gain0, sigma0, offset0}, dd[3] = {1.0, 10.0, 100.0};
     return or_nelder_mead3(gat3_obj, &c, x0, dd, max_iter, params);
 }
+
+/* RelevanceMaskStack2D_ after the tools' input step (RelevanceMaskStack2D_
+ * .java:140-285): kernelGetRelevanceMask2D.cl's local variances, then for
+ * the constants c_0 = lower, c_{i+1} = c_i + step (a float sum), n =
+ * (int) ceil((upper - lower) / step) + 1 of them (:262-263), the mask of
+ * OptimiseRelevanceMask_.getRelevanceMap (OptimiseRelevanceMask_.java:
+ * 297-380: float tile-sum noise variance, threshold nv * c or c when nv is
+ * 0, 0 where var^2 < threshold or var^2 == 0) into masks[i].  Odd square
+ * blocks only (the plugin swaps the radii).  Returns 0, -1 for another
+ * block, a step <= 0 or NaN or fewer than 1 constant, -2 if more than cap
+ * (*n holds the count either way). */
+int or_relevance_mask_stack(const float *img, int w, int h, int bw, int bh, float lower, float upper, float step,
+                            int cap, float *masks, float *constants, int *n)
+{
+    *n = 0;
+    if (bw != bh || bw % 2 == 0 || bw < 3 || !(step > 0.0f)) return -1;
+    double q = ceil((double)((upper - lower) / step));
+    if (!(q >= 0.0) || q > 1e8) return -1;
+    int count = (int)q + 1;
+    *n = count;
+    if (count > cap) return -2;
+    int64_t np = (int64_t)w * h;
+    float *var = (float *)malloc(sizeof(float) * (size_t)np);
+    or_local_var_f32(img, w, h, bw, bh, var);
+    float nv = or_noise_var(img, w, h);
+    int br = bw / 2;
+    float c = lower;
+    for (int i = 0; i < count; i++) {
+        float thr = (nv == 0.0f) ? c : nv * c;
+        float *m = masks + (size_t)i * np;
+        memset(m, 0, sizeof(float) * (size_t)np);
+        for (int y = br; y < h - br; y++)
+            for (int x = br; x < w - br; x++) {
+                float s = var[(int64_t)y * w + x];
+                float v = s * s;
+                m[(int64_t)y * w + x] = (v < thr || v == 0.0f) ? 0.0f : 1.0f;
+            }
+        constants[i] = c;
+        c += step;
+    }
+    free(var);
+    return 0;
+}

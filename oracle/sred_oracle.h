@@ -237,6 +237,8 @@ int or_relevance_mask_tool(const float *img, int w, int h, int bw, int bh, float
 void or_local_var_f32(const float *img, int w, int h, int bw, int bh, float *var);
 int or_optimise_relevance_mask(const float *img, int w, int h, int bw, int bh, float step, int max_steps,
                                float *all_ones, float *all_zeros);
+int or_relevance_mask_stack(const float *img, int w, int h, int bw, int bh, float lower, float upper, float step,
+                            int cap, float *masks, float *constants, int *n);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,7 @@ EXPORTS = [
     "sred_reference_block_3d", "sred_block_repetition_3d", "sred_block_repetition_3d_dev",
     "sred_octree_gat_3d", "sred_octree_gat_3d_dev", "sred_gat_tree_3d", "sred_gat_tree_3d_dev",
     "sred_gat_objective_3d", "sred_gat_minimize_3d", "sred_gat_minimize_3d_dev",
+    "sred_relevance_mask_stack_2d", "sred_relevance_mask_stack_2d_dev",
 ]
 
 
@@ -138,6 +139,8 @@ def load():
         "sred_octree_gat_3d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _I, _FL, _VP, _VP, _I, _VP, _VP, _VP], _I),
         "sred_gat_tree_3d": ([_VP, _F, _I64, _FL, _FL], _I),
         "sred_gat_objective_3d": ([_VP, _F, _I, _I, _I, _D, _D, _D, _VP], _I),
+        "sred_relevance_mask_stack_2d": ([_VP, _F, _I, _I, _I, _I, _FL, _FL, _FL, _I, _VP, _VP, _VP], _I),
+        "sred_relevance_mask_stack_2d_dev": ([_VP, _VP, _I, _I, _I, _I, _FL, _FL, _FL, _I, _VP, _VP, _VP], _I),
         "sred_gat_minimize_3d": ([_VP, _F, _I, _I, _I, _D, _D, _D, _I, _VP, _VP], _I),
         "sred_gat_minimize_3d_dev": ([_VP, _VP, _I, _I, _I, _D, _D, _D, _I, _VP, _VP], _I),
         "sred_gat_tree_3d_dev": ([_VP, _VP, _I64, _FL, _FL], _I),

@@ -963,6 +963,33 @@ int sred_gat_minimize_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, do
     return gat_minimize_dev(ctx, d_img, w, h, gain0, sigma0, offset0, max_iter, params, error);
 }
 
+int sred_relevance_mask_stack_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float lower,
+                                    float upper, float step, int cap, float *d_masks, float *constants, int *n_masks)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !n_masks || cap < 0 || (cap > 0 && !d_masks)) {
+        set_error("null buffer or negative capacity");
+        return SRED_ERR_ARG;
+    }
+    return relevance_mask_stack_dev(ctx, d_img, w, h, bw, bh, lower, upper, step, cap, d_masks, constants, n_masks);
+}
+
+int sred_relevance_mask_stack_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh, float lower,
+                                float upper, float step, int cap, float *masks, float *constants, int *n_masks)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!img || !n_masks || cap < 0 || (cap > 0 && !masks) || w <= 0 || h <= 0) {
+        set_error("null buffer, negative capacity or empty image");
+        return SRED_ERR_ARG;
+    }
+    const int64_t n = (int64_t)w * h;
+    SRED_TRY(upload(ctx, ctx->img, img, n));
+    SRED_TRY(ctx->stack_work.ensure(sizeof(float) * (size_t)std::max(cap, 1) * n));
+    SRED_TRY(relevance_mask_stack_dev(ctx, ctx->img.as<float>(), w, h, bw, bh, lower, upper, step, cap,
+                                      ctx->stack_work.as<float>(), constants, n_masks));
+    return download(ctx, masks, ctx->stack_work, (int64_t)*n_masks * n);
+}
+
 int sred_gat_objective_3d(sred_ctx *ctx, const float *vol, int w, int h, int d, double gain, double sigma,
                           double offset, double *error)
 {

@@ -349,6 +349,8 @@ int relevance_mask_tool_dev(sred_ctx *ctx, const float *d_img, int w, int h, int
                             float *noise_var, float *threshold);
 int optimise_relevance_mask_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float step,
                                 int64_t max_steps, float *all_ones, float *all_zeros);
+int relevance_mask_stack_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float lower,
+                             float upper, float step, int cap, float *d_masks, float *constants, int *n_masks);
 // The stack normalised as a whole, then each of its nseg frames of seg floats.
 int stack_frames_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
 void simple_regression_host(const double *x, const double *y, int64_t n, double *slope, double *intercept);

@@ -229,4 +229,44 @@ int optimise_relevance_mask_dev(sred_ctx *ctx, const float *d_img, int w, int h,
     return SRED_ERR_ARG;
 }
 
+int relevance_mask_stack_dev(sred_ctx *ctx, const float *d_img, int w, int h, int bw, int bh, float lower,
+                             float upper, float step, int cap, float *d_masks, float *constants, int *n_masks)
+{
+#pragma clang fp contract(off)
+    // RelevanceMaskStack2D_ after the input step (RelevanceMaskStack2D_.java:
+    // 140-285): the plugin's local variances (kernelGetRelevanceMask2D.cl),
+    // then one OptimiseRelevanceMask_.getRelevanceMap mask per constant
+    // lower, lower + step, ... (a float sum), (int) ceil((upper - lower) /
+    // step) + 1 of them (:262-263)
+    *n_masks = 0;
+    Geometry g;
+    SRED_TRY(check_square(w, h, bw, bh, &g));
+    const double q = std::ceil((double)((upper - lower) / step));
+    if (!(step > 0.0f) || !(q >= 0.0) || q > 1e8) {
+        set_error("the filter constants need a positive step and upper >= lower");
+        return SRED_ERR_ARG;
+    }
+    const int count = (int)q + 1;
+    *n_masks = count;
+    if (count > cap) {
+        set_error("more masks than the output holds");
+        return SRED_ERR_ARG;
+    }
+    SRED_TRY(ctx->means.ensure(sizeof(float) * g.n()));
+    SRED_TRY(ctx->stds.ensure(sizeof(float) * g.n()));
+    SRED_TRY(launch_local_stats(ctx, g, d_img, ctx->means.as<float>(), ctx->stds.as<float>(), true));
+    float nv = 0.0f;
+    SRED_TRY(noise_var_dev(ctx, d_img, w, h, &nv));  // getRelevanceMap's float tile sums
+    float c = lower;
+    for (int i = 0; i < count; i++) {
+        const float thr = (nv == 0.0f) ? c : nv * c;
+        hipLaunchKernelGGL(k_tool_mask, dim3((w + 255) / 256, h), dim3(256), 0, ctx->stream, ctx->stds.as<float>(), w,
+                           h, g.brw, thr, d_masks + (size_t)i * g.n());
+        if (constants) constants[i] = c;
+        c += step;
+    }
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
 }  // namespace sred

@@ -1,5 +1,6 @@
 """Mirrors of SReD's relevance-mask tools (RelevanceMask2D_,
-OptimiseRelevanceMask_, CalculateLocalMeans_) over the C ABI.
+RelevanceMaskStack2D_, OptimiseRelevanceMask_, CalculateLocalMeans_) over
+the C ABI.
 
 Each plugin first stabilises and normalises the image the same way
 (GATMinimizer2D from (0, 100, 0), getGAT with the fitted doubles, a double
@@ -87,3 +88,30 @@ def image_statistics_2d(image, block_width=3, block_height=3, max_iter=5000, res
         if own:
             res.close()
     return np.stack([np.asarray(means).reshape(h, w), np.asarray(stds).reshape(h, w)])
+
+
+def relevance_mask_stack_2d(image, block_width=3, block_height=3, lower=0.0, upper=1.0, step=0.1, max_iter=5000,
+                            resources=None):
+    """RelevanceMaskStack2D_.run (RelevanceMaskStack2D_.java:31-301): the
+    stabilised image and the relevance masks of the filter constants lower,
+    lower + step, ... (float sums) up to the count the plugin computes.
+    Returns (masks (n, H, W), constants (n,), prepared image)."""
+    res, own = _ctx(resources)
+    try:
+        a, _ = prepare_input(image, max_iter, res)
+        h, w = a.shape
+        n = ctypes.c_int()
+        rc = L.load().sred_relevance_mask_stack_2d(res.handle, L.fptr(a), w, h, block_width, block_height,
+                                                   float(lower), float(upper), float(step), 0, None, None,
+                                                   ctypes.byref(n))
+        if n.value < 1:
+            L.check(rc)  # raises: invalid block or constants
+        masks = np.zeros((n.value, h, w), np.float32)
+        consts = np.zeros(n.value, np.float32)
+        L.check(L.load().sred_relevance_mask_stack_2d(res.handle, L.fptr(a), w, h, block_width, block_height,
+                                                      float(lower), float(upper), float(step), n.value,
+                                                      masks.ctypes.data, consts.ctypes.data, ctypes.byref(n)))
+    finally:
+        if own:
+            res.close()
+    return masks, consts, a

@@ -60,3 +60,16 @@ def test_image_statistics_and_errors(ctx):
         tools.optimise_relevance_mask(img, 3, 3, 0.0, max_iter=50, resources=ctx)
     with pytest.raises(ValueError):
         tools.image_statistics_2d(img, 4, 3, resources=ctx)
+
+
+@pytest.mark.parametrize("b,lower,upper,step", [(3, 0.0, 1.0, 0.1), (5, 0.25, 3.0, 0.35), (3, 0.0, 40.0, 2.5)])
+def test_mask_stack_bitexact(ctx, b, lower, upper, step):
+    raw = _image(96, 80, 21)
+    masks, consts, prepared = tools.relevance_mask_stack_2d(raw, b, b, lower, upper, step, 300, ctx)
+    want_img = O.tool_prep(raw, 300)[0]
+    np.testing.assert_array_equal(prepared, want_img)
+    wm, wc = O.relevance_mask_stack(want_img, b, b, lower, upper, step)
+    np.testing.assert_array_equal(consts, wc)
+    np.testing.assert_array_equal(masks, wm)
+    with pytest.raises(sred_amd.SredError):
+        tools.relevance_mask_stack_2d(raw, 3, 5, 0.0, 1.0, 0.1, 50, ctx)

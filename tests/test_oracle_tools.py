@@ -15,7 +15,12 @@ no GPU needed.
 * the constant scan: the literal loop (masks rebuilt for every constant)
   against a reduction formulation in numpy (min and max of the squared
   variances walked along the float sequence 0, step, 2 step, ...); the
-  all-ones constant precedes the all-zeros one.
+  all-ones constant precedes the all-zeros one;
+* the mask stack (RelevanceMaskStack2D_): the count (int) ceil((upper -
+  lower) / step) + 1 in float, the constants as float sums, each mask
+  against the numpy rule on the squared local variances, and, with lower 0,
+  the last all-ones and first all-zeros masks at the constants the
+  already-pinned scan reports.
 """
 import numpy as np
 import pytest
@@ -95,3 +100,37 @@ def test_optimise_scan_against_extremes(b, step):
     assert ones <= zeros
     with pytest.raises(ValueError):
         O.optimise_relevance_mask(img, b, b, 0.0)
+
+
+@pytest.mark.parametrize("b,lower,upper,step", [(3, 0.0, 1.0, 0.1), (5, 0.25, 3.0, 0.35), (3, 0.0, 40.0, 2.5)])
+def test_mask_stack(b, lower, upper, step):
+    img = O.tool_prep(_image(60, 72, 11), 300)[0]
+    masks, consts = O.relevance_mask_stack(img, b, b, lower, upper, step)
+    f = np.float32
+    n = int(np.ceil(np.float64((f(upper) - f(lower)) / f(step)))) + 1
+    assert len(consts) == n
+    c = f(lower)
+    for i in range(n):
+        assert consts[i] == c
+        c = f(c + f(step))
+    var2 = O.local_var_f32(img, b, b) ** 2
+    nv = f(O.noise_var(img))
+    br = b // 2
+    inner = np.zeros(img.shape, bool)
+    inner[br:-br, br:-br] = True
+    for m, c in zip(masks, consts):
+        thr = c if nv == 0 else f(nv * c)
+        want = inner & ~((var2 < thr) | (var2 == 0))
+        np.testing.assert_array_equal(m, want.astype(np.float32))
+    if lower == 0.0:
+        ones, zeros = O.optimise_relevance_mask(img, b, b, step)
+        full = [i for i, m in enumerate(masks) if m[inner].all()]
+        empty = [i for i, m in enumerate(masks) if not m[inner].any()]
+        if full and full[-1] + 1 < n:
+            assert consts[full[-1]] == ones
+        if empty:
+            assert consts[empty[0]] == zeros
+    with pytest.raises(ValueError):
+        O.relevance_mask_stack(img, 3, 5, 0.0, 1.0, 0.1)
+    with pytest.raises(ValueError):
+        O.relevance_mask_stack(img, 3, 3, 1.0, 0.0, 0.1)
