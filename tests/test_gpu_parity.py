@@ -18,7 +18,7 @@ from tests.test_oracle import checkerboard, checkerboard_closed_form
 pytestmark = pytest.mark.gpu
 K = kats()
 GM = sred_amd.GLOBAL_METRICS
-MAP_TOL = 1e-4          # normalised map, absolute (north star)
+MAP_TOL = 1e-4          # normalised map, absolute (the parity bound of the maps)
 S_RTOL = {"fp32": 2e-5, "bf16x3": 2e-4}  # raw accumulators vs FP64 (S of Pearson can be small: atol too)
 
 
