@@ -8,7 +8,8 @@
 // sums of the halved sizes picked by the bits of the cell index, their
 // criteria do not depend on the decisions above them, so one launch
 // evaluates every node that could divide (one wave per node, in-order
-// chains of chain.h), and the host applies the Fisher thresholds and walks
+// chains: chain.h's, and for the first pass oc_pass1 with its double
+// terms staged by all lanes), and the host applies the Fisher thresholds and walks
 // the tree in pre-order (children x fastest, then y, then z).  What differs
 // from the quadtree, as the reference writes it:
 //   * the node variance is Utils.getMeanAndVarBlock3D's: double sums of v
@@ -78,13 +79,77 @@ __global__ __launch_bounds__(256) void k_oc_residuals(const float *__restrict__ 
     res[o] = lap * scale;
 }
 
+constexpr int kDStride = kSeg + 2;  // doubles per double term stream (streams start 4 banks apart)
+
+// Pass 1 of a node's criterion: the double sums of v (lane 0) and v*v
+// (lane 1) and the float sum of the residuals (lane 2), each in element
+// order.  All 64 lanes form the double terms while staging a block of
+// elements in LDS, so a chain step is one double add and one float add
+// (every lane of the three does both; the unused sums are discarded); the
+// next block's loads are in flight meanwhile.
+template <class Fetch>
+__device__ __forceinline__ void oc_pass1(int n, Fetch fetch, double *sd, float *sf, double &accd, float &acc)
+{
+#pragma clang fp contract(off)
+    constexpr int Q = kSeg / 64;
+    const int lane = threadIdx.x;
+    float2 raw[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int e = lane + 64 * q;
+        raw[q] = fetch(e < n ? e : -1);
+    }
+    for (int base = 0; base < n; base += kSeg) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const double dv = (double)raw[q].x;
+            sd[lane + 64 * q] = dv;
+            sd[kDStride + lane + 64 * q] = dv * dv;
+            sf[lane + 64 * q] = raw[q].y;
+        }
+        const int nb = base + kSeg;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int e = nb + lane + 64 * q;
+            raw[q] = fetch(e < n ? e : -1);
+        }
+        __syncthreads();
+        if (lane < 3) {
+            const double *d = sd + (lane == 1 ? kDStride : 0);
+            const double2 *d2 = reinterpret_cast<const double2 *>(d);
+            const float4 *f4 = reinterpret_cast<const float4 *>(sf);
+            const int cnt = min(kSeg, n - base), ng = cnt >> 2;
+#pragma unroll 4
+            for (int g = 0; g < ng; g++) {
+                const double2 a = d2[2 * g], b = d2[2 * g + 1];
+                const float4 f = f4[g];
+                accd += a.x;
+                acc += f.x;
+                accd += a.y;
+                acc += f.y;
+                accd += b.x;
+                acc += f.z;
+                accd += b.y;
+                acc += f.w;
+            }
+            for (int t = ng << 2; t < cnt; t++) {
+                accd += d[t];
+                acc += sf[t];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // stats[node] = (block variance, noise variance) of every node of levels
 // 0 .. nlev-1 (OcTree_.computeCriterion, :216-280); one wave per node.
 __global__ __launch_bounds__(64) void k_oc_criteria(const float *__restrict__ img, const float *__restrict__ res, int w,
                                                     int h, OcLevels L, float2 *__restrict__ stats)
 {
 #pragma clang fp contract(off)
-    __shared__ __align__(16) float sb[3 * kStride + kPad];
+    __shared__ __align__(16) float sb[kStride + kPad];
+    __shared__ __align__(16) double sd[2 * kDStride];
+    __shared__ __align__(16) float sf[kSeg + kPad];
     const long long node = blockIdx.x;
     int k = 0;
     while (k + 1 < L.nlev && node >= L.first[k + 1]) k++;
@@ -112,19 +177,7 @@ __global__ __launch_bounds__(64) void k_oc_criteria(const float *__restrict__ im
         }
         return r;
     };
-    auto term1 = [](float2 r, float (&t)[3]) {
-        t[0] = r.x;
-        t[1] = r.x;
-        t[2] = r.y;
-    };
-    // every lane runs the same instructions (lane 2's double sum and lanes
-    // 0-1's float sums are discarded): no divergent branch per element
-    auto op1 = [&](int c, float x) {
-        const double xd = (double)x;
-        accd += (c == 1) ? xd * xd : xd;
-        acc += x;
-    };
-    stream_chain<3>(n, fetch1, term1, op1, sb);
+    oc_pass1(n, fetch1, sd, sf, accd, acc);
     const double mean = __shfl(accd, 0) / (double)n;
     const float bvar = (float)(__shfl(accd, 1) / (double)n - mean * mean);
     const float fn = (float)n;
