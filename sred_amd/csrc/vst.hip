@@ -161,6 +161,46 @@ __global__ __launch_bounds__(kGatErrThreads) void k_gat_error(const double2 *__r
     }
 }
 
+// Volumes: the same block terms, one block per wave over grid.y workgroups
+// per set, into bterm[set][b]; k_gat_error_final then combines them in
+// k_gat_error's order (block b to lane b mod 64, then lane_tree64).
+__global__ __launch_bounds__(kGatErrThreads) void k_gat_error_part(const double2 *__restrict__ rows, int bw, int bh,
+                                                                   int bd, int nbx, int nby, int nbz, int nrows,
+                                                                   int rps, double *__restrict__ bterm)
+{
+#pragma clang fp contract(off)
+    const int set = blockIdx.x, lane = threadIdx.x & 63, nb = nbx * nby * nbz;
+    const int b = blockIdx.y * (kGatErrThreads / 64) + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const double2 *r = rows + (size_t)set * nrows * nbx;
+    const int bz = b / (nbx * nby), rem = b - bz * (nbx * nby), by = rem / nbx, bx = rem - by * nbx;
+    double mean = 0.0, sq = 0.0;
+    for (int k = lane; k < bh * bd; k += 64) {
+        const int kz = k / bh, ky = k - kz * bh;
+        const double2 q = r[((size_t)(bz * bd + kz) * rps + by * bh + ky) * nbx + bx];
+        mean += q.x;
+        sq += q.y;
+    }
+    mean = lane_tree64(mean);
+    sq = lane_tree64(sq);
+    const double n = (double)((int64_t)bw * bh * bd);
+    mean = mean / n;
+    const double var = sq / n - mean * mean;
+    const double delta = var - 1.0;
+    if (lane == 0) bterm[(size_t)set * nb + b] = (delta * delta) / (double)nb;
+}
+
+__global__ __launch_bounds__(64) void k_gat_error_final(const double *__restrict__ bterm, int nb,
+                                                        double *__restrict__ err)
+{
+#pragma clang fp contract(off)
+    const int set = blockIdx.x, lane = threadIdx.x;
+    double e = 0.0;
+    for (int q = lane; q < nb; q += 64) e += bterm[(size_t)set * nb + q];
+    e = lane_tree64(e);
+    if (lane == 0) err[set] = e;
+}
+
 static double gat_ref(double gain, double sigma, double offset)
 {
 #pragma clang fp contract(off)
@@ -192,6 +232,7 @@ struct GatEval {
     int64_t plane = 0;             // voxels per slice (0 for an image)
     bool volume = false;           // double squares, NaN for invalid parameters
     double2 *d_rows;
+    double *d_bterm = nullptr;     // volumes: per-block terms of every set
     double *h_err;  // pinned host memory written by k_gat_error
     int64_t launches = 0, pixel_evals = 0;
     double rows_ms = 0.0;  // k_gat_rows time summed over the launches (HIP events, when fit_timing is on)
@@ -213,8 +254,16 @@ struct GatEval {
             hipLaunchKernelGGL(k_gat_rows<false>, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
                                nrows, rps, plane, nsets, ps, d_rows);
         if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
-        hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw, bh, bd, nbx,
-                           nby, nbz, nrows, rps, h_err);
+        const int nb = nbx * nby * nbz;
+        if (volume) {  // many blocks: spread over workgroups, then the fixed-order sum
+            const int gy = (nb + kGatErrThreads / 64 - 1) / (kGatErrThreads / 64);
+            hipLaunchKernelGGL(k_gat_error_part, dim3(nsets, gy), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw,
+                               bh, bd, nbx, nby, nbz, nrows, rps, d_bterm);
+            hipLaunchKernelGGL(k_gat_error_final, dim3(nsets), dim3(64), 0, ctx->stream, d_bterm, nb, h_err);
+        } else {
+            hipLaunchKernelGGL(k_gat_error, dim3(nsets), dim3(kGatErrThreads), 0, ctx->stream, d_rows, bw, bh, bd,
+                               nbx, nby, nbz, nrows, rps, h_err);
+        }
         SRED_HIP_CHECK(hipGetLastError());
         SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         for (int s = 0; s < nsets; s++) f[s] = h_err[s];
@@ -300,8 +349,10 @@ int gat_setup3d(sred_ctx *ctx, const float *d_vol, int w, int h, int d, GatEval 
     e->nrows = nz * c * e->rps;
     e->plane = (int64_t)w * h;
     e->volume = true;
-    SRED_TRY(ctx->pm_part.ensure(sizeof(double2) * (size_t)kGatSets * e->nrows * e->nbx + 256));
+    const size_t rows_bytes = (sizeof(double2) * (size_t)kGatSets * e->nrows * e->nbx + 255) / 256 * 256;
+    SRED_TRY(ctx->pm_part.ensure(rows_bytes + sizeof(double) * (size_t)kGatSets * nx * ny * nz + 256));
     e->d_rows = ctx->pm_part.as<double2>();
+    e->d_bterm = reinterpret_cast<double *>(ctx->pm_part.as<char>() + rows_bytes);
     if (!ctx->h_fit) SRED_HIP_CHECK(hipHostMalloc((void **)&ctx->h_fit, sizeof(double) * kGatSets, hipHostMallocDefault));
     e->h_err = ctx->h_fit;
     SRED_TRY(time_begin(ctx));
