@@ -82,17 +82,27 @@ __device__ __forceinline__ double lane_tree64(double a)
 // wave per (j, bx) evaluates every parameter set of the launch.  Row j is
 // row j mod rps of slice j / rps (plane voxels apart); DSQ squares in double
 // (the volume objective), else in float (the image objective).
+//
+// Rows of at most 32 elements share a wave: segments of S = 2^seg_log2
+// lanes (the smallest power of 2 >= bw) each take one row, and the butterfly
+// runs within the segment.  That equals lane_tree64 over the row alone: the
+// steps across segments would add exact zeros (the transformed values and
+// their squares are >= 0).
 template <bool DSQ>
 __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, int w, int bw, int nbx, int nrows,
-                                                  int rps, int64_t plane, int nsets, GatSets ps,
+                                                  int rps, int64_t plane, int nsets, GatSets ps, int seg_log2,
                                                   double2 *__restrict__ rows)
 {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // wave-uniform
+    const int S = 1 << seg_log2, li = lane & (S - 1);
+    const int wave_id = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int t = (wave_id << (6 - seg_log2)) + (lane >> seg_log2);
     const int per_set = nrows * nbx;
-    if (t >= per_set) return;
-    const int j = t / nbx, bx = t - j * nbx;
+    if ((wave_id << (6 - seg_log2)) >= per_set) return;  // wave-uniform: no row of this wave exists
+    const bool valid = t < per_set;
+    const int tt = valid ? t : 0;
+    const int j = tt / nbx, bx = tt - j * nbx;
     const int z = j / rps, y = j - z * rps;
     double lim[kGatSets], two_g[kGatSets], rm[kGatSets], rs[kGatSets];
 #pragma unroll
@@ -102,7 +112,7 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
         rm[s] = rs[s] = 0.0;
     }
     const float *p = px + z * plane + (int64_t)y * w + (int64_t)bx * bw;
-    for (int i = lane; i < bw; i += 64) {
+    for (int i = li; i < (valid ? bw : 0); i += S) {
         const double d = (double)p[i];
 #pragma unroll
         for (int s = 0; s < kGatSets; s++) {
@@ -117,8 +127,12 @@ __global__ __launch_bounds__(256) void k_gat_rows(const float *__restrict__ px, 
 #pragma unroll
     for (int s = 0; s < kGatSets; s++) {
         if (s < nsets) {
-            const double a = lane_tree64(rm[s]), b = lane_tree64(rs[s]);
-            if (lane == 0) rows[(size_t)s * per_set + t] = make_double2(a, b);
+            double a = rm[s], b = rs[s];
+            for (int m = S >> 1; m >= 1; m >>= 1) {
+                a = a + __shfl_xor(a, m, 64);
+                b = b + __shfl_xor(b, m, 64);
+            }
+            if (li == 0 && valid) rows[(size_t)s * per_set + t] = make_double2(a, b);
         }
     }
 }
@@ -245,14 +259,16 @@ struct GatEval {
             ps.gain[s] = x[s][0];
             ps.ref[s] = gat_ref(x[s][0], x[s][1], x[s][2]);
         }
-        const int waves = nrows * nbx;
+        int seg_log2 = 0;
+        while ((1 << seg_log2) < bw && seg_log2 < 6) seg_log2++;
+        const int waves = (nrows * nbx + (64 >> seg_log2) - 1) >> (6 - seg_log2);
         if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev0, ctx->stream));
         if (volume)
             hipLaunchKernelGGL(k_gat_rows<true>, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
-                               nrows, rps, plane, nsets, ps, d_rows);
+                               nrows, rps, plane, nsets, ps, seg_log2, d_rows);
         else
             hipLaunchKernelGGL(k_gat_rows<false>, dim3((waves + 3) / 4), dim3(256), 0, ctx->stream, d_img, w, bw, nbx,
-                               nrows, rps, plane, nsets, ps, d_rows);
+                               nrows, rps, plane, nsets, ps, seg_log2, d_rows);
         if (ctx->fit_timing) SRED_HIP_CHECK(hipEventRecord(ctx->ev1, ctx->stream));
         const int nb = nbx * nby * nbz;
         if (volume) {  // many blocks: spread over workgroups, then the fixed-order sum
