@@ -12,8 +12,9 @@
 // each footprint offset is one coalesced row read per wave); the offsets and
 // the template are indexed uniformly (scalar loads).  Pearson reads its K
 // voxels in three passes (min/max; remap, local-mean subtraction, min/max;
-// both remaps again and the dot product) and recomputes the remapped values
-// bit for bit instead of holding K values per lane.  The macro literals of
+// the second remap and the dot product): up to kCacheK voxels the second
+// pass's values wait in LDS for the third, beyond it they are recomputed
+// bit for bit.  The macro literals of
 // the kernels are double (ref_mean, ref_std, EPSILON), so the remaps divide
 // in double, Pearson's quotient is double and SSIM's numerator is rounded
 // to float before its double division (kernelGetBlockSsim3D.cl:101).  The
@@ -30,7 +31,12 @@ namespace sred {
 
 namespace {
 
-template <int M>
+constexpr int kCacheK = 48;  // Pearson keeps the remapped block in LDS up to this size (48 KB per workgroup)
+
+// CACHE (Pearson, K <= kCacheK): pass 2 parks each lane's remapped,
+// mean-subtracted values in LDS ([k][thread], no bank conflicts) and pass 3
+// reads them back instead of gathering and dividing again.
+template <int M, bool CACHE>
 __global__ __launch_bounds__(256) void k_block3d(const float *__restrict__ img, const float *__restrict__ means,
                                                  const float *__restrict__ stds, int w, int h, int d, int brw,
                                                  int brh, int brd, const int *__restrict__ offs,
@@ -38,6 +44,7 @@ __global__ __launch_bounds__(256) void k_block3d(const float *__restrict__ img, 
                                                  double tstd_d, double eps_d, float *__restrict__ out)
 {
 #pragma clang fp contract(off)
+    extern __shared__ float tcache[];
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
@@ -66,13 +73,14 @@ __global__ __launch_bounds__(256) void k_block3d(const float *__restrict__ img, 
         float mn2 = FLT_MAX, mx2 = -FLT_MAX;
         for (int k = 0; k < K; k++) {
             const float t = (float)((double)(c[offs[k]] - mn) / den1) - cm;
+            if (CACHE) tcache[k * 256 + threadIdx.x] = t;
             mn2 = fminf(mn2, t);
             mx2 = fmaxf(mx2, t);
         }
         const double den2 = (double)(mx2 - mn2) + eps_d;
         float cov = 0.0f;
         for (int k = 0; k < K; k++) {
-            const float t = (float)((double)(c[offs[k]] - mn) / den1) - cm;
+            const float t = CACHE ? tcache[k * 256 + threadIdx.x] : (float)((double)(c[offs[k]] - mn) / den1) - cm;
             const float t3 = (float)((double)(t - mn2) / den2);
             cov += tmpl[k] * t3;
         }
@@ -133,20 +141,25 @@ int block_map3d_dev(sred_ctx *ctx, const Geometry &g, int metric, const float *d
     SRED_TRY(time_begin(ctx));
     switch (metric) {
     case SRED_PEARSON:
-        hipLaunchKernelGGL(k_block3d<SRED_PEARSON>, grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds, g.w, g.h,
-                           g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
+        if (g.K <= kCacheK)
+            hipLaunchKernelGGL((k_block3d<SRED_PEARSON, true>), grid, dim3(256), sizeof(float) * 256 * (size_t)g.K,
+                               ctx->stream, d_img, d_means, d_stds, g.w, g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K,
+                               tmpl_std, tm, ts, ed, d_out);
+        else
+            hipLaunchKernelGGL((k_block3d<SRED_PEARSON, false>), grid, dim3(256), 0, ctx->stream, d_img, d_means,
+                               d_stds, g.w, g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
         break;
     case SRED_SSIM:
-        hipLaunchKernelGGL(k_block3d<SRED_SSIM>, grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds, g.w, g.h,
-                           g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
+        hipLaunchKernelGGL((k_block3d<SRED_SSIM, false>), grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds, g.w,
+                           g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
         break;
     case SRED_NRMSE:
-        hipLaunchKernelGGL(k_block3d<SRED_NRMSE>, grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds, g.w, g.h,
-                           g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
+        hipLaunchKernelGGL((k_block3d<SRED_NRMSE, false>), grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds,
+                           g.w, g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
         break;
     default:
-        hipLaunchKernelGGL(k_block3d<SRED_ABSDIFF>, grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds, g.w,
-                           g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
+        hipLaunchKernelGGL((k_block3d<SRED_ABSDIFF, false>), grid, dim3(256), 0, ctx->stream, d_img, d_means, d_stds,
+                           g.w, g.h, g.d, g.brw, g.brh, g.brd, offs, tp, g.K, tmpl_std, tm, ts, ed, d_out);
         break;
     }
     SRED_HIP_CHECK(hipGetLastError());

@@ -1,7 +1,8 @@
 """GPU block repetition of volumes (BlockRepetition3D_ ->
 CLUtils.calculateBlockRepetitionMap3D, as intended) against the oracle,
 through the C ABI: the maps of all four metrics bit for bit, with and
-without a relevance constant and normalisation, for several block shapes,
+without a relevance constant and normalisation, for several block shapes
+(Pearson with and without its LDS cache),
 with flat regions (test std 0) and a flat reference block; the device entry
 with its statistics and mask outputs; the benchmark's 128 x 128 x 64 volume;
 the errors."""
@@ -41,7 +42,8 @@ def _oracle_map(metric, vol, blk, c, normalize):
 
 @pytest.mark.parametrize("metric", [O.PEARSON, O.SSIM, O.NRMSE, O.ABSDIFF])
 @pytest.mark.parametrize("b,c,normalize", [((3, 3, 3), 0.0, True), ((5, 5, 5), 0.0, True), ((7, 5, 3), 0.8, True),
-                                           ((3, 5, 7), 0.0, False), ((5, 3, 3), 1.5, False)])
+                                           ((3, 5, 7), 0.0, False), ((5, 3, 3), 1.5, False),
+                                           ((9, 7, 5), 0.0, True)])  # K = 99: Pearson without the LDS cache
 def test_block3d_bitexact(ctx, metric, b, c, normalize):
     vol = _vol(14, 30, 37, 1)
     vol[5:10, 10:20, 12:24] = 60.0  # flat region: test std 0
