@@ -188,3 +188,50 @@ def variance_stabilising_transform_3d(volume, gain=1.0, offset=10.0, sigma=100.0
     finally:
         if own:
             res.close()
+
+
+def _roi_mean_std(values):
+    """VarianceStabilisingTransform2D_.meanAndStdDev (:116-133): double sums in
+    order, mean and sqrt(sq/n - mean^2); (0, 0) for no values."""
+    a = np.asarray(values, np.float64).reshape(-1)
+    if a.size == 0:
+        return 0.0, 0.0
+    s = sq = 0.0
+    for v in a.tolist():
+        s += v
+        sq += v * v
+    mean = s / a.size
+    return mean, float(np.sqrt(sq / a.size - mean * mean))
+
+
+def variance_stabilising_transform_2d(image, gain=1.0, offset=10.0, sigma=100.0, max_iter=5000, roi=None,
+                                      resources=None):
+    """The VarianceStabilisingTransform2D_ plugin (VarianceStabilisingTransform2D_
+    .java:24-110) on a 2D image: with roi = (x, y, w, h) the initial offset and
+    noise std come from the ROI's pixels as ImageProcessor.getPixel returns
+    them (the value for 8- and 16-bit images; for a 32-bit image ImageJ's
+    FloatProcessor returns the float's raw bits as an int, and so does this
+    mirror for float input); then GATMinimizer2D from (gain, sigma, offset)
+    and getGAT with the fitted doubles.  Returns (float32 image, (gain,
+    sigma, offset))."""
+    arr = np.asarray(image)
+    if arr.ndim != 2:
+        raise ValueError("expected a 2D image")
+    if roi is not None:
+        x, y, w, h = (int(v) for v in roi)
+        vals = arr[y:y + h, x:x + w]
+        if np.issubdtype(vals.dtype, np.floating):
+            vals = np.ascontiguousarray(vals, np.float32).view(np.int32)
+        offset, sigma = _roi_mean_std(vals)
+    px = L.f32(arr)
+    hh, ww = px.shape
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        m = utils.GATMinimizer2D(px, ww, hh, gain, sigma, offset, max_iter, res)
+        m.run()
+        out = utils.get_gat_2d(px, m.gain, m.sigma, m.offset, res).reshape(hh, ww)
+        return out, (m.gain, m.sigma, m.offset)
+    finally:
+        if own:
+            res.close()

@@ -242,3 +242,14 @@ def test_host_f_quantile_and_regression():
     assert (s, i) == (2.0, 3.0)
     with pytest.raises(ValueError):
         qt.perform_linear_regression([[1.0, 2.0], [2.0, 3.0]])
+
+
+def test_vst2d_roi_statistics():
+    from sred_amd.commands import _roi_mean_std
+    assert _roi_mean_std([]) == (0.0, 0.0)
+    m, s = _roi_mean_std([2, 4, 4, 4, 5, 5, 7, 9])  # the textbook population std 2
+    assert (m, s) == (5.0, 2.0)
+    rng = np.random.default_rng(3)
+    v = rng.integers(0, 4000, 500)
+    m, s = _roi_mean_std(v)
+    assert m == pytest.approx(v.mean(), rel=1e-15) and s == pytest.approx(v.std(), rel=1e-9)

@@ -82,3 +82,21 @@ def test_gat_minimize_device_path_and_errors(ctx):
     assert ms2 > 0.0 and (launches2, evals2) == (launches, evals)
     with pytest.raises(sred_amd.SredError):
         utils.GATMinimizer2D(np.zeros((5, 40), np.float32), 40, 5, 1, 10, 100, 10, resources=ctx).run()
+
+
+def test_vst2d_plugin(ctx):
+    rng = np.random.default_rng(6)
+    img = (2.0 * rng.poisson(np.linspace(5, 80, 96 * 80).reshape(80, 96)) + rng.normal(10, 3, (80, 96))).astype(
+        np.uint16)
+    out, (g, s, o) = sred_amd.variance_stabilising_transform_2d(img, max_iter=200, roi=(4, 6, 20, 10),
+                                                                resources=ctx)
+    from sred_amd.commands import _roi_mean_std  # the plugin's sequential double sums
+    mean, std = _roi_mean_std(img[6:16, 4:24])
+    assert mean == pytest.approx(img[6:16, 4:24].mean(), rel=1e-12)
+    p, f = O.gat_minimize(img.astype(np.float32), 1.0, std, mean, 200)
+    assert (g, s, o) == tuple(p)
+    ref = (3.0 / 8.0) * g * g + s * s - g * o
+    x = img.astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        want = np.where(g * x + ref < 0.0, 0.0, (2.0 / g) * np.sqrt(g * x + ref)).astype(np.float32)
+    np.testing.assert_array_equal(out, want)
