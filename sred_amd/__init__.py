@@ -13,7 +13,8 @@ from .clutils import (BLOCK_METRICS, GLOBAL_METRICS, calculate_block_hu_map_2d, 
                       calculate_phase_correlation_map_2d, get_hu_invariants_2d, get_local_statistics_2d,
                       get_opencl_resources)
 from .commands import (block_repetition_2d, block_repetition_3d, block_repetition_timelapse_2d, global_repetition_2d,
-                       global_repetition_3d, variance_stabilising_transform_2d, variance_stabilising_transform_3d)
+                       global_repetition_3d, octree_stabilise_3d, quadtree_stabilise_2d,
+                       variance_stabilising_transform_2d, variance_stabilising_transform_3d)
 from .utils import (GATMinimizer2D, GATMinimizer3D, InputImage2D, InputImage3D, OcTree, QuadTree, ReferenceBlock2D, ReferenceBlock3D,
                     RelevanceMask, apply_mask_2d, block_size_2d, get_gat_2d, get_hu_template_2d, get_input_image_2d,
                     get_input_image_3d, get_mean_noise_var_2d, get_reference_block_2d, get_reference_block_3d,
@@ -31,5 +32,5 @@ __all__ = [
     "GATMinimizer2D", "QuadTree", "get_gat_2d", "block_repetition_timelapse_2d", "global_repetition_3d",
     "block_repetition_3d", "InputImage3D", "ReferenceBlock3D", "get_input_image_3d", "get_reference_block_3d",
     "calculate_block_repetition_map_3d", "OcTree", "GATMinimizer3D", "variance_stabilising_transform_3d",
-    "variance_stabilising_transform_2d",
+    "variance_stabilising_transform_2d", "quadtree_stabilise_2d", "octree_stabilise_3d",
 ]

@@ -235,3 +235,52 @@ def variance_stabilising_transform_2d(image, gain=1.0, offset=10.0, sigma=100.0,
     finally:
         if own:
             res.close()
+
+
+def quadtree_stabilise_2d(image, min_size=4, alpha=0.01, max_iterations=50, var_alpha=0.75, resources=None):
+    """The QuadTree_ plugin (QuadTree_.java:28-100) on a 2D image: the tree,
+    robust means (max_iterations), LTS variances (var_alpha), the regression
+    and applyGATtree with its double (g0, eDC).  Returns (float32 image,
+    (g0, eDC), leaf rectangles (x, y, w, h) -- its overlay)."""
+    arr = L.f32(np.array(image, dtype=np.float32, copy=True))
+    if arr.ndim != 2:
+        raise ValueError("expected a 2D image")
+    h, w = arr.shape
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        qt = utils.QuadTree(w, h, min_size, alpha, res)
+        qt.build_tree(arr)
+        qt.calculate_robust_means(arr, max_iterations)
+        qt.calculate_lts_variances(arr, var_alpha)
+        g0, edc = qt.perform_linear_regression(qt.collect_mean_variance_pairs())
+        out = qt.apply_gat_tree(arr, w * h, g0, edc).reshape(h, w)
+        return out, (g0, edc), qt.leaves()
+    finally:
+        if own:
+            res.close()
+
+
+def octree_stabilise_3d(volume, min_size=4, alpha=0.01, max_iterations=50, var_alpha=0.75, resources=None):
+    """The OcTree_ plugin (OcTree_.java:30-126) on a (depth, H, W) volume of at
+    least 3 slices: the tree, robust means (max_iterations), LTS variances
+    with the fraction 0.75 (the plugin reads var_alpha but passes 0.75f,
+    :88), the float (g0, eDC) and applyGATtree over every voxel.  Returns
+    (float32 volume, (g0, eDC), leaf boxes (x, y, z, w, h, d))."""
+    arr = L.f32(np.array(volume, dtype=np.float32, copy=True))
+    if arr.ndim != 3 or arr.shape[0] < 3:  # :64-67
+        raise ValueError("Image must have at least 3 slices. Please try again.")
+    d, h, w = arr.shape
+    own = resources is None
+    res = clutils.get_opencl_resources() if own else resources
+    try:
+        oc = utils.OcTree(w, h, d, min_size, alpha, res)
+        oc.build_tree(arr)
+        oc.calculate_robust_means(arr, max_iterations)
+        oc.calculate_lts_variances(arr, 0.75)
+        g0, edc = oc.perform_linear_regression(oc.collect_mean_variance_pairs())
+        out = oc.apply_gat_tree(arr, w * h * d, g0, edc).reshape(d, h, w)
+        return out, (g0, edc), oc.leaves()
+    finally:
+        if own:
+            res.close()

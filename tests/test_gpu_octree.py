@@ -133,3 +133,20 @@ def test_errors(ctx):
     assert len(oc.collect_mean_variance_pairs()) == 1
     with pytest.raises(ValueError):
         oc.perform_linear_regression(oc.collect_mean_variance_pairs())
+
+
+def test_tree_plugins(ctx):
+    img = structured(1, 150, 170, seed=12)[0]
+    out, (g0, edc), rects = sred_amd.quadtree_stabilise_2d(img, 4, 0.01, 50, 0.5, resources=ctx)
+    ref = O.quadtree_gat(img, 4, 0.01, 50, 0.5)
+    assert (g0, edc) == (ref.g0, ref.edc)
+    np.testing.assert_array_equal(rects, ref.rects)
+    np.testing.assert_array_equal(out, O.gat_tree(img, ref.g0, ref.edc))
+    vol = structured(12, 40, 36, seed=13)
+    out, (g0, edc), boxes = sred_amd.octree_stabilise_3d(vol, 4, 0.01, 50, 0.5, resources=ctx)  # 0.5 is ignored
+    ref = O.octree_gat(vol, 4, 0.01, 50, 0.75)
+    assert (g0, edc) == (ref.g0, ref.edc)
+    np.testing.assert_array_equal(boxes, ref.boxes)
+    np.testing.assert_array_equal(out, O.gat_tree3d(vol, ref.g0, ref.edc))
+    with pytest.raises(ValueError):
+        sred_amd.octree_stabilise_3d(vol[:2], resources=ctx)
