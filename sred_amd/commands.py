@@ -91,10 +91,15 @@ def global_repetition_3d(volume, block_width=5, block_height=5, block_depth=5, r
 
 
 def block_repetition_timelapse_2d(block, stack, relevance_constant=0.0, metric="Pearson's R",
-                                  normalize_output=True, resources=None):
-    """The time-lapse branch of BlockRepetition2D_ (BlockRepetition2D_.java:95-137),
-    without noise stabilisation: one repetition map per frame of a (depth, H, W)
-    stack (depth >= 3), as a float32 (depth, H, W) array."""
+                                  normalize_output=True, stabilise_noise_variance=False, gat_method="Simplex",
+                                  resources=None):
+    """The time-lapse branch of BlockRepetition2D_ (BlockRepetition2D_.java:95-137):
+    one repetition map per frame of a (depth, H, W) stack (depth >= 3), as a
+    float32 (depth, H, W) array.  The stack goes through getInputImage3D
+    (:97) first: with stabilisation and "Quad/Octree" the octree GAT of the
+    whole stack (utils.get_input_image_3d), with "Simplex" nothing (the
+    reference discards that transform); the entry then normalises the stack
+    and each frame."""
     ref = utils.get_reference_block_2d(block)
     if ref is None:
         raise ValueError("Block dimensions must be odd (e.g., 3x3 or 5x5). Please try again.")
@@ -109,6 +114,8 @@ def block_repetition_timelapse_2d(block, stack, relevance_constant=0.0, metric="
     own = resources is None
     res = clutils.get_opencl_resources() if own else resources
     try:
+        if stabilise_noise_variance and gat_method == "Quad/Octree" and d >= 3:
+            st = L.f32(utils.get_input_image_3d(st, True, gat_method, False, res).image_array.reshape(d, h, w))
         L.check(L.load().sred_block_timelapse_2d(res.handle, m, L.fptr(st), w, h, d, L.fptr(L.f32(ref.pixels)),
                                                  ref.width, ref.height, ref.mean, ref.std, float(relevance_constant),
                                                  int(bool(normalize_output)), L.fptr(out)))

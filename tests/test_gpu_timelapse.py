@@ -53,3 +53,24 @@ def test_timelapse_device_entry_and_errors(ctx):
     np.testing.assert_array_equal(d.cpu().numpy(), st)  # the input stack is not modified
     with pytest.raises(sred_amd.SredError):
         sred_amd.block_repetition_timelapse_2d(tmpl, st[:2], 0.0, "Pearson's R", True, resources=ctx)
+
+
+def test_timelapse_stabilised(ctx):
+    """getInputImage3D's stabilisation of the stack (:97): "Quad/Octree" is the
+    octree GAT of the whole stack before the maps, "Simplex" changes nothing."""
+    rng = np.random.default_rng(21)  # gain-2 counts, constant on the octree's 28 x 24 x 4 leaves
+    lam = np.kron(rng.uniform(20, 400, (4, 4, 4)), np.ones((4, 24, 28)))
+    st = (2.0 * rng.poisson(lam) + rng.normal(10.0, 3.0, lam.shape)).astype(np.float32)
+    tmpl = st[5, 20:29, 24:33].copy()
+    rb = RB(*O.reference_block(tmpl))
+    oc = O.octree_gat(st, 4, 0.01, 50, 0.75)
+    stab = O.gat_tree3d(st, oc.g0, oc.edc)
+    got = sred_amd.block_repetition_timelapse_2d(tmpl, st, 0.0, "Pearson's R", True, True, "Quad/Octree",
+                                                 resources=ctx)
+    want = O.block_timelapse(O.PEARSON, stab, rb.pixels, 9, 9, rb.mean, rb.std, 0.0, True)
+    np.testing.assert_array_equal(got, want)
+    plain = sred_amd.block_repetition_timelapse_2d(tmpl, st, 0.0, "Pearson's R", True, resources=ctx)
+    simplex = sred_amd.block_repetition_timelapse_2d(tmpl, st, 0.0, "Pearson's R", True, True, "Simplex",
+                                                     resources=ctx)
+    np.testing.assert_array_equal(simplex, plain)
+    assert not np.array_equal(got, plain)
