@@ -7,9 +7,9 @@
 // a complete 2^k x 2^k x 2^k grid of boxes W_k x H_k x D_k whose origins are
 // sums of the halved sizes picked by the bits of the cell index, their
 // criteria do not depend on the decisions above them, so one launch
-// evaluates every node that could divide (one wave per node, in-order
-// chains: chain.h's, and for the first pass oc_pass1 with its double
-// terms staged by all lanes), and the host applies the Fisher thresholds and walks
+// evaluates every node that could divide (two waves per node, in-order
+// chains: oc_pass1 with its double terms staged by all lanes on one,
+// chain.h's float chains on the other), and the host applies the Fisher thresholds and walks
 // the tree in pre-order (children x fastest, then y, then z).  What differs
 // from the quadtree, as the reference writes it:
 //   * the node variance is Utils.getMeanAndVarBlock3D's: double sums of v
@@ -119,10 +119,7 @@ __device__ __forceinline__ void oc_pass1(int n, Fetch fetch, double *sd, float *
             const double2 *d2 = reinterpret_cast<const double2 *>(d);
             const float4 *f4 = reinterpret_cast<const float4 *>(sf);
             const int cnt = min(kSeg, n - base), ng = cnt >> 2;
-#pragma unroll 4
-            for (int g = 0; g < ng; g++) {
-                const double2 a = d2[2 * g], b = d2[2 * g + 1];
-                const float4 f = f4[g];
+            auto eat4 = [&](const double2 &a, const double2 &b, const float4 &f) {
                 accd += a.x;
                 acc += f.x;
                 accd += a.y;
@@ -131,10 +128,38 @@ __device__ __forceinline__ void oc_pass1(int n, Fetch fetch, double *sd, float *
                 acc += f.z;
                 accd += b.y;
                 acc += f.w;
-            }
-            for (int t = ng << 2; t < cnt; t++) {
-                accd += d[t];
-                acc += sf[t];
+            };
+            if (cnt == kSeg) {
+                // full stage: the next 16 elements' reads are issued before the
+                // current 16 are consumed (the last prefetch reads the padding)
+                double2 DA[8], DB[8];
+                float4 FA[4], FB[4];
+#pragma unroll
+                for (int u = 0; u < 8; u++) DA[u] = d2[u];
+#pragma unroll
+                for (int u = 0; u < 4; u++) FA[u] = f4[u];
+#pragma unroll 1
+                for (int e = 0; e < kSeg; e += 32) {
+#pragma unroll
+                    for (int u = 0; u < 8; u++) DB[u] = d2[(e + 16) / 2 + u];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) FB[u] = f4[(e + 16) / 4 + u];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) eat4(DA[2 * u], DA[2 * u + 1], FA[u]);
+#pragma unroll
+                    for (int u = 0; u < 8; u++) DA[u] = d2[(e + 32) / 2 + u];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) FA[u] = f4[(e + 32) / 4 + u];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) eat4(DB[2 * u], DB[2 * u + 1], FB[u]);
+                }
+            } else {
+#pragma unroll 4
+                for (int g = 0; g < ng; g++) eat4(d2[2 * g], d2[2 * g + 1], f4[g]);
+                for (int t = ng << 2; t < cnt; t++) {
+                    accd += d[t];
+                    acc += sf[t];
+                }
             }
         }
         __syncthreads();
@@ -142,15 +167,19 @@ __device__ __forceinline__ void oc_pass1(int n, Fetch fetch, double *sd, float *
 }
 
 // stats[node] = (block variance, noise variance) of every node of levels
-// 0 .. nlev-1 (OcTree_.computeCriterion, :216-280); one wave per node.
+// 0 .. nlev-1 (OcTree_.computeCriterion, :216-280); two single-wave
+// workgroups per node, root first: role 0 (even blockIdx) the double sums
+// of v and v*v, role 1 the residual sum and then the second pass, so the
+// root's double chain runs beside its two float chains.
 __global__ __launch_bounds__(64) void k_oc_criteria(const float *__restrict__ img, const float *__restrict__ res, int w,
                                                     int h, OcLevels L, float2 *__restrict__ stats)
 {
 #pragma clang fp contract(off)
     __shared__ __align__(16) float sb[kStride + kPad];
-    __shared__ __align__(16) double sd[2 * kDStride];
+    __shared__ __align__(16) double sd[2 * kDStride + 16];  // + the read-ahead of a full stage
     __shared__ __align__(16) float sf[kSeg + kPad];
-    const long long node = blockIdx.x;
+    const long long node = blockIdx.x >> 1;
+    const int role = blockIdx.x & 1;
     int k = 0;
     while (k + 1 < L.nlev && node >= L.first[k + 1]) k++;
     const long long idx = node - L.first[k];
@@ -165,34 +194,39 @@ __global__ __launch_bounds__(64) void k_oc_criteria(const float *__restrict__ im
     }
     const int64_t plane = (int64_t)w * h;
 
-    // pass 1: double sums of v and v*v (lanes 0, 1) and the float sum of the
-    // residuals (lane 2), each in z, y, x order
-    double accd = 0.0;
+    float *out = reinterpret_cast<float *>(stats) + 2 * node + role;
+    if (role == 0) {
+        // double sums of v and v*v (lanes 0, 1) in z, y, x order (lane 2's
+        // float chain runs over zeros)
+        double accd = 0.0;
+        float acc = 0.0f;
+        auto fetch1 = [&](int e) {
+            return make_float2(e >= 0 ? img[box_voxel(e, nw, nh, x0, y0, z0, w, plane)] : 0.0f, 0.0f);
+        };
+        oc_pass1(n, fetch1, sd, sf, accd, acc);
+        const double mean = __shfl(accd, 0) / (double)n;
+        const float bvar = (float)(__shfl(accd, 1) / (double)n - mean * mean);
+        if (threadIdx.x == 0) *out = bvar;
+        return;
+    }
+
+    // pass 1 of the residuals: their float sum in z, y, x order
     float acc = 0.0f;
-    auto fetch1 = [&](int e) {
-        float2 r = make_float2(0.0f, 0.0f);
-        if (e >= 0) {
-            const int64_t p = box_voxel(e, nw, nh, x0, y0, z0, w, plane);
-            r = make_float2(img[p], res[p]);
-        }
-        return r;
-    };
-    oc_pass1(n, fetch1, sd, sf, accd, acc);
-    const double mean = __shfl(accd, 0) / (double)n;
-    const float bvar = (float)(__shfl(accd, 1) / (double)n - mean * mean);
+    auto fetch2 = [&](int e) { return e >= 0 ? res[box_voxel(e, nw, nh, x0, y0, z0, w, plane)] : 0.0f; };
+    auto term1 = [](float r, float (&t)[1]) { t[0] = r; };
+    auto add = [&](int, float x) { acc += x; };
+    stream_chain<1>(n, fetch2, term1, add, sb);
     const float fn = (float)n;
-    const float mr = __shfl(acc, 2) / fn;
+    const float mr = __shfl(acc, 0) / fn;
 
     // pass 2: sum of (float)((r - mean_r)^2), the square in double (:272-276)
     acc = 0.0f;
-    auto fetch2 = [&](int e) { return e >= 0 ? res[box_voxel(e, nw, nh, x0, y0, z0, w, plane)] : 0.0f; };
     auto term2 = [&](float r, float (&t)[1]) {
         const double dd = (double)r - (double)mr;
         t[0] = (float)(dd * dd);
     };
-    auto add = [&](int, float x) { acc += x; };
     stream_chain<1>(n, fetch2, term2, add, sb);
-    if (threadIdx.x == 0) stats[node] = make_float2(bvar, acc / (fn - 1.0f));
+    if (threadIdx.x == 0) *out = acc / (fn - 1.0f);
 }
 
 // OcTree_.calculateRobustMean (:316-366) per leaf; one wave per leaf.
@@ -326,7 +360,7 @@ int octree_gat_dev(sred_ctx *ctx, const float *d_vol, int w, int h, int d, int m
     hipLaunchKernelGGL(k_oc_residuals, dim3((w + 63) / 64, (h + 3) / 4, d), dim3(256), 0, ctx->stream, d_vol, w, h, d,
                        scale, ctx->qt_res.as<float>());
     SRED_TRY(time_begin(ctx));
-    hipLaunchKernelGGL(k_oc_criteria, dim3((unsigned)nodes), dim3(64), 0, ctx->stream, d_vol, ctx->qt_res.as<float>(),
+    hipLaunchKernelGGL(k_oc_criteria, dim3((unsigned)(2 * nodes)), dim3(64), 0, ctx->stream, d_vol, ctx->qt_res.as<float>(),
                        w, h, L, ctx->qt_nodes.as<float2>());
     SRED_TRY(time_end(ctx, nvox * L.nlev));
     SRED_HIP_CHECK(hipGetLastError());
