@@ -253,3 +253,14 @@ def test_vst2d_roi_statistics():
     v = rng.integers(0, 4000, 500)
     m, s = _roi_mean_std(v)
     assert m == pytest.approx(v.mean(), rel=1e-15) and s == pytest.approx(v.std(), rel=1e-9)
+
+
+def test_tree_plugin_argument_errors():
+    """QuadTree_ takes one 2D image; OcTree_ refuses fewer than 3 slices
+    (OcTree_.java:64-67); both before any device work."""
+    with pytest.raises(ValueError):
+        sred_amd.quadtree_stabilise_2d(np.zeros((3, 8, 8), np.float32))
+    with pytest.raises(ValueError):
+        sred_amd.octree_stabilise_3d(np.zeros((2, 8, 8), np.float32))
+    with pytest.raises(ValueError):
+        sred_amd.octree_stabilise_3d(np.zeros((8, 8), np.float32))
