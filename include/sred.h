@@ -136,6 +136,12 @@ int sred_block_repetition_2d(sred_ctx *ctx, int metric, const float *img, int w,
                              int normalize, float *out);
 
 /* ---- device path, device buffers ---- */
+/* Utils.getMeanNoiseVar2D (Utils.java:790-837) of a device image: the tile
+ * variances (getMeanAndVarBlock2D, Utils.java:854-900, float sums in the
+ * reference's order) on the GPU, the sort and the mean of the lowest
+ * (int)(0.03 nBlocks + 1) on the host, as the _dev pipelines use it; the
+ * same float as sred_noise_var_2d.  Synchronises. */
+int sred_noise_var_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var);
 /* Whole global-repetition pipeline of GlobalRepetition2D_.run
  * (GlobalRepetition2D_.java:124-145, input already normalised, VST off):
  * statistics, noise variance, relevance mask, nPixels, accumulators and

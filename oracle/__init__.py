@@ -188,6 +188,15 @@ def footprint(bw, bh):
     return k, np.array(dx[:], dtype=np.int32), np.array(dy[:], dtype=np.int32)
 
 
+OR_MAX_K = 3200  # largest footprint the C restatement holds per thread (sred_oracle.h)
+
+
+def _check_k(bw, bh):
+    k = lib().or_footprint(bw, bh, None, None)
+    if k > OR_MAX_K:
+        raise ValueError(f"block {bw}x{bh} has {k} pixels, more than the oracle's {OR_MAX_K}")
+
+
 def java_float_literal(f):
     return lib().or_java_float_literal(float(np.float32(f)))
 
@@ -210,6 +219,7 @@ def reference_block(block):
 
 
 def local_stats_f32(img, bw, bh):
+    _check_k(bw, bh)
     img = _f32(img)
     h, w = img.shape
     m = np.zeros_like(img)
@@ -278,6 +288,7 @@ def _rows(rows):
 
 
 def global_accum_f32(metric, img, means, stds, bw, bh, filter_param, threshold, rows=None, nthreads=0):
+    _check_k(bw, bh)
     img = _f32(img)
     h, w = img.shape
     S = np.zeros_like(img)
@@ -289,6 +300,7 @@ def global_accum_f32(metric, img, means, stds, bw, bh, filter_param, threshold, 
 
 
 def global_accum_f64(metric, img, stds32, bw, bh, filter_param, threshold, rows=None, nthreads=0):
+    _check_k(bw, bh)
     img = _f32(img)
     h, w = img.shape
     S = np.zeros(img.shape, np.float64)
@@ -324,6 +336,7 @@ def global_finalize_f64(metric, S, W, mask, n_pixels, c, normalize, bw, bh):
 
 def global_map(metric, img, bw, bh, c=0.0, normalize=True, fp64=False, nthreads=0):
     """Full global-repetition map (refsim float32, or oracle64 with fp64=True)."""
+    _check_k(bw, bh)
     img = _f32(img)
     h, w = img.shape
     if fp64:

@@ -90,16 +90,28 @@ int or_reference_block(const float *blk, int bw, int bh, float *px, float *mean,
     return k;
 }
 
+/* Entry points with per-thread block buffers of OR_MAX_K elements refuse
+ * larger footprints by filling their outputs with NaN (or returning -1). */
+static void or_fill_nan(float *a, float *b, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) { a[i] = NAN; b[i] = NAN; }
+}
+static void or_fill_nan64(double *a, double *b, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++) { a[i] = NAN; b[i] = NAN; }
+}
+
 /* ---- kernelGetLocalStatistics2D.cl:9-55 ---- */
 void or_local_stats_f32(const float *img, int w, int h, int bw, int bh, float *means, float *stds)
 {
     int brw = bw / 2, brh = bh / 2;
     int K = or_footprint(bw, bh, NULL, NULL);
+    if (K > OR_MAX_K) { or_fill_nan(means, stds, (int64_t)w * h); return; }
     int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
     or_footprint(bw, bh, dx, dy);
 #pragma omp parallel for schedule(static)
     for (int gy = 0; gy < h; gy++) {
-        float blk[1024];
+        float blk[OR_MAX_K];
         for (int gx = 0; gx < w; gx++) {
             int64_t o = (int64_t)gy * w + gx;
             if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { means[o] = 0.0f; stds[o] = 0.0f; continue; }
@@ -274,6 +286,7 @@ void or_global_accum_f32(int metric, const float *img, const float *means, const
 {
     const int brw = bw / 2, brh = bh / 2;
     const int K = or_footprint(bw, bh, NULL, NULL);
+    if (K > OR_MAX_K) { or_fill_nan(S, Wsum, (int64_t)w * h); return; }
     int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
     or_footprint(bw, bh, dx, dy);
     const double fp_d = or_java_float_literal(filter_param);
@@ -290,7 +303,7 @@ void or_global_accum_f32(int metric, const float *img, const float *means, const
         int64_t o = rows ? rows[r] : r;
         int gx = (int)(o % w), gy = (int)(o / w);
         float acc = 0.0f, wacc = 0.0f;
-        float rb[1024], tb[1024];
+        float rb[OR_MAX_K], tb[OR_MAX_K];
         if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) {
             /* border: map written 0 (NRMSE leaves it unwritten), weights untouched */
             S[o] = 0.0f; Wsum[o] = 0.0f;
@@ -376,6 +389,7 @@ void or_global_accum_f64(int metric, const float *img, const float *stds32, int 
 {
     const int brw = bw / 2, brh = bh / 2;
     const int K = or_footprint(bw, bh, NULL, NULL);
+    if (K > OR_MAX_K) { or_fill_nan64(S, Wsum, (int64_t)w * h); return; }
     int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
     or_footprint(bw, bh, dx, dy);
     const double fp_d = or_java_float_literal(filter_param);
@@ -414,7 +428,7 @@ void or_global_accum_f64(int metric, const float *img, const float *stds32, int 
         float rs32 = stds32[o];
         if ((double)(rs32 * rs32) < thr_d) { S[o] = 0.0; Wsum[o] = 0.0; continue; }
         double rs = sqrt(v64[o]), rm = m64[o];
-        double rb[1024];
+        double rb[OR_MAX_K];
         for (int i = 0; i < K; i++) rb[i] = (double)img[o + (int64_t)dy[i] * w + dx[i]] - rm;
         ksum acc = {0, 0}, wacc = {0, 0};
         for (int64_t j = 0; j < nval; j++) {
@@ -533,7 +547,7 @@ int or_block_raw(int metric, const float *img, const float *means, const float *
 {
     const int brw = bw / 2, brh = bh / 2;
     const int K = or_footprint(bw, bh, NULL, NULL);
-    if (K > 1024) return -1;
+    if (K > OR_MAX_K) return -1;
     int *dx = malloc(sizeof(int) * K), *dy = malloc(sizeof(int) * K);
     or_footprint(bw, bh, dx, dy);
     const double eps_d = or_java_float_literal(EPS_F);
@@ -541,7 +555,7 @@ int or_block_raw(int metric, const float *img, const float *means, const float *
     const double mean_d = or_java_float_literal(tmpl_mean);
 #pragma omp parallel for schedule(static)
     for (int gy = 0; gy < h; gy++) {
-        float cb[1024];
+        float cb[OR_MAX_K];
         for (int gx = 0; gx < w; gx++) {
             int64_t o = (int64_t)gy * w + gx;
             if (gx < brw || gx >= w - brw || gy < brh || gy >= h - brh) { out[o] = 0.0f; continue; }
@@ -1801,6 +1815,7 @@ int or_global_repetition3d(int metric, const float *img, int w, int h, int d, in
     const int brw = bw / 2, brh = bh / 2, brd = bd / 2;
     int K;
     int64_t *off = vol_offsets(w, h, bw, bh, bd, &K);
+    if (K > 4096) { free(off); return -1; }  /* per-thread block buffers below */
     const int64_t n = (int64_t)w * h * d;
     float *means = malloc(sizeof(float) * n), *stds = malloc(sizeof(float) * n), *mask = malloc(sizeof(float) * n);
     double *S = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n);

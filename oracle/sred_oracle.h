@@ -34,6 +34,11 @@ extern "C" {
 
 enum { OR_PEARSON = 0, OR_SSIM = 1, OR_NRMSE = 2, OR_ABSDIFF = 3 };
 
+/* Largest footprint the block-vector restatements hold per thread (the HIP
+ * library's cap: a 63 x 63 ellipse has 3109 pixels).  Larger footprints are
+ * refused: NaN outputs from the void entry points, -1 from the others. */
+#define OR_MAX_K 3200
+
 /* Elliptical footprint, GlobalRepetition2D_.java:102-111 and the kernel
  * predicate kernelGetGlobalPearson2D.cl:44-50.  Writes the row-major offsets
  * (dx, dy) of the K member pixels if dx/dy are non-NULL; returns K. */

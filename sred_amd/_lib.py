@@ -29,7 +29,7 @@ OPT_FIT_TIMING = 6
 EXPORTS = [
     "sred_version", "sred_last_error", "sred_ctx_create", "sred_ctx_destroy", "sred_ctx_set_stream",
     "sred_ctx_synchronize", "sred_ctx_set_chunk_cols", "sred_ctx_set_option", "sred_metric_from_name", "sred_block_size_2d",
-    "sred_normalize_input_2d", "sred_reference_block_2d", "sred_noise_var_2d", "sred_relevance_mask_2d",
+    "sred_normalize_input_2d", "sred_reference_block_2d", "sred_noise_var_2d", "sred_noise_var_2d_dev", "sred_relevance_mask_2d",
     "sred_apply_mask_2d", "sred_normalize_map_2d", "sred_local_stats_2d", "sred_global_accum_2d",
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
@@ -108,6 +108,7 @@ def load():
         "sred_normalize_input_2d": ([_F, _I64], _I),
         "sred_reference_block_2d": ([_F, _I, _I, _F, _F, _F], _I),
         "sred_noise_var_2d": ([_F, _I, _I, _F], _I),
+        "sred_noise_var_2d_dev": ([_VP, _VP, _I, _I, _F], _I),
         "sred_relevance_mask_2d": ([_F, _I, _I, _I, _I, _FL, _FL, _F, _F, _F], _I),
         "sred_apply_mask_2d": ([_F, _I, _I, _F], _I),
         "sred_normalize_map_2d": ([_F, _I, _I, _I, _I, _F], _I),

@@ -197,6 +197,18 @@ def variance_stabilising_transform_3d(volume, gain=1.0, offset=10.0, sigma=100.0
             res.close()
 
 
+def _clip_roi(roi, width, height):
+    """The rectangle ip.getRoi() returns for roi = (x, y, w, h): ImageJ's
+    ImageProcessor.setRoi keeps its intersection with the image, and an
+    empty intersection becomes the empty rectangle (0, 0, 0, 0)."""
+    x, y, w, h = (int(v) for v in roi)
+    x0, y0 = max(x, 0), max(y, 0)
+    x1, y1 = min(x + w, width), min(y + h, height)
+    if x1 <= x0 or y1 <= y0:
+        return 0, 0, 0, 0
+    return x0, y0, x1 - x0, y1 - y0
+
+
 def _roi_mean_std(values):
     """VarianceStabilisingTransform2D_.meanAndStdDev (:116-133): double sums in
     order, mean and sqrt(sq/n - mean^2); (0, 0) for no values."""
@@ -225,7 +237,7 @@ def variance_stabilising_transform_2d(image, gain=1.0, offset=10.0, sigma=100.0,
     if arr.ndim != 2:
         raise ValueError("expected a 2D image")
     if roi is not None:
-        x, y, w, h = (int(v) for v in roi)
+        x, y, w, h = _clip_roi(roi, arr.shape[1], arr.shape[0])
         vals = arr[y:y + h, x:x + w]
         if np.issubdtype(vals.dtype, np.floating):
             vals = np.ascontiguousarray(vals, np.float32).view(np.int32)

@@ -745,6 +745,16 @@ int sred_block_repetition_2d(sred_ctx *ctx, int metric, const float *img, int w,
     return SRED_OK;
 }
 
+int sred_noise_var_2d_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!d_img || !noise_var || w <= 0 || h <= 0) {
+        set_error("invalid argument");
+        return SRED_ERR_ARG;
+    }
+    return noise_var_dev(ctx, d_img, w, h, noise_var);
+}
+
 int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float *d_img, int w, int h, int bw,
                            int bh, float relevance_constant, int normalize, float *d_out, float *d_means,
                            float *d_stds, float *d_mask, float *d_S, float *d_Wsum)

@@ -189,8 +189,8 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
             set_error("null buffer in batch problem " + std::to_string(i));
             return SRED_ERR_ARG;
         }
-        if ((x.g.K + 31) / 32 > 8) {
-            set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+        if ((x.g.K + 31) / 32 > kMaxKS) {
+            set_error("the batched split-BF16 contraction supports block sizes up to K = 256 pixels");
             return SRED_ERR_ARG;
         }
         x.d_img = probs[i].d_img;

@@ -598,11 +598,19 @@ __device__ __forceinline__ void tile_to_lds(const uint4 *Uhi, const uint4 *Ulo, 
 __device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0); }
 
 // Weighted metrics (Pearson, SSIM) skip pairs whose t values are more than
-// kSkipGap apart: such a pair's weight exp2(-(t_i - t_j)^2) is below 2^-40
-// and its similarity term at most 1 in magnitude, so a row sum over N blocks
-// changes by less than N * 2^-40 (< 1e-6 for N = 10^6, against a self-pair
-// term of about 1).
-constexpr float kSkipGap = 6.3245553f;  // sqrt(40)
+// skip_gap<M>() apart.  Every skipped term is then below 2^-40 in magnitude,
+// so a row sum over N blocks changes by less than N * 2^-40 (< 1e-6 for
+// N = 10^6, against a self-pair term of about 1):
+//   Pearson: max(0, r) <= 1, weight exp2(-gap^2) < 2^-40 with gap^2 = 40;
+//   SSIM:    |2cov + c2| <= s_i^2 + s_j^2 + c2 (Cauchy-Schwarz), but the
+//            luminance factor (2 m_i m_j + c1) / (m_i^2 m_j^2 + c1) of the
+//            product denominator (kernelGetGlobalSsim2D.cl:113) reaches
+//            100.5 (at m_i m_j = 0.01, for any real means), so the term is at
+//            most 100.5 w and gap^2 = 47 gives 100.5 * 2^-47 < 2^-40.
+// The 3D weighted NRMSE (sqrt(mse) * w, unbounded for inputs outside [0, 1])
+// and the unweighted NRMSE are never skipped.
+template <int M> constexpr float skip_gap() { return M == SRED_SSIM ? 6.8556546f : 6.3245553f; }  // sqrt(47), sqrt(40)
+template <int M> constexpr bool skips() { return M == SRED_PEARSON || M == SRED_SSIM; }
 
 // two 64-column tiles in flight (copy of the next overlaps the current one)
 // while their LDS fits twice beside the column accumulator at 2 workgroups/CU
@@ -627,7 +635,7 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
 // a column panel.
 //
 // Valid blocks are ordered by ascending sigma (so ascending t): for Pearson
-// and SSIM a panel pair whose t ranges are more than kSkipGap apart writes
+// and SSIM a panel pair whose t ranges are more than skip_gap apart writes
 // zero partial sums without being evaluated, and inside an evaluated panel
 // pair a wave skips each column tile whose t range is that far from its
 // rows' range.  next[1] counts the ordered pairs skipped.
@@ -689,7 +697,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     const int p = q - c * (c + 1) / 2;
     const bool diag = (p == c);
     const int col0 = c * P, row0 = p * P;
-    if (M != SRED_NRMSE && !diag && aux[col0].x - aux[row0 + P - 1].x > kSkipGap) {
+    if (skips<M>() && !diag && aux[col0].x - aux[row0 + P - 1].x > skip_gap<M>()) {
         // every pair of the panel pair has a vanishing weight
         for (int i = tid; i < P; i += 256) {
             partial[(size_t)c * Nv_pad + row0 + i] = 0.0f;
@@ -764,10 +772,10 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     };
     auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
         const int c0 = col0 + (T % ntiles) * kBN;
-        if (M != SRED_NRMSE) {
+        if (skips<M>()) {
             const float tlo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[0].x)));
             const float thi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[kBN - 1].x)));
-            if (tlo - wt_hi > kSkipGap || wt_lo - thi > kSkipGap) {  // the tile's weights vanish for these rows
+            if (tlo - wt_hi > skip_gap<M>() || wt_lo - thi > skip_gap<M>()) {  // the tile's weights vanish for these rows
                 if (!diag) stage[T & 1][wave][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 nskip += (diag ? 1ull : 2ull) * (unsigned long long)(16 * RG * kBN);
                 if (T % ntiles == ntiles - 1) end_sweep();
@@ -835,10 +843,15 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
 
 // ---------------------------------------------------------------------------
 // 3b. plain FP32 contraction (vector ALU).  64x64 pair tiles, 4x4 pairs per
-// thread; the workgroup's 64 rows stay in LDS (k-major), each column tile is
-// staged k-major, and every k step is two broadcast-free float4 LDS reads
-// and 16 FMAs.
+// thread; every k step is two broadcast-free float4 LDS reads and 16 FMAs.
+// The block vectors (k-major) are staged f32_chunk(K4) k values at a time:
+// up to K4 = 160 (80 KiB of LDS, two workgroups per CU) one chunk holds the
+// whole block and the workgroup's 64 rows stay in LDS for the whole column
+// sweep; larger blocks (any K up to the footprint cap) go in chunks of 128
+// and reload the row chunk with each column chunk.
 // ---------------------------------------------------------------------------
+__host__ __device__ constexpr int f32_chunk(int K4) { return K4 <= 160 ? K4 : 128; }
+
 template <int M>
 __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ UT, const float4 *__restrict__ aux,
                                                       const int *__restrict__ tile_small, int K4, int Nv_pad,
@@ -846,16 +859,22 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
                                                       EpiConst ec)
 {
     extern __shared__ float sm[];
+    const int KC = f32_chunk(K4);
+    const int nkc = (K4 + KC - 1) / KC;
     float *As = sm;
-    float *Bs = sm + (size_t)K4 * 64;
+    float *Bs = sm + (size_t)KC * 64;
     const int rt = blockIdx.x % nrowtiles, ch = blockIdx.x / nrowtiles;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const int row0 = rt * 64;
-    for (int i = tid; i < K4 * 16; i += 256) {
-        int k = i >> 4, q = i & 15;
-        *reinterpret_cast<float4 *>(&As[k * 64 + q * 4]) =
-            *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + row0 + q * 4]);
-    }
+    // k rows [k0, k0 + kn) of 64 blocks starting at v0 into dst[k - k0][64]
+    auto stage = [&](float *dst, int v0, int k0, int kn) {
+        for (int i = tid; i < kn * 16; i += 256) {
+            int k = i >> 4, q = i & 15;
+            *reinterpret_cast<float4 *>(&dst[k * 64 + q * 4]) =
+                *reinterpret_cast<const float4 *>(&UT[(size_t)(k0 + k) * Nv_pad + v0 + q * 4]);
+        }
+    };
+    if (nkc == 1) stage(As, row0, 0, K4);
     float4 R[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) R[i] = row_const<M>(aux[row0 + ty * 4 + i], ec);
@@ -865,27 +884,27 @@ __global__ __launch_bounds__(256) void k_contract_f32(const float *__restrict__ 
     const int c_end = min(c_begin + cols_per_chunk, Nv_pad);
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
         const bool exact = row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
-        __syncthreads();
-        for (int i = tid; i < K4 * 16; i += 256) {
-            int k = i >> 4, q = i & 15;
-            *reinterpret_cast<float4 *>(&Bs[k * 64 + q * 4]) =
-                *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + c0 + q * 4]);
-        }
-        __syncthreads();
         float d[4][4];
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) d[i][j] = 0.0f;
+        for (int k0 = 0; k0 < K4; k0 += KC) {
+            const int kn = min(KC, K4 - k0);
+            __syncthreads();
+            if (nkc > 1) stage(As, row0, k0, kn);
+            stage(Bs, c0, k0, kn);
+            __syncthreads();
 #pragma unroll 4
-        for (int k = 0; k < K4; k++) {
-            float4 a = *reinterpret_cast<const float4 *>(&As[k * 64 + ty * 4]);
-            float4 b = *reinterpret_cast<const float4 *>(&Bs[k * 64 + tx * 4]);
-            float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+            for (int k = 0; k < kn; k++) {
+                float4 a = *reinterpret_cast<const float4 *>(&As[k * 64 + ty * 4]);
+                float4 b = *reinterpret_cast<const float4 *>(&Bs[k * 64 + tx * 4]);
+                float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+                for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) d[i][j] = fmaf(av[i], bv[j], d[i][j]);
+                    for (int j = 0; j < 4; j++) d[i][j] = fmaf(av[i], bv[j], d[i][j]);
+            }
         }
         float4 C[4];
 #pragma unroll
@@ -1149,13 +1168,12 @@ int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, 
         set_error("invalid contraction plan arguments");
         return SRED_ERR_ARG;
     }
-    q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF;
     q.KS = (K + 31) / 32;
     q.K4 = (K + 3) / 4 * 4;
-    if (q.mfma && q.KS > 8) {
-        set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
-        return SRED_ERR_ARG;
-    }
+    // the split-BF16 kernels hold a block's fragments in registers up to
+    // K = 32 * kMaxKS; larger blocks take the FP32 contraction, which stages
+    // any K through LDS in chunks
+    q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && q.KS <= kMaxKS;
     q.sym = q.mfma && symmetric;
     // symmetric panels: P rows = P columns, a multiple of 256, up to 2048,
     // small enough for at least ~2000 panel pairs
@@ -1389,11 +1407,7 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
         hipLaunchKernelGGL(k_contract_absdiff, dim3(grid), dim3(256), 0, ctx->stream, r.d_aux, Nv_pad, nrowtiles, cpc,
                            d_part, ec);
     } else {
-        size_t lds = sizeof(float) * 2 * (size_t)K4 * 64;
-        if (lds > 160 * 1024) {
-            set_error("block too large for the FP32 contraction tile");
-            return SRED_ERR_ARG;
-        }
+        const size_t lds = sizeof(float) * 2 * (size_t)f32_chunk(K4) * 64;
         if (metric == SRED_PEARSON)
             launch_f32<SRED_PEARSON>(grid, lds, ctx->stream, r.u32, r.d_aux, r.d_tsm, K4, Nv_pad, nrowtiles, cpc,
                                      d_part, ec);

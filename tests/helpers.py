@@ -59,3 +59,36 @@ def kat_reference_block(block, footprint_fn):
     s = np.float32(s / np.float32(np.float32(k) - np.float32(1)))
     s = np.float32(np.sqrt(np.float64(s)))
     return px, float(m), float(s)
+
+
+def fp64_map_at(O, metric, img, bw, bh, c, pixels):
+    """oracle64's normalised global map at the given pixels (flat indices)
+    without computing every row: the FP64 accumulators of those rows only,
+    finalised with the mask restricted to them, so the normalisation's
+    min/max come from the given pixels.  Pass the map's extreme pixels among
+    them and the values equal the full map's."""
+    import numpy as np
+
+    h, w = img.shape
+    m32, s32 = O.local_stats_f32(img, bw, bh)
+    nv = O.noise_var(img)
+    mask, thr, npx = O.relevance_mask(s32, bw // 2, bh // 2, nv, c)
+    rows = np.unique(np.asarray(pixels, np.int64))
+    S, W = O.global_accum_f64(metric, img, s32, bw, bh, nv, thr, rows=rows)
+    sub = np.zeros_like(mask)
+    sub.ravel()[rows] = mask.ravel()[rows]
+    out = O.global_finalize_f64(metric, S, W, sub, npx, c, True, bw, bh)
+    return out.ravel()[np.asarray(pixels, np.int64)], mask
+
+
+def sample_pixels(gpu_map, mask, n, seed):
+    """n seeded interior pixels with mask > 0 plus the 4 smallest and 4
+    largest values of gpu_map among them (flat indices)."""
+    import numpy as np
+
+    idx = np.flatnonzero(mask.ravel() > 0)
+    rng = np.random.default_rng(seed)
+    vals = gpu_map.ravel()[idx]
+    order = np.argsort(vals, kind="stable")
+    extremes = np.concatenate([idx[order[:4]], idx[order[-4:]]])
+    return np.unique(np.concatenate([rng.choice(idx, min(n, idx.size), replace=False), extremes]))

@@ -147,7 +147,7 @@ def test_normalise_matches_oracle_bitwise():
 # launch geometry of the global contraction (host only)
 # ---------------------------------------------------------------------------
 NV_CASES = [1, 2, 63, 64, 255, 256, 257, 1000, 3876, 8320, 27936, 63504, 254016, 1028196, 4137156]
-K_CASES = [5, 13, 29, 49, 81, 113, 149, 197]
+K_CASES = [5, 13, 29, 49, 81, 113, 149, 197, 253, 317, 709, 3001]
 
 
 @pytest.mark.parametrize("precision", [L.FP32, L.BF16X3])
@@ -157,7 +157,9 @@ def test_contraction_plan_invariants(precision, sym):
         for k in K_CASES:
             for metric in range(4):
                 p = L.global_plan(nv, k, metric, precision, sym)
-                mfma = precision == L.BF16X3 and metric != L.ABSDIFF
+                # split BF16 up to K = 256; larger blocks take the FP32 contraction
+                mfma = precision == L.BF16X3 and metric != L.ABSDIFF and (k + 31) // 32 <= 8
+                assert bool(p["flags"] & 1) == mfma
                 is_sym = bool(p["flags"] & 2)
                 assert is_sym == (mfma and sym == 1)
                 assert p["nv_pad"] >= nv and p["nv_pad"] - nv < max(p["rows"], 256)
@@ -253,6 +255,17 @@ def test_vst2d_roi_statistics():
     v = rng.integers(0, 4000, 500)
     m, s = _roi_mean_std(v)
     assert m == pytest.approx(v.mean(), rel=1e-15) and s == pytest.approx(v.std(), rel=1e-9)
+
+
+def test_vst2d_roi_clipped_to_the_image():
+    """ImageJ keeps the intersection of a ROI with the image (an empty one
+    becomes (0, 0, 0, 0)); numpy slicing alone would wrap negative starts."""
+    from sred_amd.commands import _clip_roi
+    assert _clip_roi((4, 6, 20, 10), 64, 48) == (4, 6, 20, 10)
+    assert _clip_roi((-5, -3, 20, 10), 64, 48) == (0, 0, 15, 7)
+    assert _clip_roi((50, 40, 30, 30), 64, 48) == (50, 40, 14, 8)
+    assert _clip_roi((-10, 5, 5, 5), 64, 48) == (0, 0, 0, 0)
+    assert _clip_roi((70, 5, 5, 5), 64, 48) == (0, 0, 0, 0)
 
 
 def test_tree_plugin_argument_errors():

@@ -94,3 +94,33 @@ def test_batch_errors():
         with pytest.raises(sred_amd.SredError):
             ctx.set_option(L.OPT_BATCH_PANEL, 300)
         global_batch_device(ctx, 0, [])
+
+
+BIG = [((640, 1000), 17), ((1024, 520), 13), ((700, 512), 5), ((512, 900), 9)]
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_batch_large_nonsquare_images_with_b17(metric):
+    """Images of 512-1024 px per side (non-square) at block sizes 5..17 in
+    one batch: each map equals the single-image path's and matches FP64 on
+    seeded rows plus its extremes."""
+    from tests.helpers import fp64_map_at, sample_pixels
+    torch = pytest.importorskip("torch")
+    imgs = [O.normalize_input(synthetic.spot_ring_mixture(s[1], s[0], seed=60 + i)) for i, (s, _) in enumerate(BIG)]
+    with sred_amd.Context(0) as ctx:
+        ctx.set_option(L.OPT_AUTOTUNE, 0)  # largest panel that fits (the autotuner is covered above)
+        d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+        d_outs = [torch.full_like(d, 7.0) for d in d_imgs]
+        probs = [(d.data_ptr(), im.shape[1], im.shape[0], b, b, o.data_ptr())
+                 for d, im, o, (_, b) in zip(d_imgs, imgs, d_outs, BIG)]
+        global_batch_device(ctx, metric, probs, 0.0, True)
+        torch.cuda.synchronize()
+        outs = [o.cpu().numpy() for o in d_outs]
+        for i, ((shape, b), im, out) in enumerate(zip(BIG, imgs, outs)):
+            single = run_single(ctx, metric, im, b)
+            assert np.abs(out - single).max() <= 2e-6, (shape, b)
+            m32, s32 = O.local_stats_f32(im, b, b)
+            mask, _, _ = O.relevance_mask(s32, b // 2, b // 2, O.noise_var(im), 0.0)
+            px = sample_pixels(out, mask, 40, seed=70 + i)
+            ref, _ = fp64_map_at(O, metric, im, b, b, 0.0, px)
+            assert np.abs(out.ravel()[px] - ref).max() <= MAP_TOL, (shape, b)

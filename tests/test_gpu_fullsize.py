@@ -1,0 +1,167 @@
+"""GPU parity at the benchmark workloads' full sizes, block shapes beyond the
+split-BF16 range, non-square global blocks and the device noise variance.
+
+Bars: the normalised global maps within 1e-4 absolute of the FP64 oracle
+(oracle64) and, where the whole float32 restatement (refsim) finishes in
+seconds, of refsim too; block maps and the noise variance bit-exact.  Maps
+too large for every oracle row are checked on seeded rows plus the map's
+extremes (tests.helpers.fp64_map_at), which fixes the normalisation.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import sred_amd
+from sred_amd import _lib as L
+from sred_amd import synthetic, utils
+from tests.helpers import fp64_map_at, sample_pixels
+
+pytestmark = pytest.mark.gpu
+GM = sred_amd.GLOBAL_METRICS
+BM = sred_amd.BLOCK_METRICS
+MAP_TOL = 1e-4
+
+
+def noise_var_dev(ctx, img):
+    torch = pytest.importorskip("torch")
+    d = torch.from_numpy(np.ascontiguousarray(img, np.float32)).cuda()
+    out = np.zeros(1, np.float32)
+    L.check(L.load().sred_noise_var_2d_dev(ctx.handle, d.data_ptr(), img.shape[1], img.shape[0], L.fptr(out)))
+    return out[0]
+
+
+@pytest.mark.parametrize("shape,seed", [((256, 256), 0), ((288, 352), 1), ((289, 352), 2), ((131, 97), 3),
+                                        ((1024, 1024), 4), ((8, 8), 5), ((17, 23), 6)])
+def test_noise_variance_device_bitexact(ctx, shape, seed):
+    """The device noise variance every _dev and batched pipeline uses
+    (tile variances on the GPU, sort and lowest-3% mean on the host) against
+    the oracle's Utils.getMeanNoiseVar2D, across the 8x8 / 16x16 tile switch
+    at 352 x 288 pixels."""
+    img = O.normalize_input(synthetic.spots(shape[1], shape[0], max(3, shape[0] * shape[1] // 300), seed=seed))
+    got = noise_var_dev(ctx, img)
+    want = np.float32(O.noise_var(img))
+    assert got.tobytes() == want.tobytes(), (got, want)
+    assert got.tobytes() == np.float32(utils.get_mean_noise_var_2d(img, img.shape[1], img.shape[0])).tobytes()
+
+
+# ---------------------------------------------------------------------------
+# 256 x 256, 5 x 5 global Pearson: the reference's CPU-runnable case, whole map
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def spots256():
+    img = O.normalize_input(synthetic.spots(256, 256, 200, seed=0))
+    return img, O.global_map(O.PEARSON, img, 5, 5, 0.0, True), O.global_map(O.PEARSON, img, 5, 5, 0.0, True,
+                                                                           fp64=True)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_spots256_b5_full_map(ctx, spots256, precision):
+    img, ref32, ref64 = spots256
+    out = sred_amd.global_repetition_2d(img, 5, 5, 0.0, "Pearson's R", True, precision, resources=ctx)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+    assert np.abs(out - ref32).max() <= MAP_TOL
+
+
+# ---------------------------------------------------------------------------
+# 1024 x 1024, 11 x 11: Pearson, SSIM and NRMSE in both precisions
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def lattice1024():
+    return O.normalize_input(synthetic.lattice_and_blobs(1024, seed=2))
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_lattice1024_b11_sampled_rows(ctx, lattice1024, metric, precision):
+    img = lattice1024
+    out = sred_amd.global_repetition_2d(img, 11, 11, 0.0, GM[metric], True, precision, resources=ctx)
+    m32, s32 = O.local_stats_f32(img, 11, 11)
+    mask, _, _ = O.relevance_mask(s32, 5, 5, O.noise_var(img), 0.0)
+    px = sample_pixels(out, mask, 96, seed=100 + metric)
+    ref, _ = fp64_map_at(O, metric, img, 11, 11, 0.0, px)
+    err = np.abs(out.ravel()[px] - ref)
+    assert err.max() <= MAP_TOL, (err.max(), px[np.argmax(err)])
+    # the minimum maps to 0; the maximum stays below 1 because the reference
+    # divides by (max - min + EPSILON) with a range of ~1e-6 here (maps scale
+    # as 1/nPixels)
+    assert out[mask > 0].min() == 0.0
+
+
+# ---------------------------------------------------------------------------
+# block shapes: beyond the split-BF16 range (K > 256) and non-square
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("metric", [0, 1, 2, 3])
+@pytest.mark.parametrize("b,shape", [(21, (90, 80)), (31, (72, 66))])
+def test_global_large_blocks_default_precision(ctx, metric, b, shape):
+    """Blocks the reference dialog accepts beyond K = 256 (21x21: 317 px,
+    31x31: 709 px) through the default (split-BF16) precision: the plan takes
+    the FP32 contraction, the map matches the oracle."""
+    assert sred_amd.block_size_2d(b, b) > 256
+    img = O.normalize_input(synthetic.spots(shape[1], shape[0], 12, seed=b + metric, sigma=4.0))
+    out = sred_amd.global_repetition_2d(img, b, b, 0.0, GM[metric], resources=ctx)
+    ref64 = O.global_map(metric, img, b, b, 0.0, True, fp64=True)
+    ref32 = O.global_map(metric, img, b, b, 0.0, True)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+    assert np.abs(out - ref32).max() <= MAP_TOL
+
+
+def test_global_large_block_device_entry(ctx):
+    """A 21x21 map through sred_global_map_2d_dev with BF16X3 requested."""
+    torch = pytest.importorskip("torch")
+    img = O.normalize_input(synthetic.spots(96, 84, 15, seed=7, sigma=4.0))
+    d_img = torch.from_numpy(img).cuda()
+    d_out = torch.zeros_like(d_img)
+    sred_amd.commands.global_map_device(ctx, L.PEARSON, L.BF16X3, d_img.data_ptr(), 96, 84, 21, 21, 0.0, True,
+                                        d_out.data_ptr())
+    torch.cuda.synchronize()
+    ref = O.global_map(O.PEARSON, img, 21, 21, 0.0, True, fp64=True)
+    assert np.abs(d_out.cpu().numpy() - ref).max() <= MAP_TOL
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("metric", [0, 1, 2, 3])
+@pytest.mark.parametrize("bw,bh", [(9, 5), (5, 11), (3, 7)])
+def test_global_non_square_blocks(ctx, metric, precision, bw, bh):
+    img = O.normalize_input(synthetic.spots(83, 71, 30, seed=bw * 10 + bh))
+    out = sred_amd.global_repetition_2d(img, bw, bh, 0.0, GM[metric], True, precision, resources=ctx)
+    ref64 = O.global_map(metric, img, bw, bh, 0.0, True, fp64=True)
+    ref32 = O.global_map(metric, img, bw, bh, 0.0, True)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+    assert np.abs(out - ref32).max() <= MAP_TOL
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2, 3])
+@pytest.mark.parametrize("b", [21, 31])
+def test_block_maps_large_templates_bitexact(ctx, metric, b):
+    img = synthetic.rings_and_blobs(240, 30, 30, seed=metric + b)
+    tmpl = synthetic.ring_template(b, radius=b / 3.0)
+    rb = utils.get_reference_block_2d(tmpl)
+    inp = utils.get_input_image_2d(img)
+    out = sred_amd.clutils.calculate_block_repetition_map_2d(BM[metric], inp, rb, 0.0, True, ctx)
+    ref = O.block_map(metric, inp.image_array, rb.pixels, b, b, rb.mean, rb.std, 0.0, True)
+    np.testing.assert_array_equal(out.reshape(img.shape), ref)
+
+
+# ---------------------------------------------------------------------------
+# SSIM with skipped pairs where the luminance factor is large
+# ---------------------------------------------------------------------------
+def test_ssim_skip_with_means_near_one_tenth(ctx):
+    """Two textures with local means near 0.1 (where the product-denominator
+    luminance factor (2 m_i m_j + c1)/(m_i^2 m_j^2 + c1) peaks near 100) and
+    block sigmas far apart (a quiet background and a sparse bright texture):
+    the symmetric kernel skips the cross pairs and the map matches FP64."""
+    rng = np.random.default_rng(29)
+    h, w, b = 150, 170, 5
+    img = (0.1 + 0.003 * rng.standard_normal((h, w))).astype(np.float32)
+    right = np.where(rng.random((h, w - w // 2)) < 0.05, 1.0, 0.05).astype(np.float32)
+    img[:, w // 2:] = right
+    img[0, 0], img[-1, -1] = 0.0, 1.0  # fixes the input normalisation to the identity
+    inp = utils.get_input_image_2d(img)
+    out = sred_amd.global_repetition_2d(img, b, b, 0.0, "SSIM", resources=ctx)
+    n = (h - b + 1) * (w - b + 1)
+    assert ctx.last_skipped() > 0.3 * n * n
+    m32, _ = O.local_stats_f32(inp.image_array, b, b)
+    inner = m32[2:-2, 2:-2]
+    assert np.median(inner) == pytest.approx(0.1, abs=0.03)
+    ref = O.global_map(O.SSIM, inp.image_array, b, b, 0.0, True, fp64=True)
+    assert np.abs(out - ref).max() <= MAP_TOL

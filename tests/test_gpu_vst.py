@@ -100,3 +100,9 @@ def test_vst2d_plugin(ctx):
     with np.errstate(invalid="ignore"):
         want = np.where(g * x + ref < 0.0, 0.0, (2.0 / g) * np.sqrt(g * x + ref)).astype(np.float32)
     np.testing.assert_array_equal(out, want)
+    # a ROI partly outside the image: its clipped rectangle's statistics
+    out2, (g2, s2, o2) = sred_amd.variance_stabilising_transform_2d(img, max_iter=200, roi=(-5, 70, 20, 20),
+                                                                   resources=ctx)
+    mean2, std2 = _roi_mean_std(img[70:80, 0:15])
+    p2, _ = O.gat_minimize(img.astype(np.float32), 1.0, std2, mean2, 200)
+    assert (g2, s2, o2) == tuple(p2)

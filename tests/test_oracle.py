@@ -290,3 +290,32 @@ def test_refsim_vs_oracle64_normalised_maps_are_close():
         a = O.global_map(metric, img, 5, 5, 0.0, True)
         b = O.global_map(metric, img, 5, 5, 0.0, True, fp64=True)
         assert np.abs(a - b).max() < 1e-4
+
+
+@pytest.mark.parametrize("metric", [O.PEARSON, O.SSIM, O.NRMSE, O.ABSDIFF])
+def test_global_oracle_large_blocks_match_brute_force(metric):
+    """Blocks beyond 1024 pixels (41x41: 1257 pixels) against the numpy brute
+    force, through the same per-thread buffers the parity tests use."""
+    b = 41
+    assert O.footprint(b, b)[0] > 1024
+    rng = np.random.default_rng(41)
+    img = O.normalize_input(rng.random((b + 7, b + 5)).astype(np.float32))
+    m, s = O.local_stats_f32(img, b, b)
+    nv = O.noise_var(img)
+    S64, W64 = O.global_accum_f64(metric, img, s, b, b, nv, 0.0)
+    Sb, Wb = brute_force_f64(metric, img, b)
+    np.testing.assert_allclose(S64, Sb, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(W64, Wb, rtol=1e-9, atol=1e-12)
+    S32, W32 = O.global_accum_f32(metric, img, m, s, b, b, nv, 0.0)
+    np.testing.assert_allclose(S32, S64, rtol=2e-4, atol=2e-5)
+
+
+def test_oracle_refuses_footprints_beyond_its_buffers():
+    img = np.zeros((70, 70), np.float32)
+    assert O.footprint(65, 65)[0] > O.OR_MAX_K
+    with pytest.raises(ValueError):
+        O.local_stats_f32(img, 65, 65)
+    with pytest.raises(ValueError):
+        O.global_map(O.PEARSON, img, 65, 65)
+    with pytest.raises(ValueError):
+        O.global_accum_f64(O.PEARSON, img, img, 65, 65, 0.01, 0.0)
