@@ -157,6 +157,21 @@ int sred_global_map_2d_dev(sred_ctx *ctx, int metric, int precision, const float
 int sred_block_map_2d_dev(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl,
                           int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
                           int normalize, float *d_out);
+/* sred_block_map_2d_dev without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops over frames, harnesses that forbid hipMalloc).  Same arguments and
+ * result as sred_block_map_2d_dev; every scratch buffer (template table,
+ * min/max partials, normalisation words and, for relevance_constant != 0 or
+ * NRMSE, the window statistics, mask and noise tiles) lives in one 64 MiB
+ * device array per GPU that the library holds until the process ends.
+ * Limit: calls whose scratch exceeds it are refused with SRED_ERR_ARG
+ * (relevance 0 Pearson/SSIM/AbsDiff need 12 B per 1024 pixels beyond the
+ * template; a relevance constant needs 12 B per pixel, up to about 5.5 M
+ * pixels; NRMSE at relevance 0 needs 4 B per pixel, up to about 16 M).  Calls that overlap on one GPU share the array
+ * and race; each call rewrites what it reads. */
+int sred_block_map_2d_dev_noalloc(sred_ctx *ctx, int metric, const float *d_img, int w, int h, const float *tmpl,
+                                  int bw, int bh, float tmpl_mean, float tmpl_std, float relevance_constant,
+                                  int normalize, float *d_out);
 /* The time-lapse branch of BlockRepetition2D_.run (BlockRepetition2D_.java:95-137)
  * on a stack of depth >= 3 frames (frame t at stack + t*w*h), without noise
  * stabilisation: the stack min-max normalised as a whole (getInputImage3D),

@@ -32,6 +32,7 @@ EXPORTS = [
     "sred_normalize_input_2d", "sred_reference_block_2d", "sred_noise_var_2d", "sred_noise_var_2d_dev", "sred_relevance_mask_2d",
     "sred_apply_mask_2d", "sred_normalize_map_2d", "sred_local_stats_2d", "sred_global_accum_2d",
     "sred_global_repetition_2d", "sred_block_repetition_2d", "sred_global_map_2d_dev", "sred_block_map_2d_dev",
+    "sred_block_map_2d_dev_noalloc",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
     "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_hu_invariants_2d", "sred_global_hu_2d",
@@ -119,6 +120,7 @@ def load():
         "sred_block_repetition_2d": ([_VP, _I, _F, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _F], _I),
         "sred_global_map_2d_dev": ([_VP, _I, _I, _VP, _I, _I, _I, _I, _FL, _I, _VP, _VP, _VP, _VP, _VP, _VP], _I),
         "sred_block_map_2d_dev": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
+        "sred_block_map_2d_dev_noalloc": ([_VP, _I, _VP, _I, _I, _F, _I, _I, _FL, _FL, _FL, _I, _VP], _I),
         "sred_ctx_last_contraction": ([_VP, _F, ctypes.POINTER(_I), ctypes.POINTER(_I64)], _I),
         "sred_ctx_last_skipped": ([_VP, ctypes.POINTER(_I64)], _I),
         "sred_gat_2d": ([_VP, _F, _I64, _D, _D, _D], _I),

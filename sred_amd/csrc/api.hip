@@ -29,6 +29,10 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 int DevBuf::ensure(size_t n)
 {
     if (n <= bytes && ptr) return SRED_OK;
+    if (fixed) {
+        set_error("lent scratch region too small (" + std::to_string(bytes) + " < " + std::to_string(n) + " bytes)");
+        return SRED_ERR_ARG;
+    }
     if (ptr) {
         hipError_t e = hipFree(ptr);
         ptr = nullptr;
@@ -51,7 +55,8 @@ int DevBuf::ensure(size_t n)
 
 void DevBuf::release()
 {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !fixed) (void)hipFree(ptr);
+    fixed = false;
     ptr = nullptr;
     bytes = 0;
 }
@@ -236,8 +241,8 @@ static int download(sred_ctx *ctx, float *h, const DevBuf &b, int64_t n)
 
 // Block-repetition host sequence of CLUtils.getBlock*2D on device maps:
 // Pearson/SSIM CLUtils.java:565-577, NRMSE :788-841, AbsDiff :938-975.
-static int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *tmpl,
-                          float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out)
+int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *tmpl,
+                   float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out)
 {
     const int64_t n = g.n();
     const bool need_stats = (c != 0.0f);
@@ -361,8 +366,23 @@ int sred_ctx_create(int device, sred_ctx **out)
         return SRED_ERR_ARG;
     }
     SRED_HIP_CHECK(hipSetDevice(device));
+    // set-up with the device current: the no-allocation entry point's
+    // scratch address on this device, and the timing events
+    unsigned char *scratch = nullptr;
+    SRED_TRY(noalloc_scratch_init(device, &scratch));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SRED_HIP_CHECK(hipEventCreate(&e0));
+    e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error(std::string("hipEventCreate: ") + hipGetErrorString(e));
+        return SRED_ERR_HIP;
+    }
     sred_ctx *c = new sred_ctx();
     c->device = device;
+    c->noalloc_scratch = scratch;
+    c->ev0 = e0;
+    c->ev1 = e1;
     *out = c;
     return SRED_OK;
 }

@@ -63,6 +63,7 @@ constexpr size_t kMiscBytes = 4096 + 3 * 4 * 1024;
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool fixed = false;  // lent memory (no-allocation calls): ensure() never reallocates, release() never frees
     int ensure(size_t n);
     void release();
     template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -160,6 +161,7 @@ struct sred_ctx {
     int last_nv = 0;  // valid blocks of the last global accumulation
     unsigned long long *last_skip = nullptr;  // device counter of pairs skipped by the last symmetric contraction
     double *h_fit = nullptr;  // pinned host doubles the GAT objective kernel writes its values to
+    unsigned char *noalloc_scratch = nullptr;  // this device's copy of the no-allocation scratch array
 };
 
 namespace sred {
@@ -355,6 +357,13 @@ int relevance_mask_stack_dev(sred_ctx *ctx, const float *d_img, int w, int h, in
 // The stack normalised as a whole, then each of its nseg frames of seg floats.
 int stack_frames_normalize_dev(sred_ctx *ctx, const float *d_in, float *d_out, int64_t seg, int nseg);
 void simple_regression_host(const double *x, const double *y, int64_t n, double *slope, double *intercept);
+// The block pipeline of sred_block_map_2d_dev (statistics, template
+// comparison, mask and the host steps of CLUtils.getBlock*2D) on device buffers.
+int block_pipeline(sred_ctx *ctx, const Geometry &g, int metric, const float *d_img, const float *tmpl,
+                   float tmpl_mean, float tmpl_std, float c, int normalize, float *d_out);
+// Address of `device`'s copy of the no-allocation scratch array (looked up
+// once per device, with the device current).
+int noalloc_scratch_init(int device, unsigned char **out);
 // Masked min/max normalisation (Utils.normalizeImage2D) on a device map.
 // mask may be NULL.  init_max selects -FLT_MAX (0) or Float.MIN_VALUE (1).
 int normalize_map_dev(sred_ctx *ctx, const Geometry &g, float *d_map, const float *d_mask, int init_max_min_value);
