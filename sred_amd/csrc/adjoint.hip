@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "sred_internal.h"
 
@@ -38,12 +39,14 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(const int *__restrict__ id
                                                       const float *__restrict__ stds, const float *__restrict__ S,
                                                       const float *__restrict__ W, const float *__restrict__ mask,
                                                       const float *__restrict__ dmap, const unsigned int *__restrict__ mm,
-                                                      float n_pixels, float tscale, float4 *__restrict__ info)
+                                                      float n_pixels, float tscale, float4 *__restrict__ info,
+                                                      float4 *__restrict__ info2)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= Nv_pad) return;
     if (v >= Nv) {
         info[v] = make_float4(0.0f, 0.0f, 0.0f, 1e30f);  // padding: weight exp2(-inf) = 0, a = b = 0
+        info2[v] = make_float4(1e30f, 0.0f, 0.0f, 0.0f);
         return;
     }
     int p = idx[v];
@@ -59,6 +62,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(const int *__restrict__ id
     double a = (s == 0.0f) ? 0.0 : g / D;
     double b = -g * rep * (double)n_pixels / D;
     info[v] = make_float4(s, (float)a, (float)b, s * tscale);
+    info2[v] = make_float4(s * tscale, (float)a, (float)b, s > 0.0f ? 1.0f / s : 0.0f);
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int value)
@@ -271,21 +275,52 @@ __global__ __launch_bounds__(256) void k_transpose_frag(const uint4 *__restrict_
     Vlo[t] = make_uint4(l[0], l[1], l[2], l[3]);
 }
 
-constexpr int kAdjWaves = 8;                         // waves per workgroup, 16 rows each
-constexpr int kAdjRows = kAdjWaves * 16;             // rows per workgroup
+// Rows per wave: 2 groups of 16 (every column fragment read from LDS serves
+// both); 4 waves per workgroup.
+constexpr int kAdjRG = 2;
+constexpr int kAdjRows = 4 * 16 * kAdjRG;  // rows per workgroup
+// Fast tiles: every sigma of the tile's rows and columns is 0 or at least
+// kAdjFastSigma, so delta = eps/(s_i s_j) <= 3.1e-4 and 1/(1 + delta) is
+// taken as 1 - delta (relative error delta^2 <= 1e-7); other tiles divide.
+constexpr float kAdjFastSigma = 0.018f;
+
+// small[g] = 1 if a block of [32g, 32g + 32) has 0 < sigma < kAdjFastSigma
+// (info.x = sigma).
+__global__ __launch_bounds__(256) void k_adjoint_small(const float4 *__restrict__ info, int ngroups,
+                                                       int *__restrict__ small)
+{
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / 32;
+    if (g >= ngroups) return;
+    const float s = info[g * 32 + (threadIdx.x & 31)].x;
+    const unsigned long long b = __ballot(s > 0.0f && s < kAdjFastSigma);
+    const unsigned int half = (threadIdx.x & 32) ? (unsigned int)(b >> 32) : (unsigned int)b;
+    if ((threadIdx.x & 31) == 0) small[g] = half != 0u;
+}
 
 // s_waitcnt vmcnt(0) expcnt(0) lgkmcnt(0): this wave's global->LDS copies and
 // LDS accesses are complete (issued before every barrier of the tile loop).
 __device__ __forceinline__ void adj_wait_all() { __builtin_amdgcn_s_waitcnt(0); }
 
+// The pair pass.  With u the Pearson-scaled blocks (u_i . u_j = cov/(s_i s_j)),
+// d = u_i . u_j, delta = eps/(s_i s_j), w = exp2(-(t_i - t_j)^2), w~ = w/(1 + delta)
+// and Q_ij = (a_i + a_j) [d > 0] w~ (symmetric in i, j), row i gathers
+//   G_i   = sum_j Q_ij u_j                          (second product, MFMA)
+//   q_i   = sum_j Q_ij d+ / (1 + delta)             (= sum_j Q_ij r_ij; from
+//           u_i . G_i less the per-pair correction sum_j Q_ij d+ delta/(1 + delta))
+//   e_i   = sum_j (Q_ij d+ + (b_i + b_j) w)(t_i - t_j)
+// and its block gradient is, with dL/ds_i = -q_i / s_i + cE e_i
+// (cE = -2/(H tscale)),
+//   (G_i / s_i + dL/ds_i u_i) / sqrt(K-1), centred over the K pixels.
+// info2 = (t, a, b, 1/s) per block (1/s = 0 for flat blocks and padding).
 template <int KS>
-__global__ __launch_bounds__(512, 1) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+__global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                          const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
-                                                         const float4 *__restrict__ info, const float *__restrict__ UT,
-                                                         int K, int Nv, int Nv_pad, float eps, float m2_over_H,
-                                                         float inv_sqrt_km1, int K16, float *__restrict__ Gc)
+                                                         const float4 *__restrict__ info2, const int *__restrict__ small,
+                                                         const float *__restrict__ UT, int K, int Nv, int Nv_pad,
+                                                         float eps, float cE, float inv_sqrt_km1, int K16,
+                                                         float *__restrict__ Gc)
 {
-    constexpr int KG = 2 * KS;
+    constexpr int KG = 2 * KS, RG = kAdjRG;
     constexpr int URUN = 4 * KS * 64, VRUN = 2 * KG * 64;  // uint4 of one 64-column tile
     // two tile buffers (distinct arrays: reading one does not wait for the copy into the other)
     __shared__ uint4 uh0[URUN], ul0[URUN], vh0[VRUN], vl0[VRUN];
@@ -294,26 +329,52 @@ __global__ __launch_bounds__(512, 1) void k_adjoint_mfma(const uint4 *__restrict
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lq = lane >> 4;
-    const int rowbase = blockIdx.x * kAdjRows + wave * 16;
+    const int rowbase = blockIdx.x * kAdjRows + wave * 16 * RG;
 
-    bf16x8 bh[KS], bl[KS];
+    bf16x8 bh[RG][KS], bl[RG][KS];
+    float4 R[RG];
+    float rinv[RG];
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-        const size_t o = ((size_t)(rowbase / 16) * KS + ks) * 64 + lane;
-        bh[ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
-        bl[ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
+    for (int rg = 0; rg < RG; rg++) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const size_t o = ((size_t)(rowbase / 16 + rg) * KS + ks) * 64 + lane;
+            bh[rg][ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
+            bl[rg][ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
+        }
+        R[rg] = info2[rowbase + rg * 16 + li];
+        rinv[rg] = R[rg].w;
+        R[rg].w *= eps;  // delta = (eps / s_i) (1 / s_j)
     }
-    const float4 R = info[rowbase + li];
-    const float rm = m2_over_H * R.x;  // (-2/H) s_i
-    floatx4 acc[KG];
-    float tot[KG][4];
+    const bool row_small = small[rowbase / 32] != 0;
+    floatx4 acc[KG][RG];
 #pragma unroll
-    for (int kg = 0; kg < KG; kg++) {
-        acc[kg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int kg = 0; kg < KG; kg++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) tot[kg][r] = 0.0f;
-    }
-    double gs = 0.0;
+        for (int rg = 0; rg < RG; rg++) acc[kg][rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // FP32 totals of the second product, kept in this wave's rows of Gc
+    // (K16 floats per row; only this wave touches them): acc is added every
+    // 32 tiles.  Lane l holds row i = l % 16 of each group, k = 16kg + 4(l/16) + r.
+    auto flush = [&](bool first) {
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++) {
+            float *row = Gc + (size_t)(rowbase + rg * 16 + li) * K16;
+#pragma unroll
+            for (int kg = 0; kg < KG; kg++) {
+                const int k = 16 * kg + 4 * lq;
+                if (k < K16) {
+                    float4 t = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *reinterpret_cast<const float4 *>(row + k);
+                    t.x += acc[kg][rg][0];
+                    t.y += acc[kg][rg][1];
+                    t.z += acc[kg][rg][2];
+                    t.w += acc[kg][rg][3];
+                    *reinterpret_cast<float4 *>(row + k) = t;
+                }
+                acc[kg][rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+        }
+    };
+    double gq[RG] = {0.0, 0.0}, ge[RG] = {0.0, 0.0};
     const int ntiles = Nv_pad / 64;
 
     // asynchronous copy of tile T: 16 KS runs of 64 uint4 (the column blocks'
@@ -322,75 +383,115 @@ __global__ __launch_bounds__(512, 1) void k_adjoint_mfma(const uint4 *__restrict
         typedef __attribute__((address_space(3))) void *lds_ptr;
         const int c0 = T * 64;
         const size_t ou = (size_t)(c0 / 16) * KS * 64, ov = (size_t)(c0 / 32) * KG * 64;
-        for (int run = wave; run < 16 * KS; run += kAdjWaves) {
+#pragma unroll
+        for (int i = 0; i < 4 * KS; i++) {
+            const int run = wave + 4 * i;
             const int a = run / (4 * KS), b = run - a * (4 * KS);
             const uint4 *src = a == 0 ? Uhi + ou : a == 1 ? Ulo + ou : a == 2 ? Vhi + ov : Vlo + ov;
             uint4 *dst = a == 0 ? duh : a == 1 ? dul : a == 2 ? dvh : dvl;
             __builtin_amdgcn_global_load_lds((const void *)(src + b * 64 + lane), (lds_ptr)(dst + b * 64), 16, 0, 0);
         }
-        if (wave == kAdjWaves - 1)
-            __builtin_amdgcn_global_load_lds((const void *)(info + c0 + lane), (lds_ptr)dci, 16, 0, 0);
+        if (wave == 3) __builtin_amdgcn_global_load_lds((const void *)(info2 + c0 + lane), (lds_ptr)dci, 16, 0, 0);
+    };
+
+    // first-product dot tiles of column group cg for both row groups
+    auto dots = [&](const uint4 *uh, const uint4 *ul, int cg, floatx4 (&d)[RG]) {
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++) d[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, uh[(cg * KS + ks) * 64 + lane]);
+            const bf16x8 al = __builtin_bit_cast(bf16x8, ul[(cg * KS + ks) * 64 + lane]);
+#pragma unroll
+            for (int rg = 0; rg < RG; rg++) {
+                d[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[rg][ks], d[rg], 0, 0, 0);
+                d[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[rg][ks], d[rg], 0, 0, 0);
+                d[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[rg][ks], d[rg], 0, 0, 0);
+            }
+        }
+    };
+    // pair terms of column group cg (half h of its step) for row group rg:
+    //   Q into pv[h*4 + r]; corr += Q d+ (1 - 1/(1 + delta)); se += (Q d+ + (b_i + b_j) w)(t_i - t_j)
+    // Fast tiles take 1 - 1/(1 + delta) = delta and w~ = w (1 - delta) (relative error delta^2 <= 1e-7).
+    auto pairs = [&](auto exact_tag, const float4 *ci, int cg, int h, int rg, const floatx4 &d, float (&pv)[8],
+                     float &corr, float &se) {
+        constexpr bool EXACT = decltype(exact_tag)::value;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float4 C = ci[cg * 16 + 4 * lq + r];  // column j = c0 + 16cg + 4(l/16) + r
+            const float dt = R[rg].x - C.x;
+            const float w = __builtin_amdgcn_exp2f(-(dt * dt));
+            const float delta = R[rg].w * C.w;
+            float wt, om;  // w / (1 + delta), 1 - 1/(1 + delta)
+            if (EXACT) {
+                const float kap = __builtin_amdgcn_rcpf(1.0f + delta);
+                wt = w * kap;
+                om = delta * kap;
+            } else {
+                wt = fmaf(-w, delta, w);
+                om = delta;
+            }
+            const float dv = d[r];
+            const float dp = __builtin_amdgcn_fmed3f(dv, 0.0f, FLT_MAX);
+            const float aw = (R[rg].y + C.y) * wt;
+            const float Q = dv > 0.0f ? aw : 0.0f;
+            const float Qd = Q * dp;
+            corr = fmaf(Qd, om, corr);
+            se = fmaf(fmaf(R[rg].z + C.z, w, Qd), dt, se);
+            pv[h * 4 + r] = Q;
+        }
+    };
+    // second-product MFMAs of step st for row group rg (coefficients pv)
+    auto grads = [&](const uint4 *vh, const uint4 *vl, int st, int rg, const float (&pv)[8]) {
+        unsigned int ph[4], pl[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) split_bf16x2(pv[2 * q], pv[2 * q + 1], ph[q], pl[q]);
+        const bf16x8 bH = __builtin_bit_cast(bf16x8, make_uint4(ph[0], ph[1], ph[2], ph[3]));
+        const bf16x8 bL = __builtin_bit_cast(bf16x8, make_uint4(pl[0], pl[1], pl[2], pl[3]));
+#pragma unroll
+        for (int kg = 0; kg < KG; kg++) {
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, vh[(st * KG + kg) * 64 + lane]);
+            const bf16x8 al = __builtin_bit_cast(bf16x8, vl[(st * KG + kg) * 64 + lane]);
+            acc[kg][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bH, acc[kg][rg], 0, 0, 0);
+            acc[kg][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bL, acc[kg][rg], 0, 0, 0);
+            acc[kg][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bH, acc[kg][rg], 0, 0, 0);
+        }
+    };
+    // one 32-column step: the dot tiles of its second half are issued before
+    // the pair terms of the first, and row group 0's gradient MFMAs before
+    // row group 1's last pair terms, so the matrix core has work while the
+    // vector ALU evaluates pairs
+    auto step32 = [&](auto exact_tag, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl,
+                      const float4 *ci, int st, float (&corr)[RG], float (&se)[RG]) {
+        float pv[RG][8];
+        floatx4 d0[RG], d1[RG];
+        dots(uh, ul, 2 * st, d0);
+        dots(uh, ul, 2 * st + 1, d1);
+#pragma unroll
+        for (int rg = 0; rg < RG; rg++) pairs(exact_tag, ci, 2 * st, 0, rg, d0[rg], pv[rg], corr[rg], se[rg]);
+        pairs(exact_tag, ci, 2 * st + 1, 1, 0, d1[0], pv[0], corr[0], se[0]);
+        grads(vh, vl, st, 0, pv[0]);
+        pairs(exact_tag, ci, 2 * st + 1, 1, 1, d1[1], pv[1], corr[1], se[1]);
+        grads(vh, vl, st, 1, pv[1]);
     };
 
     auto compute = [&](int T, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl, const float4 *ci) {
-        float gsum = 0.0f;
-        // two 32-column steps: the coefficients of a step feed its second-product MFMAs at once
+        const int c0 = T * 64;
+        const bool exact = row_small || (small[c0 / 32] | small[c0 / 32 + 1]) != 0;
+        float corr[RG], se[RG];
 #pragma unroll
+        for (int rg = 0; rg < RG; rg++) { corr[rg] = 0.0f; se[rg] = 0.0f; }
+#pragma unroll 1
         for (int st = 0; st < 2; st++) {
-            float pv[8];  // coefficients P of the step, B-operand element order
-#pragma unroll
-            for (int half = 0; half < 2; half++) {
-                const int cg = 2 * st + half;
-                floatx4 d = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int ks = 0; ks < KS; ks++) {
-                    const bf16x8 ah = __builtin_bit_cast(bf16x8, uh[(cg * KS + ks) * 64 + lane]);
-                    const bf16x8 al = __builtin_bit_cast(bf16x8, ul[(cg * KS + ks) * 64 + lane]);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ks], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ks], d, 0, 0, 0);
-                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ks], d, 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float4 C = ci[cg * 16 + 4 * lq + r];  // column j = c0 + 16cg + 4(l/16) + r
-                    const float si = R.x, sj = C.x;
-                    const float dt = R.w - C.w;
-                    const float w = __builtin_amdgcn_exp2f(-(dt * dt));
-                    const float A = R.y + C.y;
-                    const float ss = si * sj;
-                    const float inv = __builtin_amdgcn_rcpf(ss + eps);
-                    const float rv = d[r] * (ss * inv);  // cov / (s_i s_j + eps); 0 for flat blocks
-                    const float rr = fmaxf(rv, 0.0f);
-                    const float M = rv > 0.0f ? w * inv : 0.0f;
-                    const float P = A * M * sj;
-                    // -A M r s_j + w (-2/H)(s_i - s_j)(A r + b_i + b_j)
-                    gsum = fmaf(w * (rm - m2_over_H * sj), fmaf(A, rr, R.z + C.z), fmaf(-P, rr, gsum));
-                    pv[half * 4 + r] = P;
-                }
-            }
-            unsigned int ph[4], pl[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) split_bf16x2(pv[2 * q], pv[2 * q + 1], ph[q], pl[q]);
-            const bf16x8 bH = __builtin_bit_cast(bf16x8, make_uint4(ph[0], ph[1], ph[2], ph[3]));
-            const bf16x8 bL = __builtin_bit_cast(bf16x8, make_uint4(pl[0], pl[1], pl[2], pl[3]));
-#pragma unroll
-            for (int kg = 0; kg < KG; kg++) {
-                const bf16x8 ah = __builtin_bit_cast(bf16x8, vh[(st * KG + kg) * 64 + lane]);
-                const bf16x8 al = __builtin_bit_cast(bf16x8, vl[(st * KG + kg) * 64 + lane]);
-                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bH, acc[kg], 0, 0, 0);
-                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bL, acc[kg], 0, 0, 0);
-                acc[kg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bH, acc[kg], 0, 0, 0);
-            }
+            if (exact) step32(std::true_type{}, uh, ul, vh, vl, ci, st, corr, se);
+            else step32(std::false_type{}, uh, ul, vh, vl, ci, st, corr, se);
         }
-        gs += (double)gsum;
-        if ((T & 31) == 31 || T == ntiles - 1) {
 #pragma unroll
-            for (int kg = 0; kg < KG; kg++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) tot[kg][r] += acc[kg][r];
-                acc[kg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-            }
+        for (int rg = 0; rg < RG; rg++) {
+            gq[rg] += (double)corr[rg];
+            ge[rg] += (double)se[rg];
         }
+        if ((T & 31) == 31 || T == ntiles - 1) flush(T < 32);
     };
 
     // step T: tile T has landed (own copies waited, then the barrier), every
@@ -408,33 +509,54 @@ __global__ __launch_bounds__(512, 1) void k_adjoint_mfma(const uint4 *__restrict
         if (T + 1 < ntiles) step(T + 1, uh1, ul1, vh1, vl1, ci1, uh0, ul0, vh0, vl0, ci0);
     }
 
-    // per row i: G_k = (sum_j P_ij u_jk + dL/ds_i u_ik) / sqrt(K-1), centred over k < K.
-    // Lane l holds i = l % 16 and k = 16kg + 4(l/16) + r.
-    const int v = rowbase + li;
-    double g = gs;
-    g += __shfl_xor(g, 16);
-    g += __shfl_xor(g, 32);
-    const bool live = v < Nv && R.x != 0.0f;
-    double gk[KG][4], part = 0.0;
+    // per row i: G_k = (tot_k / s_i + dL/ds_i u_ik) / sqrt(K-1), centred over k < K.
+    // Lane l holds i = l % 16 of each row group and k = 16kg + 4(l/16) + r.
 #pragma unroll
-    for (int kg = 0; kg < KG; kg++)
+    for (int rg = 0; rg < RG; rg++) {
+        const int v = rowbase + rg * 16 + li;
+        double q = gq[rg], e = ge[rg];
+        q += __shfl_xor(q, 16);
+        q += __shfl_xor(q, 32);
+        e += __shfl_xor(e, 16);
+        e += __shfl_xor(e, 32);
+        const double is = (double)rinv[rg];
+        const bool live = v < Nv && rinv[rg] != 0.0f;
+        float *row = Gc + (size_t)v * K16;
+        // sum_j Q_ij d+_ij = u_i . G_i (Q_ij = 0 where d_ij <= 0), so
+        // q_i = sum_j Q_ij d+_ij / (1 + delta) = u_i . G_i - corr_i
+        double uk[KG][4], gt[KG][4], ug = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int k = 16 * kg + 4 * lq + r;
-            const double uik = (k < K) ? (double)UT[(size_t)k * Nv_pad + v] : 0.0;
-            gk[kg][r] = (k < K) ? ((double)tot[kg][r] + g * uik) * (double)inv_sqrt_km1 : 0.0;
-            part += gk[kg][r];
-        }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    const double mean = part / (double)K;
+        for (int kg = 0; kg < KG; kg++)
 #pragma unroll
-    for (int kg = 0; kg < KG; kg++)
+            for (int r = 0; r < 4; r++) {
+                const int k = 16 * kg + 4 * lq + r;
+                uk[kg][r] = (k < K) ? (double)UT[(size_t)k * Nv_pad + v] : 0.0;
+                gt[kg][r] = (k < K) ? (double)row[k] : 0.0;
+                ug += uk[kg][r] * gt[kg][r];
+            }
+        ug += __shfl_xor(ug, 16);
+        ug += __shfl_xor(ug, 32);
+        const double ds = -(ug - q) * is + (double)cE * e;
+        double gk[KG][4], part = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int k = 16 * kg + 4 * lq + r;
-            if (k < K16) Gc[(size_t)v * K16 + k] = (live && k < K) ? (float)(gk[kg][r] - mean) : 0.0f;
-        }
+        for (int kg = 0; kg < KG; kg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k = 16 * kg + 4 * lq + r;
+                gk[kg][r] = (k < K) ? (gt[kg][r] * is + ds * uk[kg][r]) * (double)inv_sqrt_km1 : 0.0;
+                part += gk[kg][r];
+            }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        const double mean = part / (double)K;
+#pragma unroll
+        for (int kg = 0; kg < KG; kg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k = 16 * kg + 4 * lq + r;
+                if (k < K16) row[k] = (live && k < K) ? (float)(gk[kg][r] - mean) : 0.0f;
+            }
+    }
 }
 
 // dL/dx_q = sum_k Gc[block centred at q - off_k][k] (transpose of the gather).
@@ -530,10 +652,10 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     const double H = java_float_literal(nv) + java_float_literal(kEps);
     const float tscale = (float)std::sqrt(1.4426950408889634 / H);
     const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(K - 1)));
-    SRED_TRY(ctx->aux.ensure(sizeof(float4) * (size_t)Nv_pad));
-    float4 *d_info = ctx->aux.as<float4>();
+    SRED_TRY(ctx->aux.ensure(2 * sizeof(float4) * (size_t)Nv_pad));
+    float4 *d_info = ctx->aux.as<float4>(), *d_info2 = d_info + Nv_pad;
     hipLaunchKernelGGL(k_adjoint_coef, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, d_idx, Nv, Nv_pad,
-                       d_stds, d_S, d_W, d_mask, d_dmap, d_mm, np, tscale, d_info);
+                       d_stds, d_S, d_W, d_mask, d_dmap, d_mm, np, tscale, d_info, d_info2);
     // Pearson-scaled block vectors, k-major
     SRED_TRY(upload_offsets(ctx, g, g.w, &ctx->offs2, ctx->offs2_key));
     size_t nu = (size_t)Nv_pad * K4;
@@ -547,24 +669,30 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
         SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
         SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
         SRED_TRY(ctx->adj_v.ensure(2 * sizeof(uint4) * u4));
+        const int ngroups = Nv_pad / 32;
+        SRED_TRY(ctx->tflags.ensure(sizeof(int) * (size_t)ngroups));
         uint4 *d_uhi = ctx->uhi.as<uint4>(), *d_ulo = ctx->ulo.as<uint4>();
         uint4 *d_vhi = ctx->adj_v.as<uint4>(), *d_vlo = d_vhi + u4;
+        int *d_small = ctx->tflags.as<int>();
         launch_gather_bf16(ctx, SRED_PEARSON, d_img, d_means, d_stds, d_idx, Nv, Nv_pad, K, KS,
                            ctx->offs2.as<int>(), inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), d_uhi, d_ulo);
         hipLaunchKernelGGL(k_transpose_frag, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, d_uhi,
                            d_ulo, Nv_pad, KS, d_vhi, d_vlo);
+        hipLaunchKernelGGL(k_adjoint_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_info,
+                           ngroups, d_small);
         SRED_TRY(time_begin(ctx));
-        const float m2H = (float)(-2.0 / H);
+        const float cE = (float)(-2.0 / H / (double)tscale);  // (s_i - s_j) = (t_i - t_j) / tscale
         const float *UTp = ctx->u32.as<float>();
+        const dim3 grid(Nv_pad / kAdjRows);
         if (KS == 1)
-            hipLaunchKernelGGL(k_adjoint_mfma<1>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
-                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+            hipLaunchKernelGGL(k_adjoint_mfma<1>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
+                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
         else if (KS == 2)
-            hipLaunchKernelGGL(k_adjoint_mfma<2>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
-                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+            hipLaunchKernelGGL(k_adjoint_mfma<2>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
+                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
         else
-            hipLaunchKernelGGL(k_adjoint_mfma<3>, dim3(Nv_pad / kAdjRows), dim3(512), 0, ctx->stream, d_uhi, d_ulo, d_vhi,
-                               d_vlo, d_info, UTp, K, Nv, Nv_pad, kEps, m2H, inv_sqrt_km1, K16, d_Gc);
+            hipLaunchKernelGGL(k_adjoint_mfma<3>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
+                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
         SRED_HIP_CHECK(hipGetLastError());
         SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
     } else {
