@@ -771,7 +771,6 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
             }
     };
     auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
-        const int c0 = col0 + (T % ntiles) * kBN;
         if (skips<M>()) {
             const float tlo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[0].x)));
             const float thi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[kBN - 1].x)));
@@ -782,8 +781,9 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
                 return;
             }
         }
-        const bool exact =
-            row_small || (M == SRED_PEARSON && (tile_small[c0 / 32] | tile_small[c0 / 32 + 1]) != 0);
+        // small-sigma columns (aux.z of Pearson) from the tile's constants in
+        // LDS: no global load inside the tile loop
+        const bool exact = row_small || (M == SRED_PEARSON && __ballot(ccon[lane].z != 0.0f) != 0ull);
         float4 *sw = stage[T & 1][wave];
         if (diag) {
             if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
