@@ -7,18 +7,22 @@ finalize and normalisation): sred_global_map_2d_dev.
 
 Unit: one ordered (reference block, test block) pair over the interior,
 N^2 with N = (1024-10)^2 = 1,028,196; value = N^2 / t / 1e9 (Gblock-pairs/s).
-Pairs whose Gaussian weight is below 2^-40 (t ranges more than sqrt(40)
-apart, recognised per panel pair and tile after sorting the blocks by sigma)
-are skipped inside the contraction; their count is reported and the
-roofline counts only the evaluated pairs.
+The split-BF16 contraction evaluates each unordered pair once (the per-pair
+terms are symmetric) and skips pairs whose Gaussian weight is below 2^-40
+(Pearson: t ranges more than sqrt(40) apart; SSIM: sqrt(47)), recognised per
+panel pair and tile after sorting the blocks by sigma; the skipped count is
+reported.  The roofline counts the algorithmic work of the reference's
+evaluation, N^2 * 2K flops per map, over the contraction's time (HIP events on
+its stream); the TFLOP/s the matrix cores issue and their busy fraction
+(rocprofv3 counters, committed summary) are reported beside it.
 
 The JSON line also carries: the plain-FP32 form timed the same way and the
-speed-up (and the split-BF16 form over all ordered pairs, the FP32 form's
-algorithm, for a like-for-like ratio); max/mean abs error of both normalised maps against the FP64
-oracle on sampled pixels (plus the map extremes) and between the two forms
-over the whole map; the roofline of the dominant kernel (the contraction,
-timed with HIP events on its own stream); and the oracle's float32
-restatement of the reference timed on host cores on a bounded sample.
+speed-up; the split-BF16 form over all ordered pairs (the FP32 form's
+algorithm) as all_pairs_value, for a like-for-like ratio; the SSIM and NRMSE
+maps of the same image in both precisions; max/mean abs error of every
+normalised map against the FP64 oracle on sampled pixels (plus the map
+extremes) and between the two forms over the whole map; and the oracle's
+float32 restatement of the reference timed on host cores on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -996,6 +1000,55 @@ def main():
     if args.profile:
         profile_run(args)
         return
+    global_run(args)
+
+
+def pmc_from_profile(key):
+    """A figure of the committed PMC summary of the headline contraction."""
+    p = os.path.join(ROOT, "profiles", "pmc_global_pearson_bf16x3_1024_b11.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith(key + ":"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def fp64_errors(O, metric, img, maps, rows, seed):
+    """Max/mean abs error of normalised global maps against the FP64 oracle on
+    `rows` seeded interior pixels plus each map's 4 smallest and 4 largest
+    values: the FP64 accumulators of those pixels only, finalised with the
+    relevance mask restricted to them (so the normalisation's min/max come
+    from the maps' extremes, which are included)."""
+    r = B // 2
+    m32, s32 = O.local_stats_f32(img, B, B)
+    nv = O.noise_var(img)
+    mask, thr, npx = O.relevance_mask(s32, r, r, nv, 0.0)
+    idx = np.flatnonzero(mask.ravel() > 0)
+    pick = [np.random.default_rng(seed).choice(idx, rows, replace=False)]
+    for m in maps.values():
+        order = np.argsort(m.ravel()[idx], kind="stable")
+        pick += [idx[order[:4]], idx[order[-4:]]]
+    px = np.unique(np.concatenate(pick))
+    S, Wt = O.global_accum_f64(metric, img, s32, B, B, nv, thr, rows=px)
+    sub = np.zeros_like(mask)
+    sub.ravel()[px] = 1.0
+    ref = O.global_finalize_f64(metric, S, Wt, sub, npx, 0.0, True, B, B).ravel()[px]
+    out = {}
+    for name, m in maps.items():
+        e = np.abs(m.ravel()[px].astype(np.float64) - ref)
+        out[name] = {"max": float(e.max()), "mean": float(e.mean())}
+    return int(px.size), out
+
+
+def global_run(args):
+    """The headline: global Pearson map 1024x1024, 11x11 (bf16x3 timed), with
+    the FP32 form, the all-ordered-pairs split-BF16 form, SSIM and NRMSE in
+    both precisions, every map's error against FP64, the roofline of the
+    contraction and the CPU baseline."""
     import torch
 
     import oracle as O
@@ -1012,72 +1065,61 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)  # default priority
     d_img = torch.from_numpy(img).to(dev)
-    d_out = {p: torch.zeros_like(d_img) for p in ("bf16x3", "fp32")}
+    d_out = torch.zeros_like(d_img)
     ctx = sred_amd.Context(0)
     ctx.set_stream(stream.cuda_stream)
+    maps = {}
 
-    def step(prec):
-        global_map_device(ctx, L.PEARSON, L.BF16X3 if prec == "bf16x3" else L.FP32, d_img.data_ptr(), W, H, B, B, 0.0,
-                          True, d_out[prec].data_ptr())
+    def timed(metric, prec, steps, warmup, symmetric=1):
+        ctx.set_option(L.OPT_SYMMETRIC, symmetric)
+        step = lambda: global_map_device(ctx, metric, L.BF16X3 if prec == "bf16x3" else L.FP32, d_img.data_ptr(),
+                                         W, H, B, B, 0.0, True, d_out.data_ptr())
+        try:
+            with torch.cuda.stream(stream):
+                t, k = _timed(step, steps, warmup, lambda: ctx.last_contraction()[0])
+        finally:
+            ctx.set_option(L.OPT_SYMMETRIC, 1)
+        out = {"ms_per_step": t * 1e3, "contraction_ms": k, "value": pairs / t / 1e9,
+               "pairs_evaluated": float(ctx.last_contraction()[2]), "pairs_skipped": float(ctx.last_skipped())}
+        maps[(metric, prec, symmetric)] = d_out.cpu().numpy()
+        print(f"# metric {metric} {prec}{'' if symmetric else ' all pairs'}: {t * 1e3:.1f} ms/step, "
+              f"contraction {k:.1f} ms", file=sys.stderr, flush=True)
+        return out
 
-    def timed(prec, steps, warmup):
-        for _ in range(warmup):
-            step(prec)
-        torch.cuda.synchronize()
-        kern = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step(prec)
-            kern.append(ctx.last_contraction()[0])
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps, float(np.mean(kern)), ctx.last_contraction()[2]
-
-    with torch.cuda.stream(stream):
-        t_mfma, k_mfma, pairs_eval = timed("bf16x3", args.steps, args.warmup)
-        skipped = float(ctx.last_skipped())
-        print(f"# bf16x3: {t_mfma * 1e3:.1f} ms/step, contraction {k_mfma:.1f} ms", file=sys.stderr, flush=True)
-        t_f32, k_f32, _ = timed("fp32", args.fp32_steps, 1)
-        print(f"# fp32:   {t_f32 * 1e3:.1f} ms/step, contraction {k_f32:.1f} ms", file=sys.stderr, flush=True)
-        # the same all-pairs algorithm as the FP32 form (no symmetry), for a like-for-like ratio
-        ctx.set_option(L.OPT_SYMMETRIC, 0)
-        t_full, k_full, _ = timed("bf16x3", 1, 0)
-        ctx.set_option(L.OPT_SYMMETRIC, 1)
-        print(f"# bf16x3 all pairs: {t_full * 1e3:.1f} ms/step, contraction {k_full:.1f} ms", file=sys.stderr,
-              flush=True)
-    map_mfma = d_out["bf16x3"].cpu().numpy()
-    map_f32 = d_out["fp32"].cpu().numpy()
+    head = timed(0, "bf16x3", args.steps, args.warmup)
+    f32 = timed(0, "fp32", args.fp32_steps, 1)
+    full = timed(0, "bf16x3", 1, 0, symmetric=0)
+    extra = {}
+    for metric, name in ((1, "ssim"), (2, "nrmse")):
+        extra[name] = {"bf16x3": timed(metric, "bf16x3", 2, 1), "fp32": timed(metric, "fp32", 1, 0)}
     ctx.close()
 
-    # ---- error vs the FP64 oracle on sampled pixels plus the map extremes ----
-    r = B // 2
-    rng = np.random.default_rng(123)
-    inner = np.zeros((H, W), bool)
-    inner[r:H - r, r:W - r] = True
-    flat_idx = np.flatnonzero(inner)
-    order = np.argsort(map_f32.ravel()[flat_idx])
-    extremes = np.concatenate([flat_idx[order[:4]], flat_idx[order[-4:]]])
-    sample = np.unique(np.concatenate([rng.choice(flat_idx, args.oracle_rows, replace=False), extremes]))
-    m32, s32 = O.local_stats_f32(img, B, B)
-    nv = O.noise_var(img)
+    # ---- every map's error against the FP64 oracle (sampled pixels + extremes) ----
     t0 = time.perf_counter()
-    S64, W64 = O.global_accum_f64(O.PEARSON, img, s32, B, B, nv, 0.0, rows=sample)
-    t_or = time.perf_counter() - t0
-    npx = float(np.count_nonzero((s32 * s32 > 0) & inner))
-    rep64 = S64.ravel()[sample] / (W64.ravel()[sample] * npx + 1e-7)
-    ex = np.isin(sample, extremes)
-    lo, hi = rep64[ex].min(), rep64[ex].max()
-    norm64 = (rep64 - lo) / (hi - lo + 1e-7)
-    err_mfma = np.abs(map_mfma.ravel()[sample] - norm64)
-    err_f32 = np.abs(map_f32.ravel()[sample] - norm64)
-    diff_forms = np.abs(map_mfma - map_f32)
-    print(f"# oracle64 on {sample.size} rows: {t_or:.1f} s", file=sys.stderr, flush=True)
+    errors = {}
+    for metric, name in ((0, "pearson"), (1, "ssim"), (2, "nrmse")):
+        nsamp, e = fp64_errors(O, metric, img, {p: maps[(metric, p, 1)] for p in ("bf16x3", "fp32")},
+                               args.oracle_rows, 123 + metric)
+        errors[name] = {"samples": nsamp, **{f"{p}_max": e[p]["max"] for p in e},
+                        **{f"{p}_mean": e[p]["mean"] for p in e},
+                        "bf16x3_vs_fp32_full_map_max": float(np.abs(maps[(metric, "bf16x3", 1)] -
+                                                                    maps[(metric, "fp32", 1)]).max())}
+    print(f"# oracle64 errors: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    for name in ("ssim", "nrmse"):
+        extra[name]["error_vs_fp64"] = errors[name]
+        extra[name]["speedup_vs_fp32"] = extra[name]["fp32"]["ms_per_step"] / extra[name]["bf16x3"]["ms_per_step"]
 
     # ---- CPU baseline: the oracle's float32 restatement on host cores ----
     cpu = None
     if not args.no_cpu:
         cores = host_cores()
+        m32, s32 = O.local_stats_f32(img, B, B)
+        nv = O.noise_var(img)
+        r = B // 2
+        inner = np.zeros((H, W), bool)
+        inner[r:H - r, r:W - r] = True
+        ref_rows = np.flatnonzero(inner)  # row-major from the first interior pixel
         rows_done, t_cpu, batch = 0, 0.0, 16
-        ref_rows = flat_idx  # row-major from the first interior pixel
         while t_cpu < args.cpu_seconds and rows_done < ref_rows.size:
             rows = ref_rows[rows_done:rows_done + batch]
             t0 = time.perf_counter()
@@ -1090,38 +1132,43 @@ def main():
                          f"reference pixels (row-major) x all {N} test pixels, {t_cpu:.1f} s on {cores} threads"}
         print(f"# cpu: {cpu['value']:.4f} Gpairs/s", file=sys.stderr, flush=True)
 
-    # algorithmic work of the pairs actually evaluated: pairs whose weight is
-    # below 2^-40 are skipped (their t ranges are sqrt(40) apart)
-    algo_flops = (pairs - skipped) * 2 * Kb
-    achieved = algo_flops / (k_mfma * 1e-3) / 1e12
+    # roofline of the contraction: algorithmic flops N^2 * 2K per launch (every
+    # ordered pair, as the reference evaluates them) over the kernel time
+    # measured with HIP events on its stream
+    k_mfma = head["contraction_ms"]
+    achieved = pairs * 2 * Kb / (k_mfma * 1e-3) / 1e12
     kpad = (Kb + 31) // 32 * 32
+    issued = head["pairs_evaluated"] * (1.0 - head["pairs_skipped"] / pairs) * 2 * kpad * 3 / (k_mfma * 1e-3) / 1e12
     result = {
         "metric": "Gblock-pairs/s, global Pearson map 1024² b11: MFMA vs FP32 at err≤1e-4",
-        "value": pairs / t_mfma / 1e9,
+        "value": head["value"],
         "unit": "Gblock-pairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_mfma * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16x3",
         "data": "synthetic: ring lattice (pitch 24, rotated) + 900 Gaussian blobs, noise 0.05, seed 2",
         "config": {"workload": "global repetition, Pearson, 1024x1024, 11x11 blocks (K=81), relevance 0, normalised",
-                   "image": [W, H], "block": [B, B], "K": Kb, "pairs": pairs, "precision": "bf16x3 (split BF16 on MFMA, FP32 accumulate, FP64 row sums)"},
-        "fp32": {"value": pairs / t_f32 / 1e9, "ms_per_step": t_f32 * 1e3, "contraction_ms": k_f32},
-        "speedup_vs_fp32": t_f32 / t_mfma,
+                   "image": [W, H], "block": [B, B], "K": Kb, "pairs": pairs,
+                   "precision": "bf16x3 (split BF16 on MFMA, FP32 accumulate, FP64 row sums)"},
+        "all_pairs_value": full["value"],
+        "fp32": {"value": f32["value"], "ms_per_step": f32["ms_per_step"], "contraction_ms": f32["contraction_ms"]},
+        "speedup_vs_fp32": f32["ms_per_step"] / head["ms_per_step"],
         "contraction_ms": k_mfma,
-        "pairs_skipped": skipped, "skip_fraction": skipped / pairs,
-        "bf16x3_all_pairs": {"ms_per_step": t_full * 1e3, "contraction_ms": k_full,
-                             "speedup_vs_fp32": t_f32 / t_full},
-        "error_vs_fp64": {"samples": int(sample.size), "bf16x3_max": float(err_mfma.max()),
-                          "bf16x3_mean": float(err_mfma.mean()), "fp32_max": float(err_f32.max()),
-                          "fp32_mean": float(err_f32.mean()), "bf16x3_vs_fp32_full_map_max": float(diff_forms.max())},
+        "pairs_skipped": head["pairs_skipped"], "skip_fraction": head["pairs_skipped"] / pairs,
+        "bf16x3_all_pairs": {"value": full["value"], "ms_per_step": full["ms_per_step"],
+                             "contraction_ms": full["contraction_ms"],
+                             "speedup_vs_fp32": f32["ms_per_step"] / full["ms_per_step"]},
+        "error_vs_fp64": errors["pearson"],
+        "ssim": extra["ssim"],
+        "nrmse": extra["nrmse"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic_from_profile(),
-                     "issued_tflops": float(pairs_eval) * (1.0 - skipped / pairs) * 2 * kpad * 3 / (k_mfma * 1e-3) / 1e12,
+                     "issued_tflops": issued, "mfma_busy_pmc": pmc_from_profile("mfma_busy_fraction"),
                      "kernel": "k_contract_mfma_sym<Pearson, KS=3>"},
         "cpu_baseline": cpu,
     }
