@@ -208,6 +208,10 @@ int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_pro
 /* Panel size the autotuner chose on this context for (metric, k_steps =
  * ceil(K/32)), 0 if none yet. */
 int sred_ctx_tuned_panel(sred_ctx *ctx, int metric, int k_steps, int *panel);
+/* The autotuner's timed launch (ms, HIP events) for each candidate panel size
+ * 256, 512, 1024, 2048 of that class on this context, 0 for candidates not
+ * timed (more partial-sum memory than allowed, a forced or untuned panel). */
+int sred_ctx_tuning_ms(sred_ctx *ctx, int metric, int k_steps, float *ms);
 
 /* ---- Hu template metric (kernelGetPatchHu2D.cl:11-282; the reference has
  * the kernel but no Java caller, so these entries define its host side) ---- */

@@ -35,7 +35,7 @@ EXPORTS = [
     "sred_block_map_2d_dev_noalloc",
     "sred_ctx_last_contraction", "sred_global_plan_2d", "sred_global_pearson_adjoint_2d",
     "sred_global_pearson_adjoint_2d_dev", "sred_block_hu_2d", "sred_block_hu_2d_dev", "sred_hu_prepare_template_2d",
-    "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_hu_invariants_2d", "sred_global_hu_2d",
+    "sred_global_batch_2d", "sred_ctx_tuned_panel", "sred_ctx_tuning_ms", "sred_hu_invariants_2d", "sred_global_hu_2d",
     "sred_global_hu_2d_dev", "sred_phase_correlation_2d", "sred_phase_correlation_2d_dev", "sred_ctx_last_skipped",
     "sred_gat_2d", "sred_gat_2d_dev", "sred_gat_objective_2d", "sred_gat_minimize_2d", "sred_gat_minimize_2d_dev",
     "sred_quadtree_gat_2d", "sred_quadtree_gat_2d_dev", "sred_gat_tree_2d", "sred_gat_tree_2d_dev", "sred_f_quantile",
@@ -172,6 +172,7 @@ def load():
         "sred_hu_prepare_template_2d": ([_F, _I, _I, _F], _I),
         "sred_global_batch_2d": ([_VP, _I, _I, ctypes.POINTER(GlobalProblem), _FL, _I], _I),
         "sred_ctx_tuned_panel": ([_VP, _I, _I, ctypes.POINTER(_I)], _I),
+        "sred_ctx_tuning_ms": ([_VP, _I, _I, _F], _I),
         "sred_hu_invariants_2d": ([_VP, _F, _F, _I, _I, _I, _I, _F], _I),
         "sred_global_hu_2d": ([_VP, _F, _F, _F, _I, _I, _I, _I, _VP, _I64, _F], _I),
         "sred_global_hu_2d_dev": ([_VP, _VP, _VP, _VP, _I, _I, _I, _I, _VP, _I64, _VP], _I),
@@ -260,6 +261,12 @@ class Context:
         p = ctypes.c_int()
         check(self._lib.sred_ctx_tuned_panel(self.handle, int(metric), int(k_steps), ctypes.byref(p)))
         return p.value
+
+    def tuning_ms(self, metric, k_steps):
+        """The autotuner's timed launch per candidate panel (256, 512, 1024, 2048), ms; 0 if not timed."""
+        ms = (ctypes.c_float * 4)()
+        check(self._lib.sred_ctx_tuning_ms(self.handle, int(metric), int(k_steps), ms))
+        return dict(zip((256, 512, 1024, 2048), list(ms)))
 
     def last_contraction(self):
         """(ms, launches, pairs_evaluated) of the last global contraction."""

@@ -405,6 +405,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->h_fit) (void)hipHostFree(ctx->h_fit);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     delete ctx;
     return SRED_OK;
 }
@@ -1458,6 +1459,20 @@ int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_pro
     }
     if (n == 0) return SRED_OK;
     return global_batch_dev(ctx, metric, n, problems, relevance_constant, normalize);
+}
+
+int sred_ctx_tuning_ms(sred_ctx *ctx, int metric, int k_steps, float *ms)
+{
+    SRED_TRY(check_ctx(ctx));
+    if (!ms) {
+        set_error("null output");
+        return SRED_ERR_ARG;
+    }
+    for (int i = 0; i < 4; i++) ms[i] = 0.0f;
+    auto f = ctx->tune_ms.find(std::make_pair(metric, k_steps));
+    if (f != ctx->tune_ms.end())
+        for (size_t i = 0; i < f->second.size() && i < 4; i++) ms[i] = f->second[i];
+    return SRED_OK;
 }
 
 int sred_ctx_tuned_panel(sred_ctx *ctx, int metric, int k_steps, int *panel)

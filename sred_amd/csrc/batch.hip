@@ -6,10 +6,12 @@
 // that of sred_global_map_2d_dev (GlobalRepetition2D_.java:124-145) with the
 // symmetric split-BF16 contraction:
 //
-//   front   (per image, host-driven, small synchronisations): statistics,
-//           noise variance, relevance mask and nPixels, compaction, weight
-//           sums, per-block scalars and the gathered block matrix (padded to
-//           a multiple of 2048 rows so any panel size up to 2048 applies);
+//   front   (three stages over all images, two synchronisations per batch):
+//           statistics and noise tiles; after the first synchronisation the
+//           noise variances (host sort), relevance masks and compactions;
+//           after the second nPixels, weight sums, per-block scalars and the
+//           gathered block matrices (padded to a multiple of 2048 rows so any
+//           panel size up to 2048 applies);
 //   graph   (one hipGraphLaunch): per block-size class (K_pad = 32*KS) and
 //           memory wave, ONE persistent contraction kernel whose workgroups
 //           walk the panel pairs of every image of the class (SymProblem
@@ -39,7 +41,7 @@ BatchStore::~BatchStore() { release(); }
 void BatchStore::release()
 {
     DevBuf *b[] = {&means, &stds, &mask, &S, &W, &counts, &valid_idx, &aux, &tflags, &u32, &uhi, &ulo, &offs2,
-                   &wmom, &wpart, &sortbuf};
+                   &wmom, &wpart, &sortbuf, &tile_var, &count};
     for (DevBuf *x : b) x->release();
     offs2_key[0] = offs2_key[1] = offs2_key[2] = -1;
 }
@@ -149,6 +151,8 @@ int class_panel(sred_ctx *ctx, int metric, int KS, std::vector<Item> &it, const 
     const EpiConst ec = epi_const();
     float best_ms = 1e30f;
     int best = cand.back();
+    std::vector<float> &times = ctx->tune_ms[key];
+    times.assign(sizeof kPanels / sizeof kPanels[0], 0.0f);
     for (int P : cand) {
         std::vector<SymProblem> tab;
         const int64_t nitems = fill_table(it, sub, P, ctx->batch_part.as<float>(), tab);
@@ -160,6 +164,8 @@ int class_panel(sred_ctx *ctx, int metric, int KS, std::vector<Item> &it, const 
         float ms = 0.0f;
         SRED_HIP_CHECK(hipEventSynchronize(ctx->ev1));
         SRED_HIP_CHECK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        for (size_t c = 0; c < times.size(); c++)
+            if (kPanels[c] == P) times[c] = ms;
         if (ms < best_ms) {
             best_ms = ms;
             best = P;
@@ -181,7 +187,10 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
     std::vector<Item> it(n);
     ctx->last_skip = nullptr;
     while ((int)ctx->batch_store.size() < n) ctx->batch_store.push_back(new BatchStore());
-    // ---- front, per image
+    // ---- front: three stages over all images, two host synchronisations
+    std::vector<int> nb(n);
+    std::vector<size_t> tile_off(n);
+    size_t tile_floats = 0;
     for (int i = 0; i < n; i++) {
         Item &x = it[i];
         SRED_TRY(make_geometry(probs[i].w, probs[i].h, probs[i].bw, probs[i].bh, &x.g));
@@ -193,9 +202,30 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
             set_error("the batched split-BF16 contraction supports block sizes up to K = 256 pixels");
             return SRED_ERR_ARG;
         }
+        int tbw, tbh;
+        nb[i] = noise_tiles(x.g.w, x.g.h, &tbw, &tbh);
+        if (nb[i] < 0) return SRED_ERR_ARG;
+        tile_off[i] = tile_floats;
+        tile_floats += (size_t)(nb[i] + 1 + 3) / 4 * 4;
         x.d_img = probs[i].d_img;
         x.d_out = probs[i].d_out;
         x.st = ctx->batch_store[i];
+    }
+    // pinned host staging: tile variances, mask counts, compaction totals
+    const size_t stage_bytes = sizeof(float) * tile_floats + sizeof(unsigned long long) * n + 2 * sizeof(int2) * n;
+    if (ctx->h_stage_bytes < stage_bytes) {
+        if (ctx->h_stage) SRED_HIP_CHECK(hipHostFree(ctx->h_stage));
+        ctx->h_stage = nullptr;
+        ctx->h_stage_bytes = 0;
+        SRED_HIP_CHECK(hipHostMalloc(&ctx->h_stage, stage_bytes, hipHostMallocDefault));
+        ctx->h_stage_bytes = stage_bytes;
+    }
+    float *h_tiles = static_cast<float *>(ctx->h_stage);
+    unsigned long long *h_count = reinterpret_cast<unsigned long long *>(h_tiles + tile_floats);
+    int2 *h_tot = reinterpret_cast<int2 *>(h_count + n);
+    // stage 1: local statistics and noise tiles of every image
+    for (int i = 0; i < n; i++) {
+        Item &x = it[i];
         const int64_t np = x.g.n();
         BatchStore &st = *x.st;
         SRED_TRY(st.means.ensure(sizeof(float) * np));
@@ -203,18 +233,36 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
         SRED_TRY(st.mask.ensure(sizeof(float) * np));
         SRED_TRY(st.S.ensure(sizeof(float) * np));
         SRED_TRY(st.W.ensure(sizeof(float) * np));
+        SRED_TRY(st.count.ensure(sizeof(unsigned long long)));
         SRED_TRY(launch_local_stats(ctx, x.g, x.d_img, st.means.as<float>(), st.stds.as<float>()));
+        SRED_TRY(noise_var_enqueue(ctx, st.tile_var, x.d_img, x.g.w, x.g.h, h_tiles + tile_off[i]));
+    }
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    // stage 2: noise variances (host), relevance masks with their counts, compactions
+    std::vector<GlobalParams> par(n);
+    for (int i = 0; i < n; i++) {
+        Item &x = it[i];
+        BatchStore &st = *x.st;
         float nv;
-        SRED_TRY(noise_var_dev(ctx, x.d_img, x.g.w, x.g.h, &nv));
+        SRED_TRY(noise_var_finish(h_tiles + tile_off[i], nb[i], &nv));
         const float thr = nv * c;
-        SRED_TRY(relevance_mask_dev(ctx, x.g, st.stds.as<float>(), thr, st.mask.as<float>(), &x.n_pixels));
-        GlobalParams p;
+        SRED_TRY(relevance_mask_enqueue(ctx, x.g, st.stds.as<float>(), thr, st.mask.as<float>(),
+                                        st.count.as<unsigned long long>(), h_count + i));
+        GlobalParams &p = par[i];
         p.metric = metric;
         p.precision = SRED_BF16X3;
         p.filter_param = nv;
         p.threshold = thr;
-        SRED_TRY(global_prepare(ctx, st.bufs(), x.g, p, x.d_img, st.means.as<float>(), st.stds.as<float>(),
-                                st.S.as<float>(), st.W.as<float>(), 1, 0, kPadRows, &x.r));
+        SRED_TRY(global_prepare_front(ctx, st.bufs(), x.g, p, st.stds.as<float>(), h_tot + 2 * i));
+    }
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    // stage 3: nPixels, weight sums, per-block scalars, gathered block matrices
+    for (int i = 0; i < n; i++) {
+        Item &x = it[i];
+        BatchStore &st = *x.st;
+        x.n_pixels = n_pixels_of(h_count[i]);
+        SRED_TRY(global_prepare_back(ctx, st.bufs(), x.g, par[i], x.d_img, st.means.as<float>(), st.stds.as<float>(),
+                                     st.S.as<float>(), st.W.as<float>(), 1, 0, kPadRows, h_tot + 2 * i, &x.r));
     }
     // ---- classes, panel sizes, waves
     std::map<int, std::vector<int>> cls;  // KS -> images with Nv > 0

@@ -1226,25 +1226,18 @@ GlobalBufs ctx_global_bufs(sred_ctx *ctx)
 // small-sigma flags and the gathered block matrix.  pad_rows > 0 pads the
 // split-BF16 block matrix and scalars to a multiple of pad_rows rows, so any
 // panel size dividing it can be used afterwards.
-int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p, const float *d_img,
-                   const float *d_means, const float *d_stds, float *d_S, float *d_W, int symmetric, int panel,
-                   int pad_rows, GlobalPrepared *out)
+// Step 1 of a global problem, enqueued only: the compaction of the valid
+// blocks, and a copy of its totals (valid and flat counts, the bits of the
+// largest valid sigma) into h_tot[2] (host memory; read after the stream has
+// been synchronised).
+int global_prepare_front(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p,
+                         const float *d_stds, int2 *h_tot)
 {
-    const int metric = p.metric;
-    if (metric < 0 || metric > 3) {
+    if (p.metric < 0 || p.metric > 3) {
         set_error("unknown metric");
         return SRED_ERR_UNKNOWN_METRIC;
     }
-    GlobalPrepared r{};
     const double thr_d = java_float_literal(p.threshold);
-    const double fp_d = java_float_literal(p.filter_param);
-    const double eps_d = java_float_literal(kEps);
-    r.tscale_d = std::sqrt(1.4426950408889634 / (fp_d + eps_d));
-    const float tscale = (float)r.tscale_d;
-    const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(g.K - 1)));
-    const float inv_sqrt_k = (float)(1.0 / std::sqrt((double)g.K));
-
-    // 1. compaction
     const int64_t n = g.n();
     const int nblk = (int)((n + kCompactTile - 1) / kCompactTile);
     SRED_TRY(b.counts->ensure(sizeof(int2) * (3 * (size_t)nblk + 4)));
@@ -1259,9 +1252,35 @@ int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const 
     hipLaunchKernelGGL(k_compact_scatter, dim3(nblk), dim3(kCompactBlock), 0, ctx->stream, d_stds, g.w, g.h, g.brw,
                        g.brh, g.d, g.brd, thr_d, p.drop_flat ? 1 : 0, d_offsets, d_idx);
     SRED_HIP_CHECK(hipGetLastError());
+    SRED_HIP_CHECK(hipMemcpyAsync(h_tot, d_offsets + nblk, 2 * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+    return SRED_OK;
+}
+
+int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p, const float *d_img,
+                   const float *d_means, const float *d_stds, float *d_S, float *d_W, int symmetric, int panel,
+                   int pad_rows, GlobalPrepared *out)
+{
     int2 tot[2];
-    SRED_HIP_CHECK(hipMemcpyAsync(tot, d_offsets + nblk, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
+    SRED_TRY(global_prepare_front(ctx, b, g, p, d_stds, tot));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return global_prepare_back(ctx, b, g, p, d_img, d_means, d_stds, d_S, d_W, symmetric, panel, pad_rows, tot, out);
+}
+
+// Steps 1-2 after the compaction totals tot[2] have reached the host.
+int global_prepare_back(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p,
+                        const float *d_img, const float *d_means, const float *d_stds, float *d_S, float *d_W,
+                        int symmetric, int panel, int pad_rows, const int2 *tot, GlobalPrepared *out)
+{
+    const int metric = p.metric;
+    GlobalPrepared r{};
+    const double fp_d = java_float_literal(p.filter_param);
+    const double eps_d = java_float_literal(kEps);
+    r.tscale_d = std::sqrt(1.4426950408889634 / (fp_d + eps_d));
+    const float tscale = (float)r.tscale_d;
+    const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(g.K - 1)));
+    const float inv_sqrt_k = (float)(1.0 / std::sqrt((double)g.K));
+    const int64_t n = g.n();
+    int *d_idx = b.valid_idx->as<int>();
     const int Nv = tot[0].x;
     float sigma_max;
     std::memcpy(&sigma_max, &tot[1].x, sizeof sigma_max);

@@ -113,6 +113,7 @@ namespace sred {
 // Per-image scratch of a batched global call (owned by the context).
 struct BatchStore {
     DevBuf means, stds, mask, S, W, counts, valid_idx, aux, tflags, u32, uhi, ulo, offs2, wmom, wpart, sortbuf;
+    DevBuf tile_var, count;  // noise tiles, relevance-mask pixel count
     int offs2_key[3] = {-1, -1, -1};
     ~BatchStore();
     void release();
@@ -145,6 +146,7 @@ struct sred_ctx {
     sred::DevBuf batch_part, batch_tab;  // partial sums of a wave; problem tables + item counter
     size_t batch_ctr_off = 0;
     std::map<std::pair<int, int>, int> tuned_panel;  // (metric, KS) -> panel size
+    std::map<std::pair<int, int>, std::vector<float>> tune_ms;  // (metric, KS) -> ms per candidate panel (0: not timed)
     int autotune = 1;
     int batch_panel = 0;       // forced panel size (0: autotuner)
     int64_t batch_mb = 16384;  // partial-sum memory per wave
@@ -162,6 +164,8 @@ struct sred_ctx {
     unsigned long long *last_skip = nullptr;  // device counter of pairs skipped by the last symmetric contraction
     double *h_fit = nullptr;  // pinned host doubles the GAT objective kernel writes its values to
     unsigned char *noalloc_scratch = nullptr;  // this device's copy of the no-allocation scratch array
+    void *h_stage = nullptr;  // pinned host staging of batched readbacks
+    size_t h_stage_bytes = 0;
 };
 
 namespace sred {
@@ -232,6 +236,27 @@ struct GlobalPrepared {
 int global_prepare(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p, const float *d_img,
                    const float *d_means, const float *d_stds, float *d_S, float *d_W, int symmetric, int panel,
                    int pad_rows, GlobalPrepared *out);
+// global_prepare in two parts around its one synchronisation: the
+// compaction (enqueued, totals copied to h_tot[2]) and the rest once h_tot
+// holds them (batched callers synchronise once for many problems).
+int global_prepare_front(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p,
+                         const float *d_stds, int2 *h_tot);
+int global_prepare_back(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, const GlobalParams &p,
+                        const float *d_img, const float *d_means, const float *d_stds, float *d_S, float *d_W,
+                        int symmetric, int panel, int pad_rows, const int2 *tot, GlobalPrepared *out);
+// The noise variance in two parts: the tile variances enqueued (into `tiles`,
+// copied with the non-finite flag to h_tiles[0..nb], host memory), then the
+// host sort and mean once the stream has been synchronised.  noise_tiles
+// gives nb (or -1 for an image smaller than one tile).
+int noise_tiles(int w, int h, int *tbw, int *tbh);
+int noise_var_enqueue(sred_ctx *ctx, DevBuf &tiles, const float *d_img, int w, int h, float *h_tiles,
+                      bool double_sums = false);
+int noise_var_finish(float *h_tiles, int nb, float *noise_var);
+// The relevance mask with its pixel count copied to *h_count (enqueued);
+// n_pixels_of turns the count into Java's float nPixels.
+int relevance_mask_enqueue(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
+                           unsigned long long *d_count, unsigned long long *h_count);
+float n_pixels_of(unsigned long long count);
 int global_reduce_dev(sred_ctx *ctx, const GlobalPrepared &r, int metric, const float *d_part, int nchunks,
                       int Nv_pad, const float *d_stds, float *d_S);
 EpiConst epi_const();

@@ -114,18 +114,29 @@ __global__ void k_tile_var(const float *__restrict__ img, int w, int tbw, int tb
     out[t] = var;
 }
 
-int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var, bool double_sums)
+// Tile count of Utils.getMeanNoiseVar2D: 8x8 tiles, 16x16 above the CIF
+// size (Utils.java:793-801).
+int noise_tiles(int w, int h, int *tbw, int *tbh)
 {
-#pragma clang fp contract(off)
-    int tbw = 8, tbh = 8;
-    if ((int64_t)w * h > 352 * 288) { tbw = 16; tbh = 16; }  // CIF threshold, Utils.java:793-801
-    int nbx = w / tbw, nby = h / tbh, nb = nbx * nby;
+    *tbw = 8;
+    *tbh = 8;
+    if ((int64_t)w * h > 352 * 288) { *tbw = 16; *tbh = 16; }
+    const int nb = (w / *tbw) * (h / *tbh);
     if (nb <= 0) {
         set_error("image smaller than one noise tile (the reference indexes an empty variance array)");
-        return SRED_ERR_ARG;
+        return -1;
     }
-    SRED_TRY(ctx->tile_var.ensure(sizeof(float) * nb + 64));
-    float *d_var = ctx->tile_var.as<float>();
+    return nb;
+}
+
+int noise_var_enqueue(sred_ctx *ctx, DevBuf &tiles, const float *d_img, int w, int h, float *h_tiles, bool double_sums)
+{
+    int tbw, tbh;
+    const int nb = noise_tiles(w, h, &tbw, &tbh);
+    if (nb < 0) return SRED_ERR_ARG;
+    const int nbx = w / tbw;
+    SRED_TRY(tiles.ensure(sizeof(float) * nb + 64));
+    float *d_var = tiles.as<float>();
     int *d_bad = reinterpret_cast<int *>(d_var + nb);
     SRED_TRY(zero_dev(ctx, d_bad, sizeof(int)));
     if (double_sums)
@@ -135,24 +146,41 @@ int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_
         hipLaunchKernelGGL(k_tile_var<float>, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, d_img, w, tbw, tbh,
                            nbx, nb, d_var, d_bad);
     SRED_HIP_CHECK(hipGetLastError());
-    ctx->host_tile_var.resize(nb + 1);
-    SRED_HIP_CHECK(hipMemcpyAsync(ctx->host_tile_var.data(), d_var, sizeof(float) * (nb + 1), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    SRED_HIP_CHECK(hipMemcpyAsync(h_tiles, d_var, sizeof(float) * (nb + 1), hipMemcpyDeviceToHost, ctx->stream));
+    return SRED_OK;
+}
+
+// The host part of Utils.getMeanNoiseVar2D (Utils.java:818-836) on the tile
+// variances and the non-finite flag (h_tiles[nb]) the enqueue copied back;
+// sorts h_tiles in place.
+int noise_var_finish(float *h_tiles, int nb, float *noise_var)
+{
+#pragma clang fp contract(off)
     int bad;
-    memcpy(&bad, &ctx->host_tile_var[nb], sizeof(int));
+    memcpy(&bad, &h_tiles[nb], sizeof(int));
     if (bad) {
         set_error("Input contains invalid (NaN or Infinite) values.");
         return SRED_ERR_NONFINITE;
     }
-    std::sort(ctx->host_tile_var.begin(), ctx->host_tile_var.begin() + nb);
+    std::sort(h_tiles, h_tiles + nb);
     int nvars = (int)(0.03f * (float)nb + 1.0f);
     float nv = 0.0f;
-    for (int i = 0; i < nvars && i < nb; i++) nv += ctx->host_tile_var[i];
+    for (int i = 0; i < nvars && i < nb; i++) nv += h_tiles[i];
     nv = fabsf(nv / (float)nvars);
     nv = (1.0f + 0.001f * (nv - 40.0f)) * nv;
     *noise_var = nv;
     return SRED_OK;
+}
+
+int noise_var_dev(sred_ctx *ctx, const float *d_img, int w, int h, float *noise_var, bool double_sums)
+{
+    int tbw, tbh;
+    const int nb = noise_tiles(w, h, &tbw, &tbh);
+    if (nb < 0) return SRED_ERR_ARG;
+    ctx->host_tile_var.resize(nb + 1);
+    SRED_TRY(noise_var_enqueue(ctx, ctx->tile_var, d_img, w, h, ctx->host_tile_var.data(), double_sums));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return noise_var_finish(ctx->host_tile_var.data(), nb, noise_var);
 }
 
 // ---------------------------------------------------------------------------
@@ -188,20 +216,29 @@ __global__ __launch_bounds__(256) void k_relevance_mask(const float *__restrict_
     if (threadIdx.x == 0 && red[0]) atomicAdd(count, (unsigned long long)red[0]);
 }
 
-int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
-                       float *n_pixels)
+int relevance_mask_enqueue(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
+                           unsigned long long *d_count, unsigned long long *h_count)
 {
-    SRED_TRY(ctx->misc.ensure(kMiscBytes));
-    unsigned long long *d_count = ctx->misc.as<unsigned long long>();
     SRED_TRY(zero_dev(ctx, d_count, sizeof(unsigned long long)));
     int blocks = (int)std::min<int64_t>((g.n() + 255) / 256, 4096);
     hipLaunchKernelGGL(k_relevance_mask, dim3(blocks), dim3(256), 0, ctx->stream, d_stds, g.w, g.h, g.brw, g.brh,
                        threshold, d_mask, d_count);
     SRED_HIP_CHECK(hipGetLastError());
+    SRED_HIP_CHECK(hipMemcpyAsync(h_count, d_count, sizeof *h_count, hipMemcpyDeviceToHost, ctx->stream));
+    return SRED_OK;
+}
+
+// nPixels as Java counts it: float += 1.0f, which stops growing at 2^24.
+float n_pixels_of(unsigned long long count) { return (float)std::min<unsigned long long>(count, 16777216ull); }
+
+int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, float threshold, float *d_mask,
+                       float *n_pixels)
+{
+    SRED_TRY(ctx->misc.ensure(kMiscBytes));
     unsigned long long cnt = 0;
-    SRED_HIP_CHECK(hipMemcpyAsync(&cnt, d_count, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+    SRED_TRY(relevance_mask_enqueue(ctx, g, d_stds, threshold, d_mask, ctx->misc.as<unsigned long long>(), &cnt));
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    *n_pixels = (float)std::min<unsigned long long>(cnt, 16777216ull);
+    *n_pixels = n_pixels_of(cnt);
     return SRED_OK;
 }
 
