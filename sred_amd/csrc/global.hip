@@ -554,12 +554,14 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
                 for (int r = 0; r < 4; r++) epilogue<M, EXACT, COLS>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], ec, &cs);
             colsum[cg] = cs;
         }
-        if (cg >= 0 && cg + 1 < 4) {
-            // per MFMA of the next group, a few VALU/transcendental ops of this one
+        if (M != SRED_SSIM && cg >= 0 && cg + 1 < 4) {
+            // per MFMA of the next group, two VALU/transcendental ops of this
+            // one (measured best for Pearson; SSIM's longer epilogue is left
+            // to the compiler's own order, also measured faster)
 #pragma unroll
             for (int i = 0; i < 3 * KS * RG; i++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x402, 4, 0);  // VALU | TRANS
+                __builtin_amdgcn_sched_group_barrier(0x402, 2, 0);  // VALU | TRANS
             }
         }
     }
