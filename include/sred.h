@@ -42,7 +42,9 @@ extern "C" {
 
 /* ---- contraction precision of the global maps ---- */
 #define SRED_FP32 0   /* plain FP32 vector-ALU contraction */
-#define SRED_BF16X3 1 /* split-BF16 (hi*hi + hi*lo + lo*hi) on MFMA, FP32 accumulate */
+#define SRED_BF16X3 1 /* split-BF16 (hi*hi + hi*lo + lo*hi) on MFMA, FP32 accumulate; blocks of more than
+                         256 pixels (21x21 and larger) take the FP32 contraction, so every odd block size
+                         the reference accepts works in either precision */
 
 /* Largest supported block side (pixels); K <= 3200. */
 #define SRED_MAX_BLOCK_SIDE 63
@@ -506,9 +508,9 @@ int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision
 int sred_ctx_last_contraction(sred_ctx *ctx, float *ms, int *launches, int64_t *pairs_evaluated);
 /* Ordered block pairs whose evaluation the last global call (single or
  * batched, split-BF16 symmetric contraction, Pearson or SSIM) skipped because
- * their t = sigma * sqrt(log2(e) / (h + eps)) differ by more than sqrt(40),
- * i.e. their weight exp(-(s_i - s_j)^2 / (h + eps)) is below 2^-40; 0 for
- * other paths.  Synchronises.  No counterpart in the reference (measurement). */
+ * their t = sigma * sqrt(log2(e) / (h + eps)) differ by more than sqrt(40)
+ * (Pearson) or sqrt(47) (SSIM, whose luminance factor reaches 100.5), so that
+ * every skipped term is below 2^-40; 0 for other paths.  Synchronises.  No counterpart in the reference (measurement). */
 int sred_ctx_last_skipped(sred_ctx *ctx, int64_t *pairs_skipped);
 
 #ifdef __cplusplus

@@ -277,7 +277,7 @@ class Context:
         return ms.value, n.value, p.value
 
     def last_skipped(self):
-        """Ordered block pairs the last global call skipped (weight < 2^-40)."""
+        """Ordered block pairs the last global call skipped (each term below 2^-40)."""
         p = ctypes.c_int64()
         check(self._lib.sred_ctx_last_skipped(self.handle, ctypes.byref(p)))
         return p.value
