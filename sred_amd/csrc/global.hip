@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "sred_internal.h"
 
@@ -712,8 +713,22 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
     if (!diag)
         for (int i = tid; i < P; i += 256) colacc[i] = 0.0f;
 
-    const int ntiles = P / kBN;  // multiple of 4: the step count below is even
+    const int ntiles = P / kBN;  // multiple of 4 (the step count below is even), at most 64
     const int NT = ntiles * (P / BM);
+    // per column tile of the panel, held by lane (tile index): its t range
+    // and whether it holds a small-sigma block (exact Pearson epilogue); read
+    // back with readlane, so the tile loop waits on no memory for them
+    float tile_tlo = 0.0f, tile_thi = 0.0f;
+    unsigned long long tile_exact = 0ull;
+    if (lane < ntiles) {
+        const int cc = col0 + lane * kBN;
+        if (skips<M>()) {
+            tile_tlo = aux[cc].x;
+            tile_thi = aux[cc + kBN - 1].x;
+        }
+        if (M == SRED_PEARSON) tile_exact = (tile_small[cc / 32] | tile_small[cc / 32 + 1]) != 0;
+    }
+    if (M == SRED_PEARSON) tile_exact = __ballot(tile_exact != 0ull);
 
     bf16x8 ahi[RG][KS], alo[RG][KS];
     float4 R[RG][4];
@@ -772,10 +787,14 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
                 if (li == 0) partial[(size_t)c * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = s;
             }
     };
-    auto compute = [&](int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
-        if (skips<M>()) {
-            const float tlo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[0].x)));
-            const float thi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ccon[kBN - 1].x)));
+    // SKIP (a std::bool_constant): whether this panel pair may hold tiles
+    // whose weights vanish (decided once per panel pair below), so pairs
+    // that cannot skip run a tile loop without the test
+    auto compute = [&](auto skip_tag, int T, const uint4 *chi, const uint4 *clo, const float4 *ccon) {
+        const int tt = T % ntiles;
+        if (skips<M>() && decltype(skip_tag)::value) {
+            const float tlo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tile_tlo), tt));
+            const float thi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tile_thi), tt));
             if (tlo - wt_hi > skip_gap<M>() || wt_lo - thi > skip_gap<M>()) {  // the tile's weights vanish for these rows
                 if (!diag) stage[T & 1][wave][lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 nskip += (diag ? 1ull : 2ull) * (unsigned long long)(16 * RG * kBN);
@@ -783,9 +802,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
                 return;
             }
         }
-        // small-sigma columns (aux.z of Pearson) from the tile's constants in
-        // LDS: no global load inside the tile loop
-        const bool exact = row_small || (M == SRED_PEARSON && __ballot(ccon[lane].z != 0.0f) != 0ull);
+        const bool exact = row_small || (M == SRED_PEARSON && ((tile_exact >> tt) & 1ull) != 0ull);
         float4 *sw = stage[T & 1][wave];
         if (diag) {
             if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
@@ -797,6 +814,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
         if (T % ntiles == ntiles - 1) end_sweep();
     };
 
+    auto tiles = [&](auto skip_tag) {
     if constexpr (DB) {
         // step T: wait for tile T's copy, barrier, fold tile T-1's column sums,
         // start sweep if needed, issue tile T+1's copy, compute tile T
@@ -812,7 +830,7 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
                 wait_vm_all();
             }
             if (T + 1 < NT) tile_to_lds<KS>(Uhi, Ulo, aux, col0 + ((T + 1) % ntiles) * kBN, nhi, nlo, ncon, wave, lane);
-            compute(T, chi, clo, ccon);
+            compute(skip_tag, T, chi, clo, ccon);
         };
         tile_to_lds<KS>(Uhi, Ulo, aux, col0, hi0, lo0, con0, wave, lane);
         for (int T = 0; T < NT; T += 2) {
@@ -828,9 +846,15 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *
             tile_to_lds<KS>(Uhi, Ulo, aux, col0 + (T % ntiles) * kBN, hi0, lo0, con0, wave, lane);
             wait_vm_all();
             __syncthreads();
-            compute(T, hi0, lo0, con0);
+            compute(skip_tag, T, hi0, lo0, con0);
         }
     }
+    };
+    // the largest t difference of the panel pair (rows and columns are each
+    // in ascending t)
+    const float span = fmaxf(aux[col0 + P - 1].x - aux[row0].x, aux[row0 + P - 1].x - aux[col0].x);
+    if (skips<M>() && span > skip_gap<M>()) tiles(std::true_type{});
+    else tiles(std::false_type{});
     if (!diag) {
         wait_vm_all();
         __syncthreads();
