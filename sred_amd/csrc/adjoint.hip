@@ -239,11 +239,33 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 // hi = RNE bf16 pair of (a, b) (one v_cvt_pk_bf16_f32), lo = the same of the remainders
-__device__ __forceinline__ void split_bf16x2(float a, float b, unsigned int &hi, unsigned int &lo)
+// The BF16 pairs (-1, 0) and (0, -1) as 32-bit words in SGPRs, for
+// split_bf16x2_dot below.  They are set by s_mov_b32 in inline asm so that the
+// compiler cannot fold them: it encodes the pair (-1, 0) (word 0x0000bf80)
+// as the 32-bit inline constant -1.0 (word 0xbf800000), which
+// v_dot2c_f32_bf16 reads as the pair (0, -1).
+struct Bf16Neg {
+    bf16x2 x, y;
+};
+__device__ __forceinline__ Bf16Neg bf16_neg_units()
+{
+    unsigned int kx, ky;
+    asm volatile("s_mov_b32 %0, 0xbf80" : "=s"(kx));
+    asm volatile("s_mov_b32 %0, 0xbf800000" : "=s"(ky));
+    return Bf16Neg{__builtin_bit_cast(bf16x2, kx), __builtin_bit_cast(bf16x2, ky)};
+}
+// (a, b) -> BF16 pairs hi = rne(a, b), lo = rne(a - hi.x, b - hi.y), the
+// residuals taken by v_dot2_f32_bf16 (hi . (-1, 0) + a): one instruction
+// each instead of a widening shift or mask and a subtraction.  The residual
+// of a round-to-nearest BF16 is exactly representable in FP32, so this gives
+// the bits of the subtraction form.
+__device__ __forceinline__ void split_bf16x2_dot(float a, float b, const Bf16Neg &n, unsigned int &hi,
+                                                 unsigned int &lo)
 {
     const bf16x2 h = __builtin_convertvector(floatx2{a, b}, bf16x2);
-    const floatx2 hf = __builtin_convertvector(h, floatx2);
-    const bf16x2 l = __builtin_convertvector(floatx2{a - hf.x, b - hf.y}, bf16x2);
+    const float la = __builtin_amdgcn_fdot2_f32_bf16(h, n.x, a, false);
+    const float lb = __builtin_amdgcn_fdot2_f32_bf16(h, n.y, b, false);
+    const bf16x2 l = __builtin_convertvector(floatx2{la, lb}, bf16x2);
     hi = __builtin_bit_cast(unsigned int, h);
     lo = __builtin_bit_cast(unsigned int, l);
 }
@@ -347,6 +369,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         R[rg].w *= eps;  // delta = (eps / s_i) (1 / s_j)
     }
     const bool row_small = small[rowbase / 32] != 0;
+    const Bf16Neg negu = bf16_neg_units();
     floatx4 acc[KG][RG];
 #pragma unroll
     for (int kg = 0; kg < KG; kg++)
@@ -432,10 +455,9 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
                 om = delta;
             }
             const float dv = d[r];
-            const float dp = __builtin_amdgcn_fmed3f(dv, 0.0f, FLT_MAX);
             const float aw = (R[rg].y + C.y) * wt;
             const float Q = dv > 0.0f ? aw : 0.0f;
-            const float Qd = Q * dp;
+            const float Qd = Q * dv;  // = Q max(d, 0): Q is 0 where d <= 0
             corr = fmaf(Qd, om, corr);
             se = fmaf(fmaf(R[rg].z + C.z, w, Qd), dt, se);
             pv[h * 4 + r] = Q;
@@ -445,7 +467,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
     auto grads = [&](const uint4 *vh, const uint4 *vl, int st, int rg, const float (&pv)[8]) {
         unsigned int ph[4], pl[4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) split_bf16x2(pv[2 * q], pv[2 * q + 1], ph[q], pl[q]);
+        for (int q = 0; q < 4; q++) split_bf16x2_dot(pv[2 * q], pv[2 * q + 1], negu, ph[q], pl[q]);
         const bf16x8 bH = __builtin_bit_cast(bf16x8, make_uint4(ph[0], ph[1], ph[2], ph[3]));
         const bf16x8 bL = __builtin_bit_cast(bf16x8, make_uint4(pl[0], pl[1], pl[2], pl[3]));
 #pragma unroll
