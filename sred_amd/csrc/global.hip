@@ -519,30 +519,54 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 // the tile (the k-groups and waves are summed when the tile is folded, so no
 // cross-lane operation is needed).  B fragments are read from the
 // fragment-major LDS tile: 64 consecutive uint4 per (group, k-step).
+// Pearson and the unweighted NRMSE with K <= 96 read the B fragments of
+// group cg+2 into a second register set while group cg+1's MFMAs and group
+// cg's epilogue run, so no MFMA waits on LDS latency (measured 2.8% faster at
+// 1024^2, 11x11); the other cases read each group's fragments just before
+// its MFMAs (with the second set SSIM spills and runs 4% slower, and so do
+// all metrics beyond K = 96).
 template <int M, int KS, int RG, bool EXACT, bool COLS>
 __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
                                          const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
                                          const float4 (&R)[RG][4], float (&accS)[RG][4], float4 *stage_w, int lane,
                                          const EpiConst &ec)
 {
-    const int li = lane & 15, lk = lane >> 4;
+    constexpr bool PF = (M == SRED_PEARSON || M == SRED_NRMSE) && KS <= 3;
+    const int li = lane & 15;
     floatx4 acc[2][RG];
     float colsum[4];
+    uint4 bh[PF ? 2 : 1][KS], bl[PF ? 2 : 1][KS];
+    if (PF) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            bh[0][ks] = lds_hi[ks * 64 + lane];
+            bl[0][ks] = lds_lo[ks * 64 + lane];
+        }
+    }
 #pragma unroll
     for (int cg = -1; cg < 4; cg++) {
+        if (PF && cg + 2 < 4) {  // B fragments of group cg+2, into the set group cg used
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                bh[(cg + 2) & 1][ks] = lds_hi[((cg + 2) * KS + ks) * 64 + lane];
+                bl[(cg + 2) & 1][ks] = lds_lo[((cg + 2) * KS + ks) * 64 + lane];
+            }
+        }
         if (cg + 1 < 4) {  // issue group cg+1
             floatx4(&a)[RG] = acc[(cg + 1) & 1];
 #pragma unroll
             for (int rg = 0; rg < RG; rg++) a[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
-                bf16x8 bh = __builtin_bit_cast(bf16x8, lds_hi[((cg + 1) * KS + ks) * 64 + lane]);
-                bf16x8 bl = __builtin_bit_cast(bf16x8, lds_lo[((cg + 1) * KS + ks) * 64 + lane]);
+                const bf16x8 h = __builtin_bit_cast(
+                    bf16x8, PF ? bh[(cg + 1) & 1][ks] : lds_hi[((cg + 1) * KS + ks) * 64 + lane]);
+                const bf16x8 l = __builtin_bit_cast(
+                    bf16x8, PF ? bl[(cg + 1) & 1][ks] : lds_lo[((cg + 1) * KS + ks) * 64 + lane]);
 #pragma unroll
                 for (int rg = 0; rg < RG; rg++) {
-                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bh, a[rg], 0, 0, 0);
-                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], bl, a[rg], 0, 0, 0);
-                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], bh, a[rg], 0, 0, 0);
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], h, a[rg], 0, 0, 0);
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[rg][ks], l, a[rg], 0, 0, 0);
+                    a[rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[rg][ks], h, a[rg], 0, 0, 0);
                 }
             }
         }
@@ -556,9 +580,11 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
             colsum[cg] = cs;
         }
         if (M != SRED_SSIM && cg >= 0 && cg + 1 < 4) {
-            // per MFMA of the next group, two VALU/transcendental ops of this
-            // one (measured best for Pearson; SSIM's longer epilogue is left
-            // to the compiler's own order, also measured faster)
+            // the prefetch reads first, then per MFMA of the next group two
+            // VALU/transcendental ops of this one (measured best for Pearson;
+            // SSIM's longer epilogue is left to the compiler's own order,
+            // also measured faster)
+            if (PF && cg + 2 < 4) __builtin_amdgcn_sched_group_barrier(0x100, 2 * KS, 0);  // DS reads
 #pragma unroll
             for (int i = 0; i < 3 * KS * RG; i++) {
                 __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
