@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(const int *__restrict__ id
                                                       const float *__restrict__ W, const float *__restrict__ mask,
                                                       const float *__restrict__ dmap, const unsigned int *__restrict__ mm,
                                                       float n_pixels, float tscale, float4 *__restrict__ info,
-                                                      float4 *__restrict__ info2)
+                                                      float4 *__restrict__ info2, unsigned int *__restrict__ bmax)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= Nv_pad) return;
@@ -63,6 +63,8 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(const int *__restrict__ id
     double b = -g * rep * (double)n_pixels / D;
     info[v] = make_float4(s, (float)a, (float)b, s * tscale);
     info2[v] = make_float4(s * tscale, (float)a, (float)b, s > 0.0f ? 1.0f / s : 0.0f);
+    // max |b| over the blocks (bits of a non-negative float order as unsigned; NaN above every finite value)
+    atomicMax(bmax, __float_as_uint(fabsf((float)b)));
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int value)
@@ -270,11 +272,11 @@ __device__ __forceinline__ void split_bf16x2_dot(float a, float b, const Bf16Neg
     lo = __builtin_bit_cast(unsigned int, l);
 }
 
+// V for the k-groups kg < KG (16 k values each; KG <= 2 KS).
 __global__ __launch_bounds__(256) void k_transpose_frag(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
-                                                        int Nv_pad, int KS, uint4 *__restrict__ Vhi,
+                                                        int Nv_pad, int KS, int KG, uint4 *__restrict__ Vhi,
                                                         uint4 *__restrict__ Vlo)
 {
-    const int KG = 2 * KS;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)(Nv_pad / 32) * KG * 64) return;
     const int lane = (int)(t & 63);
@@ -305,6 +307,8 @@ constexpr int kAdjRows = 4 * 16 * kAdjRG;  // rows per workgroup
 // kAdjFastSigma, so delta = eps/(s_i s_j) <= 3.1e-4 and 1/(1 + delta) is
 // taken as 1 - delta (relative error delta^2 <= 1e-7); other tiles divide.
 constexpr float kAdjFastSigma = 0.018f;
+constexpr float kLn2f = 0.693147180559945309f;
+constexpr float kLog2ef = 1.442695040888963407f;
 
 // small[g] = 1 if a block of [32g, 32g + 32) has 0 < sigma < kAdjFastSigma
 // (info.x = sigma).
@@ -329,21 +333,36 @@ __device__ __forceinline__ void adj_wait_all() { __builtin_amdgcn_s_waitcnt(0); 
 //   G_i   = sum_j Q_ij u_j                          (second product, MFMA)
 //   q_i   = sum_j Q_ij d+ / (1 + delta)             (= sum_j Q_ij r_ij; from
 //           u_i . G_i less the per-pair correction sum_j Q_ij d+ delta/(1 + delta))
-//   e_i   = sum_j (Q_ij d+ + (b_i + b_j) w)(t_i - t_j)
+//   e_i   = sum_j Q_ij d+ (t_i - t_j) + F_i
 // and its block gradient is, with dL/ds_i = -q_i / s_i + cE e_i
 // (cE = -2/(H tscale)),
 //   (G_i / s_i + dL/ds_i u_i) / sqrt(K-1), centred over the K pixels.
-// info2 = (t, a, b, 1/s) per block (1/s = 0 for flat blocks and padding).
-template <int KS>
+// F_i = sum_j (b_i + b_j) w_ij (t_i - t_j), the part of e_i that depends on the
+// sigmas only, comes from the binned Gauss expansion (adjoint_weight_terms_dev,
+// FP64) and is added here per row, so the pair loop carries no weight-sum term.
+//
+// The u_j are centred (sum_k u_jk = 0), so G_i is too: the second product
+// runs over the k-groups kg < KG = ceil((K - 1) / 16) only, and when K - 1 is a
+// multiple of 16 the last component is G_{i,K-1} = -sum_{k < K-1} G_ik (at
+// K = 49 three k-groups instead of four).  Its error is that of the split
+// fragments summed over k (about sqrt(K) 2^-17 relative), still far below
+// the FP32 rounding of the forward the gradient is checked against.
+//
+// info2 = (t, a, b, 1/s) per block (1/s = 0 for flat blocks and padding);
+// F[v] for v < Nv.
+template <int KS, int KG>
 __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                          const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
                                                          const float4 *__restrict__ info2, const int *__restrict__ small,
-                                                         const float *__restrict__ UT, int K, int Nv, int Nv_pad,
-                                                         float eps, float cE, float inv_sqrt_km1, int K16,
-                                                         float *__restrict__ Gc)
+                                                         const double *__restrict__ F, const float *__restrict__ UT,
+                                                         int K, int Nv, int Nv_pad, float eps, float cE,
+                                                         float inv_sqrt_km1, int K16, float *__restrict__ Gc)
 {
-    constexpr int KG = 2 * KS, RG = kAdjRG;
+    constexpr int RG = kAdjRG;
+    constexpr int KGA = 2 * KS;                            // k-groups of a block (all K values)
     constexpr int URUN = 4 * KS * 64, VRUN = 2 * KG * 64;  // uint4 of one 64-column tile
+    constexpr int NRUN = 8 * KS + 4 * KG;                  // runs of 64 uint4 copied per tile
+    static_assert(NRUN % 4 == 0, "tile runs split over 4 waves");
     // two tile buffers (distinct arrays: reading one does not wait for the copy into the other)
     __shared__ uint4 uh0[URUN], ul0[URUN], vh0[VRUN], vl0[VRUN];
     __shared__ uint4 uh1[URUN], ul1[URUN], vh1[VRUN], vl1[VRUN];
@@ -366,7 +385,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         }
         R[rg] = info2[rowbase + rg * 16 + li];
         rinv[rg] = R[rg].w;
-        R[rg].w *= eps;  // delta = (eps / s_i) (1 / s_j)
+        R[rg].w *= eps * kLog2ef;  // delta log2(e) = (eps log2(e) / s_i) (1 / s_j)
     }
     const bool row_small = small[rowbase / 32] != 0;
     const Bf16Neg negu = bf16_neg_units();
@@ -385,14 +404,12 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
 #pragma unroll
             for (int kg = 0; kg < KG; kg++) {
                 const int k = 16 * kg + 4 * lq;
-                if (k < K16) {
-                    float4 t = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *reinterpret_cast<const float4 *>(row + k);
-                    t.x += acc[kg][rg][0];
-                    t.y += acc[kg][rg][1];
-                    t.z += acc[kg][rg][2];
-                    t.w += acc[kg][rg][3];
-                    *reinterpret_cast<float4 *>(row + k) = t;
-                }
+                float4 t = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : *reinterpret_cast<const float4 *>(row + k);
+                t.x += acc[kg][rg][0];
+                t.y += acc[kg][rg][1];
+                t.z += acc[kg][rg][2];
+                t.w += acc[kg][rg][3];
+                *reinterpret_cast<float4 *>(row + k) = t;
                 acc[kg][rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
             }
         }
@@ -400,19 +417,23 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
     double gq[RG] = {0.0, 0.0}, ge[RG] = {0.0, 0.0};
     const int ntiles = Nv_pad / 64;
 
-    // asynchronous copy of tile T: 16 KS runs of 64 uint4 (the column blocks'
-    // fragments hi/lo and their transposed fragments hi/lo) and the info run
+    // asynchronous copy of tile T: the column blocks' fragments hi/lo (4 KS
+    // runs of 64 uint4 each), their transposed fragments hi/lo (2 KG runs
+    // each) and the info run
     auto copy = [&](int T, uint4 *duh, uint4 *dul, uint4 *dvh, uint4 *dvl, float4 *dci) {
         typedef __attribute__((address_space(3))) void *lds_ptr;
         const int c0 = T * 64;
         const size_t ou = (size_t)(c0 / 16) * KS * 64, ov = (size_t)(c0 / 32) * KG * 64;
 #pragma unroll
-        for (int i = 0; i < 4 * KS; i++) {
+        for (int i = 0; i < NRUN / 4; i++) {
             const int run = wave + 4 * i;
-            const int a = run / (4 * KS), b = run - a * (4 * KS);
-            const uint4 *src = a == 0 ? Uhi + ou : a == 1 ? Ulo + ou : a == 2 ? Vhi + ov : Vlo + ov;
-            uint4 *dst = a == 0 ? duh : a == 1 ? dul : a == 2 ? dvh : dvl;
-            __builtin_amdgcn_global_load_lds((const void *)(src + b * 64 + lane), (lds_ptr)(dst + b * 64), 16, 0, 0);
+            const uint4 *src;
+            uint4 *dst;
+            if (run < 4 * KS) { src = Uhi + ou + run * 64; dst = duh + run * 64; }
+            else if (run < 8 * KS) { src = Ulo + ou + (run - 4 * KS) * 64; dst = dul + (run - 4 * KS) * 64; }
+            else if (run < 8 * KS + 2 * KG) { src = Vhi + ov + (run - 8 * KS) * 64; dst = dvh + (run - 8 * KS) * 64; }
+            else { src = Vlo + ov + (run - 8 * KS - 2 * KG) * 64; dst = dvl + (run - 8 * KS - 2 * KG) * 64; }
+            __builtin_amdgcn_global_load_lds((const void *)(src + lane), (lds_ptr)dst, 16, 0, 0);
         }
         if (wave == 3) __builtin_amdgcn_global_load_lds((const void *)(info2 + c0 + lane), (lds_ptr)dci, 16, 0, 0);
     };
@@ -434,8 +455,10 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         }
     };
     // pair terms of column group cg (half h of its step) for row group rg:
-    //   Q into pv[h*4 + r]; corr += Q d+ (1 - 1/(1 + delta)); se += (Q d+ + (b_i + b_j) w)(t_i - t_j)
-    // Fast tiles take 1 - 1/(1 + delta) = delta and w~ = w (1 - delta) (relative error delta^2 <= 1e-7).
+    //   Q into pv[h*4 + r]; corr += Q d+ (1 - 1/(1 + delta)) log2(e); se += Q d+ (t_i - t_j)
+    // Fast tiles take w~ = exp2(-(t_i - t_j)^2 - delta log2(e)) = w e^-delta and
+    // 1 - 1/(1 + delta) = delta (relative errors delta^2/2 and delta, delta <= 3.1e-4);
+    // exact tiles divide.  corr is kept in log2(e) units (scaled by ln 2 per tile).
     auto pairs = [&](auto exact_tag, const float4 *ci, int cg, int h, int rg, const floatx4 &d, float (&pv)[8],
                      float &corr, float &se) {
         constexpr bool EXACT = decltype(exact_tag)::value;
@@ -443,25 +466,48 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         for (int r = 0; r < 4; r++) {
             const float4 C = ci[cg * 16 + 4 * lq + r];  // column j = c0 + 16cg + 4(l/16) + r
             const float dt = R[rg].x - C.x;
-            const float w = __builtin_amdgcn_exp2f(-(dt * dt));
-            const float delta = R[rg].w * C.w;
-            float wt, om;  // w / (1 + delta), 1 - 1/(1 + delta)
+            const float dl = R[rg].w * C.w;  // delta log2(e)
+            float wt, om;                    // w / (1 + delta), (1 - 1/(1 + delta)) log2(e)
             if (EXACT) {
-                const float kap = __builtin_amdgcn_rcpf(1.0f + delta);
+                const float w = __builtin_amdgcn_exp2f(-(dt * dt));
+                const float kap = __builtin_amdgcn_rcpf(fmaf(dl, kLn2f, 1.0f));
                 wt = w * kap;
-                om = delta * kap;
+                om = dl * kap;
             } else {
-                wt = fmaf(-w, delta, w);
-                om = delta;
+                wt = __builtin_amdgcn_exp2f(-fmaf(dt, dt, dl));
+                om = dl;
             }
             const float dv = d[r];
             const float aw = (R[rg].y + C.y) * wt;
             const float Q = dv > 0.0f ? aw : 0.0f;
             const float Qd = Q * dv;  // = Q max(d, 0): Q is 0 where d <= 0
             corr = fmaf(Qd, om, corr);
-            se = fmaf(fmaf(R[rg].z + C.z, w, Qd), dt, se);
+            se = fmaf(Qd, dt, se);
             pv[h * 4 + r] = Q;
         }
+    };
+    // the same for one pair (r)
+    auto pair1 = [&](auto exact_tag, const float4 &C, int h, int rg, int r, float dv, float (&pv)[8], float &corr,
+                     float &se) {
+        constexpr bool EXACT = decltype(exact_tag)::value;
+        const float dt = R[rg].x - C.x;
+        const float dl = R[rg].w * C.w;
+        float wt, om;
+        if (EXACT) {
+            const float w = __builtin_amdgcn_exp2f(-(dt * dt));
+            const float kap = __builtin_amdgcn_rcpf(fmaf(dl, kLn2f, 1.0f));
+            wt = w * kap;
+            om = dl * kap;
+        } else {
+            wt = __builtin_amdgcn_exp2f(-fmaf(dt, dt, dl));
+            om = dl;
+        }
+        const float aw = (R[rg].y + C.y) * wt;
+        const float Q = dv > 0.0f ? aw : 0.0f;
+        const float Qd = Q * dv;
+        corr = fmaf(Qd, om, corr);
+        se = fmaf(Qd, dt, se);
+        pv[h * 4 + r] = Q;
     };
     // second-product MFMAs of step st for row group rg (coefficients pv)
     auto grads = [&](const uint4 *vh, const uint4 *vl, int st, int rg, const float (&pv)[8]) {
@@ -479,22 +525,93 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
             acc[kg][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bH, acc[kg][rg], 0, 0, 0);
         }
     };
-    // one 32-column step: the dot tiles of its second half are issued before
-    // the pair terms of the first, and row group 0's gradient MFMAs before
-    // row group 1's last pair terms, so the matrix core has work while the
-    // vector ALU evaluates pairs
-    auto step32 = [&](auto exact_tag, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl,
-                      const float4 *ci, int st, float (&corr)[RG], float (&se)[RG]) {
-        float pv[RG][8];
-        floatx4 d0[RG], d1[RG];
-        dots(uh, ul, 2 * st, d0);
-        dots(uh, ul, 2 * st + 1, d1);
+    // one 64-column tile as two 32-column steps, software-pipelined so that
+    // the matrix core has independent work beside every stretch of pair terms:
+    //   dots(step 0) | pairs(step 0) with dots(step 1) | pairs(step 1) with
+    //   grads(step 0) | grads(step 1)
+    // Inside the two middle phases the MFMAs are spread evenly between the 16
+    // pairs of the lane, with scheduling fences (which only LDS reads may
+    // cross) after each pair: the compiler would otherwise issue all MFMAs of
+    // a phase first.
+    auto tile64 = [&](auto exact_tag, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl,
+                      const float4 *ci, float (&corr)[RG], float (&se)[RG]) {
+        constexpr int MDC = 3 * KS * RG;  // MFMAs of the dot tiles of one column group
+        constexpr int MD = 2 * MDC;       // ... of one step
+        constexpr int MG = 3 * KG * RG;   // MFMAs of the gradient product of one step
+        float pv0[RG][8], pv1[RG][8];
+        floatx4 d0[RG], d1[RG], dn[2][RG];
+        dots(uh, ul, 0, d0);
+        dots(uh, ul, 1, d1);
+        __builtin_amdgcn_sched_barrier(0);
+        // phase 1: pairs of step 0, dots of step 1 (column groups 2, 3)
+        bf16x8 fh[2][KS], fl[2][KS];
 #pragma unroll
-        for (int rg = 0; rg < RG; rg++) pairs(exact_tag, ci, 2 * st, 0, rg, d0[rg], pv[rg], corr[rg], se[rg]);
-        pairs(exact_tag, ci, 2 * st + 1, 1, 0, d1[0], pv[0], corr[0], se[0]);
-        grads(vh, vl, st, 0, pv[0]);
-        pairs(exact_tag, ci, 2 * st + 1, 1, 1, d1[1], pv[1], corr[1], se[1]);
-        grads(vh, vl, st, 1, pv[1]);
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                fh[c][ks] = __builtin_bit_cast(bf16x8, uh[((2 + c) * KS + ks) * 64 + lane]);
+                fl[c][ks] = __builtin_bit_cast(bf16x8, ul[((2 + c) * KS + ks) * 64 + lane]);
+            }
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int rg = 0; rg < RG; rg++) dn[c][rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        auto dot_mfma = [&](int m) {
+            const int c = m / MDC, ks = (m % MDC) / (3 * RG), rg = (m % (3 * RG)) / 3, sp = m % 3;
+            dn[c][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sp == 2 ? fl[c][ks] : fh[c][ks],
+                                                                sp == 1 ? bl[rg][ks] : bh[rg][ks], dn[c][rg], 0, 0, 0);
+        };
+        float4 C[2][4];  // column constants of the step's 8 columns of this lane
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) C[c][r] = ci[c * 16 + 4 * lq + r];
+#pragma unroll
+        for (int q = 0; q < 8 * RG; q++) {  // q = (rg, column group, r)
+            const int rg = q / 8, cg = (q / 4) % 2, r = q % 4;
+#pragma unroll
+            for (int m = MD * q / (8 * RG); m < MD * (q + 1) / (8 * RG); m++) dot_mfma(m);
+            pair1(exact_tag, C[cg][r], cg, rg, r, cg == 0 ? d0[rg][r] : d1[rg][r], pv0[rg], corr[rg], se[rg]);
+            __builtin_amdgcn_sched_barrier(0x100);
+        }
+        // phase 2: pairs of step 1, gradient product of step 0
+        bf16x8 gH[RG], gL[RG], vfh[KG], vfl[KG];
+        {
+#pragma unroll
+            for (int rg = 0; rg < RG; rg++) {
+                unsigned int ph[4], pl[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) split_bf16x2_dot(pv0[rg][2 * q], pv0[rg][2 * q + 1], negu, ph[q], pl[q]);
+                gH[rg] = __builtin_bit_cast(bf16x8, make_uint4(ph[0], ph[1], ph[2], ph[3]));
+                gL[rg] = __builtin_bit_cast(bf16x8, make_uint4(pl[0], pl[1], pl[2], pl[3]));
+            }
+#pragma unroll
+            for (int kg = 0; kg < KG; kg++) {
+                vfh[kg] = __builtin_bit_cast(bf16x8, vh[kg * 64 + lane]);
+                vfl[kg] = __builtin_bit_cast(bf16x8, vl[kg * 64 + lane]);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) C[c][r] = ci[(2 + c) * 16 + 4 * lq + r];
+        }
+        __builtin_amdgcn_sched_barrier(0x100);
+        auto grad_mfma = [&](int m) {
+            const int rg = m / (3 * KG), kg = (m % (3 * KG)) / 3, sp = m % 3;
+            acc[kg][rg] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sp == 2 ? vfl[kg] : vfh[kg],
+                                                                  sp == 1 ? gL[rg] : gH[rg], acc[kg][rg], 0, 0, 0);
+        };
+#pragma unroll
+        for (int q = 0; q < 8 * RG; q++) {
+            const int rg = q / 8, cg = (q / 4) % 2, r = q % 4;
+#pragma unroll
+            for (int m = MG * q / (8 * RG); m < MG * (q + 1) / (8 * RG); m++) grad_mfma(m);
+            pair1(exact_tag, C[cg][r], cg, rg, r, dn[cg][rg][r], pv1[rg], corr[rg], se[rg]);
+            __builtin_amdgcn_sched_barrier(0x100);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        grads(vh, vl, 1, 0, pv1[0]);
+        grads(vh, vl, 1, 1, pv1[1]);
     };
 
     auto compute = [&](int T, const uint4 *uh, const uint4 *ul, const uint4 *vh, const uint4 *vl, const float4 *ci) {
@@ -503,11 +620,8 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         float corr[RG], se[RG];
 #pragma unroll
         for (int rg = 0; rg < RG; rg++) { corr[rg] = 0.0f; se[rg] = 0.0f; }
-#pragma unroll 1
-        for (int st = 0; st < 2; st++) {
-            if (exact) step32(std::true_type{}, uh, ul, vh, vl, ci, st, corr, se);
-            else step32(std::false_type{}, uh, ul, vh, vl, ci, st, corr, se);
-        }
+        if (exact) tile64(std::true_type{}, uh, ul, vh, vl, ci, corr, se);
+        else tile64(std::false_type{}, uh, ul, vh, vl, ci, corr, se);
 #pragma unroll
         for (int rg = 0; rg < RG; rg++) {
             gq[rg] += (double)corr[rg];
@@ -533,6 +647,9 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
 
     // per row i: G_k = (tot_k / s_i + dL/ds_i u_ik) / sqrt(K-1), centred over k < K.
     // Lane l holds i = l % 16 of each row group and k = 16kg + 4(l/16) + r.
+    // The second product wrote tot_k for k < 16 KG (at least k < K - 1);
+    // tot_{K-1} beyond that is minus the sum of the others.
+    const bool derive_last = (K - 1) == 16 * KG;
 #pragma unroll
     for (int rg = 0; rg < RG; rg++) {
         const int v = rowbase + rg * 16 + li;
@@ -541,27 +658,40 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         q += __shfl_xor(q, 32);
         e += __shfl_xor(e, 16);
         e += __shfl_xor(e, 32);
+        q *= (double)kLn2f;  // the correction, from log2(e) units
+        if (v < Nv) e += F[v];
         const double is = (double)rinv[rg];
         const bool live = v < Nv && rinv[rg] != 0.0f;
         float *row = Gc + (size_t)v * K16;
-        // sum_j Q_ij d+_ij = u_i . G_i (Q_ij = 0 where d_ij <= 0), so
-        // q_i = sum_j Q_ij d+_ij / (1 + delta) = u_i . G_i - corr_i
-        double uk[KG][4], gt[KG][4], ug = 0.0;
+        double uk[KGA][4], gt[KGA][4], ug = 0.0, gsum = 0.0;
 #pragma unroll
-        for (int kg = 0; kg < KG; kg++)
+        for (int kg = 0; kg < KGA; kg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int k = 16 * kg + 4 * lq + r;
                 uk[kg][r] = (k < K) ? (double)UT[(size_t)k * Nv_pad + v] : 0.0;
-                gt[kg][r] = (k < K) ? (double)row[k] : 0.0;
-                ug += uk[kg][r] * gt[kg][r];
+                gt[kg][r] = (k < K && kg < KG) ? (double)row[k] : 0.0;
+                gsum += gt[kg][r];
             }
+        if (derive_last) {
+            gsum += __shfl_xor(gsum, 16);
+            gsum += __shfl_xor(gsum, 32);
+#pragma unroll
+            for (int kg = 0; kg < KGA; kg++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (16 * kg + 4 * lq + r == K - 1) gt[kg][r] = -gsum;
+        }
+#pragma unroll
+        for (int kg = 0; kg < KGA; kg++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) ug += uk[kg][r] * gt[kg][r];
         ug += __shfl_xor(ug, 16);
         ug += __shfl_xor(ug, 32);
         const double ds = -(ug - q) * is + (double)cE * e;
-        double gk[KG][4], part = 0.0;
+        double gk[KGA][4], part = 0.0;
 #pragma unroll
-        for (int kg = 0; kg < KG; kg++)
+        for (int kg = 0; kg < KGA; kg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int k = 16 * kg + 4 * lq + r;
@@ -572,7 +702,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         part += __shfl_xor(part, 32);
         const double mean = part / (double)K;
 #pragma unroll
-        for (int kg = 0; kg < KG; kg++)
+        for (int kg = 0; kg < KGA; kg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int k = 16 * kg + 4 * lq + r;
@@ -628,6 +758,41 @@ static int launch_adjoint(int M16, dim3 grid, size_t lds, hipStream_t st, const 
     return SRED_OK;
 }
 
+template <int KS, int KG>
+static void launch_adjoint_mfma_t(dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo, const uint4 *vhi,
+                                  const uint4 *vlo, const float4 *info2, const int *small, const double *F,
+                                  const float *UT, int K, int Nv, int Nv_pad, float eps, float cE, float isk, int K16,
+                                  float *Gc)
+{
+    hipLaunchKernelGGL((k_adjoint_mfma<KS, KG>), grid, dim3(256), 0, st, uhi, ulo, vhi, vlo, info2, small, F, UT, K,
+                       Nv, Nv_pad, eps, cE, isk, K16, Gc);
+}
+
+// k-groups KG = ceil((K - 1) / 16) of the second product: 1..2 (KS = 1), 2..4 (KS = 2), 4..6 (KS = 3)
+static int launch_adjoint_mfma(int KS, int KG, dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo,
+                               const uint4 *vhi, const uint4 *vlo, const float4 *info2, const int *small,
+                               const double *F, const float *UT, int K, int Nv, int Nv_pad, float eps, float cE,
+                               float isk, int K16, float *Gc)
+{
+#define SRED_ADJ_CASE(ks, kg)                                                                                        \
+    if (KS == ks && KG == kg) {                                                                                      \
+        launch_adjoint_mfma_t<ks, kg>(grid, st, uhi, ulo, vhi, vlo, info2, small, F, UT, K, Nv, Nv_pad, eps, cE, isk, \
+                                      K16, Gc);                                                                      \
+        return SRED_OK;                                                                                              \
+    }
+    SRED_ADJ_CASE(1, 1)
+    SRED_ADJ_CASE(1, 2)
+    SRED_ADJ_CASE(2, 2)
+    SRED_ADJ_CASE(2, 3)
+    SRED_ADJ_CASE(2, 4)
+    SRED_ADJ_CASE(3, 4)
+    SRED_ADJ_CASE(3, 5)
+    SRED_ADJ_CASE(3, 6)
+#undef SRED_ADJ_CASE
+    set_error("internal: no adjoint pair kernel for this block size");
+    return SRED_ERR_ARG;
+}
+
 int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_dmap,
                                float *d_grad)
 {
@@ -672,12 +837,18 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     const int *d_idx = ctx->valid_idx.as<int>();
 
     const double H = java_float_literal(nv) + java_float_literal(kEps);
-    const float tscale = (float)std::sqrt(1.4426950408889634 / H);
+    const double tscale_d = std::sqrt(1.4426950408889634 / H);
+    const float tscale = (float)tscale_d;
     const float inv_sqrt_km1 = (float)(1.0 / std::sqrt((double)(K - 1)));
     SRED_TRY(ctx->aux.ensure(2 * sizeof(float4) * (size_t)Nv_pad));
     float4 *d_info = ctx->aux.as<float4>(), *d_info2 = d_info + Nv_pad;
+    // F (FP64 per block) and the bits of max |b| behind it
+    SRED_TRY(ctx->adj_f.ensure(sizeof(double) * ((size_t)Nv_pad + 1)));
+    double *d_F = ctx->adj_f.as<double>();
+    unsigned int *d_bmax = reinterpret_cast<unsigned int *>(d_F + Nv_pad);
+    SRED_TRY(zero_dev(ctx, d_bmax, sizeof(double)));
     hipLaunchKernelGGL(k_adjoint_coef, dim3((Nv_pad + 255) / 256), dim3(256), 0, ctx->stream, d_idx, Nv, Nv_pad,
-                       d_stds, d_S, d_W, d_mask, d_dmap, d_mm, np, tscale, d_info, d_info2);
+                       d_stds, d_S, d_W, d_mask, d_dmap, d_mm, np, tscale, d_info, d_info2, d_bmax);
     // Pearson-scaled block vectors, k-major
     SRED_TRY(upload_offsets(ctx, g, g.w, &ctx->offs2, ctx->offs2_key));
     size_t nu = (size_t)Nv_pad * K4;
@@ -687,34 +858,33 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)Nv_pad * K16));
     float *d_Gc = ctx->partial.as<float>();
     if (mfma) {
-        const size_t u4 = (size_t)Nv_pad * KS * 4;  // uint4 per array (fragment-major, and transposed)
+        // second product over k < 16 KG (the last component follows from sum_k u_k = 0)
+        const int KG = std::max(1, (K - 1 + 15) / 16);
+        const size_t u4 = (size_t)Nv_pad * KS * 4;  // uint4 per fragment-major array
+        const size_t v4 = (size_t)Nv_pad * KG * 2;  // uint4 per transposed array
         SRED_TRY(ctx->uhi.ensure(sizeof(uint4) * u4));
         SRED_TRY(ctx->ulo.ensure(sizeof(uint4) * u4));
-        SRED_TRY(ctx->adj_v.ensure(2 * sizeof(uint4) * u4));
+        SRED_TRY(ctx->adj_v.ensure(2 * sizeof(uint4) * v4));
+        // the sigma-only terms of e_i (Gauss expansion, FP64)
+        SRED_TRY(adjoint_weight_terms_dev(ctx, ctx->wmom, ctx->wpart, d_stds, d_idx, d_info2, d_bmax, Nv, tscale_d,
+                                          ctx->last_sigma_max, d_F));
         const int ngroups = Nv_pad / 32;
         SRED_TRY(ctx->tflags.ensure(sizeof(int) * (size_t)ngroups));
         uint4 *d_uhi = ctx->uhi.as<uint4>(), *d_ulo = ctx->ulo.as<uint4>();
-        uint4 *d_vhi = ctx->adj_v.as<uint4>(), *d_vlo = d_vhi + u4;
+        uint4 *d_vhi = ctx->adj_v.as<uint4>(), *d_vlo = d_vhi + v4;
         int *d_small = ctx->tflags.as<int>();
         launch_gather_bf16(ctx, SRED_PEARSON, d_img, d_means, d_stds, d_idx, Nv, Nv_pad, K, KS,
                            ctx->offs2.as<int>(), inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), d_uhi, d_ulo);
-        hipLaunchKernelGGL(k_transpose_frag, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, d_uhi,
-                           d_ulo, Nv_pad, KS, d_vhi, d_vlo);
+        hipLaunchKernelGGL(k_transpose_frag, dim3((unsigned)((v4 + 255) / 256)), dim3(256), 0, ctx->stream, d_uhi,
+                           d_ulo, Nv_pad, KS, KG, d_vhi, d_vlo);
         hipLaunchKernelGGL(k_adjoint_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_info,
                            ngroups, d_small);
         SRED_TRY(time_begin(ctx));
         const float cE = (float)(-2.0 / H / (double)tscale);  // (s_i - s_j) = (t_i - t_j) / tscale
         const float *UTp = ctx->u32.as<float>();
         const dim3 grid(Nv_pad / kAdjRows);
-        if (KS == 1)
-            hipLaunchKernelGGL(k_adjoint_mfma<1>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
-                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
-        else if (KS == 2)
-            hipLaunchKernelGGL(k_adjoint_mfma<2>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
-                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
-        else
-            hipLaunchKernelGGL(k_adjoint_mfma<3>, grid, dim3(256), 0, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2,
-                               d_small, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc);
+        SRED_TRY(launch_adjoint_mfma(KS, KG, grid, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2, d_small, d_F, UTp,
+                                     K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc));
         SRED_HIP_CHECK(hipGetLastError());
         SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
     } else {

@@ -1336,6 +1336,7 @@ int global_prepare_back(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, c
     const int Nv = tot[0].x;
     float sigma_max;
     std::memcpy(&sigma_max, &tot[1].x, sizeof sigma_max);
+    ctx->last_sigma_max = sigma_max;
     r.Nv = Nv;
     // both-flat pairs give a flat row the flat-column count; rows the 3D kernels skip score 0
     r.flat_count = p.skip_flat_rows ? 0.0f : (float)std::min(tot[0].y, 16777216);

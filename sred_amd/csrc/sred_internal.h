@@ -133,6 +133,7 @@ struct sred_ctx {
     sred::DevBuf sortbuf;      // valid blocks sorted by sigma
     sred::DevBuf mmpart;       // per-workgroup min/max partials of a block map
     sred::DevBuf adj_v;        // transposed split-BF16 fragments of the adjoint
+    sred::DevBuf adj_f;        // the adjoint's sigma-only terms (FP64 per block) and max |b|
     sred::DevBuf pm_rows, pm_cols, pm_part, pm_spec;  // all-pairs vector maps (Hu, phase correlation)
     sred::DevBuf qt_res, qt_nodes, qt_leaves, qt_seg, qt_tmp;  // quadtree GAT estimate
     sred::DevBuf stack_work, seg_mm;                           // time-lapse stacks
@@ -161,6 +162,7 @@ struct sred_ctx {
     int last_launches = 0;
     int64_t last_pairs = 0;
     int last_nv = 0;  // valid blocks of the last global accumulation
+    float last_sigma_max = 0.0f;  // largest valid block sigma of the last global accumulation
     unsigned long long *last_skip = nullptr;  // device counter of pairs skipped by the last symmetric contraction
     double *h_fit = nullptr;  // pinned host doubles the GAT objective kernel writes its values to
     unsigned char *noalloc_scratch = nullptr;  // this device's copy of the no-allocation scratch array
@@ -271,6 +273,12 @@ int sort_valid_by_sigma(sred_ctx *ctx, DevBuf &buf, const float *d_stds, const i
 // Hermite expansion in FP64; sigma_max bounds the valid blocks' sigmas.
 int weight_sums_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_stds, const int *d_idx, int Nv,
                     double tscale, float sigma_max, float *d_W);
+// The sigma-only term of the global Pearson adjoint, F_v = sum_j (b_v + b_j)
+// w_vj (t_v - t_j) over the Nv valid blocks in adjoint order (info2[v].z = b_v,
+// *d_bmax = bits of max |b|), by the same expansion; FP64 results.
+int adjoint_weight_terms_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const float *d_stds, const int *d_idx,
+                             const float4 *d_info2, const unsigned int *d_bmax, int Nv, double tscale,
+                             float sigma_max, double *d_F);
 // Persistent symmetric split-BF16 contraction over nitems panel pairs of the
 // nprob problems in d_probs (device table), all with K_pad = 32*KS and panel
 // size P (metric Pearson, SSIM or NRMSE).
