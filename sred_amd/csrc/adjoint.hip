@@ -304,9 +304,11 @@ __global__ __launch_bounds__(256) void k_transpose_frag(const uint4 *__restrict_
 constexpr int kAdjRG = 2;
 constexpr int kAdjRows = 4 * 16 * kAdjRG;  // rows per workgroup
 // Fast tiles: every sigma of the tile's rows and columns is 0 or at least
-// kAdjFastSigma, so delta = eps/(s_i s_j) <= 3.1e-4 and 1/(1 + delta) is
-// taken as 1 - delta (relative error delta^2 <= 1e-7); other tiles divide.
-constexpr float kAdjFastSigma = 0.018f;
+// kAdjFastSigma, so delta = eps/(s_i s_j) <= 1e-3; there the weight w/(1 + delta)
+// is taken as w e^-delta (relative error delta^2/2 <= 5e-7) and the correction
+// factor delta/(1 + delta) as delta (error delta^2 <= 1e-6 relative to q_i);
+// other tiles divide.
+constexpr float kAdjFastSigma = 0.01f;
 constexpr float kLn2f = 0.693147180559945309f;
 constexpr float kLog2ef = 1.442695040888963407f;
 
@@ -457,7 +459,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
     // pair terms of column group cg (half h of its step) for row group rg:
     //   Q into pv[h*4 + r]; corr += Q d+ (1 - 1/(1 + delta)) log2(e); se += Q d+ (t_i - t_j)
     // Fast tiles take w~ = exp2(-(t_i - t_j)^2 - delta log2(e)) = w e^-delta and
-    // 1 - 1/(1 + delta) = delta (relative errors delta^2/2 and delta, delta <= 3.1e-4);
+    // 1 - 1/(1 + delta) = delta (relative errors delta^2/2 and delta, delta <= 1e-3);
     // exact tiles divide.  corr is kept in log2(e) units (scaled by ln 2 per tile).
     auto pairs = [&](auto exact_tag, const float4 *ci, int cg, int h, int rg, const floatx4 &d, float (&pv)[8],
                      float &corr, float &se) {
@@ -834,7 +836,15 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     const int KS = (K + 31) / 32;
     const bool mfma = KS <= 3;  // split-BF16 MFMA pass (its accumulators fit the registers up to K = 96)
     const int Nv_pad = mfma ? (Nv + kAdjRows - 1) / kAdjRows * kAdjRows : (Nv + 63) / 64 * 64;
+    // blocks in ascending sigma: the few tiles that need the exact small-sigma
+    // form then hold the small-sigma blocks only (any order gives the same
+    // gradient up to rounding; pix2v below follows it)
     const int *d_idx = ctx->valid_idx.as<int>();
+    if (Nv > 1) {
+        int *sorted = nullptr;
+        SRED_TRY(sort_valid_by_sigma(ctx, ctx->sortbuf, d_stds, d_idx, Nv, &sorted));
+        d_idx = sorted;
+    }
 
     const double H = java_float_literal(nv) + java_float_literal(kEps);
     const double tscale_d = std::sqrt(1.4426950408889634 / H);
