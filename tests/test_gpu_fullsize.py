@@ -165,3 +165,26 @@ def test_ssim_skip_with_means_near_one_tenth(ctx):
     assert np.median(inner) == pytest.approx(0.1, abs=0.03)
     ref = O.global_map(O.SSIM, inp.image_array, b, b, 0.0, True, fp64=True)
     assert np.abs(out - ref).max() <= MAP_TOL
+
+
+# ---------------------------------------------------------------------------
+# weight sums with more than 2^21 blocks in one sigma bin
+# ---------------------------------------------------------------------------
+def test_weight_sums_many_blocks_in_one_bin(ctx):
+    """1600 x 1600 pure noise with 11 x 11 blocks: 2.53e6 valid blocks whose
+    sigmas (t = sigma sqrt(log2 e / h) close to 1.2) fall in one or two bins
+    of the weight-sum expansion.  The bin's count moment then exceeds 2^21
+    blocks, where 2^-42 fixed point would pass 2^63; the scale steps up to
+    2^-40 and the Pearson map still matches FP64 on sampled rows."""
+    rng = np.random.default_rng(31)
+    img = O.normalize_input((0.5 + 0.05 * rng.standard_normal((1600, 1600))).astype(np.float32))
+    out = sred_amd.global_repetition_2d(img, 11, 11, 0.0, "Pearson's R", True, "bf16x3", resources=ctx)
+    m32, s32 = O.local_stats_f32(img, 11, 11)
+    t = s32[5:-5, 5:-5].astype(np.float64) * np.sqrt(np.log2(np.e) / (O.noise_var(img) + 1e-7))
+    counts = np.bincount(np.floor(t.ravel() / 0.5 + 0.5).astype(np.int64))
+    assert counts.max() > 2 ** 21
+    mask, _, _ = O.relevance_mask(s32, 5, 5, O.noise_var(img), 0.0)
+    px = sample_pixels(out, mask, 24, seed=131)
+    ref, _ = fp64_map_at(O, 0, img, 11, 11, 0.0, px)
+    err = np.abs(out.ravel()[px] - ref)
+    assert err.max() <= MAP_TOL, (err.max(), px[np.argmax(err)])
