@@ -949,6 +949,25 @@ def adjoint_run(args):
     kgrad = (Kb - 1 + 15) // 16 * 16
     issued = float(pairs_adj) * 2 * 3 * (kpad + kgrad) / (k * 1e-3) / 1e12
     g = d_grad.cpu().numpy()
+    cpu = None
+    if not args.no_cpu:
+        # CPU baseline: the FP64 torch autograd restatement of the map and its
+        # gradient (the reference has no adjoint), on a 120x120 crop of the
+        # same image (N = 12,544 blocks: its N x N pair matrices take ~1.3 GB
+        # each), forward + backward per call
+        from oracle import torch_ref as T
+        crop, uc = img[:120, :120], u[:120, :120]
+        Nc = (120 - 2 * (BB // 2)) ** 2
+        T.pearson_map_grad(crop, BB, uc.astype(np.float64))
+        reps, t0 = 2, time.perf_counter()
+        for _ in range(reps):
+            T.pearson_map_grad(crop, BB, uc.astype(np.float64))
+        tc = (time.perf_counter() - t0) / reps
+        cpu = {"value": float(Nc) ** 2 / tc / 1e9, "unit": "Gblock-pairs/s", "cores": torch.get_num_threads(),
+               "kind": "port", "sample": f"FP64 torch autograd restatement of the map and its gradient "
+               f"(oracle/torch_ref.py) on a 120x120 crop of the same image and weights, 9x9 blocks, "
+               f"{tc:.2f} s per forward + backward"}
+        print(f"# cpu: {cpu['value']:.4f} Gpairs/s ({tc:.2f} s per call)", file=sys.stderr)
     print(json.dumps({
         "metric": "Gblock-pairs/s, global Pearson map + adjoint 512² b9", "value": float(N) ** 2 / t / 1e9,
         "unit": "Gblock-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
@@ -959,7 +978,7 @@ def adjoint_run(args):
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / PEAK_BF16_TFLOPS, "traffic": None, "issued_tflops": issued,
                      "kernel": f"k_adjoint_mfma<KS={kpad // 32}, KG={kgrad // 16}>"},
-        "cpu_baseline": None}))
+        "cpu_baseline": cpu}))
 
 
 def profile_run(args):

@@ -168,7 +168,7 @@ constexpr int kAdjChunkBins = 128;        // bins per pass (LDS table 2 x 17 x 1
 __device__ __forceinline__ double adj_b_scale(unsigned int bmax_bits)
 {
     const float bmax = __uint_as_float(bmax_bits);
-    if (!(bmax <= 3.4e38f)) return __longlong_as_double(0x7ff8000000000000LL);
+    if (bmax_bits >= 0x7f800000u) return __longlong_as_double(0x7ff8000000000000LL);  // inf or NaN
     return bmax > 0.0f ? exp2(-ceil(log2((double)bmax))) : 0.0;
 }
 
