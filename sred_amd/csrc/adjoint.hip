@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float *__restrict__ U
 // u_{j,k}, k = 16kg + l % 16, j = 32jg + jmap(l / 16, e), jmap(q, e) = 4q + e
 // for e < 4 and 16 + 4q + e - 4 beyond.  P is split into hi + lo BF16 and the
 // second product uses three MFMAs as well; its FP32 accumulators are flushed
-// into FP32 totals every 32 tiles, dL/ds_i into FP64 every tile.
+// into FP32 totals every 128 tiles, dL/ds_i into FP64 every tile.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -398,7 +398,9 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
         for (int rg = 0; rg < RG; rg++) acc[kg][rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     // FP32 totals of the second product, kept in this wave's rows of Gc
     // (K16 floats per row; only this wave touches them): acc is added every
-    // 32 tiles.  Lane l holds row i = l % 16 of each group, k = 16kg + 4(l/16) + r.
+    // 128 tiles (8192 columns; the row's load waits for the tile copy in
+    // flight, so the adds are kept rare).  Lane l holds row i = l % 16 of
+    // each group, k = 16kg + 4(l/16) + r.
     auto flush = [&](bool first) {
 #pragma unroll
         for (int rg = 0; rg < RG; rg++) {
@@ -629,7 +631,7 @@ __global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict
             gq[rg] += (double)corr[rg];
             ge[rg] += (double)se[rg];
         }
-        if ((T & 31) == 31 || T == ntiles - 1) flush(T < 32);
+        if ((T & 127) == 127 || T == ntiles - 1) flush(T < 128);
     };
 
     // step T: tile T has landed (own copies waited, then the barrier), every
