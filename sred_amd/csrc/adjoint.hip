@@ -352,8 +352,10 @@ __device__ __forceinline__ void adj_wait_all() { __builtin_amdgcn_s_waitcnt(0); 
 //
 // info2 = (t, a, b, 1/s) per block (1/s = 0 for flat blocks and padding);
 // F[v] for v < Nv.
+// Up to K = 64 two workgroups share a CU; beyond, the row fragments and
+// accumulators of three k-steps need more registers than half a SIMD's file.
 template <int KS, int KG>
-__global__ __launch_bounds__(256, 2) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
+__global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                          const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
                                                          const float4 *__restrict__ info2, const int *__restrict__ small,
                                                          const double *__restrict__ F, const float *__restrict__ UT,
