@@ -45,6 +45,34 @@ def test_adjoint_matches_fp64_autograd(ctx, shape, b, seed):
     assert np.median(np.abs(g - g64)) / scale < 2e-5
 
 
+@pytest.mark.parametrize("kind,b", [("lattice", 9), ("noise", 7), ("lattice", 11)])
+def test_adjoint_matches_fp64_autograd_other_textures(ctx, kind, b):
+    """Textures other than spots: a crop of the ring lattice and blobs image
+    (large block sigmas, many near-equal pairs) and pure Gaussian noise (every
+    block sigma close to the others, so every tile takes the fast weight form).
+
+    On pure noise the correlations cluster around 0, where max(0, r) has its
+    kink: the ~1e-6 error of the split-BF16 dot products flips [r > 0] for a
+    few pairs, and each flip moves a pixel's gradient by ~1e-3 of the largest
+    entry.  The FP64 reference itself moves by 2.0-2.8e-3 of max|grad| (median
+    1e-8) when its r is perturbed by +-1e-6, so there the largest deviation is
+    held to 1e-2 and the median to the usual 2e-5."""
+    if kind == "lattice":
+        img = O.normalize_input(synthetic.lattice_and_blobs(1024, seed=2)[:48, :52])
+    else:
+        rng = np.random.default_rng(44)
+        img = O.normalize_input((0.5 + 0.05 * rng.standard_normal((44, 40))).astype(np.float32))
+    u = np.random.default_rng(45).random(img.shape).astype(np.float32)
+    _, g64, _ = T.pearson_map_grad(img, b, u.astype(np.float64))
+    g = gpu_grad(ctx, img, b, u)
+    scale = np.abs(g64).max()
+    err = np.abs(g - g64)
+    print(f"adjoint vs FP64 autograd, {kind} {img.shape} b{b}: max {err.max() / scale:.2e}, "
+          f"median {np.median(err) / scale:.2e}")
+    assert err.max() / scale < (1e-2 if kind == "noise" else 2e-3), err.max() / scale
+    assert np.median(err) / scale < 2e-5
+
+
 def test_adjoint_linear_in_weights_full_size(ctx):
     img = O.normalize_input(synthetic.spots(512, 512, 800, seed=4))
     rng = np.random.default_rng(40)
