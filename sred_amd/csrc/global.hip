@@ -647,8 +647,12 @@ template <int M> constexpr float skip_gap() { return M == SRED_SSIM ? 6.8556546f
 template <int M> constexpr bool skips() { return M == SRED_PEARSON || M == SRED_SSIM; }
 
 // two 64-column tiles in flight (copy of the next overlaps the current one)
-// while their LDS fits twice beside the column accumulator at 2 workgroups/CU
-constexpr bool sym_double_buffer(int ks) { return ks <= 3; }
+// while their LDS fits twice beside the column accumulator at 2 workgroups/CU:
+// up to K_pad = 128, where the tiles take 64 KiB and the panel is held to
+// 1024 columns (4 KiB accumulator; measured 3% faster than one buffer at
+// P = 2048, while two buffers at P = 2048 leave one workgroup per CU and
+// run 35% slower)
+constexpr bool sym_double_buffer(int ks) { return ks <= 4; }
 
 // ---------------------------------------------------------------------------
 // 3a'. symmetric split-BF16 MFMA contraction.  Every per-pair term is
@@ -1232,9 +1236,10 @@ int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, 
     // any K through LDS in chunks
     q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && q.KS <= kMaxKS;
     q.sym = q.mfma && symmetric;
-    // symmetric panels: P rows = P columns, a multiple of 256, up to 2048,
-    // small enough for at least ~2000 panel pairs
-    q.P = q.sym ? std::min(2048, std::max(256, ((Nv + 63) / 64 + 255) / 256 * 256)) : 0;
+    // symmetric panels: P rows = P columns, a multiple of 256, up to 2048
+    // (1024 at K_pad = 128, whose two tile buffers leave room for a 4 KiB
+    // column accumulator only), small enough for at least ~2000 panel pairs
+    q.P = q.sym ? std::min(q.KS == 4 ? 1024 : 2048, std::max(256, ((Nv + 63) / 64 + 255) / 256 * 256)) : 0;
     if (q.sym && panel > 0) {
         if (panel % 256 != 0 || panel > 2048) {
             set_error("panel size must be a multiple of 256, at most 2048");
