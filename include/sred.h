@@ -43,8 +43,9 @@ extern "C" {
 /* ---- contraction precision of the global maps ---- */
 #define SRED_FP32 0   /* plain FP32 vector-ALU contraction */
 #define SRED_BF16X3 1 /* split-BF16 (hi*hi + hi*lo + lo*hi) on MFMA, FP32 accumulate; blocks of more than
-                         256 pixels (21x21 and larger) take the FP32 contraction, so every odd block size
-                         the reference accepts works in either precision */
+                         256 pixels (21x21 and larger; 257 is reflected into 256 dimensions) take the FP32
+                         contraction, so every odd block size the reference accepts works in either
+                         precision */
 
 /* Largest supported block side (pixels); K <= 3200. */
 #define SRED_MAX_BLOCK_SIDE 63
@@ -204,7 +205,7 @@ typedef struct sred_global_problem {
  * every image's reduction, finalize and normalisation.  Panel sizes come from
  * an autotuner (timed on the first batch holding a (metric, K_pad) class and
  * cached in the context) unless SRED_OPT_BATCH_PANEL forces one.  Metrics:
- * Pearson, SSIM, NRMSE (K <= 256).  Synchronises. */
+ * Pearson, SSIM, NRMSE (K <= 257).  Synchronises. */
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
                          float relevance_constant, int normalize);
 /* Panel size the autotuner chose on this context for (metric, k_steps =
@@ -496,7 +497,8 @@ int sred_global_pearson_adjoint_2d_dev(sred_ctx *ctx, const float *d_img, int w,
  * block_pixels (K) pixels (no counterpart in the reference; host only).
  * out[8]: padded block count, rows per panel (symmetric) or per workgroup,
  * columns per chunk, chunks, workgroups, partial-sum slots, block pairs
- * evaluated, flags (1 = MFMA, 2 = symmetric). */
+ * evaluated, flags (1 = MFMA, 2 = symmetric, 4 = blocks of K = 32m + 1
+ * reflected into K - 1 dimensions). */
 int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision, int symmetric, int chunk_cols,
                         int64_t *out);
 /* Duration (ms, HIP events on the context's stream) and launch count of the

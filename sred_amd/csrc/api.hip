@@ -1560,7 +1560,7 @@ int sred_global_plan_2d(int n_valid, int block_pixels, int metric, int precision
     out[4] = pl.grid;
     out[5] = pl.partial_elems;
     out[6] = pl.pairs_evaluated;
-    out[7] = (pl.mfma ? 1 : 0) | (pl.sym ? 2 : 0);
+    out[7] = (pl.mfma ? 1 : 0) | (pl.sym ? 2 : 0) | (pl.reflect ? 4 : 0);
     return SRED_OK;
 }
 

@@ -219,11 +219,30 @@ __device__ __forceinline__ float row_scale(int metric, float sigma, float inv_sq
 // columns (B) alike, so every fragment load -- global or LDS -- is 64
 // consecutive uint4 per wave, and a 64-column tile is one contiguous 4*KS KiB
 // run that can be copied straight into LDS.
+//
+// Reflection (reflect != 0, blocks of K = 32m + 1 pixels): the block vectors
+// u are centred, so they are orthogonal to 1/sqrt(K) (1, ..., 1).  The
+// Householder reflection H that swaps that unit vector with e_{K-1} keeps
+// every dot product and sends u to H u, whose last component is
+// s / sqrt(K) with s = sum_k u_k (zero but for the rounding of the centring,
+// so dropping it changes a dot product by s_i s_j / K, second order in that
+// rounding).  The other components are u_k + c,
+//   c = (u_{K-1} - s / sqrt(K)) / (sqrt(K) - 1),
+// one shift for the whole block; the K - 1 of them fit m k-steps of 32.
+__device__ __forceinline__ float reflect_shift(const float *img, int p, const int *offs, int K, float mu, float sc)
+{
+    float s = 0.0f;
+    for (int k = 0; k < K; k++) s += (img[p + offs[k]] - mu) * sc;
+    const float ulast = (img[p + offs[K - 1]] - mu) * sc;
+    const float rk = sqrtf((float)K);
+    return (ulast - s / rk) / (rk - 1.0f);
+}
+
 __global__ __launch_bounds__(256) void k_gather_bf16(int metric, const float *__restrict__ img,
                                                      const float *__restrict__ means, const float *__restrict__ stds,
                                                      const int *__restrict__ idx, int Nv, int Nv_pad, int K, int KS,
                                                      const int *__restrict__ offs, float inv_sqrt_km1,
-                                                     float inv_sqrt_k, uint4 *__restrict__ Uhi,
+                                                     float inv_sqrt_k, int reflect, uint4 *__restrict__ Uhi,
                                                      uint4 *__restrict__ Ulo)
 {
     const int per_group = KS * 64;  // uint4 per 16 blocks
@@ -237,10 +256,12 @@ __global__ __launch_bounds__(256) void k_gather_bf16(int metric, const float *__
         int p = idx[v];
         float mu = means[p];
         float sc = row_scale(metric, stds[p], inv_sqrt_km1, inv_sqrt_k);
+        const int Kd = reflect ? K - 1 : K;  // dimensions stored
+        const float c = reflect ? reflect_shift(img, p, offs, K, mu, sc) : 0.0f;
         for (int e = 0; e < 8; e++) {
             int k = q * 8 + e;
             float u = 0.0f;
-            if (k < K) u = (img[p + offs[k]] - mu) * sc;  // test_block[i] - test_mean (kernel line 84-87)
+            if (k < Kd) u = (img[p + offs[k]] - mu) * sc + c;  // test_block[i] - test_mean (kernel line 84-87)
             hi[e] = f2bf(u);
             lo[e] = f2bf(u - bf2f(hi[e]));
         }
@@ -278,7 +299,7 @@ __global__ __launch_bounds__(256) void k_gather_f32(int metric, const float *__r
 __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict__ img, const float *__restrict__ means,
                                              const float *__restrict__ stds, const int *__restrict__ idx, int Nv,
                                              int Nv_pad, int K, const int *__restrict__ offs, float tscale,
-                                             float inv_sqrt_km1, float inv_sqrt_k, int split_bf16,
+                                             float inv_sqrt_km1, float inv_sqrt_k, int split_bf16, int reflect,
                                              float4 *__restrict__ aux)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -305,8 +326,10 @@ __global__ __launch_bounds__(256) void k_aux(int metric, const float *__restrict
         // hi*hi + hi*lo + lo*hi, so the norm omits lo*lo too and identical
         // blocks give mse = 0 up to FP32 rounding
         float sc = row_scale(metric, s, inv_sqrt_km1, inv_sqrt_k), n = 0.0f;
-        for (int k = 0; k < K; k++) {
-            float u = (img[p + offs[k]] - mu) * sc;
+        const int Kd = reflect ? K - 1 : K;  // the reflected blocks' components (k_gather_bf16)
+        const float c = reflect ? reflect_shift(img, p, offs, K, mu, sc) : 0.0f;
+        for (int k = 0; k < Kd; k++) {
+            float u = (img[p + offs[k]] - mu) * sc + c;
             if (split_bf16) {
                 float hi = bf2f(f2bf(u)), lo = bf2f(f2bf(u - hi));
                 n += hi * hi + 2.0f * hi * lo;
@@ -1193,7 +1216,7 @@ void launch_gather_bf16(sred_ctx *ctx, int metric, const float *d_img, const flo
 {
     size_t u4 = (size_t)Nv_pad * KS * 4;
     hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
-                       d_means, d_stds, d_idx, Nv, Nv_pad, K, KS, d_offs, inv_sqrt_km1, inv_sqrt_k, Uhi, Ulo);
+                       d_means, d_stds, d_idx, Nv, Nv_pad, K, KS, d_offs, inv_sqrt_km1, inv_sqrt_k, 0, Uhi, Ulo);
 }
 
 void launch_gather_f32(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
@@ -1229,12 +1252,16 @@ int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, 
         set_error("invalid contraction plan arguments");
         return SRED_ERR_ARG;
     }
-    q.KS = (K + 31) / 32;
+    // split BF16 stores K - 1 reflected components when that saves a k-step
+    // (K = 32m + 1; k_gather_bf16)
+    q.reflect = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && K > 1 && (K - 1) % 32 == 0;
+    q.KS = q.reflect ? (K - 1) / 32 : (K + 31) / 32;
     q.K4 = (K + 3) / 4 * 4;
     // the split-BF16 kernels hold a block's fragments in registers up to
-    // K = 32 * kMaxKS; larger blocks take the FP32 contraction, which stages
-    // any K through LDS in chunks
+    // K_pad = 32 * kMaxKS; larger blocks take the FP32 contraction, which
+    // stages any K through LDS in chunks
     q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && q.KS <= kMaxKS;
+    q.reflect = q.reflect && q.mfma;
     q.sym = q.mfma && symmetric;
     // symmetric panels: P rows = P columns, a multiple of 256, up to 2048
     // (1024 at K_pad = 128, whose two tile buffers leave room for a 4 KiB
@@ -1382,7 +1409,8 @@ int global_prepare_back(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, c
     SRED_TRY(b.aux->ensure(sizeof(float4) * (size_t)Nv_rows));
     float4 *d_aux = b.aux->as<float4>();
     hipLaunchKernelGGL(k_aux, dim3((Nv_rows + 255) / 256), dim3(256), 0, ctx->stream, metric, d_img, d_means, d_stds,
-                       d_idx, Nv, Nv_rows, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, mfma ? 1 : 0, d_aux);
+                       d_idx, Nv, Nv_rows, g.K, d_offs, tscale, inv_sqrt_km1, inv_sqrt_k, mfma ? 1 : 0,
+                       pl.reflect ? 1 : 0, d_aux);
     const int ngroups = Nv_rows / 32;
     SRED_TRY(b.tflags->ensure(sizeof(int) * (size_t)std::max(ngroups, 1)));
     hipLaunchKernelGGL(k_tile_small, dim3((ngroups * 32 + 255) / 256), dim3(256), 0, ctx->stream, d_aux, ngroups,
@@ -1395,7 +1423,7 @@ int global_prepare_back(sred_ctx *ctx, const GlobalBufs &b, const Geometry &g, c
         SRED_TRY(b.ulo->ensure(sizeof(uint4) * u4));
         hipLaunchKernelGGL(k_gather_bf16, dim3((unsigned)((u4 + 255) / 256)), dim3(256), 0, ctx->stream, metric, d_img,
                            d_means, d_stds, d_idx, Nv, Nv_rows, g.K, pl.KS, d_offs, inv_sqrt_km1, inv_sqrt_k,
-                           b.uhi->as<uint4>(), b.ulo->as<uint4>());
+                           pl.reflect ? 1 : 0, b.uhi->as<uint4>(), b.ulo->as<uint4>());
         r.uhi = b.uhi->as<uint4>();
         r.ulo = b.ulo->as<uint4>();
     } else if (metric != SRED_ABSDIFF) {

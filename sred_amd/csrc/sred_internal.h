@@ -195,6 +195,10 @@ int relevance_mask_dev(sred_ctx *ctx, const Geometry &g, const float *d_stds, fl
 
 struct ContractionPlan {
     bool mfma, sym;
+    // split-BF16 blocks of K = 32m + 1 pixels are reflected into K - 1
+    // dimensions (the centred vectors are orthogonal to the all-ones vector),
+    // so they take m k-steps instead of m + 1
+    bool reflect;
     int KS, K4, P, Nv_pad, bm, nrowtiles, cols_per_chunk, nchunks;
     int64_t grid, partial_elems, pairs_evaluated;
 };

@@ -130,6 +130,22 @@ def test_global_non_square_blocks(ctx, metric, precision, bw, bh):
     assert np.abs(out - ref32).max() <= MAP_TOL
 
 
+@pytest.mark.parametrize("metric", [0, 1, 2])
+@pytest.mark.parametrize("bw,bh", [(3, 31), (15, 7), (9, 17)])
+def test_global_reflected_blocks(ctx, metric, bw, bh):
+    """Blocks of K = 32m + 1 pixels (3x31: 33, 15x7: 65, 9x17: 97), which
+    the split-BF16 gather reflects into K - 1 dimensions (one k-step fewer):
+    the maps match FP64 and the FP32 contraction, which keeps all K."""
+    K = sred_amd.block_size_2d(bw, bh)
+    assert (K - 1) % 32 == 0 and L.global_plan(10000, K, metric, L.BF16X3)["flags"] & 4
+    img = O.normalize_input(synthetic.spots(90, 84, 30, seed=bw * 100 + bh, sigma=3.0))
+    out = sred_amd.global_repetition_2d(img, bw, bh, 0.0, GM[metric], True, "bf16x3", resources=ctx)
+    f32 = sred_amd.global_repetition_2d(img, bw, bh, 0.0, GM[metric], True, "fp32", resources=ctx)
+    ref64 = O.global_map(metric, img, bw, bh, 0.0, True, fp64=True)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+    assert np.abs(out - f32).max() <= MAP_TOL
+
+
 @pytest.mark.parametrize("metric", [0, 1, 2, 3])
 @pytest.mark.parametrize("b", [21, 31])
 def test_block_maps_large_templates_bitexact(ctx, metric, b):

@@ -51,7 +51,7 @@ def test_global_pearson3d(ctx, precision, b, c):
 
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 @pytest.mark.parametrize("metric", ["SSIM", "NRMSE", "Abs. diff. of StdDevs"])
-@pytest.mark.parametrize("b,c", [((3, 3, 3), 0.0), ((5, 5, 3), 0.8)])
+@pytest.mark.parametrize("b,c", [((3, 3, 3), 0.0), ((5, 5, 3), 0.8), ((5, 5, 5), 0.0)])
 def test_global_metrics3d(ctx, precision, metric, b, c):
     vol = _blobs(16, 26, 30, 5)
     vol[4:12, 6:16, 8:20] = 0.5  # flat region: flat rows, flat columns
@@ -98,3 +98,19 @@ def test_errors(ctx):
                                                    0.0, 1, L.fptr(np.zeros_like(vol))))
     assert e.value.code == L.SRED_ERR_UNKNOWN_METRIC
     assert L.load().sred_block_size_3d(5, 5, 5) == 33
+
+
+@pytest.mark.parametrize("metric", ["Pearson's R", "SSIM", "NRMSE"])
+def test_global3d_257_pixel_blocks(ctx, metric):
+    """9x9x9 blocks hold 257 pixels: reflected into 256 dimensions they stay
+    on the split-BF16 contraction; the map matches FP64."""
+    assert O.footprint3d(9, 9, 9)[0] == 257
+    vol = _blobs(18, 22, 20, 7)
+    got = sred_amd.global_repetition_3d(vol, 9, 9, 9, 0.0, metric, True, "bf16x3", resources=ctx)
+    m = {"Pearson's R": O.PEARSON, "SSIM": O.SSIM, "NRMSE": O.NRMSE}[metric]
+    inp = O.normalize_input(vol)
+    r64 = (O.global_pearson3d(inp, 9, 9, 9, fp64=True) if m == O.PEARSON
+           else O.global_repetition3d(inp, m, 9, 9, 9, fp64=True))
+    e64 = np.abs(got - r64.out).max()
+    print(f"3d {metric} 9x9x9 bf16x3: max|err| vs FP64 {e64:.2e}", flush=True)
+    assert e64 <= 1e-4
