@@ -43,9 +43,10 @@ extern "C" {
 /* ---- contraction precision of the global maps ---- */
 #define SRED_FP32 0   /* plain FP32 vector-ALU contraction */
 #define SRED_BF16X3 1 /* split-BF16 (hi*hi + hi*lo + lo*hi) on MFMA, FP32 accumulate; blocks of more than
-                         256 pixels (21x21 and larger; 257 is reflected into 256 dimensions) take the FP32
+                         384 pixels (25x25 and larger; 385 is reflected into 384 dimensions) take the FP32
                          contraction, so every odd block size the reference accepts works in either
-                         precision */
+                         precision (the non-symmetric comparison contraction, SRED_OPT_SYMMETRIC = 0,
+                         stops at 256 pixels) */
 
 /* Largest supported block side (pixels); K <= 3200. */
 #define SRED_MAX_BLOCK_SIDE 63
@@ -205,7 +206,7 @@ typedef struct sred_global_problem {
  * every image's reduction, finalize and normalisation.  Panel sizes come from
  * an autotuner (timed on the first batch holding a (metric, K_pad) class and
  * cached in the context) unless SRED_OPT_BATCH_PANEL forces one.  Metrics:
- * Pearson, SSIM, NRMSE (K <= 257).  Synchronises. */
+ * Pearson, SSIM, NRMSE (K <= 385).  Synchronises. */
 int sred_global_batch_2d(sred_ctx *ctx, int metric, int n, const sred_global_problem *problems,
                          float relevance_constant, int normalize);
 /* Panel size the autotuner chose on this context for (metric, k_steps =

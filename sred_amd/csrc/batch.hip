@@ -200,8 +200,8 @@ int global_batch_dev(sred_ctx *ctx, int metric, int n, const sred_global_problem
         }
         // K = 32m + 1 is stored reflected in K - 1 dimensions (plan_contraction_p)
         const int kd = (x.g.K > 1 && (x.g.K - 1) % 32 == 0) ? x.g.K - 1 : x.g.K;
-        if ((kd + 31) / 32 > kMaxKS) {
-            set_error("the batched split-BF16 contraction supports block sizes up to K = 257 pixels");
+        if ((kd + 31) / 32 > kMaxKSSym) {
+            set_error("the batched split-BF16 contraction supports block sizes up to K = 385 pixels");
             return SRED_ERR_ARG;
         }
         int tbw, tbh;

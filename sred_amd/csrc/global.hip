@@ -431,12 +431,13 @@ constexpr int kBN = 64;
 // (SSIM carries four row constants per row, so it keeps 4 row groups only up to K_pad = 96).
 constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3 : 4) ? 4 : 2; }
 // the symmetric kernel also carries column sums and a second accumulator set
-// (software pipelining): 2 row groups at every K_pad of the split-BF16 range.
-// Beyond K_pad = 160 the registers then spill a little, but every B fragment
-// read from LDS feeds twice the MFMAs; measured on a 768^2 lattice image,
-// against one row group: 15x17 blocks (K_pad 192) Pearson 147 -> 113 ms,
-// 17x17 (224) 166 -> 134 ms, 19x19 (256) 319 -> 305 ms; SSIM and NRMSE gain
-// 15-30%.
+// (software pipelining): 2 row groups up to K_pad = 256.  Beyond K_pad = 160
+// the registers then spill a little, but every B fragment read from LDS feeds
+// twice the MFMAs; measured on a 768^2 lattice image, against one row group:
+// 15x17 blocks (K_pad 192) Pearson 147 -> 113 ms, 17x17 (224) 166 -> 134 ms,
+// 19x19 (256) 319 -> 305 ms; SSIM and NRMSE gain 15-30%.  K_pad 288-384
+// (21x21, 23x23 blocks) keeps one row group: their tiles leave one workgroup
+// per CU.
 constexpr int rg_sym(int metric, int ks) { return ks <= 8 ? 2 : 1; }
 
 template <int M, int KS>
@@ -709,7 +710,7 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 4; }
 // publishes tile T and retires every read of the buffer the next copy reuses.
 // ---------------------------------------------------------------------------
 template <int M, int KS>
-__global__ __launch_bounds__(256, 2) void k_contract_mfma_sym(const SymProblem *__restrict__ probs, int nprob,
+__global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(const SymProblem *__restrict__ probs, int nprob,
                                                               long long nitems, unsigned long long *__restrict__ next,
                                                               int P, EpiConst ec)
 {
@@ -1145,8 +1146,12 @@ static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, cons
     case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
     case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
     case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 9: return launch_sym_ks<M, 9>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 10: return launch_sym_ks<M, 10>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 11: return launch_sym_ks<M, 11>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 12: return launch_sym_ks<M, 12>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
     default:
-        set_error("split-BF16 contraction supports block sizes up to K = 256 pixels");
+        set_error("the symmetric split-BF16 contraction supports block sizes up to K = 384 pixels");
         return SRED_ERR_ARG;
     }
 }
@@ -1258,9 +1263,10 @@ int plan_contraction_p(int Nv, int K, int metric, int precision, int symmetric, 
     q.KS = q.reflect ? (K - 1) / 32 : (K + 31) / 32;
     q.K4 = (K + 3) / 4 * 4;
     // the split-BF16 kernels hold a block's fragments in registers up to
-    // K_pad = 32 * kMaxKS; larger blocks take the FP32 contraction, which
+    // K_pad = 32 * kMaxKS (the symmetric one, with one row group beyond 256,
+    // up to 32 * kMaxKSSym); larger blocks take the FP32 contraction, which
     // stages any K through LDS in chunks
-    q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && q.KS <= kMaxKS;
+    q.mfma = (precision == SRED_BF16X3) && metric != SRED_ABSDIFF && q.KS <= (symmetric ? kMaxKSSym : kMaxKS);
     q.reflect = q.reflect && q.mfma;
     q.sym = q.mfma && symmetric;
     // symmetric panels: P rows = P columns, a multiple of 256, up to 2048

@@ -54,6 +54,7 @@ void set_error(const std::string &msg);
 constexpr float kEps = 1e-7f;           // Utils.EPSILON, Utils.java:20
 constexpr int kMaxK = 3200;             // footprint size cap (63x63 ellipse has 3109 px)
 constexpr int kMaxKS = 8;               // split-BF16 kernels: K <= 32 * kMaxKS (larger blocks use FP32)
+constexpr int kMaxKSSym = 12;           // the symmetric split-BF16 kernel (one row group beyond 8 k-steps)
 // Context scratch `misc`: [0, 1024) relevance-mask count, [1024, 2048)
 // normalisation min/max, [2048, 4096) mask check, [4096, ...) min/max
 // partials of up to 1024 workgroups (3 words each).

@@ -147,7 +147,7 @@ def test_normalise_matches_oracle_bitwise():
 # launch geometry of the global contraction (host only)
 # ---------------------------------------------------------------------------
 NV_CASES = [1, 2, 63, 64, 255, 256, 257, 1000, 3876, 8320, 27936, 63504, 254016, 1028196, 4137156]
-K_CASES = [5, 13, 29, 33, 49, 65, 81, 113, 149, 197, 253, 257, 317, 709, 3001]
+K_CASES = [5, 13, 29, 33, 49, 65, 81, 113, 149, 197, 253, 257, 289, 317, 377, 384, 385, 386, 441, 709, 3001]
 
 
 @pytest.mark.parametrize("precision", [L.FP32, L.BF16X3])
@@ -157,11 +157,12 @@ def test_contraction_plan_invariants(precision, sym):
         for k in K_CASES:
             for metric in range(4):
                 p = L.global_plan(nv, k, metric, precision, sym)
-                # split BF16 up to K_pad = 256, blocks of K = 32m + 1 reflected into
-                # K - 1 dimensions (257 included); larger blocks take the FP32 contraction
+                # split BF16 up to K_pad = 256 (384 for the symmetric contraction),
+                # blocks of K = 32m + 1 reflected into K - 1 dimensions (257 and 385
+                # included); larger blocks take the FP32 contraction
                 split = precision == L.BF16X3 and metric != L.ABSDIFF
                 kd = k - 1 if split and (k - 1) % 32 == 0 else k
-                mfma = split and (kd + 31) // 32 <= 8
+                mfma = split and (kd + 31) // 32 <= (12 if sym else 8)
                 assert bool(p["flags"] & 1) == mfma
                 assert bool(p["flags"] & 4) == (mfma and kd == k - 1)
                 is_sym = bool(p["flags"] & 2)

@@ -88,15 +88,19 @@ def test_lattice1024_b11_sampled_rows(ctx, lattice1024, metric, precision):
 
 
 # ---------------------------------------------------------------------------
-# block shapes: beyond the split-BF16 range (K > 256) and non-square
+# block shapes: beyond K = 256 and non-square
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("metric", [0, 1, 2, 3])
-@pytest.mark.parametrize("b,shape", [(21, (90, 80)), (31, (72, 66))])
+@pytest.mark.parametrize("b,shape", [(21, (90, 80)), (23, (84, 92)), (31, (72, 66))])
 def test_global_large_blocks_default_precision(ctx, metric, b, shape):
-    """Blocks the reference dialog accepts beyond K = 256 (21x21: 317 px,
-    31x31: 709 px) through the default (split-BF16) precision: the plan takes
-    the FP32 contraction, the map matches the oracle."""
-    assert sred_amd.block_size_2d(b, b) > 256
+    """Blocks the reference dialog accepts beyond K = 256 through the default
+    (split-BF16) precision: 21x21 (317 px) and 23x23 (377 px) take the
+    symmetric split-BF16 contraction with one row group (Pearson, SSIM,
+    NRMSE), 31x31 (709 px) and AbsDiffStds the FP32 contraction; every map
+    matches the oracle."""
+    K = sred_amd.block_size_2d(b, b)
+    assert K > 256
+    assert bool(L.global_plan(10000, K, metric, L.BF16X3)["flags"] & 1) == (K <= 384 and metric != L.ABSDIFF)
     img = O.normalize_input(synthetic.spots(shape[1], shape[0], 12, seed=b + metric, sigma=4.0))
     out = sred_amd.global_repetition_2d(img, b, b, 0.0, GM[metric], resources=ctx)
     ref64 = O.global_map(metric, img, b, b, 0.0, True, fp64=True)
@@ -142,6 +146,21 @@ def test_global_reflected_blocks(ctx, metric, bw, bh):
     out = sred_amd.global_repetition_2d(img, bw, bh, 0.0, GM[metric], True, "bf16x3", resources=ctx)
     f32 = sred_amd.global_repetition_2d(img, bw, bh, 0.0, GM[metric], True, "fp32", resources=ctx)
     ref64 = O.global_map(metric, img, bw, bh, 0.0, True, fp64=True)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+    assert np.abs(out - f32).max() <= MAP_TOL
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_global_385_pixel_blocks(ctx, metric):
+    """11x51 blocks hold 385 pixels: reflected into 384 dimensions they take
+    the symmetric split-BF16 contraction at its largest size (12 k-steps);
+    the maps match FP64 and the FP32 contraction, which keeps all 385."""
+    K = sred_amd.block_size_2d(11, 51)
+    assert K == 385 and L.global_plan(10000, K, metric, L.BF16X3)["flags"] & 7 == 7
+    img = O.normalize_input(synthetic.spots(80, 120, 16, seed=metric + 51, sigma=4.0))
+    out = sred_amd.global_repetition_2d(img, 11, 51, 0.0, GM[metric], True, "bf16x3", resources=ctx)
+    f32 = sred_amd.global_repetition_2d(img, 11, 51, 0.0, GM[metric], True, "fp32", resources=ctx)
+    ref64 = O.global_map(metric, img, 11, 51, 0.0, True, fp64=True)
     assert np.abs(out - ref64).max() <= MAP_TOL
     assert np.abs(out - f32).max() <= MAP_TOL
 
