@@ -42,29 +42,42 @@ __global__ __launch_bounds__(256) void k_adjoint_coef(const int *__restrict__ id
                                                       float n_pixels, float tscale, float4 *__restrict__ info,
                                                       float4 *__restrict__ info2, unsigned int *__restrict__ bmax)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    // Nv_pad is a multiple of 64: the threads past it are whole waves
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= Nv_pad) return;
+    unsigned int mb = 0u, ma = 0u;
     if (v >= Nv) {
         info[v] = make_float4(0.0f, 0.0f, 0.0f, 1e30f);  // padding: weight exp2(-inf) = 0, a = b = 0
         info2[v] = make_float4(1e30f, 0.0f, 0.0f, 0.0f);
-        return;
+    } else {
+        int p = idx[v];
+        // normalisation constants of the forward map (ordered-int min/max)
+        unsigned int lo = mm[0], hi = mm[1];
+        float mn = __uint_as_float((lo & 0x80000000u) ? (lo & 0x7fffffffu) : ~lo);
+        float mx = __uint_as_float((hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi);
+        double g = (double)dmap[p];
+        if (mask[p] > 0.0f) g /= ((double)mx - (double)mn + 1e-7);
+        double D = (double)W[p] * (double)n_pixels + 1e-7;
+        double rep = (double)S[p] / D;
+        float s = stds[p];
+        double a = (s == 0.0f) ? 0.0 : g / D;
+        double b = -g * rep * (double)n_pixels / D;
+        info[v] = make_float4(s, (float)a, (float)b, s * tscale);
+        info2[v] = make_float4(s * tscale, (float)a, (float)b, s > 0.0f ? 1.0f / s : 0.0f);
+        mb = __float_as_uint(fabsf((float)b));
+        ma = __float_as_uint(fabsf((float)a));
     }
-    int p = idx[v];
-    // normalisation constants of the forward map (ordered-int min/max)
-    unsigned int lo = mm[0], hi = mm[1];
-    float mn = __uint_as_float((lo & 0x80000000u) ? (lo & 0x7fffffffu) : ~lo);
-    float mx = __uint_as_float((hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi);
-    double g = (double)dmap[p];
-    if (mask[p] > 0.0f) g /= ((double)mx - (double)mn + 1e-7);
-    double D = (double)W[p] * (double)n_pixels + 1e-7;
-    double rep = (double)S[p] / D;
-    float s = stds[p];
-    double a = (s == 0.0f) ? 0.0 : g / D;
-    double b = -g * rep * (double)n_pixels / D;
-    info[v] = make_float4(s, (float)a, (float)b, s * tscale);
-    info2[v] = make_float4(s * tscale, (float)a, (float)b, s > 0.0f ? 1.0f / s : 0.0f);
-    // max |b| over the blocks (bits of a non-negative float order as unsigned; NaN above every finite value)
-    atomicMax(bmax, __float_as_uint(fabsf((float)b)));
+    // bmax[0] = max |b|, bmax[1] = max |a| over the blocks (bits of a non-negative
+    // float order as unsigned; NaN above every finite value), one atomic per wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        mb = max(mb, (unsigned int)__shfl_xor((int)mb, o));
+        ma = max(ma, (unsigned int)__shfl_xor((int)ma, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(bmax, mb);
+        atomicMax(bmax + 1, ma);
+    }
 }
 
 __global__ void k_fill_int(int *__restrict__ p, int64_t n, int value)
@@ -312,6 +325,26 @@ constexpr float kAdjFastSigma = 0.01f;
 constexpr float kLn2f = 0.693147180559945309f;
 constexpr float kLog2ef = 1.442695040888963407f;
 
+// Scale exponent of the dot tiles: 40, lowered for max|a| >= 2^50 so that
+// 2^E Q d stays below 2^91 (sums of 64 such in FP32); 0 for inf or NaN.
+__device__ __forceinline__ int adj_dot_exponent(unsigned int amax_bits)
+{
+    if (amax_bits == 0u) return 40;
+    if (amax_bits >= 0x7f800000u) return 0;
+    int ex;
+    (void)frexpf(__uint_as_float(amax_bits), &ex);  // max|a| < 2^ex
+    return min(max(90 - ex, 0), 40);
+}
+
+// v * s for a power of two s that keeps every element finite (exact)
+__device__ __forceinline__ bf16x8 scale_bf16x8(bf16x8 v, float s)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)v[e] * s);
+    return r;
+}
+
 // small[g] = 1 if a block of [32g, 32g + 32) has 0 < sigma < kAdjFastSigma
 // (info.x = sigma).
 __global__ __launch_bounds__(256) void k_adjoint_small(const float4 *__restrict__ info, int ngroups,
@@ -358,7 +391,9 @@ template <int KS, int KG>
 __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                          const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
                                                          const float4 *__restrict__ info2, const int *__restrict__ small,
-                                                         const double *__restrict__ F, const float *__restrict__ UT,
+                                                         const double *__restrict__ F,
+                                                         const unsigned int *__restrict__ amax_bits,
+                                                         const float *__restrict__ UT,
                                                          int K, int Nv, int Nv_pad, float eps, float cE,
                                                          float inv_sqrt_km1, int K16, float *__restrict__ Gc)
 {
@@ -376,6 +411,14 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
     const int li = lane & 15, lq = lane >> 4;
     const int rowbase = blockIdx.x * kAdjRows + wave * 16 * RG;
 
+    // The row fragments carry a factor 2^E, so the dot tiles hold 2^E d and
+    // [d > 0] w~ is one v_med3(2^E d, 0, w~) (0 <= w~ <= 1): the two agree
+    // unless 0 < d < 2^-E, far inside the rounding of d (E = 40 but for loss
+    // coefficients beyond 2^50; a pair that close to the kink of max(0, r) may
+    // take either side).  corr and se then carry 2^E as well, removed per row
+    // at the end.
+    const int E = adj_dot_exponent(*amax_bits);
+    const float dscale = __builtin_amdgcn_ldexpf(1.0f, E);
     bf16x8 bh[RG][KS], bl[RG][KS];
     float4 R[RG];
     float rinv[RG];
@@ -384,8 +427,8 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const size_t o = ((size_t)(rowbase / 16 + rg) * KS + ks) * 64 + lane;
-            bh[rg][ks] = __builtin_bit_cast(bf16x8, Uhi[o]);
-            bl[rg][ks] = __builtin_bit_cast(bf16x8, Ulo[o]);
+            bh[rg][ks] = scale_bf16x8(__builtin_bit_cast(bf16x8, Uhi[o]), dscale);
+            bl[rg][ks] = scale_bf16x8(__builtin_bit_cast(bf16x8, Ulo[o]), dscale);
         }
         R[rg] = info2[rowbase + rg * 16 + li];
         rinv[rg] = R[rg].w;
@@ -460,39 +503,12 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
             }
         }
     };
-    // pair terms of column group cg (half h of its step) for row group rg:
-    //   Q into pv[h*4 + r]; corr += Q d+ (1 - 1/(1 + delta)) log2(e); se += Q d+ (t_i - t_j)
+    // pair terms of one pair (column constants C, row group rg, r of the
+    // lane's four columns) into pv[h * 4 + r]:
+    //   Q; corr += Q d+ (1 - 1/(1 + delta)) log2(e); se += Q d+ (t_i - t_j)
     // Fast tiles take w~ = exp2(-(t_i - t_j)^2 - delta log2(e)) = w e^-delta and
     // 1 - 1/(1 + delta) = delta (relative errors delta^2/2 and delta, delta <= 1e-3);
-    // exact tiles divide.  corr is kept in log2(e) units (scaled by ln 2 per tile).
-    auto pairs = [&](auto exact_tag, const float4 *ci, int cg, int h, int rg, const floatx4 &d, float (&pv)[8],
-                     float &corr, float &se) {
-        constexpr bool EXACT = decltype(exact_tag)::value;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const float4 C = ci[cg * 16 + 4 * lq + r];  // column j = c0 + 16cg + 4(l/16) + r
-            const float dt = R[rg].x - C.x;
-            const float dl = R[rg].w * C.w;  // delta log2(e)
-            float wt, om;                    // w / (1 + delta), (1 - 1/(1 + delta)) log2(e)
-            if (EXACT) {
-                const float w = __builtin_amdgcn_exp2f(-(dt * dt));
-                const float kap = __builtin_amdgcn_rcpf(fmaf(dl, kLn2f, 1.0f));
-                wt = w * kap;
-                om = dl * kap;
-            } else {
-                wt = __builtin_amdgcn_exp2f(-fmaf(dt, dt, dl));
-                om = dl;
-            }
-            const float dv = d[r];
-            const float aw = (R[rg].y + C.y) * wt;
-            const float Q = dv > 0.0f ? aw : 0.0f;
-            const float Qd = Q * dv;  // = Q max(d, 0): Q is 0 where d <= 0
-            corr = fmaf(Qd, om, corr);
-            se = fmaf(Qd, dt, se);
-            pv[h * 4 + r] = Q;
-        }
-    };
-    // the same for one pair (r)
+    // exact tiles divide.  corr is kept in log2(e) units (scaled by ln 2 per row).
     auto pair1 = [&](auto exact_tag, const float4 &C, int h, int rg, int r, float dv, float (&pv)[8], float &corr,
                      float &se) {
         constexpr bool EXACT = decltype(exact_tag)::value;
@@ -508,9 +524,9 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
             wt = __builtin_amdgcn_exp2f(-fmaf(dt, dt, dl));
             om = dl;
         }
-        const float aw = (R[rg].y + C.y) * wt;
-        const float Q = dv > 0.0f ? aw : 0.0f;
-        const float Qd = Q * dv;
+        // [d > 0] w~ as one v_med3 (dv = 2^E d, 0 <= w~ <= 1)
+        const float Q = (R[rg].y + C.y) * __builtin_amdgcn_fmed3f(dv, 0.0f, wt);
+        const float Qd = Q * dv;  // 2^E Q max(d, 0)
         corr = fmaf(Qd, om, corr);
         se = fmaf(Qd, dt, se);
         pv[h * 4 + r] = Q;
@@ -664,7 +680,9 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
         q += __shfl_xor(q, 32);
         e += __shfl_xor(e, 16);
         e += __shfl_xor(e, 32);
-        q *= (double)kLn2f;  // the correction, from log2(e) units
+        const double unscale = __builtin_amdgcn_ldexp(1.0, -E);
+        q *= (double)kLn2f * unscale;  // the correction, from log2(e) units
+        e *= unscale;
         if (v < Nv) e += F[v];
         const double is = (double)rinv[rg];
         const bool live = v < Nv && rinv[rg] != 0.0f;
@@ -767,24 +785,24 @@ static int launch_adjoint(int M16, dim3 grid, size_t lds, hipStream_t st, const 
 template <int KS, int KG>
 static void launch_adjoint_mfma_t(dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo, const uint4 *vhi,
                                   const uint4 *vlo, const float4 *info2, const int *small, const double *F,
-                                  const float *UT, int K, int Nv, int Nv_pad, float eps, float cE, float isk, int K16,
-                                  float *Gc)
+                                  const unsigned int *amax, const float *UT, int K, int Nv, int Nv_pad, float eps,
+                                  float cE, float isk, int K16, float *Gc)
 {
-    hipLaunchKernelGGL((k_adjoint_mfma<KS, KG>), grid, dim3(256), 0, st, uhi, ulo, vhi, vlo, info2, small, F, UT, K,
-                       Nv, Nv_pad, eps, cE, isk, K16, Gc);
+    hipLaunchKernelGGL((k_adjoint_mfma<KS, KG>), grid, dim3(256), 0, st, uhi, ulo, vhi, vlo, info2, small, F, amax,
+                       UT, K, Nv, Nv_pad, eps, cE, isk, K16, Gc);
 }
 
 // k-groups KG = ceil((K - 1) / 16) of the second product: 1..2 (KS = 1), 2..4 (KS = 2), 4..6 (KS = 3)
 static int launch_adjoint_mfma(int KS, int KG, dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo,
                                const uint4 *vhi, const uint4 *vlo, const float4 *info2, const int *small,
-                               const double *F, const float *UT, int K, int Nv, int Nv_pad, float eps, float cE,
-                               float isk, int K16, float *Gc)
+                               const double *F, const unsigned int *amax, const float *UT, int K, int Nv, int Nv_pad,
+                               float eps, float cE, float isk, int K16, float *Gc)
 {
-#define SRED_ADJ_CASE(ks, kg)                                                                                        \
-    if (KS == ks && KG == kg) {                                                                                      \
-        launch_adjoint_mfma_t<ks, kg>(grid, st, uhi, ulo, vhi, vlo, info2, small, F, UT, K, Nv, Nv_pad, eps, cE, isk, \
-                                      K16, Gc);                                                                      \
-        return SRED_OK;                                                                                              \
+#define SRED_ADJ_CASE(ks, kg)                                                                                 \
+    if (KS == ks && KG == kg) {                                                                               \
+        launch_adjoint_mfma_t<ks, kg>(grid, st, uhi, ulo, vhi, vlo, info2, small, F, amax, UT, K, Nv, Nv_pad, \
+                                      eps, cE, isk, K16, Gc);                                                 \
+        return SRED_OK;                                                                                       \
     }
     SRED_ADJ_CASE(1, 1)
     SRED_ADJ_CASE(1, 2)
@@ -897,8 +915,8 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
         const float cE = (float)(-2.0 / H / (double)tscale);  // (s_i - s_j) = (t_i - t_j) / tscale
         const float *UTp = ctx->u32.as<float>();
         const dim3 grid(Nv_pad / kAdjRows);
-        SRED_TRY(launch_adjoint_mfma(KS, KG, grid, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2, d_small, d_F, UTp,
-                                     K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc));
+        SRED_TRY(launch_adjoint_mfma(KS, KG, grid, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2, d_small, d_F,
+                                     d_bmax + 1, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc));
         SRED_HIP_CHECK(hipGetLastError());
         SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
     } else {
