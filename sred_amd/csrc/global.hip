@@ -374,7 +374,12 @@ template <int M> __device__ __forceinline__ float4 row_const(float4 a, const Epi
 }
 template <int M> __device__ __forceinline__ float4 col_const(float4 a, const EpiConst &) { return a; }
 
-template <int M, bool EXACT = true, bool COLS = false>
+// FOLD (the symmetric kernel): the MFMA accumulators did not start at zero,
+// so `dot` already holds the metric's affine part (see sym_row_setup):
+//   SSIM:  dot = u_i.u_j + c2/2, and R.y = 4 m_i (the factor 2 of 2cov + c2
+//          moved into the luminance term: (4 m_i m_j + 2 c1)(cov + c2/2));
+//   NRMSE: dot = |u_i|^2 - 2 u_i.u_j (row fragments scaled by -2).
+template <int M, bool EXACT = true, bool COLS = false, bool FOLD = false>
 __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float4 &C, float &S, const EpiConst &e,
                                          float *cS = nullptr)
 {
@@ -390,20 +395,20 @@ __device__ __forceinline__ void epilogue(float dot, const float4 &R, const float
         if (COLS) *cS = fmaf(r, wt, *cS);
     } else if (M == SRED_SSIM) {
         const float w = __builtin_amdgcn_exp2f(-(d * d));
-        float a = fmaf(R.y, C.y, e.c1);
-        float b = fmaf(2.0f, dot, e.c2);
+        float a = fmaf(R.y, C.y, FOLD ? 2.0f * e.c1 : e.c1);
+        float b = FOLD ? dot : fmaf(2.0f, dot, e.c2);
         float c = fmaf(R.z, C.z, e.c1);
         float t = fmaxf((a * b) * __builtin_amdgcn_rcpf(c * (R.w + C.w)) * w, 0.0f);
         S += t;
         if (COLS) *cS += t;
     } else if (M == SRED_NRMSE) {  // unweighted (kernelGetGlobalNrmse2D.cl)
-        float mse = fmaf(-2.0f, dot, R.y + C.y);
+        float mse = FOLD ? dot + C.y : fmaf(-2.0f, dot, R.y + C.y);
         float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f));
         S += t;
         if (COLS) *cS += t;
     } else if (M == kNrmseWeighted) {  // sqrt(mse) * weight (kernelGetGlobalNrmse3D.cl:111)
         const float w = __builtin_amdgcn_exp2f(-(d * d));
-        float mse = fmaf(-2.0f, dot, R.y + C.y);
+        float mse = FOLD ? dot + C.y : fmaf(-2.0f, dot, R.y + C.y);
         float t = __builtin_amdgcn_sqrtf(fmaxf(mse, 0.0f)) * w;
         S += t;
         if (COLS) *cS += t;
@@ -538,6 +543,50 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
         }
 }
 
+// v * s for a power of two s (exact in BF16 while the result stays finite)
+__device__ __forceinline__ bf16x8 bf16x8_scaled(bf16x8 v, float s)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)v[e] * s);
+    return r;
+}
+
+// Folds each metric's affine part into the symmetric kernel's MFMAs (one
+// vector instruction less per pair in the epilogue<..., FOLD = true>):
+//   SSIM:  accumulators start at c2/2, R.y = 2 m_i becomes 4 m_i;
+//   NRMSE (and the weighted 3D form): row fragments times -2 (a power of
+//          two: exact), accumulators start at the row's |u_i|^2;
+//   Pearson: accumulators start at 0.
+// Measured against the unfolded form: 1024^2 11x11 SSIM 274.6 -> 268.7 ms,
+// NRMSE 256.2 -> 251.6 ms, 768^2 13x13 -2% / -1%.  SSIM beyond K_pad = 128
+// keeps the unfolded form (the start values cost registers there: 17x17
+// spilled more and ran 2% slower).
+template <int M, int KS> constexpr bool sym_fold() { return M == SRED_SSIM ? KS <= 4 : M != SRED_PEARSON; }
+template <int M, int KS, int RG>
+__device__ __forceinline__ void sym_row_setup(bf16x8 (&ahi)[RG][KS], bf16x8 (&alo)[RG][KS], float4 (&R)[RG][4],
+                                              floatx4 (&cinit)[RG], const EpiConst &ec)
+{
+#pragma unroll
+    for (int rg = 0; rg < RG; rg++) {
+        if (M == SRED_NRMSE || M == kNrmseWeighted) {
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                ahi[rg][ks] = bf16x8_scaled(ahi[rg][ks], -2.0f);
+                alo[rg][ks] = bf16x8_scaled(alo[rg][ks], -2.0f);
+            }
+            cinit[rg] = floatx4{R[rg][0].y, R[rg][1].y, R[rg][2].y, R[rg][3].y};
+        } else if (M == SRED_SSIM && sym_fold<M, KS>()) {
+            const float h = 0.5f * ec.c2;
+            cinit[rg] = floatx4{h, h, h, h};
+#pragma unroll
+            for (int r = 0; r < 4; r++) R[rg][r].y *= 2.0f;
+        } else {
+            cinit[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One 64-column tile of one wave (16*RG rows): the MFMAs of column group
 // cg+1 are issued before the epilogue of group cg, so the matrix core works
@@ -557,8 +606,8 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
 template <int M, int KS, int RG, bool EXACT, bool COLS>
 __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_lo, const float4 *ccon,
                                          const bf16x8 (&ahi)[RG][KS], const bf16x8 (&alo)[RG][KS],
-                                         const float4 (&R)[RG][4], float (&accS)[RG][4], float4 *stage_w, int lane,
-                                         const EpiConst &ec)
+                                         const float4 (&R)[RG][4], const floatx4 (&cinit)[RG], float (&accS)[RG][4],
+                                         float4 *stage_w, int lane, const EpiConst &ec)
 {
     constexpr bool PF = (M == SRED_PEARSON || M == SRED_NRMSE) && KS <= 3;
     const int li = lane & 15;
@@ -584,7 +633,7 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
         if (cg + 1 < 4) {  // issue group cg+1
             floatx4(&a)[RG] = acc[(cg + 1) & 1];
 #pragma unroll
-            for (int rg = 0; rg < RG; rg++) a[rg] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int rg = 0; rg < RG; rg++) a[rg] = cinit[rg];
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) {
                 const bf16x8 h = __builtin_bit_cast(
@@ -605,7 +654,9 @@ __device__ __forceinline__ void sym_tile(const uint4 *lds_hi, const uint4 *lds_l
 #pragma unroll
             for (int rg = 0; rg < RG; rg++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) epilogue<M, EXACT, COLS>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], ec, &cs);
+                for (int r = 0; r < 4; r++)
+                    epilogue<M, EXACT, COLS, sym_fold<M, KS>()>(acc[cg & 1][rg][r], R[rg][r], C, accS[rg][r], ec,
+                                                                &cs);
             colsum[cg] = cs;
         }
         if (M != SRED_SSIM && cg >= 0 && cg + 1 < 4) {
@@ -791,6 +842,7 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
 
     bf16x8 ahi[RG][KS], alo[RG][KS];
     float4 R[RG][4];
+    floatx4 cinit[RG];  // MFMA accumulator start values (the folded affine part of the metric)
     float accS[RG][4];
     bool row_small = false;
     int rowbase = 0;
@@ -824,6 +876,7 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
         for (int rg = 0; rg < RG; rg++)
 #pragma unroll
             for (int r = 0; r < 4; r++) R[rg][r] = row_const<M>(aux[rowbase + rg * 16 + lk * 4 + r], ec);
+        sym_row_setup<M, KS, RG>(ahi, alo, R, cinit, ec);
         row_small = false;
         if (M == SRED_PEARSON)
             for (int g = 0; g < RG / 2; g++) row_small |= tile_small[rowbase / 32 + g] != 0;
@@ -864,11 +917,11 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
         const bool exact = row_small || (M == SRED_PEARSON && ((tile_exact >> tt) & 1ull) != 0ull);
         float4 *sw = stage[T & 1][wave];
         if (diag) {
-            if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
-            else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
+            if (exact) sym_tile<M, KS, RG, true, false>(chi, clo, ccon, ahi, alo, R, cinit, accS, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, false>(chi, clo, ccon, ahi, alo, R, cinit, accS, sw, lane, ec);
         } else {
-            if (exact) sym_tile<M, KS, RG, true, true>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
-            else sym_tile<M, KS, RG, false, true>(chi, clo, ccon, ahi, alo, R, accS, sw, lane, ec);
+            if (exact) sym_tile<M, KS, RG, true, true>(chi, clo, ccon, ahi, alo, R, cinit, accS, sw, lane, ec);
+            else sym_tile<M, KS, RG, false, true>(chi, clo, ccon, ahi, alo, R, cinit, accS, sw, lane, ec);
         }
         if (T % ntiles == ntiles - 1) end_sweep();
     };
