@@ -124,3 +124,33 @@ def test_batch_large_nonsquare_images_with_b17(metric):
             px = sample_pixels(out, mask, 40, seed=70 + i)
             ref, _ = fp64_map_at(O, metric, im, b, b, 0.0, px)
             assert np.abs(out.ravel()[px] - ref).max() <= MAP_TOL, (shape, b)
+
+
+LARGE_BLOCKS = [((84, 90), 21, 21), ((120, 80), 11, 51), ((96, 100), 23, 23), ((60, 70), 5, 5)]
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_batch_blocks_beyond_256_pixels(metric):
+    """Blocks of 317 (21x21), 385 (11x51, reflected into 384 dimensions) and
+    377 (23x23) pixels in one batch beside a 5x5 class: the 10- and 12-k-step
+    classes of the symmetric kernel; each map equals the single-image path's
+    and matches FP64."""
+    torch = pytest.importorskip("torch")
+    imgs = [O.normalize_input(synthetic.spots(s[1], s[0], 14, seed=80 + i, sigma=4.0))
+            for i, (s, _, _) in enumerate(LARGE_BLOCKS)]
+    with sred_amd.Context(0) as ctx:
+        d_imgs = [torch.from_numpy(im).cuda() for im in imgs]
+        d_outs = [torch.full_like(d, 7.0) for d in d_imgs]
+        probs = [(d.data_ptr(), im.shape[1], im.shape[0], bw, bh, o.data_ptr())
+                 for d, im, o, (_, bw, bh) in zip(d_imgs, imgs, d_outs, LARGE_BLOCKS)]
+        global_batch_device(ctx, metric, probs, 0.0, True)
+        torch.cuda.synchronize()
+        for (shape, bw, bh), d, im, o in zip(LARGE_BLOCKS, d_imgs, imgs, d_outs):
+            single = torch.zeros_like(d)
+            global_map_device(ctx, metric, L.BF16X3, d.data_ptr(), im.shape[1], im.shape[0], bw, bh, 0.0, True,
+                              single.data_ptr())
+            torch.cuda.synchronize()
+            out = o.cpu().numpy()
+            assert np.abs(out - single.cpu().numpy()).max() <= 2e-6, (shape, bw, bh)
+            ref = O.global_map(metric, im, bw, bh, 0.0, True, fp64=True)
+            assert np.abs(out - ref).max() <= MAP_TOL, (shape, bw, bh)
