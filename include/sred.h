@@ -75,13 +75,19 @@ int sred_ctx_set_chunk_cols(sred_ctx *ctx, int cols);
  * multiple of 256 up to 2048; 0 = off) and SRED_OPT_BATCH_MEMORY_MB (partial
  * sums per wave, default 16384); SRED_OPT_FIT_TIMING (0, the default; 1: a GAT
  * fit brackets every objective launch's row kernel with HIP events, summed
- * for sred_ctx_last_contraction, at about 4 us per launch). */
+ * for sred_ctx_last_contraction, at about 4 us per launch);
+ * SRED_OPT_PARTIAL_MEMORY_MB (partial sums of one global call's symmetric
+ * contraction, default 16384: beyond it, from about 3e6 blocks (1730^2
+ * pixels) on, the call runs the blocks as super-panels, one launch per
+ * super-panel pair, with FP64 row totals in between, so any image size
+ * fits). */
 #define SRED_OPT_CHUNK_COLS 1
 #define SRED_OPT_SYMMETRIC 2
 #define SRED_OPT_AUTOTUNE 3
 #define SRED_OPT_BATCH_PANEL 4
 #define SRED_OPT_BATCH_MEMORY_MB 5
 #define SRED_OPT_FIT_TIMING 6
+#define SRED_OPT_PARTIAL_MEMORY_MB 7
 int sred_ctx_set_option(sred_ctx *ctx, int option, int value);
 
 /* Metric from the reference's dialog string; block != 0 uses the block

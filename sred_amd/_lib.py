@@ -24,6 +24,7 @@ PEARSON, SSIM, NRMSE, ABSDIFF = 0, 1, 2, 3
 FP32, BF16X3 = 0, 1
 OPT_CHUNK_COLS, OPT_SYMMETRIC, OPT_AUTOTUNE, OPT_BATCH_PANEL, OPT_BATCH_MEMORY_MB = 1, 2, 3, 4, 5
 OPT_FIT_TIMING = 6
+OPT_PARTIAL_MEMORY_MB = 7
 
 # every symbol include/sred.h declares
 EXPORTS = [

@@ -398,7 +398,7 @@ int sred_ctx_destroy(sred_ctx *ctx)
                       &ctx->wmom, &ctx->wpart, &ctx->symtab, &ctx->batch_part, &ctx->batch_tab, &ctx->pm_rows,
                       &ctx->pm_cols, &ctx->pm_part, &ctx->pm_spec, &ctx->sortbuf, &ctx->mmpart, &ctx->adj_v,
                       &ctx->qt_res, &ctx->qt_nodes, &ctx->qt_leaves, &ctx->qt_seg, &ctx->qt_tmp,
-                      &ctx->stack_work, &ctx->seg_mm, &ctx->tmpl3d, &ctx->offs3d};
+                      &ctx->stack_work, &ctx->seg_mm, &ctx->tmpl3d, &ctx->offs3d, &ctx->adj_f, &ctx->acc64};
     for (DevBuf *b : bufs) b->release();
     for (BatchStore *b : ctx->batch_store) delete b;
     ctx->batch_store.clear();
@@ -461,6 +461,10 @@ int sred_ctx_set_option(sred_ctx *ctx, int option, int value)
         return SRED_OK;
     case SRED_OPT_FIT_TIMING:
         ctx->fit_timing = value != 0;
+        return SRED_OK;
+    case SRED_OPT_PARTIAL_MEMORY_MB:
+        if (value < 1) break;
+        ctx->partial_mb = value;
         return SRED_OK;
     default:
         break;

@@ -802,19 +802,29 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
     const int *__restrict__ tile_small = probs[qlo].tsm;
     float *__restrict__ partial = probs[qlo].partial;
     const int Nv_pad = probs[qlo].Nv_pad;
-    // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
+    const int npr = probs[qlo].npr, coff = probs[qlo].coff;
+    float *__restrict__ pcol = npr > 0 ? probs[qlo].pcol : partial;
+    const int ldc = npr > 0 ? probs[qlo].ldc : Nv_pad;
+    // (p, c) from the chunk-major index q: triangular q = c(c+1)/2 + p, or
+    // rectangular q = c npr + p
     const int q = (int)(item - probs[qlo].item0);
-    int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
-    while ((c + 1) * (c + 2) / 2 <= q) c++;
-    while (c * (c + 1) / 2 > q) c--;
-    const int p = q - c * (c + 1) / 2;
-    const bool diag = (p == c);
-    const int col0 = c * P, row0 = p * P;
+    int c, p;
+    if (npr > 0) {
+        c = q / npr;
+        p = q - c * npr;
+    } else {
+        c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+        while ((c + 1) * (c + 2) / 2 <= q) c++;
+        while (c * (c + 1) / 2 > q) c--;
+        p = q - c * (c + 1) / 2;
+    }
+    const bool diag = npr == 0 && p == c;
+    const int col0 = coff + c * P, row0 = p * P;
     if (skips<M>() && !diag && aux[col0].x - aux[row0 + P - 1].x > skip_gap<M>()) {
         // every pair of the panel pair has a vanishing weight
         for (int i = tid; i < P; i += 256) {
             partial[(size_t)c * Nv_pad + row0 + i] = 0.0f;
-            partial[(size_t)p * Nv_pad + col0 + i] = 0.0f;
+            pcol[(size_t)p * ldc + (col0 - coff) + i] = 0.0f;
         }
         if (tid == 0) atomicAdd(next + 1, 2ull * (unsigned long long)P * (unsigned long long)P);
         continue;
@@ -973,7 +983,7 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
         fold(NT - 1);
         wait_vm_all();
         __syncthreads();
-        for (int i = tid; i < P; i += 256) partial[(size_t)p * Nv_pad + col0 + i] = colacc[i];
+        for (int i = tid; i < P; i += 256) pcol[(size_t)p * ldc + (col0 - coff) + i] = colacc[i];
     }
     if (nskip && lane == 0) atomicAdd(next + 1, nskip);
   }
@@ -1127,6 +1137,32 @@ __global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ partia
     // both-flat pairs contribute 1*weight with weight exp(-0) = 1
     // (kernelGetGlobalPearson2D.cl:101-102, kernelGetGlobalSsim2D.cl:108-109);
     // the 3D kernels skip flat rows (flat_count 0, every metric)
+    if (flat_rows && stds[p] == 0.0f) s = (double)flat_count;
+    S_map[p] = (float)(s * s_scale);
+}
+
+// Row totals of a contraction run in super-panels: acc[v] += the nslots
+// partials of row v (slot stride ld), in slot order, in FP64.
+__global__ __launch_bounds__(256) void k_accum_partials(const float *__restrict__ part, int nslots, int ld, int n,
+                                                        double *__restrict__ acc)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nslots; c++) s += (double)part[(size_t)c * ld + v];
+    acc[v] += s;
+}
+
+// k_reduce for FP64 row totals
+__global__ __launch_bounds__(256) void k_reduce_acc64(const double *__restrict__ acc, int Nv,
+                                                      const int *__restrict__ idx, const float *__restrict__ stds,
+                                                      int flat_rows, float flat_count, double s_scale,
+                                                      float *__restrict__ S_map)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= Nv) return;
+    double s = acc[v];
+    const int p = idx[v];
     if (flat_rows && stds[p] == 0.0f) s = (double)flat_count;
     S_map[p] = (float)(s * s_scale);
 }
@@ -1514,6 +1550,83 @@ int global_reduce_dev(sred_ctx *ctx, const GlobalPrepared &r, int metric, const 
 
 EpiConst epi_const() { return EpiConst{kEps, 0.0001f, 0.0009f, (float)((double)kEps * 1.4426950408889634)}; }
 
+// The symmetric contraction of a call whose partial sums (npanels x Nv_pad
+// floats) exceed `budget` floats: the blocks (in sigma order) are cut into
+// super-panels of S blocks, S the largest multiple of the panel size P with
+// 2 (S/P) S floats within the budget (at least P), and each super-panel's
+// triangle and each pair of super-panels' rectangle is one persistent launch
+// over the same panel pairs the whole call would evaluate.  After each launch
+// its partials are added, slot by slot in FP64, to the row totals of its
+// blocks (rows and, for a rectangle, columns); the launches run in a fixed
+// order, so the result is deterministic.  The partial-sum buffer is reused.
+static int sym_superpanels(sred_ctx *ctx, const GlobalPrepared &r, int cm, int metric, const EpiConst &ec,
+                           int64_t budget, const float *d_stds, float *d_S)
+{
+    const ContractionPlan &pl = r.pl;
+    const int P = pl.P, KS = pl.KS, Nv_pad = pl.Nv_pad;
+    int64_t nps = 1;  // panels per super-panel: 2 (S/P) S = 2 nps^2 P floats of partials
+    while (2 * (nps + 1) * (nps + 1) * (int64_t)P <= budget) nps++;
+    const int S = (int)std::min<int64_t>(nps * P, Nv_pad);
+    const int m = (Nv_pad + S - 1) / S;
+    SRED_TRY(ctx->partial.ensure(sizeof(float) * 2 * (size_t)(S / P) * S));
+    SRED_TRY(ctx->acc64.ensure(sizeof(double) * (size_t)Nv_pad));
+    float *d_part = ctx->partial.as<float>();
+    double *d_acc = ctx->acc64.as<double>();
+    SRED_TRY(zero_dev(ctx, d_acc, sizeof(double) * (size_t)Nv_pad));
+    // every super-panel pair's table entry, uploaded once (the table outlives the launches)
+    ctx->host_symtab.clear();
+    std::vector<long long> nitems;
+    for (int a = 0; a < m; a++)
+        for (int b = a; b < m; b++) {
+            const int A0 = a * S, SA = std::min(S, Nv_pad - A0), B0 = b * S, SB = std::min(S, Nv_pad - B0);
+            const size_t fo = (size_t)(A0 / 16) * KS * 64;
+            SymProblem q{r.uhi + fo, r.ulo + fo, r.d_aux + A0, r.d_tsm + A0 / 32, d_part, SA, SA / P, 0};
+            if (a != b) {
+                q.pcol = d_part + (size_t)(SB / P) * SA;
+                q.ldc = SB;
+                q.coff = B0 - A0;
+                q.npr = SA / P;
+                nitems.push_back((long long)(SA / P) * (SB / P));
+            } else {
+                nitems.push_back((long long)(SA / P) * (SA / P + 1) / 2);
+            }
+            ctx->host_symtab.push_back(q);
+        }
+    const size_t ntab = ctx->host_symtab.size();
+    const size_t tab_bytes = (sizeof(SymProblem) * ntab + 127) / 128 * 128;
+    SRED_TRY(ctx->symtab.ensure(tab_bytes + 256));
+    SRED_HIP_CHECK(hipMemcpyAsync(ctx->symtab.ptr, ctx->host_symtab.data(), sizeof(SymProblem) * ntab,
+                                  hipMemcpyHostToDevice, ctx->stream));
+    SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const SymProblem *d_tab = ctx->symtab.as<SymProblem>();
+    unsigned long long *d_next = reinterpret_cast<unsigned long long *>(ctx->symtab.as<char>() + tab_bytes);
+    SRED_TRY(zero_dev(ctx, d_next + 1, sizeof(unsigned long long)));
+    ctx->last_skip = d_next + 1;
+    SRED_TRY(time_begin(ctx));
+    size_t t = 0;
+    for (int a = 0; a < m; a++)
+        for (int b = a; b < m; b++, t++) {
+            const int A0 = a * S, SA = std::min(S, Nv_pad - A0), B0 = b * S, SB = std::min(S, Nv_pad - B0);
+            SRED_TRY(launch_sym_metric(cm, KS, nitems[t], ctx->stream, d_tab + t, 1, d_next, P, ec));
+            SRED_HIP_CHECK(hipGetLastError());
+            const int nrow_slots = (a == b) ? SA / P : SB / P;
+            hipLaunchKernelGGL(k_accum_partials, dim3((SA + 255) / 256), dim3(256), 0, ctx->stream, d_part,
+                               nrow_slots, SA, SA, d_acc + A0);
+            if (a != b)
+                hipLaunchKernelGGL(k_accum_partials, dim3((SB + 255) / 256), dim3(256), 0, ctx->stream,
+                                   d_part + (size_t)(SB / P) * SA, SA / P, SB, SB, d_acc + B0);
+            SRED_HIP_CHECK(hipGetLastError());
+        }
+    SRED_TRY(time_end(ctx, pl.pairs_evaluated));
+    ctx->last_launches = (int)ntab;
+    const double s_scale = (metric == SRED_ABSDIFF) ? 1.0 / (double)(float)r.tscale_d : 1.0;
+    const int flat_rows = r.zero_flat_rows || metric == SRED_PEARSON || metric == SRED_SSIM;
+    hipLaunchKernelGGL(k_reduce_acc64, dim3((r.Nv + 255) / 256), dim3(256), 0, ctx->stream, d_acc, r.Nv, r.d_idx,
+                       d_stds, flat_rows, r.flat_count, s_scale, d_S);
+    SRED_HIP_CHECK(hipGetLastError());
+    return SRED_OK;
+}
+
 int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, const float *d_img,
                      const float *d_means, const float *d_stds, float *d_S, float *d_W)
 {
@@ -1534,6 +1647,8 @@ int global_accum_dev(sred_ctx *ctx, const Geometry &g, const GlobalParams &p, co
     const int nrowtiles = pl.nrowtiles, cpc = pl.cols_per_chunk, nchunks = pl.nchunks;
     const int grid = (int)pl.grid;
     const EpiConst ec = epi_const();
+    const int64_t budget = ctx->partial_mb << 18;  // floats
+    if (pl.sym && pl.partial_elems > budget) return sym_superpanels(ctx, r, cm, metric, ec, budget, d_stds, d_S);
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)pl.partial_elems));
     float *d_part = ctx->partial.as<float>();
 

@@ -107,6 +107,13 @@ struct SymProblem {
     float *partial;
     int Nv_pad, npanels;
     long long item0;  // index of this problem's first panel pair in the launch's work list
+    // Rectangular problem (npr > 0): the panel pairs of npr row panels (blocks
+    // [0, npr P) of the arrays above) against the column panels starting at
+    // block coff (past the rows); row sums go to partial[c][Nv_pad rows],
+    // column sums to pcol[p][ldc columns].  npr = 0: the triangle of
+    // npanels panels, column sums in partial as well.
+    float *pcol = nullptr;
+    int ldc = 0, coff = 0, npr = 0;
 };
 }  // namespace sred
 
@@ -143,6 +150,8 @@ struct sred_ctx {
     std::vector<float> host_tmpl3d;            // reference block last copied into tmpl3d
     sred::DevBuf symtab;       // problem table of the symmetric contraction
     std::vector<sred::SymProblem> host_symtab;
+    sred::DevBuf acc64;        // FP64 row totals of a contraction run in super-panels
+    int64_t partial_mb = 16384;  // partial-sum memory of one global call before it runs in super-panels
     // batched global calls
     std::vector<sred::BatchStore *> batch_store;
     sred::DevBuf batch_part, batch_tab;  // partial sums of a wave; problem tables + item counter

@@ -223,3 +223,48 @@ def test_weight_sums_many_blocks_in_one_bin(ctx):
     ref, _ = fp64_map_at(O, 0, img, 11, 11, 0.0, px)
     err = np.abs(out.ravel()[px] - ref)
     assert err.max() <= MAP_TOL, (err.max(), px[np.argmax(err)])
+
+
+# ---------------------------------------------------------------------------
+# super-panels: a call whose partial sums exceed the memory budget
+# ---------------------------------------------------------------------------
+def _superpanel_run(img, metric, c, budget_mb):
+    """(one-launch map, super-panel map, launches) on a fresh context."""
+    with sred_amd.Context(0) as c2:
+        one = sred_amd.global_repetition_2d(img, 5, 5, c, GM[metric], resources=c2)
+        _, launches1, pairs1 = c2.last_contraction()
+        skip1 = c2.last_skipped()
+        c2.set_option(L.OPT_PARTIAL_MEMORY_MB, budget_mb)
+        out = sred_amd.global_repetition_2d(img, 5, 5, c, GM[metric], resources=c2)
+        _, launches, pairs = c2.last_contraction()
+        assert launches1 == 1 and pairs == pairs1
+        assert c2.last_skipped() == skip1
+    return one, out, launches
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_global_superpanels_small(metric):
+    """Partial-sum budget 1 MB on a 300 x 260 image (up to 60 panels of 1280
+    blocks, up to 17.6 MB of partial sums in one launch): super-panels of 10
+    panels, so a few tens of launches of triangles and rectangles.  The map
+    equals the
+    one-launch map to FP32 rounding (only the FP64 order of the row totals
+    differs), skips the same pairs, and matches FP64 (relevance constant 0.7)."""
+    img = O.normalize_input(synthetic.spots(300, 260, 120, seed=90 + metric))
+    one, out, launches = _superpanel_run(img, metric, 0.7, 1)
+    assert launches > 1
+    assert np.abs(out - one).max() <= 2e-6
+    ref64 = O.global_map(metric, img, 5, 5, 0.7, True, fp64=True)
+    assert np.abs(out - ref64).max() <= MAP_TOL
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_global_superpanels_uneven(metric):
+    """700 x 700, 5x5 (484,416 blocks in 237 panels of 2048; one launch needs
+    439 MB of partial sums): with a 1 MB budget the blocks run as 30
+    super-panels of 8 panels, the last of 5 (465 launches); the map equals
+    the one-launch map to FP32 rounding and skips the same pairs."""
+    img = O.normalize_input(synthetic.spot_ring_mixture(700, 700, seed=95 + metric))
+    one, out, launches = _superpanel_run(img, metric, 0.0, 1)
+    assert launches == 30 * 31 // 2
+    assert np.abs(out - one).max() <= 2e-6
