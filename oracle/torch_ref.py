@@ -19,20 +19,27 @@ import torch
 import oracle as O
 
 
+def _block(b):
+    """(bw, bh) of a block given as one odd side or a (width, height) pair."""
+    return (b, b) if np.isscalar(b) else (int(b[0]), int(b[1]))
+
+
 def pearson_map_t(x, b, h_d=None, stds32=None, n_pixels=None, minmax=None):
-    """Normalised global Pearson map of x (H, W) float64 -> (H, W) float64.
+    """Normalised global Pearson map of x (H, W) float64 -> (H, W) float64,
+    with b x b blocks or, for b = (bw, bh), bw wide and bh high.
 
     Returns (map, aux) where aux holds the fixed quantities used.
     """
     H, W = x.shape
-    r = b // 2
-    k, dx, dy = O.footprint(b, b)
+    bw, bh = _block(b)
+    rw, rh = bw // 2, bh // 2
+    k, dx, dy = O.footprint(bw, bh)
     x32 = x.detach().numpy().astype(np.float32)
     if stds32 is None:
-        _, stds32 = O.local_stats_f32(x32, b, b)
+        _, stds32 = O.local_stats_f32(x32, bw, bh)
     if h_d is None:
         h_d = O.java_float_literal(O.noise_var(x32))
-    ys, xs = np.mgrid[r:H - r, r:W - r]
+    ys, xs = np.mgrid[rh:H - rh, rw:W - rw]
     ys, xs = ys.ravel(), xs.ravel()
     idx = torch.from_numpy((ys[:, None] + dy[None, :]) * W + (xs[:, None] + dx[None, :]))
     A = x.reshape(-1)[idx]                                   # N x K

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float *__restrict__ U
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 MFMA form of the adjoint pass (blocks up to K = 96).
+// Split-BF16 MFMA form of the adjoint pass (blocks up to K = 128; 13x13 is 113).
 //
 // Per 64-column tile a wave (32 rows i) computes the transposed dot-product
 // tile D[j][i] = u_j . u_i with v_mfma_f32_16x16x32_bf16 (A = the column
@@ -386,7 +386,8 @@ __device__ __forceinline__ void adj_wait_all() { __builtin_amdgcn_s_waitcnt(0); 
 // info2 = (t, a, b, 1/s) per block (1/s = 0 for flat blocks and padding);
 // F[v] for v < Nv.
 // Up to K = 64 two workgroups share a CU; beyond, the row fragments and
-// accumulators of three k-steps need more registers than half a SIMD's file.
+// accumulators of three or four k-steps need more registers than half a
+// SIMD's file (K = 128: 420 VGPRs, 130 KiB of LDS).
 template <int KS, int KG>
 __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
                                                          const uint4 *__restrict__ Vhi, const uint4 *__restrict__ Vlo,
@@ -792,7 +793,8 @@ static void launch_adjoint_mfma_t(dim3 grid, hipStream_t st, const uint4 *uhi, c
                        UT, K, Nv, Nv_pad, eps, cE, isk, K16, Gc);
 }
 
-// k-groups KG = ceil((K - 1) / 16) of the second product: 1..2 (KS = 1), 2..4 (KS = 2), 4..6 (KS = 3)
+// k-groups KG = ceil((K - 1) / 16) of the second product: 1..2 (KS = 1), 2..4 (KS = 2), 4..6 (KS = 3),
+// 6..8 (KS = 4)
 static int launch_adjoint_mfma(int KS, int KG, dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo,
                                const uint4 *vhi, const uint4 *vlo, const float4 *info2, const int *small,
                                const double *F, const unsigned int *amax, const float *UT, int K, int Nv, int Nv_pad,
@@ -812,6 +814,9 @@ static int launch_adjoint_mfma(int KS, int KG, dim3 grid, hipStream_t st, const 
     SRED_ADJ_CASE(3, 4)
     SRED_ADJ_CASE(3, 5)
     SRED_ADJ_CASE(3, 6)
+    SRED_ADJ_CASE(4, 6)
+    SRED_ADJ_CASE(4, 7)
+    SRED_ADJ_CASE(4, 8)
 #undef SRED_ADJ_CASE
     set_error("internal: no adjoint pair kernel for this block size");
     return SRED_ERR_ARG;
@@ -856,7 +861,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     SRED_TRY(zero_dev(ctx, d_grad, sizeof(float) * n));
     if (Nv == 0) return SRED_OK;
     const int KS = (K + 31) / 32;
-    const bool mfma = KS <= 3;  // split-BF16 MFMA pass (its accumulators fit the registers up to K = 96)
+    const bool mfma = KS <= 4;  // split-BF16 MFMA pass (its fragments and accumulators fit up to K = 128)
     const int Nv_pad = mfma ? (Nv + kAdjRows - 1) / kAdjRows * kAdjRows : (Nv + 63) / 64 * 64;
     // blocks in ascending sigma: the few tiles that need the exact small-sigma
     // form then hold the small-sigma blocks only (any order gives the same

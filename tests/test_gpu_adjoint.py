@@ -24,15 +24,19 @@ pytestmark = pytest.mark.gpu
 
 
 def gpu_grad(ctx, img, b, weights):
+    """The adjoint through the C-ABI; b is one odd side or (width, height)."""
     h, w = img.shape
+    bw, bh = (b, b) if np.isscalar(b) else b
     g = np.zeros_like(img)
     L.check(L.load().sred_global_pearson_adjoint_2d(ctx.handle, L.fptr(np.ascontiguousarray(img, np.float32)), w, h,
-                                                   b, b, L.fptr(np.ascontiguousarray(weights, np.float32)), L.fptr(g)))
+                                                   bw, bh, L.fptr(np.ascontiguousarray(weights, np.float32)),
+                                                   L.fptr(g)))
     return g
 
 
 @pytest.mark.parametrize("shape,b,seed", [((26, 28), 5, 4), ((36, 40), 9, 5), ((33, 30), 7, 6), ((40, 44), 11, 7),
-                                         ((24, 22), 3, 8)])
+                                         ((24, 22), 3, 8), ((46, 44), 13, 9), ((44, 38), (9, 17), 10),
+                                         ((47, 40), (11, 17), 11), ((30, 34), (7, 3), 12)])
 def test_adjoint_matches_fp64_autograd(ctx, shape, b, seed):
     img = O.normalize_input(synthetic.spots(shape[1], shape[0], 10, seed=seed))
     u = np.random.default_rng(seed + 36).random(shape).astype(np.float32)
