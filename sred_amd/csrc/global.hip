@@ -442,8 +442,15 @@ constexpr int rg_for(int metric, int ks) { return ks <= (metric == SRED_SSIM ? 3
 // 15x17 blocks (K_pad 192) Pearson 147 -> 113 ms, 17x17 (224) 166 -> 134 ms,
 // 19x19 (256) 319 -> 305 ms; SSIM and NRMSE gain 15-30%.  K_pad 288-384
 // (21x21, 23x23 blocks) keeps one row group: their tiles leave one workgroup
-// per CU.
-constexpr int rg_sym(int metric, int ks) { return ks <= 8 ? 2 : 1; }
+// per CU.  Four row groups (half the tile copies, folds and barriers per
+// pair) where that was measured faster at 1024^2: Pearson at K_pad = 32
+// (5x5: 157.5 -> 145.5 ms) and NRMSE at K_pad = 64 (9x9: 189 -> 172 ms); the
+// other cases spill more and ran slower with four (Pearson 9x9 +5%, SSIM
+// 9x9 +47%, NRMSE 5x5 +2%, 11x11 +10%, 13x13 2.2x).
+constexpr int rg_sym(int metric, int ks)
+{
+    return ((metric == SRED_PEARSON && ks == 1) || (metric == SRED_NRMSE && ks == 2)) ? 4 : ks <= 8 ? 2 : 1;
+}
 
 template <int M, int KS>
 __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restrict__ Uhi, const uint4 *__restrict__ Ulo,
