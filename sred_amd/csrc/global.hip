@@ -767,7 +767,7 @@ constexpr bool sym_double_buffer(int ks) { return ks <= 4; }
 // not depend on the copy into the other); one barrier per step then both
 // publishes tile T and retires every read of the buffer the next copy reuses.
 // ---------------------------------------------------------------------------
-template <int M, int KS>
+template <int M, int KS, bool RECT>
 __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(const SymProblem *__restrict__ probs, int nprob,
                                                               long long nitems, unsigned long long *__restrict__ next,
                                                               int P, EpiConst ec)
@@ -809,9 +809,12 @@ __global__ __launch_bounds__(256, KS <= 8 ? 2 : 1) void k_contract_mfma_sym(cons
     const int *__restrict__ tile_small = probs[qlo].tsm;
     float *__restrict__ partial = probs[qlo].partial;
     const int Nv_pad = probs[qlo].Nv_pad;
-    const int npr = probs[qlo].npr, coff = probs[qlo].coff;
-    float *__restrict__ pcol = npr > 0 ? probs[qlo].pcol : partial;
-    const int ldc = npr > 0 ? probs[qlo].ldc : Nv_pad;
+    // RECT: the launch may hold rectangular problems (super-panel pairs); the
+    // other instantiation keeps those fields out of its registers (measured
+    // 1% at 1024^2, 11x11)
+    const int npr = RECT ? probs[qlo].npr : 0, coff = RECT ? probs[qlo].coff : 0;
+    float *__restrict__ pcol = (RECT && npr > 0) ? probs[qlo].pcol : partial;
+    const int ldc = (RECT && npr > 0) ? probs[qlo].ldc : Nv_pad;
     // (p, c) from the chunk-major index q: triangular q = c(c+1)/2 + p, or
     // rectangular q = c npr + p
     const int q = (int)(item - probs[qlo].item0);
@@ -1216,7 +1219,7 @@ static void launch_mfma_ks(int grid, hipStream_t st, const uint4 *hi, const uint
 // resident ones find the counter exhausted and exit.
 template <int M, int KS>
 static int launch_sym_ks(long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob,
-                         unsigned long long *d_next, int P, EpiConst ec)
+                         unsigned long long *d_next, int P, EpiConst ec, bool rect)
 {
     int dev = 0, cus = 0;
     SRED_HIP_CHECK(hipGetDevice(&dev));
@@ -1224,28 +1227,32 @@ static int launch_sym_ks(long long nitems, size_t lds, hipStream_t st, const Sym
     const int grid = (int)std::min<long long>(nitems, 8LL * std::max(cus, 1));
     if (grid <= 0) return SRED_OK;
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned int *>(d_next), (int64_t)2);
-    hipLaunchKernelGGL((k_contract_mfma_sym<M, KS>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems, d_next,
-                       P, ec);
+    if (rect)
+        hipLaunchKernelGGL((k_contract_mfma_sym<M, KS, true>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems,
+                           d_next, P, ec);
+    else
+        hipLaunchKernelGGL((k_contract_mfma_sym<M, KS, false>), dim3(grid), dim3(256), lds, st, d_probs, nprob, nitems,
+                           d_next, P, ec);
     return SRED_OK;
 }
 
 template <int M>
 static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      unsigned long long *d_next, int P, EpiConst ec)
+                      unsigned long long *d_next, int P, EpiConst ec, bool rect)
 {
     switch (KS) {
-    case 1: return launch_sym_ks<M, 1>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 2: return launch_sym_ks<M, 2>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 3: return launch_sym_ks<M, 3>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 4: return launch_sym_ks<M, 4>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 5: return launch_sym_ks<M, 5>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 9: return launch_sym_ks<M, 9>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 10: return launch_sym_ks<M, 10>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 11: return launch_sym_ks<M, 11>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    case 12: return launch_sym_ks<M, 12>(nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    case 1: return launch_sym_ks<M, 1>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 2: return launch_sym_ks<M, 2>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 3: return launch_sym_ks<M, 3>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 4: return launch_sym_ks<M, 4>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 5: return launch_sym_ks<M, 5>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 6: return launch_sym_ks<M, 6>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 7: return launch_sym_ks<M, 7>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 8: return launch_sym_ks<M, 8>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 9: return launch_sym_ks<M, 9>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 10: return launch_sym_ks<M, 10>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 11: return launch_sym_ks<M, 11>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    case 12: return launch_sym_ks<M, 12>(nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
     default:
         set_error("the symmetric split-BF16 contraction supports block sizes up to K = 384 pixels");
         return SRED_ERR_ARG;
@@ -1253,14 +1260,15 @@ static int launch_sym(int KS, long long nitems, size_t lds, hipStream_t st, cons
 }
 
 int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      unsigned long long *d_next, int P, EpiConst ec)
+                      unsigned long long *d_next, int P, EpiConst ec, bool rect)
 {
     const size_t lds = sizeof(float) * (size_t)P;
-    if (metric == SRED_PEARSON) return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
-    if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
+    if (metric == SRED_PEARSON)
+        return launch_sym<SRED_PEARSON>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    if (metric == SRED_SSIM) return launch_sym<SRED_SSIM>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
+    if (metric == SRED_NRMSE) return launch_sym<SRED_NRMSE>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
     if (metric == kNrmseWeighted)
-        return launch_sym<kNrmseWeighted>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec);
+        return launch_sym<kNrmseWeighted>(KS, nitems, lds, st, d_probs, nprob, d_next, P, ec, rect);
     set_error("the symmetric contraction takes Pearson, SSIM or NRMSE");
     return SRED_ERR_ARG;
 }
@@ -1614,7 +1622,7 @@ static int sym_superpanels(sred_ctx *ctx, const GlobalPrepared &r, int cm, int m
     for (int a = 0; a < m; a++)
         for (int b = a; b < m; b++, t++) {
             const int A0 = a * S, SA = std::min(S, Nv_pad - A0), B0 = b * S, SB = std::min(S, Nv_pad - B0);
-            SRED_TRY(launch_sym_metric(cm, KS, nitems[t], ctx->stream, d_tab + t, 1, d_next, P, ec));
+            SRED_TRY(launch_sym_metric(cm, KS, nitems[t], ctx->stream, d_tab + t, 1, d_next, P, ec, true));
             SRED_HIP_CHECK(hipGetLastError());
             const int nrow_slots = (a == b) ? SA / P : SB / P;
             hipLaunchKernelGGL(k_accum_partials, dim3((SA + 255) / 256), dim3(256), 0, ctx->stream, d_part,

@@ -297,8 +297,9 @@ int adjoint_weight_terms_dev(sred_ctx *ctx, DevBuf &wmom, DevBuf &wpart, const f
 // nprob problems in d_probs (device table), all with K_pad = 32*KS and panel
 // size P (metric Pearson, SSIM or NRMSE).
 // d_next: a 64-bit device counter the launch zeroes and its workgroups claim items from.
+// rect: the table may hold rectangular problems (SymProblem::npr > 0).
 int launch_sym_metric(int metric, int KS, long long nitems, hipStream_t st, const SymProblem *d_probs, int nprob,
-                      unsigned long long *d_next, int P, EpiConst ec);
+                      unsigned long long *d_next, int P, EpiConst ec, bool rect = false);
 // Split-BF16 gather of the valid blocks, fragment-major (Uhi/Ulo[Nv_pad/16][KS][64]).
 void launch_gather_bf16(sred_ctx *ctx, int metric, const float *d_img, const float *d_means, const float *d_stds,
                         const int *d_idx, int Nv, int Nv_pad, int K, int KS, const int *d_offs, float inv_sqrt_km1,
