@@ -489,10 +489,11 @@ int sred_phase_correlation_2d_dev(sred_ctx *ctx, const float *d_img, const float
 /* ---- differentiation (no counterpart in the reference) ---- */
 /* Gradient of L = sum_i dL_dmap[i] * map[i] of the global Pearson map
  * (relevance constant 0, normalised output; the forward of
- * sred_global_map_2d_dev with the FP32 contraction) with respect to the
- * normalised input pixels, by a hand-written adjoint.  Held fixed: the noise
- * variance, nPixels, the relevance mask and the output min/max; flat blocks
- * (std == 0) pass no gradient through their statistics.  K <= 160. */
+ * sred_global_map_2d_dev with the split-BF16 contraction) with respect to
+ * the normalised input pixels, by a hand-written adjoint.  Held fixed: the
+ * noise variance, nPixels, the relevance mask and the output min/max; flat
+ * blocks (std == 0) pass no gradient through their statistics.  K <= 160
+ * (the pair pass on MFMA up to K = 128, on the vector ALU beyond). */
 int sred_global_pearson_adjoint_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh,
                                    const float *dL_dmap, float *dL_dimg);
 /* Same on device buffers; d_map (optional) receives the forward map. */
