@@ -114,3 +114,23 @@ def test_global3d_257_pixel_blocks(ctx, metric):
     e64 = np.abs(got - r64.out).max()
     print(f"3d {metric} 9x9x9 bf16x3: max|err| vs FP64 {e64:.2e}", flush=True)
     assert e64 <= 1e-4
+
+
+@pytest.mark.parametrize("metric", ["Pearson's R", "SSIM", "NRMSE"])
+def test_global3d_superpanels(metric):
+    """A volume's map with the partial-sum budget cut to 1 MB (9,408 blocks in
+    37 panels of 256: super-panels of 21 panels, so two triangles and one
+    rectangle; NRMSE takes the weighted 3D epilogue) equals the one-launch
+    map to FP32 rounding and matches FP64."""
+    vol = _blobs(16, 26, 30, 7)
+    with sred_amd.Context(0) as c2:
+        one = sred_amd.global_repetition_3d(vol, 3, 3, 3, 0.0, metric, True, "bf16x3", resources=c2)
+        c2.set_option(L.OPT_PARTIAL_MEMORY_MB, 1)
+        got = sred_amd.global_repetition_3d(vol, 3, 3, 3, 0.0, metric, True, "bf16x3", resources=c2)
+        assert c2.last_contraction()[1] == 3
+    assert np.abs(got - one).max() <= 2e-6
+    m = {"Pearson's R": O.PEARSON, "SSIM": O.SSIM, "NRMSE": O.NRMSE}[metric]
+    inp = O.normalize_input(vol)
+    r64 = (O.global_pearson3d(inp, 3, 3, 3, fp64=True) if m == O.PEARSON
+           else O.global_repetition3d(inp, m, 3, 3, 3, fp64=True))
+    assert np.abs(got - r64.out).max() <= 1e-4
