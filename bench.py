@@ -355,7 +355,8 @@ def pairmaps_run(args):
         "config": {"workload": name + ", every interior pixel against every interior pixel", "pairs": pairs},
         "kernel_ms": k, "max_rel_err_vs_oracle_8_rows": rel,
         "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                     "traffic": None, "kernel": "k_hu_pairs" if hu else "k_phase_pairs<5, 5, true>",
+                     "traffic": None,
+                     "kernel": "k_hu_pairs_sym (each unordered pair once)" if hu else "k_phase_pairs<5, 5, true>",
                      "flops_per_pair": flops_pair},
         "cpu_baseline": cpu}))
 

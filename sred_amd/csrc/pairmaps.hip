@@ -24,7 +24,9 @@
 // owns one row and walks a chunk of columns in order with the column data
 // wave-uniform (scalar loads), so a row's sum is a sequential per-lane sum;
 // chunks of columns run as a second grid dimension when rows alone do not
-// fill the GPU, and their FP64 partial sums are added in chunk order.
+// fill the GPU, and their FP64 partial sums are added in chunk order.  The
+// full Hu map (every interior pixel a row) evaluates each unordered pair
+// once instead (k_hu_pairs_sym).
 //
 //   * Hu: per pair one reciprocal and one exp2 on FP32 (transcendental-rate
 //     bound, ~15 VALU issue slots per pair), 64-column FP32 sub-sums added
@@ -216,6 +218,90 @@ __global__ __launch_bounds__(256) void k_hu_pairs(const float2 *__restrict__ col
     part[(size_t)blockIdx.y * R + r] = acc;
 }
 
+// Symmetric form of k_hu_pairs for the full map (every interior pixel a
+// row).  The pair term is symmetric in (i, j) bit for bit (|hu_j - hu_i|,
+// |s_i - s_j| and s_i + s_j are), so each unordered pair of panels (p, c),
+// c >= p, of kHuPanel interior pixels is evaluated once: one workgroup per
+// panel pair, thread t holding rows p P + t + 256 k (k < kHuRows).  Per
+// column j of panel c (wave-uniform) a thread adds its rows' terms to their
+// FP32 sub-sums (into FP64 every 64 columns) and, off the diagonal, to one
+// column sum, which the wave reduces (butterfly) into its own LDS row of
+// column sums.  Row totals go to part[c][i], column totals (the four waves'
+// sums in FP64) to part[p][j]: one writer per slot, summed in slot order by
+// k_pm_finish.  Diagonal panels evaluate their ordered pairs for the rows
+// only.  FULL: every row of the panel is an interior pixel (all but the
+// last row panel).
+constexpr int kHuPanel = 2048, kHuRows = kHuPanel / 256;
+template <bool FULL>
+__device__ __forceinline__ void hu_sym_panel(const float2 *__restrict__ cols, int N, float eps, int p, int c,
+                                             double *__restrict__ part, float (*colacc)[kHuPanel])
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool diag = p == c;
+    const int row0 = p * kHuPanel, col0 = c * kHuPanel, ncol = min(kHuPanel, N - col0);
+    float hi[kHuRows], si[kHuRows], r32[kHuRows];
+    double r64[kHuRows];
+#pragma unroll
+    for (int k = 0; k < kHuRows; k++) {
+        const int r = row0 + tid + 256 * k;
+        const float2 me = (FULL || r < N) ? cols[r] : make_float2(0.0f, 0.0f);
+        hi[k] = me.x;
+        si[k] = me.y;
+        r32[k] = 0.0f;
+        r64[k] = 0.0;
+    }
+    for (int j0 = 0; j0 < ncol; j0 += 64) {
+        const int jn = min(j0 + 64, ncol);
+        for (int jj = j0; jj < jn; jj++) {
+            const float2 cj = cols[col0 + jj];
+            float csum = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kHuRows; k++) {
+                const float q = fabsf(si[k] - cj.y) * __builtin_amdgcn_rcpf((si[k] + cj.y) + eps);
+                const float pw = __builtin_amdgcn_exp2f(fmaf(-kLog2_100, q, kLog2_100));  // 100^(1 - q)
+                const float a = fabsf(cj.x - hi[k]), b = pw - 1.0f;
+                r32[k] = fmaf(a, b, r32[k]);
+                if (!diag) csum = (FULL || row0 + tid + 256 * k < N) ? fmaf(a, b, csum) : csum;
+            }
+            if (!diag) {
+#pragma unroll
+                for (int m = 32; m >= 1; m >>= 1) csum += __shfl_xor(csum, m);
+                if (lane == 0) colacc[wave][jj] = csum;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kHuRows; k++) {
+            r64[k] += (double)r32[k];
+            r32[k] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kHuRows; k++) {
+        const int r = row0 + tid + 256 * k;
+        if (FULL || r < N) part[(size_t)c * N + r] = r64[k];
+    }
+    if (!diag) {
+        __syncthreads();
+        for (int jj = tid; jj < ncol; jj += 256)
+            part[(size_t)p * N + col0 + jj] =
+                (((double)colacc[0][jj] + (double)colacc[1][jj]) + (double)colacc[2][jj]) + (double)colacc[3][jj];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hu_pairs_sym(const float2 *__restrict__ cols, int N, float eps,
+                                                      double *__restrict__ part)
+{
+    __shared__ float colacc[4][kHuPanel];
+    // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
+    const int q = blockIdx.x;
+    int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+    while ((c + 1) * (c + 2) / 2 <= q) c++;
+    while (c * (c + 1) / 2 > q) c--;
+    const int p = q - c * (c + 1) / 2;
+    if ((p + 1) * kHuPanel <= N) hu_sym_panel<true>(cols, N, eps, p, c, part, colacc);
+    else hu_sym_panel<false>(cols, N, eps, p, c, part, colacc);
+}
+
 static int hu_invariants_launch(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means,
                                 const float *d_stds, float *d_hu_map, float2 *d_cols)
 {
@@ -264,6 +350,22 @@ int global_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
     SRED_TRY(ctx->pm_cols.ensure(sizeof(float2) * (size_t)ncols));
     float2 *d_cols = ctx->pm_cols.as<float2>();
     SRED_TRY(hu_invariants_launch(ctx, g, d_img, d_means, d_stds, nullptr, d_cols));
+    // the full map: each unordered panel pair once, while its partial sums
+    // (npanels x N doubles) stay within the context's partial-sum budget
+    const int64_t npan = (ncols + kHuPanel - 1) / kHuPanel;
+    if (!pr.d_rowpix && npan * ncols * (int64_t)sizeof(double) <= (ctx->partial_mb << 20)) {
+        SRED_TRY(ctx->pm_part.ensure(sizeof(double) * (size_t)npan * ncols));
+        double *d_part = ctx->pm_part.as<double>();
+        SRED_TRY(time_begin(ctx));
+        hipLaunchKernelGGL(k_hu_pairs_sym, dim3((unsigned)(npan * (npan + 1) / 2)), dim3(256), 0, ctx->stream, d_cols,
+                           ncols, kEps, d_part);
+        SRED_HIP_CHECK(hipGetLastError());
+        SRED_TRY(time_end(ctx, (int64_t)pr.R * ncols));
+        hipLaunchKernelGGL(k_pm_finish, dim3((pr.R + 255) / 256), dim3(256), 0, ctx->stream, d_part, pr.R, (int)npan,
+                           pr.d_rowpix, iw, g.w, g.brw, g.brh, 1.0 / 99.0, d_out);
+        SRED_HIP_CHECK(hipGetLastError());
+        return SRED_OK;
+    }
     plan_chunks(&pr, ncols, 16384);
     SRED_TRY(ctx->pm_part.ensure(sizeof(double) * (size_t)pr.nchunks * pr.R));
     double *d_part = ctx->pm_part.as<double>();
