@@ -268,3 +268,22 @@ def test_global_superpanels_uneven(metric):
     one, out, launches = _superpanel_run(img, metric, 0.0, 1)
     assert launches == 30 * 31 // 2
     assert np.abs(out - one).max() <= 2e-6
+
+
+def test_global_2048_b11_default_budget():
+    """2048 x 2048, 11x11 (4.16e6 blocks): its partial sums (34 GB in one
+    launch) exceed the default 16 GB budget, so the call runs as two
+    super-panels (three launches); the map matches FP64 on 8 seeded pixels
+    plus its extremes."""
+    from tests.helpers import fp64_map_at, sample_pixels
+    img = O.normalize_input(synthetic.lattice_and_blobs(2048, seed=2, n_blobs=3600))
+    with sred_amd.Context(0) as c2:
+        out = sred_amd.global_repetition_2d(img, 11, 11, 0.0, "Pearson's R", resources=c2)
+        assert c2.last_contraction()[1] == 3
+    m32, s32 = O.local_stats_f32(img, 11, 11)
+    mask, _, _ = O.relevance_mask(s32, 5, 5, O.noise_var(img), 0.0)
+    px = sample_pixels(out, mask, 8, seed=11)
+    ref, _ = fp64_map_at(O, 0, img, 11, 11, 0.0, px)
+    err = np.abs(out.ravel()[px] - ref).max()
+    print(f"2048^2 b11 Pearson (super-panels) vs FP64 at {px.size} pixels: max {err:.2e}")
+    assert err <= MAP_TOL
