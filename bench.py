@@ -356,7 +356,9 @@ def pairmaps_run(args):
         "kernel_ms": k, "max_rel_err_vs_oracle_8_rows": rel,
         "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
                      "traffic": None,
-                     "kernel": "k_hu_pairs_sym (each unordered pair once)" if hu else "k_phase_pairs<5, 5, true>",
+                     "kernel": ("k_hu_pairs_sym" if hu else "k_phase_pairs_sym<5, 5, true>")
+                               + " (each unordered pair once)",
+                     "issued_tflops": ach / 2.0,
                      "flops_per_pair": flops_pair},
         "cpu_baseline": cpu}))
 

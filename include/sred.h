@@ -475,7 +475,8 @@ int sred_global_hu_2d_dev(sred_ctx *ctx, const float *d_img, const float *d_mean
  * kernel does not compile (nUnique undefined, :27) and its DFT uses integer
  * phases (:66-67); this is textbook phase correlation on the same blocks,
  * FP64: b = (pixel - mean) / (std + 1e-7) on the bw x bh rectangle, unit
- * spectra U = F / (|F| + 1e-7), R = U_i conj(U_j), r = Re IDFT(R); the first
+ * spectra U = F / (|F| + 1e-7), R = U_i conj(U_j) with i the lower interior
+ * index of the pair (one displacement per pair), r = Re IDFT(R); the first
  * strict positive maximum of r in raster order gives the displacement d,
  * wrapped to [-bw/2, bw/2] x [-bh/2, bh/2] (0 if r has no positive value);
  * map[i] = sum over interior j of |d_ij| * w_ij with the kernel's weight

@@ -892,7 +892,10 @@ void or_global_hu_sums(const float *hu, const float *stds, int w, int h, int bw,
  *   F        = 2D DFT of b, F(k) = sum b(x) exp(-2 pi i (kx x / bw + ky y / bh));
  *   U        = F / (|F| + eps), the block's unit spectrum (0 where F is 0);
  *   R(k)     = U_i(k) conj(U_j(k)), the normalised cross power spectrum
- *              (F_i conj F_j / |F_i conj F_j| wherever F_i, F_j >> eps);
+ *              (F_i conj F_j / |F_i conj F_j| wherever F_i, F_j >> eps),
+ *              with i the lower and j the higher interior index of the pair,
+ *              so each unordered pair has one displacement (|d| is the same
+ *              for map[i] and map[j]);
  *   r(x, y)  = Re IDFT(R) = (1/n) Re sum_k R(k) exp(+2 pi i (...));
  *   peak     = first strict maximum of r over (y, x) in raster order, counted
  *              only if positive (:145-157, "zero ... no correlation");
@@ -1043,7 +1046,9 @@ void or_phase_map(const float *img, const float *means, const float *stds, int w
             double acc = 0.0;
             for (int64_t j = 0; j < ni; j++) {
                 int dx, dy;
-                phase_peak(sre + i * n, sim + i * n, sre + j * n, sim + j * n, bw, bh, c, s, Rr, Ri, NULL, &dx, &dy);
+                const int64_t lo = j < i ? j : i, hi = j < i ? i : j;  /* lower interior index first */
+                phase_peak(sre + lo * n, sim + lo * n, sre + hi * n, sim + hi * n, bw, bh, c, s, Rr, Ri, NULL, &dx,
+                           &dy);
                 int64_t q = (int64_t)(brh + j / iw) * w + (brw + j % iw);
                 acc += sqrt((double)(dx * dx + dy * dy)) * or_phase_weight(stds[p], stds[q]);
             }

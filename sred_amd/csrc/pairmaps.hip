@@ -10,7 +10,8 @@
 //     undefined at :27) and its DFT phases use integer division (:66-67).
 //     The specification implemented here is textbook phase correlation on the
 //     same blocks: b = (pixel - mean) / (std + eps) on the rectangle, unit
-//     spectra U = F / (|F| + eps) of the 2D DFT F, R = U_i conj(U_j),
+//     spectra U = F / (|F| + eps) of the 2D DFT F, R = U_i conj(U_j) with i
+//     the lower interior index of the pair (one displacement per pair),
 //     r = Re IDFT(R), the first strict positive maximum of r in raster order,
 //     its coordinates wrapped to signed displacements d (a circular shift s
 //     of block j against block i peaks at -s), map[i] = sum over interior j
@@ -442,15 +443,84 @@ __global__ __launch_bounds__(256) void k_phase_rows(const double2 *__restrict__ 
 // wave-uniform.  UREG: the row spectrum is held in registers (small blocks)
 // or read per pair from rspec (coalesced over lanes).  Products and sums are
 // contracted to FMAs (r only decides the peak; the weight is exact either way).
+// Displacement |d| of a pair from the half-plane spectra of its lower (lo)
+// and higher (hi) interior index, given as functions of m returning (re, im):
+// R = U_lo conj(U_hi), r = Re IDFT(R), the first strict positive maximum of
+// r in raster order, wrapped to a signed displacement.
+template <int BW, int BH, class LO, class HI>
+__device__ __forceinline__ double phase_dist(LO &&lo_at, HI &&hi_at, const PhaseTw &tw)
+{
+    constexpr int BRW = BW / 2, BRH = BH / 2, NT = BW * (BRH + 1);
+    double Rr[NT], Ri[NT];
+#pragma unroll
+    for (int m = 0; m < NT; m++) {
+        const double2 u = lo_at(m);
+        const double2 c = hi_at(m);
+        Rr[m] = u.x * c.x + u.y * c.y;
+        Ri[m] = u.y * c.x - u.x * c.y;
+    }
+    double best = 0.0;
+    int px = 0, py = 0;
+#pragma unroll
+    for (int y = 0; y < BH; y++) {
+        double Vr[BW], Vi[BW];
+#pragma unroll
+        for (int kx = 0; kx < BW; kx++) {
+            double vr = Rr[kx], vi = Ri[kx];
+#pragma unroll
+            for (int ky = 1; ky <= BRH; ky++) {
+                const int q = (ky * y) % BH;
+                const double rr = Rr[ky * BW + kx], ri = Ri[ky * BW + kx];
+                if (q == 0) {
+                    vr += 2.0 * rr;
+                    vi += 2.0 * ri;
+                } else {
+                    vr += rr * tw.ch2[q] - ri * tw.sh2[q];
+                    vi += rr * tw.sh2[q] + ri * tw.ch2[q];
+                }
+            }
+            Vr[kx] = vr;
+            Vi[kx] = vi;
+        }
+#pragma unroll
+        for (int x = 0; x < BW; x++) {
+            double v = Vr[0];
+#pragma unroll
+            for (int kx = 1; kx < BW; kx++) {
+                const int q = (kx * x) % BW;
+                if (q == 0)
+                    v += Vr[kx];
+                else
+                    v += Vr[kx] * tw.cw[q] - Vi[kx] * tw.sw[q];
+            }
+            if (v > best) {
+                best = v;
+                px = x;
+                py = y;
+            }
+        }
+    }
+    const int dx = px <= BRW ? px : px - BW, dy = py <= BRH ? py : py - BH;
+    return sqrt((double)(dx * dx + dy * dy));
+}
+
+// kernelGetPhaseCorrelationMap.cl's weight (:167-178), symmetric in (s_i, s_j)
+__device__ __forceinline__ float phase_weight(float si, float sj)
+{
+    return (si == sj) ? 1.0f : 1.0f - (2.0f * (fabsf(si - sj) / (fabsf(si) + fabsf(sj))));
+}
+
 template <int BW, int BH, bool UREG>
 __global__ __launch_bounds__(64) void k_phase_pairs(const double2 *__restrict__ spec, const float *__restrict__ sig,
                                                     int ncols, const double2 *__restrict__ rspec,
                                                     const float *__restrict__ rsig, int R, int chunk, PhaseTw tw,
+                                                    const int *__restrict__ rowpix, int iw, int w, int brw, int brh,
                                                     double *__restrict__ part)
 {
-    constexpr int BRW = BW / 2, BRH = BH / 2, NT = BW * (BRH + 1);
+    constexpr int BRH = BH / 2, NT = BW * (BRH + 1);
     const int r = blockIdx.x * 64 + threadIdx.x;
     if (r >= R) return;
+    const int ri = interior_index(row_pixel(rowpix, r, iw, w, brw, brh), iw, w, brw, brh);
     double ur[UREG ? NT : 1], ui[UREG ? NT : 1];
     if constexpr (UREG) {
 #pragma unroll
@@ -462,74 +532,88 @@ __global__ __launch_bounds__(64) void k_phase_pairs(const double2 *__restrict__ 
     }
     const float si = rsig[r];
     const int c0 = blockIdx.y * chunk, c1 = min(c0 + chunk, ncols);
+    auto row_at = [&](int m) -> double2 {
+        if constexpr (UREG) return make_double2(ur[m], ui[m]);
+        else return rspec[(size_t)m * R + r];
+    };
     double acc = 0.0;
     for (int j = c0; j < c1; j++) {
         const double2 *uj = spec + (size_t)j * NT;
-        // R = U_i conj(U_j) on the half-plane
-        double Rr[NT], Ri[NT];
-#pragma unroll
-        for (int m = 0; m < NT; m++) {
-            double a, b;
-            if constexpr (UREG) {
-                a = ur[m];
-                b = ui[m];
-            } else {
-                const double2 u = rspec[(size_t)m * R + r];
-                a = u.x;
-                b = u.y;
-            }
-            const double2 c = uj[m];
-            Rr[m] = a * c.x + b * c.y;
-            Ri[m] = b * c.x - a * c.y;
-        }
-        double best = 0.0;
-        int px = 0, py = 0;
-#pragma unroll
-        for (int y = 0; y < BH; y++) {
-            double Vr[BW], Vi[BW];
-#pragma unroll
-            for (int kx = 0; kx < BW; kx++) {
-                double vr = Rr[kx], vi = Ri[kx];
-#pragma unroll
-                for (int ky = 1; ky <= BRH; ky++) {
-                    const int q = (ky * y) % BH;
-                    const double rr = Rr[ky * BW + kx], ri = Ri[ky * BW + kx];
-                    if (q == 0) {
-                        vr += 2.0 * rr;
-                        vi += 2.0 * ri;
-                    } else {
-                        vr += rr * tw.ch2[q] - ri * tw.sh2[q];
-                        vi += rr * tw.sh2[q] + ri * tw.ch2[q];
-                    }
-                }
-                Vr[kx] = vr;
-                Vi[kx] = vi;
-            }
-#pragma unroll
-            for (int x = 0; x < BW; x++) {
-                double v = Vr[0];
-#pragma unroll
-                for (int kx = 1; kx < BW; kx++) {
-                    const int q = (kx * x) % BW;
-                    if (q == 0)
-                        v += Vr[kx];
-                    else
-                        v += Vr[kx] * tw.cw[q] - Vi[kx] * tw.sw[q];
-                }
-                if (v > best) {
-                    best = v;
-                    px = x;
-                    py = y;
-                }
-            }
-        }
-        const int dx = px <= BRW ? px : px - BW, dy = py <= BRH ? py : py - BH;
-        const double dist = sqrt((double)(dx * dx + dy * dy));
-        const float sj = sig[j];
-        const float wgt = (si == sj) ? 1.0f : 1.0f - (2.0f * (fabsf(si - sj) / (fabsf(si) + fabsf(sj))));
-        acc += dist * (double)wgt;
+        // the pair's lower interior index leads (per lane: j < ri swaps the roles)
+        const bool swap = j < ri;
+        const double dist = phase_dist<BW, BH>([&](int m) { return swap ? uj[m] : row_at(m); },
+                                               [&](int m) { return swap ? row_at(m) : uj[m]; }, tw);
+        acc += dist * (double)phase_weight(si, sig[j]);
     }
     part[(size_t)blockIdx.y * R + r] = acc;
+}
+
+// Symmetric form of k_phase_pairs for the full map: the pair's displacement
+// is taken with its lower interior index first, so each unordered pair has
+// one term, evaluated once here.  One workgroup per pair of 256-pixel panels
+// (p, c), c >= p, thread t holding row p P + t; per column j of panel c
+// (wave-uniform) the thread adds the term to its row and the wave reduces
+// it (butterfly, FP64) into its own LDS row of column sums.  Off the
+// diagonal every row index is below every column index; on it only pairs
+// i < j are taken.  Row totals go to part[c][i] and the four waves' column
+// totals to part[p][j] (on the diagonal both to part[p][i]): one writer per
+// slot, summed in slot order by k_pm_finish.
+constexpr int kPhPanel = 256;
+template <int BW, int BH, bool UREG>
+__global__ __launch_bounds__(256) void k_phase_pairs_sym(const double2 *__restrict__ spec,
+                                                         const float *__restrict__ sig, int N,
+                                                         const double2 *__restrict__ rspec, PhaseTw tw,
+                                                         double *__restrict__ part)
+{
+    constexpr int NT = BW * (BH / 2 + 1);
+    __shared__ double colacc[4][kPhPanel];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // (p, c) from the chunk-major triangular index q = c(c+1)/2 + p
+    const int q = blockIdx.x;
+    int c = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+    while ((c + 1) * (c + 2) / 2 <= q) c++;
+    while (c * (c + 1) / 2 > q) c--;
+    const int p = q - c * (c + 1) / 2;
+    const bool diag = p == c;
+    const int i = p * kPhPanel + tid;
+    const bool live = i < N;
+    double ur[UREG ? NT : 1], ui[UREG ? NT : 1];
+    if constexpr (UREG) {
+#pragma unroll
+        for (int m = 0; m < NT; m++) {
+            const double2 u = live ? rspec[(size_t)m * N + i] : make_double2(0.0, 0.0);
+            ur[m] = u.x;
+            ui[m] = u.y;
+        }
+    }
+    auto row_at = [&](int m) -> double2 {
+        if constexpr (UREG) return make_double2(ur[m], ui[m]);
+        else return rspec[(size_t)m * N + i];
+    };
+    const float si = live ? sig[i] : 0.0f;
+    const int col0 = c * kPhPanel, ncol = min(kPhPanel, N - col0);
+    double racc = 0.0;
+    for (int jj = 0; jj < ncol; jj++) {
+        const int j = col0 + jj;
+        const double2 *uj = spec + (size_t)j * NT;
+        double t = 0.0;
+        if (live && (!diag || i < j))
+            t = phase_dist<BW, BH>(row_at, [&](int m) { return uj[m]; }, tw) * (double)phase_weight(si, sig[j]);
+        racc += t;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) t += __shfl_xor(t, m);
+        if (lane == 0) colacc[wave][jj] = t;
+    }
+    __syncthreads();
+    if (diag) {
+        if (live)
+            part[(size_t)p * N + i] =
+                racc + (((colacc[0][tid] + colacc[1][tid]) + colacc[2][tid]) + colacc[3][tid]);
+    } else {
+        if (live) part[(size_t)c * N + i] = racc;
+        for (int jj = tid; jj < ncol; jj += 256)
+            part[(size_t)p * N + col0 + jj] = ((colacc[0][jj] + colacc[1][jj]) + colacc[2][jj]) + colacc[3][jj];
+    }
 }
 
 struct PhaseLaunch {
@@ -542,6 +626,8 @@ struct PhaseLaunch {
     PairRows pr;
     PhaseTw tw;
     double *part;
+    int brw, brh;
+    int npan;  // > 0: the symmetric kernel over npan panels (part is [npan][ncols])
 };
 
 template <int BW, int BH>
@@ -556,8 +642,13 @@ static int phase_run(sred_ctx *ctx, const PhaseLaunch &L, int brw, int brh)
                        L.pr.d_rowpix, L.pr.R, L.iw, L.w, brw, brh, L.rspec, L.rsig);
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_begin(ctx));
-    hipLaunchKernelGGL((k_phase_pairs<BW, BH, UREG>), dim3((L.pr.R + 63) / 64, L.pr.nchunks), dim3(64), 0,
-                       ctx->stream, L.spec, L.sig, L.ncols, L.rspec, L.rsig, L.pr.R, L.pr.chunk, L.tw, L.part);
+    if (L.npan > 0)
+        hipLaunchKernelGGL((k_phase_pairs_sym<BW, BH, UREG>), dim3((unsigned)((int64_t)L.npan * (L.npan + 1) / 2)),
+                           dim3(256), 0, ctx->stream, L.spec, L.sig, L.ncols, L.rspec, L.tw, L.part);
+    else
+        hipLaunchKernelGGL((k_phase_pairs<BW, BH, UREG>), dim3((L.pr.R + 63) / 64, L.pr.nchunks), dim3(64), 0,
+                           ctx->stream, L.spec, L.sig, L.ncols, L.rspec, L.rsig, L.pr.R, L.pr.chunk, L.tw,
+                           L.pr.d_rowpix, L.iw, L.w, L.brw, L.brh, L.part);
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_end(ctx, (int64_t)L.pr.R * L.ncols));
     return SRED_OK;
@@ -614,11 +705,18 @@ int phase_map_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
         L.tw.sh2[q] = 2.0 * std::sin(two_pi * (double)q / (double)g.bh);
     }
     plan_chunks(&L.pr, L.ncols, 4096);
+    L.brw = g.brw;
+    L.brh = g.brh;
+    // the full map: each unordered panel pair once, while its partial sums
+    // (npanels x ncols doubles) stay within the context's partial-sum budget
+    const int64_t npan = (L.ncols + kPhPanel - 1) / kPhPanel;
+    L.npan = (!L.pr.d_rowpix && npan * L.ncols * (int64_t)sizeof(double) <= (ctx->partial_mb << 20)) ? (int)npan : 0;
+    const int nslots = L.npan > 0 ? L.npan : L.pr.nchunks;
     // scratch: [twiddles | spec | rspec | part | sig | rsig]
     const size_t b_tw = sizeof(double) * 2 * n;
     const size_t b_spec = sizeof(double2) * (size_t)L.ncols * NT;
     const size_t b_rspec = sizeof(double2) * (size_t)L.pr.R * NT;
-    const size_t b_part = sizeof(double) * (size_t)L.pr.nchunks * L.pr.R;
+    const size_t b_part = sizeof(double) * (size_t)nslots * L.pr.R;
     const size_t b_sig = sizeof(float) * ((size_t)L.ncols + 64), b_rsig = sizeof(float) * ((size_t)L.pr.R + 64);
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     SRED_TRY(ctx->pm_spec.ensure(al(b_tw) + al(b_spec) + al(b_rspec) + al(b_part) + al(b_sig) + al(b_rsig)));
@@ -650,8 +748,8 @@ int phase_map_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
         (void)hipStreamSynchronize(ctx->stream);  // tab must outlive the copy
         return rc;
     }
-    hipLaunchKernelGGL(k_pm_finish, dim3((L.pr.R + 255) / 256), dim3(256), 0, ctx->stream, L.part, L.pr.R,
-                       L.pr.nchunks, L.pr.d_rowpix, iw, g.w, g.brw, g.brh, 1.0, d_out);
+    hipLaunchKernelGGL(k_pm_finish, dim3((L.pr.R + 255) / 256), dim3(256), 0, ctx->stream, L.part, L.pr.R, nslots,
+                       L.pr.d_rowpix, iw, g.w, g.brw, g.brh, 1.0, d_out);
     SRED_HIP_CHECK(hipGetLastError());
     SRED_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // tab must outlive the copy
     return SRED_OK;

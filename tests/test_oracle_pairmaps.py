@@ -229,8 +229,12 @@ def test_phase_map_numpy_bruteforce():
     F = np.fft.fft2(blocks)
     U = F / (np.abs(F) + EPS)
     want = np.zeros((h, w))
+    idx = np.arange(len(pix))
     for i, (y, x) in enumerate(pix):
-        r = np.fft.ifft2(U[i][None] * np.conj(U)).real.reshape(len(pix), -1)
+        # each pair's lower interior index leads: R = U_lo conj(U_hi)
+        lo_first = np.fft.ifft2(U[i][None] * np.conj(U)).real.reshape(len(pix), -1)
+        hi_first = np.fft.ifft2(U * np.conj(U[i])[None]).real.reshape(len(pix), -1)
+        r = np.where((idx >= i)[:, None], lo_first, hi_first)
         k = np.argmax(r, axis=1)
         pos = r[np.arange(len(pix)), k] > 0
         py, px = np.divmod(k, bw)
