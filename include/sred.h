@@ -494,7 +494,7 @@ int sred_phase_correlation_2d_dev(sred_ctx *ctx, const float *d_img, const float
  * the normalised input pixels, by a hand-written adjoint.  Held fixed: the
  * noise variance, nPixels, the relevance mask and the output min/max; flat
  * blocks (std == 0) pass no gradient through their statistics.  K <= 160
- * (the pair pass on MFMA up to K = 128, on the vector ALU beyond). */
+ * (the pair pass in split BF16 on MFMA). */
 int sred_global_pearson_adjoint_2d(sred_ctx *ctx, const float *img, int w, int h, int bw, int bh,
                                    const float *dL_dmap, float *dL_dimg);
 /* Same on device buffers; d_map (optional) receives the forward map. */

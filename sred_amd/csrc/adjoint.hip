@@ -92,148 +92,8 @@ __global__ void k_pix2v(const int *__restrict__ idx, int Nv, int *__restrict__ p
     if (v < Nv) pix2v[idx[v]] = v;
 }
 
-// Row-stationary adjoint pass: 64 rows per workgroup against every column
-// tile.  UT is the Pearson-scaled block matrix u = (x - mu)/(s sqrt(K-1)),
-// k-major [K4][Nv_pad]; Gc receives, per row, the centred gradient of its K
-// block pixels ([Nv_pad][K16], zero for flat and padding rows).
-template <int M16>
-__global__ __launch_bounds__(256) void k_adjoint_f32(const float *__restrict__ UT, const float4 *__restrict__ info,
-                                                     int K, int K4, int Nv, int Nv_pad, float eps, float m2_over_H,
-                                                     float inv_sqrt_km1, float *__restrict__ Gc)
-{
-    constexpr int K16 = 16 * M16;            // k values: tx + 16 m, m < M16
-    extern __shared__ float sm[];
-    float *As = sm;                          // [K4][64]
-    float *Bs = As + (size_t)K4 * 64;        // [K4][64]
-    float *Bt = Bs + (size_t)K4 * 64;        // [64][K16 + 1]
-    float *Pt = Bt + (size_t)64 * (K16 + 1); // [64][65]
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const int row0 = blockIdx.x * 64;
-
-    for (int i = tid; i < K4 * 16; i += 256) {
-        int k = i >> 4, q = i & 15;
-        *reinterpret_cast<float4 *>(&As[k * 64 + q * 4]) =
-            *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + row0 + q * 4]);
-    }
-    float4 Ri[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) Ri[i] = info[row0 + ty * 4 + i];
-
-    double gs[4] = {0, 0, 0, 0};
-    double ga[4][M16];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int m = 0; m < M16; m++) ga[i][m] = 0.0;
-
-    for (int c0 = 0; c0 < Nv_pad; c0 += 64) {
-        __syncthreads();
-        for (int i = tid; i < K4 * 16; i += 256) {
-            int k = i >> 4, q = i & 15;
-            float4 v = *reinterpret_cast<const float4 *>(&UT[(size_t)k * Nv_pad + c0 + q * 4]);
-            *reinterpret_cast<float4 *>(&Bs[k * 64 + q * 4]) = v;
-            Bt[(q * 4 + 0) * (K16 + 1) + k] = v.x;
-            Bt[(q * 4 + 1) * (K16 + 1) + k] = v.y;
-            Bt[(q * 4 + 2) * (K16 + 1) + k] = v.z;
-            Bt[(q * 4 + 3) * (K16 + 1) + k] = v.w;
-        }
-        for (int i = tid; i < 64 * (K16 - K4); i += 256) {  // zero k padding of Bt
-            int j = i / (K16 - K4), k = K4 + i % (K16 - K4);
-            Bt[j * (K16 + 1) + k] = 0.0f;
-        }
-        __syncthreads();
-        float d[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) d[i][j] = 0.0f;
-        for (int k = 0; k < K4; k++) {
-            float4 a = *reinterpret_cast<const float4 *>(&As[k * 64 + ty * 4]);
-            float4 b = *reinterpret_cast<const float4 *>(&Bs[k * 64 + tx * 4]);
-            float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) d[i][j] = fmaf(av[i], bv[j], d[i][j]);
-        }
-        float4 Cj[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) Cj[j] = info[c0 + tx * 4 + j];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            float gsi = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float si = Ri[i].x, sj = Cj[j].x;
-                const float dt = Ri[i].w - Cj[j].w;
-                const float w = __builtin_amdgcn_exp2f(-(dt * dt));
-                const float A = Ri[i].y + Cj[j].y;
-                float r = 0.0f, M = 0.0f;
-                if (si != 0.0f && sj != 0.0f) {
-                    const float ss = si * sj;
-                    const float den = ss + eps;
-                    r = d[i][j] * ss / den;      // cov / (s_i s_j + eps)
-                    if (r > 0.0f) M = w / den;
-                    else r = 0.0f;
-                }
-                Pt[(ty * 4 + i) * 65 + tx * 4 + j] = A * M * sj;
-                gsi += -A * M * r * sj + w * m2_over_H * (si - sj) * (A * r + Ri[i].z + Cj[j].z);
-            }
-            gs[i] += (double)gsi;
-        }
-        __syncthreads();
-        float gt[4][M16];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int m = 0; m < M16; m++) gt[i][m] = 0.0f;
-        for (int j = 0; j < 64; j++) {
-            float bj[M16];
-#pragma unroll
-            for (int m = 0; m < M16; m++) bj[m] = Bt[j * (K16 + 1) + tx + 16 * m];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float pij = Pt[(ty * 4 + i) * 65 + j];
-#pragma unroll
-                for (int m = 0; m < M16; m++) gt[i][m] = fmaf(pij, bj[m], gt[i][m]);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int m = 0; m < M16; m++) ga[i][m] += (double)gt[i][m];
-    }
-    // dL/ds_i: reduce over the 16 column lanes
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int m = 1; m < 16; m <<= 1) gs[i] += __shfl_xor(gs[i], m);
-    // per row: G_k = (sum_j P''_ij u_jk + dL/ds_i u_ik) / sqrt(K-1), centred over k < K
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int v = row0 + ty * 4 + i;
-        const bool live = v < Nv && Ri[i].x != 0.0f;
-        double gk[M16], part = 0.0;
-#pragma unroll
-        for (int m = 0; m < M16; m++) {
-            int k = tx + 16 * m;
-            double uik = (k < K4) ? (double)As[k * 64 + ty * 4 + i] : 0.0;
-            gk[m] = (k < K) ? (ga[i][m] + gs[i] * uik) * (double)inv_sqrt_km1 : 0.0;
-            part += gk[m];
-        }
-#pragma unroll
-        for (int msk = 1; msk < 16; msk <<= 1) part += __shfl_xor(part, msk);
-        const double mean = part / (double)K;
-#pragma unroll
-        for (int m = 0; m < M16; m++) {
-            int k = tx + 16 * m;
-            Gc[(size_t)v * K16 + k] = (live && k < K) ? (float)(gk[m] - mean) : 0.0f;
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Split-BF16 MFMA form of the adjoint pass (blocks up to K = 128; 13x13 is 113).
+// The adjoint pass, split-BF16 on MFMA (every block the adjoint takes, up to K = 160).
 //
 // Per 64-column tile a wave (32 rows i) computes the transposed dot-product
 // tile D[j][i] = u_j . u_i with v_mfma_f32_16x16x32_bf16 (A = the column
@@ -404,9 +264,12 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
     constexpr int NRUN = 8 * KS + 4 * KG;                  // runs of 64 uint4 copied per tile
     static_assert(NRUN % 4 == 0, "tile runs split over 4 waves");
     // two tile buffers (distinct arrays: reading one does not wait for the copy into the other)
+    // two tile buffers (distinct arrays: reading one does not wait for the
+    // copy into the other) up to K = 128; one beyond (K = 160: 81 KiB)
+    constexpr bool DB = KS <= 4;
     __shared__ uint4 uh0[URUN], ul0[URUN], vh0[VRUN], vl0[VRUN];
-    __shared__ uint4 uh1[URUN], ul1[URUN], vh1[VRUN], vl1[VRUN];
-    __shared__ float4 ci0[64], ci1[64];
+    __shared__ uint4 uh1[DB ? URUN : 1], ul1[DB ? URUN : 1], vh1[DB ? VRUN : 1], vl1[DB ? VRUN : 1];
+    __shared__ float4 ci0[64], ci1[DB ? 64 : 1];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lq = lane >> 4;
@@ -662,10 +525,21 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
         if (T + 1 < ntiles) copy(T + 1, nuh, nul, nvh, nvl, nci);
         compute(T, uh, ul, vh, vl, ci);
     };
-    copy(0, uh0, ul0, vh0, vl0, ci0);
-    for (int T = 0; T < ntiles; T += 2) {
-        step(T, uh0, ul0, vh0, vl0, ci0, uh1, ul1, vh1, vl1, ci1);
-        if (T + 1 < ntiles) step(T + 1, uh1, ul1, vh1, vl1, ci1, uh0, ul0, vh0, vl0, ci0);
+    if constexpr (DB) {
+        copy(0, uh0, ul0, vh0, vl0, ci0);
+        for (int T = 0; T < ntiles; T += 2) {
+            step(T, uh0, ul0, vh0, vl0, ci0, uh1, ul1, vh1, vl1, ci1);
+            if (T + 1 < ntiles) step(T + 1, uh1, ul1, vh1, vl1, ci1, uh0, ul0, vh0, vl0, ci0);
+        }
+    } else {
+        for (int T = 0; T < ntiles; T++) {
+            adj_wait_all();
+            __syncthreads();  // every wave is done with tile T - 1
+            copy(T, uh0, ul0, vh0, vl0, ci0);
+            adj_wait_all();
+            __syncthreads();  // tile T has landed
+            compute(T, uh0, ul0, vh0, vl0, ci0);
+        }
     }
 
     // per row i: G_k = (tot_k / s_i + dL/ds_i u_ik) / sqrt(K-1), centred over k < K.
@@ -755,34 +629,6 @@ __global__ __launch_bounds__(256) void k_adjoint_scatter(const float *__restrict
     }
 }
 
-template <int M16>
-static void launch_adjoint_m(dim3 grid, size_t lds, hipStream_t st, const float *UT, const float4 *info, int K, int K4,
-                             int Nv, int Nv_pad, float eps, float m2H, float isk, float *Gc)
-{
-    hipLaunchKernelGGL((k_adjoint_f32<M16>), grid, dim3(256), lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc);
-}
-
-static int launch_adjoint(int M16, dim3 grid, size_t lds, hipStream_t st, const float *UT, const float4 *info, int K,
-                          int K4, int Nv, int Nv_pad, float eps, float m2H, float isk, float *Gc)
-{
-    switch (M16) {
-    case 1: launch_adjoint_m<1>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 2: launch_adjoint_m<2>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 3: launch_adjoint_m<3>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 4: launch_adjoint_m<4>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 5: launch_adjoint_m<5>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 6: launch_adjoint_m<6>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 7: launch_adjoint_m<7>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 8: launch_adjoint_m<8>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 9: launch_adjoint_m<9>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    case 10: launch_adjoint_m<10>(grid, lds, st, UT, info, K, K4, Nv, Nv_pad, eps, m2H, isk, Gc); break;
-    default:
-        set_error("the adjoint supports blocks up to K = 160 pixels");
-        return SRED_ERR_ARG;
-    }
-    return SRED_OK;
-}
-
 template <int KS, int KG>
 static void launch_adjoint_mfma_t(dim3 grid, hipStream_t st, const uint4 *uhi, const uint4 *ulo, const uint4 *vhi,
                                   const uint4 *vlo, const float4 *info2, const int *small, const double *F,
@@ -817,6 +663,9 @@ static int launch_adjoint_mfma(int KS, int KG, dim3 grid, hipStream_t st, const 
     SRED_ADJ_CASE(4, 6)
     SRED_ADJ_CASE(4, 7)
     SRED_ADJ_CASE(4, 8)
+    SRED_ADJ_CASE(5, 8)
+    SRED_ADJ_CASE(5, 9)
+    SRED_ADJ_CASE(5, 10)
 #undef SRED_ADJ_CASE
     set_error("internal: no adjoint pair kernel for this block size");
     return SRED_ERR_ARG;
@@ -829,11 +678,6 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     const int K = g.K, K4 = (K + 3) / 4 * 4, K16 = (K + 15) / 16 * 16;
     if (K16 / 16 > 10) {
         set_error("the adjoint supports blocks up to K = 160 pixels");
-        return SRED_ERR_ARG;
-    }
-    const size_t lds = sizeof(float) * ((size_t)2 * K4 * 64 + (size_t)64 * (K16 + 1) + 64 * 65);
-    if (lds > 160 * 1024) {
-        set_error("block too large for the adjoint tile");
         return SRED_ERR_ARG;
     }
     // forward: statistics, noise variance, mask, accumulators and the normalised map
@@ -861,8 +705,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
     SRED_TRY(zero_dev(ctx, d_grad, sizeof(float) * n));
     if (Nv == 0) return SRED_OK;
     const int KS = (K + 31) / 32;
-    const bool mfma = KS <= 4;  // split-BF16 MFMA pass (its fragments and accumulators fit up to K = 128)
-    const int Nv_pad = mfma ? (Nv + kAdjRows - 1) / kAdjRows * kAdjRows : (Nv + 63) / 64 * 64;
+    const int Nv_pad = (Nv + kAdjRows - 1) / kAdjRows * kAdjRows;
     // blocks in ascending sigma: the few tiles that need the exact small-sigma
     // form then hold the small-sigma blocks only (any order gives the same
     // gradient up to rounding; pix2v below follows it)
@@ -894,7 +737,7 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
                       inv_sqrt_km1, (float)(1.0 / std::sqrt((double)K)), ctx->u32.as<float>());
     SRED_TRY(ctx->partial.ensure(sizeof(float) * (size_t)Nv_pad * K16));
     float *d_Gc = ctx->partial.as<float>();
-    if (mfma) {
+    {
         // second product over k < 16 KG (the last component follows from sum_k u_k = 0)
         const int KG = std::max(1, (K - 1 + 15) / 16);
         const size_t u4 = (size_t)Nv_pad * KS * 4;  // uint4 per fragment-major array
@@ -922,12 +765,6 @@ int global_pearson_adjoint_dev(sred_ctx *ctx, const Geometry &g, const float *d_
         const dim3 grid(Nv_pad / kAdjRows);
         SRED_TRY(launch_adjoint_mfma(KS, KG, grid, ctx->stream, d_uhi, d_ulo, d_vhi, d_vlo, d_info2, d_small, d_F,
                                      d_bmax + 1, UTp, K, Nv, Nv_pad, kEps, cE, inv_sqrt_km1, K16, d_Gc));
-        SRED_HIP_CHECK(hipGetLastError());
-        SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
-    } else {
-        SRED_TRY(time_begin(ctx));
-        SRED_TRY(launch_adjoint(K16 / 16, dim3(Nv_pad / 64), lds, ctx->stream, ctx->u32.as<float>(), d_info, K, K4,
-                                Nv, Nv_pad, kEps, (float)(-2.0 / H), inv_sqrt_km1, d_Gc));
         SRED_HIP_CHECK(hipGetLastError());
         SRED_TRY(time_end(ctx, (int64_t)Nv_pad * Nv_pad));
     }

@@ -36,7 +36,8 @@ def gpu_grad(ctx, img, b, weights):
 
 @pytest.mark.parametrize("shape,b,seed", [((26, 28), 5, 4), ((36, 40), 9, 5), ((33, 30), 7, 6), ((40, 44), 11, 7),
                                          ((24, 22), 3, 8), ((46, 44), 13, 9), ((44, 38), (9, 17), 10),
-                                         ((47, 40), (11, 17), 11), ((30, 34), (7, 3), 12)])
+                                         ((47, 40), (11, 17), 11), ((30, 34), (7, 3), 12), ((48, 50), 15, 13),
+                                         ((46, 44), (13, 15), 14)])
 def test_adjoint_matches_fp64_autograd(ctx, shape, b, seed):
     img = O.normalize_input(synthetic.spots(shape[1], shape[0], 10, seed=seed))
     u = np.random.default_rng(seed + 36).random(shape).astype(np.float32)
