@@ -108,9 +108,7 @@ __global__ void k_pix2v(const int *__restrict__ idx, int Nv, int *__restrict__ p
 // second product uses three MFMAs as well; its FP32 accumulators are flushed
 // into FP32 totals every 128 tiles, dL/ds_i into FP64 every tile.
 // ---------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 // hi = RNE bf16 pair of (a, b) (one v_cvt_pk_bf16_f32), lo = the same of the remainders
@@ -194,15 +192,6 @@ __device__ __forceinline__ int adj_dot_exponent(unsigned int amax_bits)
     int ex;
     (void)frexpf(__uint_as_float(amax_bits), &ex);  // max|a| < 2^ex
     return min(max(90 - ex, 0), 40);
-}
-
-// v * s for a power of two s that keeps every element finite (exact)
-__device__ __forceinline__ bf16x8 scale_bf16x8(bf16x8 v, float s)
-{
-    bf16x8 r;
-#pragma unroll
-    for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)v[e] * s);
-    return r;
 }
 
 // small[g] = 1 if a block of [32g, 32g + 32) has 0 < sigma < kAdjFastSigma
@@ -291,8 +280,8 @@ __global__ __launch_bounds__(256, KS <= 2 ? 2 : 1) void k_adjoint_mfma(const uin
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const size_t o = ((size_t)(rowbase / 16 + rg) * KS + ks) * 64 + lane;
-            bh[rg][ks] = scale_bf16x8(__builtin_bit_cast(bf16x8, Uhi[o]), dscale);
-            bl[rg][ks] = scale_bf16x8(__builtin_bit_cast(bf16x8, Ulo[o]), dscale);
+            bh[rg][ks] = bf16x8_scaled(__builtin_bit_cast(bf16x8, Uhi[o]), dscale);
+            bl[rg][ks] = bf16x8_scaled(__builtin_bit_cast(bf16x8, Ulo[o]), dscale);
         }
         R[rg] = info2[rowbase + rg * 16 + li];
         rinv[rg] = R[rg].w;

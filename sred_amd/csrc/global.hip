@@ -46,9 +46,6 @@
 
 namespace sred {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
 constexpr float kPadT = 1e30f;  // t of padding columns: their weight is exp2(-inf) = 0
 // Pearson fast path: with every sigma of a tile >= kFastSigma (or 0),
 // delta = eps/(s_i s_j) <= 3.1e-4 and exp(-delta) differs from 1/(1 + delta)
@@ -548,15 +545,6 @@ __global__ __launch_bounds__(256, 2) void k_contract_mfma(const uint4 *__restric
             for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
             if (li == 0) partial[(size_t)ch * Nv_pad + rowbase + rg * 16 + lk * 4 + r] = s;
         }
-}
-
-// v * s for a power of two s (exact in BF16 while the result stays finite)
-__device__ __forceinline__ bf16x8 bf16x8_scaled(bf16x8 v, float s)
-{
-    bf16x8 r;
-#pragma unroll
-    for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)v[e] * s);
-    return r;
 }
 
 // Folds each metric's affine part into the symmetric kernel's MFMAs (one

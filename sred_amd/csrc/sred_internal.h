@@ -90,6 +90,19 @@ double java_float_literal(float f);
 }  // namespace sred
 
 namespace sred {
+// Operand and accumulator vectors of the v_mfma_f32_16x16x32_bf16 kernels.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// v * s for a power of two s (exact in BF16 while every result stays finite)
+__device__ __forceinline__ bf16x8 bf16x8_scaled(bf16x8 v, float s)
+{
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; e++) r[e] = (__bf16)((float)v[e] * s);
+    return r;
+}
+
 // Epilogue constants of the global contractions.
 struct EpiConst {
     float eps;  // 1e-7
