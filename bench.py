@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: bf16x3 steps only, then a 256 MiB device copy (PMC calibration)")
+    ap.add_argument("--profile-block", type=int, default=B,
+                    help="block side of the Pearson map a --profile run evaluates (odd, default 11)")
     ap.add_argument("--workload", choices=["global", "block", "adjoint", "batch", "hu", "phase", "vst", "quadtree",
                                            "timelapse", "tools", "vol", "block3d", "octree", "vst3d"],
                     default="global",
@@ -995,12 +997,15 @@ def profile_run(args):
     from sred_amd import synthetic
     from sred_amd.commands import global_map_device
 
+    bb = args.profile_block
+    if bb < 3 or bb % 2 == 0:
+        raise SystemExit("--profile-block must be odd and at least 3")
     img = sred_amd.get_input_image_2d(synthetic.lattice_and_blobs(W, seed=2)).image_array
     d_img = torch.from_numpy(img).cuda()
     d_out = torch.zeros_like(d_img)
     with sred_amd.Context(0) as ctx:
         for i in range(args.warmup + args.steps):
-            global_map_device(ctx, L.PEARSON, L.BF16X3, d_img.data_ptr(), W, H, B, B, 0.0, True, d_out.data_ptr())
+            global_map_device(ctx, L.PEARSON, L.BF16X3, d_img.data_ptr(), W, H, bb, bb, 0.0, True, d_out.data_ptr())
             torch.cuda.synchronize()
             print(f"# step {i}: contraction {ctx.last_contraction()[0]:.2f} ms", file=sys.stderr, flush=True)
     n = 64 * 1024 * 1024  # 256 MiB of float32
