@@ -144,7 +144,12 @@ __global__ __launch_bounds__(256) void k_pm_finish(const double *__restrict__ pa
 // integer, so the powers are exact), mu_00 += patch; mu_00^2 rounded to
 // float, "+ EPSILON" in double (an unsuffixed macro literal), quotient
 // rounded to float.  Writes the map (border 0) and/or the column record
-// (hu, std) of every interior pixel (either pointer may be null).
+// (hu, std + EPSILON/2) of every interior pixel (either pointer may be null):
+// with s' = s + EPSILON/2 the pair weight's denominator |s_i| + |s_j| + EPSILON
+// (:141, stds are non-negative) is s'_i + s'_j, one addition per pair fewer
+// and still symmetric in (i, j) bit for bit; the rounding of s' moves
+// |s_i - s_j| by at most one ulp of the larger s (a relative weight change
+// below 2e-7 at s ~ 0.05).
 __global__ __launch_bounds__(256) void k_hu_invariants(const float *__restrict__ img, const float *__restrict__ means,
                                                        const float *__restrict__ stds, int w, int h, int bw, int bh,
                                                        float xavg, float yavg, double eps,
@@ -188,7 +193,7 @@ __global__ __launch_bounds__(256) void k_hu_invariants(const float *__restrict__
     const float inv02 = (float)((double)mu02 / ((double)m2 + eps));
     const float hu = inv20 + inv02;
     if (hu_map) hu_map[p] = hu;
-    if (cols) cols[(int64_t)(gy - brh) * (w - 2 * brw) + (gx - brw)] = make_float2(hu, stds[p]);
+    if (cols) cols[(int64_t)(gy - brh) * (w - 2 * brw) + (gx - brw)] = make_float2(hu, stds[p] + 0.5f * kEps);
 }
 
 // Row sums of |hu_j - hu_i| (100^sim - 1) over the column chunk blockIdx.y
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void k_hu_invariants(const float *__restrict__
 // wave-uniform addresses.
 __global__ __launch_bounds__(256) void k_hu_pairs(const float2 *__restrict__ cols, int ncols,
                                                   const int *__restrict__ rowpix, int R, int iw, int w, int brw,
-                                                  int brh, int chunk, float eps, double *__restrict__ part)
+                                                  int brh, int chunk, double *__restrict__ part)
 {
     const int r = blockIdx.x * 256 + threadIdx.x;
     if (r >= R) return;
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(256) void k_hu_pairs(const float2 *__restrict__ col
 #pragma unroll 8
         for (int j = j0; j < jn; j++) {
             const float2 cj = cols[j];
-            const float q = fabsf(si - cj.y) * __builtin_amdgcn_rcpf((si + cj.y) + eps);
+            const float q = fabsf(si - cj.y) * __builtin_amdgcn_rcpf(si + cj.y);
             const float pw = __builtin_amdgcn_exp2f(fmaf(-kLog2_100, q, kLog2_100));  // 100^(1 - q)
             a = fmaf(fabsf(cj.x - hi), pw - 1.0f, a);
         }
@@ -234,7 +239,7 @@ __global__ __launch_bounds__(256) void k_hu_pairs(const float2 *__restrict__ col
 // last row panel).
 constexpr int kHuPanel = 2048, kHuRows = kHuPanel / 256;
 template <bool FULL>
-__device__ __forceinline__ void hu_sym_panel(const float2 *__restrict__ cols, int N, float eps, int p, int c,
+__device__ __forceinline__ void hu_sym_panel(const float2 *__restrict__ cols, int N, int p, int c,
                                              double *__restrict__ part, float (*colacc)[kHuPanel])
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -258,7 +263,7 @@ __device__ __forceinline__ void hu_sym_panel(const float2 *__restrict__ cols, in
             float csum = 0.0f;
 #pragma unroll
             for (int k = 0; k < kHuRows; k++) {
-                const float q = fabsf(si[k] - cj.y) * __builtin_amdgcn_rcpf((si[k] + cj.y) + eps);
+                const float q = fabsf(si[k] - cj.y) * __builtin_amdgcn_rcpf(si[k] + cj.y);
                 const float pw = __builtin_amdgcn_exp2f(fmaf(-kLog2_100, q, kLog2_100));  // 100^(1 - q)
                 const float a = fabsf(cj.x - hi[k]), b = pw - 1.0f;
                 r32[k] = fmaf(a, b, r32[k]);
@@ -289,7 +294,7 @@ __device__ __forceinline__ void hu_sym_panel(const float2 *__restrict__ cols, in
     }
 }
 
-__global__ __launch_bounds__(256) void k_hu_pairs_sym(const float2 *__restrict__ cols, int N, float eps,
+__global__ __launch_bounds__(256) void k_hu_pairs_sym(const float2 *__restrict__ cols, int N,
                                                       double *__restrict__ part)
 {
     __shared__ float colacc[4][kHuPanel];
@@ -299,8 +304,8 @@ __global__ __launch_bounds__(256) void k_hu_pairs_sym(const float2 *__restrict__
     while ((c + 1) * (c + 2) / 2 <= q) c++;
     while (c * (c + 1) / 2 > q) c--;
     const int p = q - c * (c + 1) / 2;
-    if ((p + 1) * kHuPanel <= N) hu_sym_panel<true>(cols, N, eps, p, c, part, colacc);
-    else hu_sym_panel<false>(cols, N, eps, p, c, part, colacc);
+    if ((p + 1) * kHuPanel <= N) hu_sym_panel<true>(cols, N, p, c, part, colacc);
+    else hu_sym_panel<false>(cols, N, p, c, part, colacc);
 }
 
 static int hu_invariants_launch(sred_ctx *ctx, const Geometry &g, const float *d_img, const float *d_means,
@@ -359,7 +364,7 @@ int global_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
         double *d_part = ctx->pm_part.as<double>();
         SRED_TRY(time_begin(ctx));
         hipLaunchKernelGGL(k_hu_pairs_sym, dim3((unsigned)(npan * (npan + 1) / 2)), dim3(256), 0, ctx->stream, d_cols,
-                           ncols, kEps, d_part);
+                           ncols, d_part);
         SRED_HIP_CHECK(hipGetLastError());
         SRED_TRY(time_end(ctx, (int64_t)pr.R * ncols));
         hipLaunchKernelGGL(k_pm_finish, dim3((pr.R + 255) / 256), dim3(256), 0, ctx->stream, d_part, pr.R, (int)npan,
@@ -372,7 +377,7 @@ int global_hu_dev(sred_ctx *ctx, const Geometry &g, const float *d_img, const fl
     double *d_part = ctx->pm_part.as<double>();
     SRED_TRY(time_begin(ctx));
     hipLaunchKernelGGL(k_hu_pairs, dim3((pr.R + 255) / 256, pr.nchunks), dim3(256), 0, ctx->stream, d_cols, ncols,
-                       pr.d_rowpix, pr.R, iw, g.w, g.brw, g.brh, pr.chunk, kEps, d_part);
+                       pr.d_rowpix, pr.R, iw, g.w, g.brw, g.brh, pr.chunk, d_part);
     SRED_HIP_CHECK(hipGetLastError());
     SRED_TRY(time_end(ctx, (int64_t)pr.R * ncols));
     hipLaunchKernelGGL(k_pm_finish, dim3((pr.R + 255) / 256), dim3(256), 0, ctx->stream, d_part, pr.R, pr.nchunks,
