@@ -94,6 +94,28 @@ def test_global_hu_full_size_sampled_rows(ctx):
     np.testing.assert_allclose(full.reshape(-1)[rows], want, rtol=2e-5)
 
 
+
+def test_global_hu_full_map_ordered_fallback():
+    """A full Hu map whose panel-pair partial sums (npanels x N doubles, panels
+    of 2048 interior pixels) exceed a 1 MB budget runs the ordered-pair
+    kernel over every row instead of the symmetric one: the two maps agree to
+    the FP32 sub-sum order, and the ordered map matches the oracle on sampled
+    rows."""
+    shape, b = (300, 280), 5
+    img = O.normalize_input(synthetic.spots(shape[1], shape[0], 60, seed=31))
+    st, m, s = _stats(img, b, b)
+    n_int = (shape[0] - 2 * (b // 2)) * (shape[1] - 2 * (b // 2))
+    assert -(-n_int // 2048) * n_int * 8 > (1 << 20)
+    with sred_amd.Context(0) as c2:
+        sym = sred_amd.calculate_global_hu_map_2d(_inp(img), st, b, b, c2).reshape(shape)
+        c2.set_option(L.OPT_PARTIAL_MEMORY_MB, 1)
+        ordered = sred_amd.calculate_global_hu_map_2d(_inp(img), st, b, b, c2).reshape(shape)
+    np.testing.assert_allclose(ordered, sym, rtol=1e-5, atol=1e-6 * np.abs(sym).max())
+    rng = np.random.default_rng(31)
+    rows = (rng.integers(b, shape[0] - b, 16) * shape[1] + rng.integers(b, shape[1] - b, 16)).astype(np.int32)
+    want = O.global_hu_map(img, m, s, b, b, rows=rows).reshape(-1)[rows]
+    np.testing.assert_allclose(ordered.reshape(-1)[rows], want, rtol=2e-5)
+
 # ---------------------------------------------------------------- phase correlation
 
 @pytest.mark.parametrize("shape,bw,bh,seed", [((24, 22), 3, 3, 1), ((30, 27), 5, 5, 2), ((26, 31), 3, 7, 3),
@@ -161,6 +183,28 @@ def test_phase_full_size_sampled_rows(ctx):
     want = O.phase_map(img, m, s, b, b, rows=rows).reshape(-1)[rows]
     np.testing.assert_allclose(full[rows], want, rtol=1e-7)
 
+
+
+def test_phase_full_map_ordered_fallback():
+    """A full phase correlation map beyond a 1 MB partial-sum budget (panels
+    of 256 interior pixels) runs the ordered-pair kernel, each lane taking
+    the lower interior index of its pair as the leading spectrum: the same
+    per-pair values as the symmetric kernel, so the maps agree to the FP64
+    summation order; sampled rows match the oracle."""
+    shape, b = (150, 160), 5
+    img = O.normalize_input(synthetic.spots(shape[1], shape[0], 30, seed=33))
+    st, m, s = _stats(img, b, b)
+    n_int = (shape[0] - 2 * (b // 2)) * (shape[1] - 2 * (b // 2))
+    assert -(-n_int // 256) * n_int * 8 > (1 << 20)
+    with sred_amd.Context(0) as c2:
+        sym = sred_amd.calculate_phase_correlation_map_2d(_inp(img), st, b, b, c2).reshape(shape)
+        c2.set_option(L.OPT_PARTIAL_MEMORY_MB, 1)
+        ordered = sred_amd.calculate_phase_correlation_map_2d(_inp(img), st, b, b, c2).reshape(shape)
+    np.testing.assert_allclose(ordered, sym, rtol=1e-12, atol=0)
+    rng = np.random.default_rng(33)
+    rows = (rng.integers(b, shape[0] - b, 8) * shape[1] + rng.integers(b, shape[1] - b, 8)).astype(np.int32)
+    want = O.phase_map(img, m, s, b, b, rows=rows).reshape(-1)[rows]
+    np.testing.assert_allclose(ordered.reshape(-1)[rows], want, rtol=1e-7)
 
 def test_pairmaps_device_path(ctx):
     torch = pytest.importorskip("torch")
